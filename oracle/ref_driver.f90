@@ -1,0 +1,137 @@
+!===============================================================================!
+! swref: a test driver around the UNCHANGED reference modules of sw-mpdec
+! (fv-solver-sw/src/*.f90, compiled where they lie by oracle/Makefile).
+!
+! This program is test infrastructure.  It replaces only the reference's
+! main program and parameter module (src/main.f90, src/parameters.f90): it
+! sets the module globals that InitializeParameters would set, installs an
+! initial state read from a binary file instead of FillInitialConditions,
+! and then drives the reference's own public entry points:
+!   FVTimeDerivative  (src/finitevolume.f90:402)   -> mode 1
+!   TimeStep          (src/equation.f90:791)       -> modes 2 and 3
+!   TimeDiscretization(src/timediscretization.f90:24), one CFL step per call
+!                     (tEnd = dt_min, nOutputFiles = 1)  -> mode 2
+!   jacobi            (src/jacobi_iteration.f90:52) -> mode 4
+! With WELLBALANCED builds the well-balanced reference state is installed the
+! way InitializeWBVariables does it (src/finitevolume.f90:373-392).
+!
+! Usage: swref <input.bin> <output.bin>
+! Input (stream, little endian):
+!   int32 mode, nx, nsteps, bathy_flag, has_wb, reserved
+!   real64 x0, x1, gravity, cfl
+!   real64 U(3,nx,nx)              [mode 1-3]   or  Mass(5nx^2), Diag(nx^2), rhs(nx^2) [mode 4]
+!   real64 Uwb(3,nx,nx)            [if has_wb]
+!===============================================================================!
+PROGRAM swref
+USE MOD_FiniteVolume2D_vars
+USE MOD_FiniteVolume2D,     ONLY: InitializeFiniteVolume
+USE MOD_FiniteVolume2D,     ONLY: FVTimeDerivative
+USE MOD_Mesh,               ONLY: BuildMesh
+USE MOD_Equation,           ONLY: TimeStep
+USE MOD_TimeDiscretization, ONLY: TimeDiscretization
+USE MOD_JacobiIteration,    ONLY: jacobi
+IMPLICIT NONE
+INTEGER            :: mode, nx, nsteps, bflag, haswb, reserved, k, nrow
+REAL               :: x0, x1, grav, cflin, dtmin
+REAL, ALLOCATABLE  :: Uic(:,:,:), Uwb(:,:,:), Mass(:), Diag(:), rhs(:), sol(:)
+CHARACTER(LEN=1024):: fin, fout
+
+CALL get_command_argument(1, fin)
+CALL get_command_argument(2, fout)
+
+OPEN(10, FILE=TRIM(fin), ACCESS='stream', FORM='unformatted', STATUS='old')
+READ(10) mode, nx, nsteps, bflag, haswb, reserved
+READ(10) x0, x1, grav, cflin
+
+#ifdef WELLBALANCED
+IF (haswb .NE. 1) THEN
+  WRITE(*,*) "swref: this build is WELLBALANCED, input must carry a WB state"
+  STOP 2
+END IF
+#else
+IF (haswb .NE. 0) THEN
+  WRITE(*,*) "swref: this build is not WELLBALANCED"
+  STOP 2
+END IF
+#endif
+
+! Module globals as InitializeParameters (src/parameters.f90:24-171) sets them.
+nElemsX = nx
+nElemsY = nx
+MESH_X0 = (/x0, x0/)
+MESH_X1 = (/x1, x1/)
+Gravity = grav
+CFL     = cflin
+BoundaryConditionsType = (/1,1,1,1/)
+BathymetryFlag    = bflag
+Reconstruction    = 4
+ReconstructionFix = Reconstruction
+timescheme        = 5
+WhichOutput       = 0   ! no snapshot files
+nOutputFiles      = 1
+InitialCondition  = 4   ! only consulted by WriteSolutionToDisk's error field
+
+CALL InitializeFiniteVolume()
+CALL BuildMesh()
+
+OPEN(11, FILE=TRIM(fout), ACCESS='stream', FORM='unformatted', STATUS='replace')
+
+IF (mode .EQ. 4) THEN
+  nrow = nx*nx
+  ALLOCATE(Mass(5*nrow), Diag(nrow), rhs(nrow), sol(nrow))
+  READ(10) Mass, Diag, rhs
+  sol = 0.
+  CALL jacobi(RowStart, ColumnsVector, Mass, Diag, rhs, sol)
+  WRITE(11) sol
+  CLOSE(11)
+  CLOSE(10)
+  STOP
+END IF
+
+ALLOCATE(Uic(3,nx,nx))
+READ(10) Uic
+IF (haswb .EQ. 1) THEN
+  ALLOCATE(Uwb(3,nx,nx))
+  READ(10) Uwb
+END IF
+CLOSE(10)
+
+#ifdef WELLBALANCED
+U = 0.
+U(1:3,1:nx,1:nx) = Uwb
+CALL FVTimeDerivative(0.)
+UtWB = Ut
+FXWB = FX
+FYWB = FY
+SWB  = S
+#endif
+
+U = 0.
+U(1:3,1:nx,1:nx) = Uic
+
+SELECT CASE (mode)
+  CASE (1)
+    CALL FVTimeDerivative(0.)
+    WRITE(11) Ut, FX, FY, S, ProductionSparse, DestructionSparse
+    WRITE(11) WeightsGPBnd, MESH_DX, MeshGP
+  CASE (2)
+    DO k = 1, nsteps
+      dtmin = TimeStep()
+      WRITE(11) dtmin, LambdaMaxX, LambdaMaxY
+      ! A non-finite dt would make TimeDiscretization loop forever
+      ! (t never reaches tEnd); stop after recording it.
+      IF (.NOT. (dtmin .LT. HUGE(1.0))) EXIT
+      tEnd = dtmin
+      CALL TimeDiscretization()
+      WRITE(11) U(1:3,1:nx,1:nx)
+    END DO
+  CASE (3)
+    dtmin = TimeStep()
+    WRITE(11) dtmin, LambdaMaxX, LambdaMaxY
+  CASE DEFAULT
+    WRITE(*,*) "swref: unknown mode"
+    STOP 2
+END SELECT
+CLOSE(11)
+
+END PROGRAM swref

@@ -1,0 +1,784 @@
+/*
+ * swm_oracle.c -- TEST ORACLE.  A plain-C restatement of the reference
+ * sw-mpdec mPDeC5 / WENO5 PATANKAR time step (accdavlo/sw-mpdec,
+ * fv-solver-sw/src/).  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it; the product library never does.
+ *
+ * It follows the reference's loops, array bounds and expression order
+ * literally (Fortran evaluates left to right, honouring parentheses; every
+ * expression below keeps that order) and must be compiled with
+ * -ffp-contract=off and without fast-math so that it is bitwise equal to a
+ * contraction-free build of the reference.  MAX/MIN follow the reference
+ * compiler's lowering: MAX(a,b) = (a > b) ? a : b, MIN(a,b) = (a < b) ? a : b,
+ * MIN(a,b,c) = MIN(MIN(a,b),c).  That fixes the NaN behaviour the dry-bed
+ * cases depend on.
+ *
+ * Parity pinning (see tests/test_oracle_*.py):
+ *  - every function below is compared bitwise with the reference built from
+ *    its own sources (oracle/_ref via oracle/Makefile) and with fixtures that
+ *    build wrote under tests/golden/;
+ *  - closed forms: WENO5 sweeps reproduce quadratics exactly up to rounding,
+ *    PATANKAR_DIV, Gauss/Lobatto constants, constant states give Ut = 0,
+ *    lake at rest gives Ut = 0 with the well-balanced variant, Jacobi matches
+ *    a dense solve, mass is conserved.
+ *
+ * Not restated: the Jameson shock indicator (src/shocksindicator.f90).  With
+ * ReconstructionFix == Reconstruction (src/parameters.f90:289-290) both
+ * ReconstructionX and ReconstructionFixX call WENO_XDIR(...,4) on disjoint
+ * cell sets that cover every cell (src/reconstruction.f90:96-108,243-255), so
+ * the indicator cannot change any result.
+ */
+#include "swm_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NVAR 3
+#define NGP 4
+#define WENOEPS 1.0E-24   /* src/finitevolume_vars.f90:97 */
+#define MIN_DEPTH 1.0E-6  /* src/finitevolume_vars.f90:104 */
+#define MIN_TIMESTEP 1.0E-30 /* src/finitevolume_vars.f90:106 */
+#define WLOBATTO (1. / 12.)  /* src/finitevolume_vars.f90:96 */
+
+static inline double fmaxr(double a, double b) { return (a > b) ? a : b; }
+static inline double fminr(double a, double b) { return (a < b) ? a : b; }
+
+struct orc_state {
+  int nx, ny;
+  double dx, dy, g, cfl;
+  int wb, bathy;
+  double wbnd[NGP], wgp[NGP][NGP];
+  int max_iter;
+  double tol;
+  /* U, V: (3, -2:nx+3, -2:ny+3) */
+  double *U, *V;
+  /* WM, WP: (3, 4, 0:nx+1, 0:ny+1) */
+  double *WM, *WP;
+  double *FX, *FY;       /* (3,0:nx,1:ny), (3,1:nx,0:ny) */
+  double *FluxX, *FluxY; /* (3,4,0:nx,1:ny), (3,4,1:nx,0:ny) */
+  double *S, *Ut;        /* (3,nx,ny) */
+  double *FXWB, *FYWB, *SWB;
+  double *Prod, *Dest;   /* (5 nx ny) */
+  double *dbdx, *dbdy;   /* 16 planes of nx*ny */
+  /* DeC storage: Ua, Up, FUp (4,3,nx,ny); ProdUp, DestUp (4, 5 nx ny) */
+  double *Ua, *Up, *FUp, *ProdUp, *DestUp;
+  /* Patankar / Jacobi scratch */
+  double *Mass, *Diag, *rhs, *haVec, *solp, *prod;
+};
+
+/* ---- index helpers mirroring the Fortran bounds ------------------------- */
+static inline size_t iU(const orc_t *o, int v, int i, int j) {
+  /* U(1:3, -2:nx+3, -2:ny+3), v 0-based, i/j Fortran indices */
+  return (size_t)v + NVAR * ((size_t)(i + 2) + (size_t)(o->nx + 6) * (size_t)(j + 2));
+}
+static inline size_t iW(const orc_t *o, int v, int g, int i, int j) {
+  /* WM(1:3, 1:4, 0:nx+1, 0:ny+1), v/g 0-based */
+  return (size_t)v + NVAR * ((size_t)g + NGP * ((size_t)i + (size_t)(o->nx + 2) * (size_t)j));
+}
+static inline size_t iFX(const orc_t *o, int v, int i, int j) {
+  return (size_t)v + NVAR * ((size_t)i + (size_t)(o->nx + 1) * (size_t)(j - 1));
+}
+static inline size_t iFY(const orc_t *o, int v, int i, int j) {
+  return (size_t)v + NVAR * ((size_t)(i - 1) + (size_t)o->nx * (size_t)j);
+}
+static inline size_t iFluxX(const orc_t *o, int v, int g, int i, int j) {
+  return (size_t)v + NVAR * ((size_t)g + NGP * ((size_t)i + (size_t)(o->nx + 1) * (size_t)(j - 1)));
+}
+static inline size_t iFluxY(const orc_t *o, int v, int g, int i, int j) {
+  return (size_t)v + NVAR * ((size_t)g + NGP * ((size_t)(i - 1) + (size_t)o->nx * (size_t)j));
+}
+static inline size_t iC(const orc_t *o, int v, int i, int j) { /* (3,nx,ny) */
+  return (size_t)v + NVAR * ((size_t)(i - 1) + (size_t)o->nx * (size_t)(j - 1));
+}
+static inline size_t iN(const orc_t *o, int m, int v, int i, int j) { /* (4,3,nx,ny), m 0-based */
+  return (size_t)m + 4 * ((size_t)v + NVAR * ((size_t)(i - 1) + (size_t)o->nx * (size_t)(j - 1)));
+}
+/* GlobalElem (src/mesh.f90:157-170), 1-based result */
+static inline int global_elem(const orc_t *o, int ii, int jj) {
+  int mj = (jj - 1) % o->ny; if (mj < 0) mj += o->ny;
+  int mi = (ii - 1) % o->nx; if (mi < 0) mi += o->nx;
+  return mj * o->nx + mi + 1;
+}
+
+/* ---- equation.f90 -------------------------------------------------------- */
+
+/* ConsToPrim, src/equation.f90:912-945 */
+static inline void cons_to_prim(const double *c, double *p) {
+  double h = c[0], hvx = c[1], hvy = c[2];
+  double ht = h + MIN_DEPTH / h;
+  p[0] = h;
+  p[1] = hvx / ht;
+  p[2] = hvy / ht;
+}
+
+/* BoundaryConditions, periodic branches + ConsToPrim, src/equation.f90:515-785 */
+static void boundary_conditions(orc_t *o) {
+  const int nx = o->nx, ny = o->ny, ng = 2;
+  double *U = o->U;
+  /* left (:555-561) */
+  for (int jj = 1; jj <= ny; ++jj)
+    for (int ii = 0; ii <= ng; ++ii)
+      for (int v = 0; v < NVAR; ++v) U[iU(o, v, -ng + ii, jj)] = U[iU(o, v, nx - ng + ii, jj)];
+  /* right (:600-606) */
+  for (int jj = 1; jj <= ny; ++jj)
+    for (int ii = 1; ii <= ng + 1; ++ii)
+      for (int v = 0; v < NVAR; ++v) U[iU(o, v, nx + ii, jj)] = U[iU(o, v, ii, jj)];
+  /* top (:645-651) */
+  for (int ii = 1; ii <= nx; ++ii)
+    for (int jj = 1; jj <= ng + 1; ++jj)
+      for (int v = 0; v < NVAR; ++v) U[iU(o, v, ii, ny + jj)] = U[iU(o, v, ii, jj)];
+  /* bottom (:690-696) */
+  for (int ii = 1; ii <= nx; ++ii)
+    for (int jj = 0; jj <= ng; ++jj)
+      for (int v = 0; v < NVAR; ++v) U[iU(o, v, ii, -ng + jj)] = U[iU(o, v, ii, ny - ng + jj)];
+  /* left corners (:735-747) */
+  for (int jj = -ng; jj <= 0; ++jj)
+    for (int ii = 0; ii <= ng; ++ii)
+      for (int v = 0; v < NVAR; ++v) U[iU(o, v, -ng + ii, jj)] = U[iU(o, v, nx - ng + ii, jj)];
+  for (int jj = ny + 1; jj <= ny + 1 + ng; ++jj)
+    for (int ii = 0; ii <= ng; ++ii)
+      for (int v = 0; v < NVAR; ++v) U[iU(o, v, -ng + ii, jj)] = U[iU(o, v, nx - ng + ii, jj)];
+  /* right corners (:759-771) */
+  for (int jj = -ng; jj <= 0; ++jj)
+    for (int ii = 1; ii <= ng + 1; ++ii)
+      for (int v = 0; v < NVAR; ++v) U[iU(o, v, nx + ii, jj)] = U[iU(o, v, ii, jj)];
+  for (int jj = ny + 1; jj <= ny + 1 + ng; ++jj)
+    for (int ii = 1; ii <= ng + 1; ++ii)
+      for (int v = 0; v < NVAR; ++v) U[iU(o, v, nx + ii, jj)] = U[iU(o, v, ii, jj)];
+  /* V = ConsToPrim(U) over the padded grid (:778-782) */
+  for (int jj = -ng; jj <= ny + ng + 1; ++jj)
+    for (int ii = -ng; ii <= nx + ng + 1; ++ii)
+      cons_to_prim(&U[iU(o, 0, ii, jj)], &o->V[iU(o, 0, ii, jj)]);
+}
+
+/* EvaluateFlux1D, src/equation.f90:988-1019 */
+static inline void evaluate_flux_1d(const double *p, double g, double *f) {
+  double h = p[0], vx = p[1], vy = p[2];
+  f[0] = h * vx;
+  f[1] = h * (vx * vx) + (0.5 * g) * (h * h);
+  f[2] = (h * vx) * vy;
+}
+
+/* RiemannSolver + RiemannSolverByRusanov, src/equation.f90:1025-1106.
+ * n, t: NormVect/TangVect of the face (src/mesh.f90:74-110). */
+static void riemann(const double *pl, const double *pr, const double n[2],
+                    const double t[2], double g, double *flux) {
+  double pll[3], prr[3], cll[3], crr[3], fl[3], fr[3];
+  pll[0] = pl[0];
+  pll[1] = n[0] * pl[1] + n[1] * pl[2];
+  pll[2] = t[0] * pl[1] + t[1] * pl[2];
+  prr[0] = pr[0];
+  prr[1] = n[0] * pr[1] + n[1] * pr[2];
+  prr[2] = t[0] * pr[1] + t[1] * pr[2];
+  /* PrimToCons, :951-982 */
+  cll[0] = pll[0]; cll[1] = pll[0] * pll[1]; cll[2] = pll[0] * pll[2];
+  crr[0] = prr[0]; crr[1] = prr[0] * prr[1]; crr[2] = prr[0] * prr[2];
+  evaluate_flux_1d(pll, g, fl);
+  evaluate_flux_1d(prr, g, fr);
+  /* WaveSpeeds1D fastest = vx + SQRT(g*h), :869-871 */
+  double fastl = pll[1] + sqrt(g * pll[0]);
+  double fastr = prr[1] + sqrt(g * prr[0]);
+  double lmax = fmaxr(fabs(fastl), fabs(fastr));
+  double f[3];
+  for (int v = 0; v < NVAR; ++v) f[v] = 0.5 * ((fl[v] + fr[v]) - lmax * (crr[v] - cll[v]));
+  /* rotate back, :1067-1068 (right-hand side evaluated before assignment) */
+  flux[0] = f[0];
+  flux[1] = n[0] * f[1] + t[0] * f[2];
+  flux[2] = n[1] * f[1] + t[1] * f[2];
+}
+
+/* ---- reconstruction.f90 --------------------------------------------------- */
+
+/* WENO5_FirstSweep, src/reconstruction.f90:673-755.  q[0..4] = Q(-2..2). */
+void orc_weno5_first(const double q[5], double *wm, double *wp) {
+  const double qm2 = q[0], qm1 = q[1], q0 = q[2], qp1 = q[3], qp2 = q[4];
+  double b1 = (1.0 / 3.0) * (4.0 * qm2 * qm2 - 19.0 * qm2 * qm1 + 25.0 * qm1 * qm1
+                             + 11.0 * qm2 * q0 - 31.0 * qm1 * q0 + 10.0 * q0 * q0);
+  double b2 = (1.0 / 3.0) * (4.0 * qm1 * qm1 - 13.0 * qm1 * q0 + 13.0 * q0 * q0
+                             + 5.0 * qm1 * qp1 - 13.0 * q0 * qp1 + 4.0 * qp1 * qp1);
+  double b3 = (1.0 / 3.0) * (10.0 * q0 * q0 - 31.0 * q0 * qp1 + 25.0 * qp1 * qp1
+                             + 11.0 * q0 * qp2 - 19.0 * qp1 * qp2 + 4.0 * qp2 * qp2);
+  double d1 = (WENOEPS + b1) * (WENOEPS + b1);
+  double d2 = (WENOEPS + b2) * (WENOEPS + b2);
+  double d3 = (WENOEPS + b3) * (WENOEPS + b3);
+  double a1, a2, a3, o1, o2, o3, w1, w2, w3;
+  /* WM: x_{i-1/2} */
+  a1 = (3.0 / 10.0) / d1;
+  a2 = (3.0 / 5.0) / d2;
+  a3 = (1.0 / 10.0) / d3;
+  o1 = a1 / (a1 + a2 + a3);
+  o2 = a2 / (a1 + a2 + a3);
+  o3 = a3 / (a1 + a2 + a3);
+  w1 = (1.0 / 6.0) * (-qm2 + 5.0 * qm1 + 2.0 * q0);
+  w2 = (1.0 / 6.0) * (2.0 * qm1 + 5.0 * q0 - qp1);
+  w3 = (1.0 / 6.0) * (11.0 * q0 - 7.0 * qp1 + 2.0 * qp2);
+  *wm = o1 * w1 + o2 * w2 + o3 * w3;
+  /* WP: x_{i+1/2} */
+  a1 = (1.0 / 10.0) / d1;
+  a2 = (3.0 / 5.0) / d2;
+  a3 = (3.0 / 10.0) / d3;
+  o1 = a1 / (a1 + a2 + a3);
+  o2 = a2 / (a1 + a2 + a3);
+  o3 = a3 / (a1 + a2 + a3);
+  w1 = (1.0 / 6.0) * (2.0 * qm2 - 7.0 * qm1 + 11.0 * q0);
+  w2 = (1.0 / 6.0) * (-qm1 + 5.0 * q0 + 2.0 * qp1);
+  w3 = (1.0 / 6.0) * (2.0 * q0 + 5.0 * qp1 - qp2);
+  *wp = o1 * w1 + o2 * w2 + o3 * w3;
+}
+
+/* Linear weights and candidate-polynomial rows of WENO5_SecondSweep,
+ * src/reconstruction.f90:1036-1132 (decimal literals as written there; the
+ * signs of the literals are folded into the coefficients, which is exact). */
+static const double SS_GAMMA[4][3] = {
+    {0.2658420974778319, 0.6112504900322486, 0.1229074124899195},
+    {0.1281641584355059, 0.5219691498503563, 0.3498666917141379},
+    {0.3498666917141379, 0.5219691498503563, 0.1281641584355059},
+    {0.1229074124899195, 0.6112504900322486, 0.2658420974778319}};
+static const double SS_W[4][3][5] = {
+    {{-0.1642562761719537, 0.7590807081409336, 0.4051755680310201, 0.0000000000000000, 0.0000000000000000},
+     {+0.0000000000000000, 0.2663118796250726, 0.8979443965468811, -0.1642562761719537, 0.0000000000000000},
+     {+0.0000000000000000, 0.0000000000000000, 1.6968800354220990, -0.9631919150471715, 0.2663118796250726}},
+    {{-0.1122135388132497, 0.3944175994189276, 0.7177959393943222, 0.0000000000000000, 0.0000000000000000},
+     {+0.0000000000000000, 0.0577769829791784, 1.0544365558340714, -0.1122135388132497, 0.0000000000000000},
+     {+0.0000000000000000, 0.0000000000000000, 1.2277675047716066, -0.2855444877507849, 0.0577769829791784}},
+    {{+0.0577769829791784, -0.2855444877507849, 1.2277675047716066, 0.0000000000000000, 0.0000000000000000},
+     {+0.0000000000000000, -0.1122135388132497, 1.0544365558340714, 0.0577769829791784, 0.0000000000000000},
+     {+0.0000000000000000, 0.0000000000000000, 0.7177959393943222, 0.3944175994189276, -0.1122135388132497}},
+    {{+0.2663118796250726, -0.9631919150471715, 1.6968800354220990, 0.0000000000000000, 0.0000000000000000},
+     {+0.0000000000000000, -0.1642562761719537, 0.8979443965468811, 0.2663118796250726, 0.0000000000000000},
+     {+0.0000000000000000, 0.0000000000000000, 0.4051755680310201, 0.7590807081409336, -0.1642562761719537}}};
+
+/* WENO5_SecondSweep, src/reconstruction.f90:999-1136 */
+void orc_weno5_second(const double q[5], double w[4]) {
+  const double qm2 = q[0], qm1 = q[1], q0 = q[2], qp1 = q[3], qp2 = q[4];
+  double b1 = (1.0 / 3.0) * (4.0 * qm2 * qm2 - 19.0 * qm2 * qm1 + 25.0 * qm1 * qm1
+                             + 11.0 * qm2 * q0 - 31.0 * qm1 * q0 + 10.0 * q0 * q0);
+  double b2 = (1.0 / 3.0) * (4.0 * qm1 * qm1 - 13.0 * qm1 * q0 + 13.0 * q0 * q0
+                             + 5.0 * qm1 * qp1 - 13.0 * q0 * qp1 + 4.0 * qp1 * qp1);
+  double b3 = (1.0 / 3.0) * (10.0 * q0 * q0 - 31.0 * q0 * qp1 + 25.0 * qp1 * qp1
+                             + 11.0 * q0 * qp2 - 19.0 * qp1 * qp2 + 4.0 * qp2 * qp2);
+  double d[3] = {(WENOEPS + b1) * (WENOEPS + b1), (WENOEPS + b2) * (WENOEPS + b2),
+                 (WENOEPS + b3) * (WENOEPS + b3)};
+  for (int g = 0; g < 4; ++g) {
+    double a1 = SS_GAMMA[g][0] / d[0];
+    double a2 = SS_GAMMA[g][1] / d[1];
+    double a3 = SS_GAMMA[g][2] / d[2];
+    double o1 = a1 / (a1 + a2 + a3);
+    double o2 = a2 / (a1 + a2 + a3);
+    double o3 = a3 / (a1 + a2 + a3);
+    double W[3];
+    for (int k = 0; k < 3; ++k) {
+      const double *c = SS_W[g][k];
+      W[k] = c[0] * qm2 + c[1] * qm1 + c[2] * q0 + c[3] * qp1 + c[4] * qp2;
+    }
+    w[g] = (o1 * W[0] + o2 * W[1] + o3 * W[2]);
+  }
+}
+
+/* WENO_XDIR with WhichReconstruction=4, src/reconstruction.f90:408-458.
+ * Stencil V(:, ii-2:ii+2, jj-2:jj+2) of the padded prim array. */
+static void weno_xdir(const orc_t *o, int ii, int jj, double *wm, double *wp) {
+  for (int v = 0; v < NVAR; ++v) {
+    double vm[5], vp[5], q[5];
+    for (int r = -2; r <= 2; ++r) {
+      for (int s = -2; s <= 2; ++s) q[s + 2] = o->V[iU(o, v, ii + s, jj + r)];
+      orc_weno5_first(q, &vm[r + 2], &vp[r + 2]);
+    }
+    double w[4];
+    orc_weno5_second(vm, w);
+    for (int g = 0; g < NGP; ++g) wm[iW(o, v, g, 0, 0)] = w[g];
+    orc_weno5_second(vp, w);
+    for (int g = 0; g < NGP; ++g) wp[iW(o, v, g, 0, 0)] = w[g];
+  }
+}
+
+/* WENO_YDIR with WhichReconstruction=4, src/reconstruction.f90:464-514 */
+static void weno_ydir(const orc_t *o, int ii, int jj, double *wm, double *wp) {
+  for (int v = 0; v < NVAR; ++v) {
+    double vm[5], vp[5], q[5];
+    for (int r = -2; r <= 2; ++r) {
+      for (int s = -2; s <= 2; ++s) q[s + 2] = o->V[iU(o, v, ii + r, jj + s)];
+      orc_weno5_first(q, &vm[r + 2], &vp[r + 2]);
+    }
+    double w[4];
+    orc_weno5_second(vm, w);
+    for (int g = 0; g < NGP; ++g) wm[iW(o, v, g, 0, 0)] = w[g];
+    orc_weno5_second(vp, w);
+    for (int g = 0; g < NGP; ++g) wp[iW(o, v, g, 0, 0)] = w[g];
+  }
+}
+
+/* ReconstructionX (+ReconstructionFixX), src/reconstruction.f90:96-108 */
+static void reconstruction_x(orc_t *o) {
+  for (int jj = 1; jj <= o->ny; ++jj)
+    for (int ii = 0; ii <= o->nx + 1; ++ii)
+      weno_xdir(o, ii, jj, &o->WM[iW(o, 0, 0, ii, jj)], &o->WP[iW(o, 0, 0, ii, jj)]);
+}
+/* ReconstructionY (+ReconstructionFixY), src/reconstruction.f90:164-176 */
+static void reconstruction_y(orc_t *o) {
+  for (int jj = 0; jj <= o->ny + 1; ++jj)
+    for (int ii = 1; ii <= o->nx; ++ii)
+      weno_ydir(o, ii, jj, &o->WM[iW(o, 0, 0, ii, jj)], &o->WP[iW(o, 0, 0, ii, jj)]);
+}
+
+/* ---- finitevolume.f90 ----------------------------------------------------- */
+
+/* PositivityLimiterX/Y body, src/finitevolume.f90:671-691 and :732-751 */
+static void limit_cell(const orc_t *o, double vh, double *wm, double *wp) {
+  double alpha = 0., beta = 0.;
+  for (int g = 0; g < NGP; ++g) {
+    alpha = alpha + o->wbnd[g] * wm[iW(o, 0, g, 0, 0)];
+    beta = beta + o->wbnd[g] * wp[iW(o, 0, g, 0, 0)];
+  }
+  double csi = (vh - WLOBATTO * alpha - WLOBATTO * beta) / (1. - 2. * WLOBATTO);
+  for (int g = 0; g < NGP; ++g) {
+    double *m = &wm[iW(o, 0, g, 0, 0)], *p = &wp[iW(o, 0, g, 0, 0)];
+    double mmin = fminr(fminr(csi, *m), *p);
+    double theta;
+    if (vh == mmin)
+      theta = 1.;
+    else
+      theta = fminr(1., fabs((vh - MIN_DEPTH) / (vh - mmin)));
+    *p = vh + theta * (*p - vh);
+    *m = vh + theta * (*m - vh);
+  }
+}
+
+/* PositivityLimiterX, src/finitevolume.f90:646-698: ii = 0..nx only */
+static void positivity_limiter_x(orc_t *o) {
+  for (int jj = 1; jj <= o->ny; ++jj)
+    for (int ii = 0; ii <= o->nx; ++ii)
+      limit_cell(o, o->V[iU(o, 0, ii, jj)], &o->WM[iW(o, 0, 0, ii, jj)], &o->WP[iW(o, 0, 0, ii, jj)]);
+}
+/* PositivityLimiterY, src/finitevolume.f90:704-757: jj = 0..ny only */
+static void positivity_limiter_y(orc_t *o) {
+  for (int jj = 0; jj <= o->ny; ++jj)
+    for (int ii = 1; ii <= o->nx; ++ii)
+      limit_cell(o, o->V[iU(o, 0, ii, jj)], &o->WM[iW(o, 0, 0, ii, jj)], &o->WP[iW(o, 0, 0, ii, jj)]);
+}
+
+static const double NX_N[2] = {1.0, 0.0}, NX_T[2] = {0.0, 1.0};  /* src/mesh.f90:74-92 */
+static const double NY_N[2] = {0.0, 1.0}, NY_T[2] = {-1.0, 0.0}; /* src/mesh.f90:80-110 */
+
+/* NumericalFluxFX, src/finitevolume.f90:546-590 */
+static void numerical_flux_fx(orc_t *o) {
+  const int nx = o->nx, ny = o->ny;
+  for (int jj = 1; jj <= ny; ++jj)
+    for (int ii = 0; ii <= nx; ++ii)
+      for (int g = 0; g < NGP; ++g) {
+        double pl[3], pr[3];
+        for (int v = 0; v < NVAR; ++v) {
+          pl[v] = o->WP[iW(o, v, g, ii, jj)];
+          pr[v] = o->WM[iW(o, v, g, ii + 1, jj)];
+        }
+        riemann(pl, pr, NX_N, NX_T, o->g, &o->FluxX[iFluxX(o, 0, g, ii, jj)]);
+      }
+  /* FX = 0; DO iGP: FX = FX + w(iGP)*FluxX(:,iGP) (:571, :585-587) */
+  for (int jj = 1; jj <= ny; ++jj)
+    for (int ii = 0; ii <= nx; ++ii)
+      for (int v = 0; v < NVAR; ++v) {
+        double acc = 0.0;
+        for (int g = 0; g < NGP; ++g) acc = acc + o->wbnd[g] * o->FluxX[iFluxX(o, v, g, ii, jj)];
+        o->FX[iFX(o, v, ii, jj)] = acc;
+      }
+}
+
+/* NumericalFluxFY, src/finitevolume.f90:596-640 */
+static void numerical_flux_fy(orc_t *o) {
+  const int nx = o->nx, ny = o->ny;
+  for (int jj = 0; jj <= ny; ++jj)
+    for (int ii = 1; ii <= nx; ++ii)
+      for (int g = 0; g < NGP; ++g) {
+        double pl[3], pr[3];
+        for (int v = 0; v < NVAR; ++v) {
+          pl[v] = o->WP[iW(o, v, g, ii, jj)];
+          pr[v] = o->WM[iW(o, v, g, ii, jj + 1)];
+        }
+        riemann(pl, pr, NY_N, NY_T, o->g, &o->FluxY[iFluxY(o, 0, g, ii, jj)]);
+      }
+  for (int jj = 0; jj <= ny; ++jj)
+    for (int ii = 1; ii <= nx; ++ii)
+      for (int v = 0; v < NVAR; ++v) {
+        double acc = 0.0;
+        for (int g = 0; g < NGP; ++g) acc = acc + o->wbnd[g] * o->FluxY[iFluxY(o, v, g, ii, jj)];
+        o->FY[iFY(o, v, ii, jj)] = acc;
+      }
+}
+
+/* SourceTerms with Reconstruction=4, src/equation.f90:284-392, and
+ * SourceFunc (:398-413) with the bathymetry gradient taken from the tables.
+ * Only Q(1)=h enters SourceFunc, so only h is reconstructed here; the other
+ * variables' reconstructions in the reference are discarded unused. */
+static void source_terms(orc_t *o) {
+  const int nx = o->nx, ny = o->ny;
+  const size_t plane = (size_t)nx * ny;
+  for (size_t k = 0; k < (size_t)NVAR * plane; ++k) o->S[k] = 0.0;
+  if (!o->bathy) return;
+  /* Vtemp(1, ii, jj, 1:4) for ii = -2..nx+3 */
+  const int W = nx + 6;
+  double *vt = (double *)malloc(sizeof(double) * (size_t)W * ny * 4);
+  for (int jj = 1; jj <= ny; ++jj)
+    for (int ii = -2; ii <= nx + 3; ++ii) {
+      double q[5];
+      for (int s = -2; s <= 2; ++s) q[s + 2] = o->U[iU(o, 0, ii, jj + s)];
+      orc_weno5_second(q, &vt[4 * ((size_t)(ii + 2) + (size_t)W * (jj - 1))]);
+    }
+  for (int jj = 1; jj <= ny; ++jj)
+    for (int ii = 1; ii <= nx; ++ii) {
+      double hgp[4][4]; /* [yg][xg] */
+      for (int yg = 0; yg < 4; ++yg) {
+        double q[5];
+        for (int s = -2; s <= 2; ++s) q[s + 2] = vt[4 * ((size_t)(ii + s + 2) + (size_t)W * (jj - 1)) + yg];
+        orc_weno5_second(q, hgp[yg]);
+      }
+      size_t c = (size_t)(jj - 1) * nx + (ii - 1);
+      double s2 = 0.0, s3 = 0.0, s1 = 0.0;
+      /* S = S + WeightsGP(iGP,jGP)*SW(:,iGP,jGP), iGP = x-GP outer (:380-388) */
+      for (int xg = 0; xg < 4; ++xg)
+        for (int yg = 0; yg < 4; ++yg) {
+          double h = hgp[yg][xg];
+          double bx = o->dbdx[(size_t)(yg * 4 + xg) * plane + c];
+          double by = o->dbdy[(size_t)(yg * 4 + xg) * plane + c];
+          double sw2 = -o->g * h * bx;
+          double sw3 = -o->g * h * by;
+          s1 = s1 + o->wgp[xg][yg] * 0.;
+          s2 = s2 + o->wgp[xg][yg] * sw2;
+          s3 = s3 + o->wgp[xg][yg] * sw3;
+        }
+      o->S[iC(o, 0, ii, jj)] = s1;
+      o->S[iC(o, 1, ii, jj)] = s2;
+      o->S[iC(o, 2, ii, jj)] = s3;
+    }
+  free(vt);
+}
+
+/* UpdateTimeDerivative, PATANKAR branch, src/finitevolume.f90:446-540 */
+static void update_time_derivative(orc_t *o) {
+  const int nx = o->nx, ny = o->ny;
+  if (o->wb) {
+    size_t nfx = (size_t)NVAR * (nx + 1) * ny, nfy = (size_t)NVAR * nx * (ny + 1);
+    for (size_t k = 0; k < nfx; ++k) o->FX[k] = o->FX[k] - o->FXWB[k];
+    for (size_t k = 0; k < nfy; ++k) o->FY[k] = o->FY[k] - o->FYWB[k];
+    for (size_t k = 0; k < (size_t)NVAR * nx * ny; ++k) o->S[k] = o->S[k] - o->SWB[k];
+  }
+  const double dx = o->dx, dy = o->dy;
+  size_t sv = 0;
+  for (int jj = 1; jj <= ny; ++jj)
+    for (int ii = 1; ii <= nx; ++ii) {
+      double f;
+      f = o->FY[iFY(o, 0, ii, jj - 1)];
+      o->Prod[sv] = fmaxr(f, 0.) / dy; o->Dest[sv] = -(fminr(f, 0.) / dy); ++sv;
+      f = o->FX[iFX(o, 0, ii - 1, jj)];
+      o->Prod[sv] = fmaxr(f, 0.) / dx; o->Dest[sv] = -(fminr(f, 0.) / dx); ++sv;
+      o->Prod[sv] = 0.; o->Dest[sv] = 0.; ++sv;
+      f = -o->FX[iFX(o, 0, ii, jj)];
+      o->Prod[sv] = fmaxr(f, 0.) / dx; o->Dest[sv] = -(fminr(f, 0.) / dx); ++sv;
+      f = -o->FY[iFY(o, 0, ii, jj)];
+      o->Prod[sv] = fmaxr(f, 0.) / dy; o->Dest[sv] = -(fminr(f, 0.) / dy); ++sv;
+      for (int v = 1; v < NVAR; ++v)
+        o->Ut[iC(o, v, ii, jj)] = o->S[iC(o, v, ii, jj)]
+                                  - (o->FX[iFX(o, v, ii, jj)] - o->FX[iFX(o, v, ii - 1, jj)]) / dx
+                                  - (o->FY[iFY(o, v, ii, jj)] - o->FY[iFY(o, v, ii, jj - 1)]) / dy;
+    }
+}
+
+/* FVTimeDerivative, src/finitevolume.f90:402-440 */
+static void fv_time_derivative(orc_t *o) {
+  boundary_conditions(o);
+  reconstruction_x(o);
+  positivity_limiter_x(o);
+  numerical_flux_fx(o);
+  reconstruction_y(o);
+  positivity_limiter_y(o);
+  numerical_flux_fy(o);
+  source_terms(o);
+  update_time_derivative(o);
+}
+
+/* ---- public API ----------------------------------------------------------- */
+
+orc_t *orc_create(int nx, int ny, double dx, double dy, double gravity, double cfl,
+                  int wellbalanced, const double wbnd[4]) {
+  if (nx < 3 || ny < 3) return NULL;
+  orc_t *o = (orc_t *)calloc(1, sizeof(orc_t));
+  o->nx = nx; o->ny = ny; o->dx = dx; o->dy = dy; o->g = gravity; o->cfl = cfl;
+  o->wb = wellbalanced ? 1 : 0;
+  o->max_iter = 1000; o->tol = 1.e-16;
+  for (int i = 0; i < 4; ++i) o->wbnd[i] = wbnd[i];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) o->wgp[i][j] = wbnd[i] * wbnd[j]; /* src/mesh.f90:136 */
+  size_t pad = (size_t)NVAR * (nx + 6) * (ny + 6);
+  size_t nc = (size_t)nx * ny;
+  o->U = calloc(pad, sizeof(double));
+  o->V = calloc(pad, sizeof(double));
+  o->WM = calloc((size_t)NVAR * NGP * (nx + 2) * (ny + 2), sizeof(double));
+  o->WP = calloc((size_t)NVAR * NGP * (nx + 2) * (ny + 2), sizeof(double));
+  o->FX = calloc((size_t)NVAR * (nx + 1) * ny, sizeof(double));
+  o->FY = calloc((size_t)NVAR * nx * (ny + 1), sizeof(double));
+  o->FluxX = calloc((size_t)NVAR * NGP * (nx + 1) * ny, sizeof(double));
+  o->FluxY = calloc((size_t)NVAR * NGP * nx * (ny + 1), sizeof(double));
+  o->S = calloc(NVAR * nc, sizeof(double));
+  o->Ut = calloc(NVAR * nc, sizeof(double));
+  o->FXWB = calloc((size_t)NVAR * (nx + 1) * ny, sizeof(double));
+  o->FYWB = calloc((size_t)NVAR * nx * (ny + 1), sizeof(double));
+  o->SWB = calloc(NVAR * nc, sizeof(double));
+  o->Prod = calloc(5 * nc, sizeof(double));
+  o->Dest = calloc(5 * nc, sizeof(double));
+  o->dbdx = calloc(16 * nc, sizeof(double));
+  o->dbdy = calloc(16 * nc, sizeof(double));
+  o->Ua = calloc(4 * NVAR * nc, sizeof(double));
+  o->Up = calloc(4 * NVAR * nc, sizeof(double));
+  o->FUp = calloc(4 * NVAR * nc, sizeof(double));
+  o->ProdUp = calloc(4 * 5 * nc, sizeof(double));
+  o->DestUp = calloc(4 * 5 * nc, sizeof(double));
+  o->Mass = calloc(5 * nc, sizeof(double));
+  o->Diag = calloc(nc, sizeof(double));
+  o->rhs = calloc(nc, sizeof(double));
+  o->haVec = calloc(nc, sizeof(double));
+  o->solp = calloc(nc, sizeof(double));
+  o->prod = calloc(nc, sizeof(double));
+  return o;
+}
+
+void orc_destroy(orc_t *o) {
+  if (!o) return;
+  double *p[] = {o->U, o->V, o->WM, o->WP, o->FX, o->FY, o->FluxX, o->FluxY, o->S, o->Ut,
+                 o->FXWB, o->FYWB, o->SWB, o->Prod, o->Dest, o->dbdx, o->dbdy, o->Ua, o->Up,
+                 o->FUp, o->ProdUp, o->DestUp, o->Mass, o->Diag, o->rhs, o->haVec, o->solp,
+                 o->prod};
+  for (size_t k = 0; k < sizeof(p) / sizeof(p[0]); ++k) free(p[k]);
+  free(o);
+}
+
+void orc_set_bathy(orc_t *o, const double *dbdx, const double *dbdy) {
+  size_t n = 16 * (size_t)o->nx * o->ny;
+  if (dbdx && dbdy) {
+    memcpy(o->dbdx, dbdx, n * sizeof(double));
+    memcpy(o->dbdy, dbdy, n * sizeof(double));
+    o->bathy = 1;
+  } else {
+    o->bathy = 0;
+  }
+}
+
+void orc_set_U(orc_t *o, const double *U) {
+  for (int jj = 1; jj <= o->ny; ++jj)
+    for (int ii = 1; ii <= o->nx; ++ii)
+      for (int v = 0; v < NVAR; ++v) o->U[iU(o, v, ii, jj)] = U[iC(o, v, ii, jj)];
+}
+
+void orc_get_U(const orc_t *o, double *U) {
+  for (int jj = 1; jj <= o->ny; ++jj)
+    for (int ii = 1; ii <= o->nx; ++ii)
+      for (int v = 0; v < NVAR; ++v) U[iC(o, v, ii, jj)] = o->U[iU(o, v, ii, jj)];
+}
+
+void orc_init_wb(orc_t *o, const double *Uwb) {
+  /* FXWB/FYWB/SWB are zero while the WB state's own FVTD runs (:387) */
+  size_t nfx = (size_t)NVAR * (o->nx + 1) * o->ny, nfy = (size_t)NVAR * o->nx * (o->ny + 1);
+  memset(o->FXWB, 0, nfx * sizeof(double));
+  memset(o->FYWB, 0, nfy * sizeof(double));
+  memset(o->SWB, 0, (size_t)NVAR * o->nx * o->ny * sizeof(double));
+  orc_set_U(o, Uwb);
+  fv_time_derivative(o);
+  memcpy(o->FXWB, o->FX, nfx * sizeof(double));
+  memcpy(o->FYWB, o->FY, nfy * sizeof(double));
+  memcpy(o->SWB, o->S, (size_t)NVAR * o->nx * o->ny * sizeof(double));
+}
+
+void orc_fvtd(orc_t *o, double *Ut, double *FX, double *FY, double *S, double *Prod,
+              double *Dest) {
+  fv_time_derivative(o);
+  size_t nc = (size_t)o->nx * o->ny;
+  if (Ut) memcpy(Ut, o->Ut, NVAR * nc * sizeof(double));
+  if (FX) memcpy(FX, o->FX, (size_t)NVAR * (o->nx + 1) * o->ny * sizeof(double));
+  if (FY) memcpy(FY, o->FY, (size_t)NVAR * o->nx * (o->ny + 1) * sizeof(double));
+  if (S) memcpy(S, o->S, NVAR * nc * sizeof(double));
+  if (Prod) memcpy(Prod, o->Prod, 5 * nc * sizeof(double));
+  if (Dest) memcpy(Dest, o->Dest, 5 * nc * sizeof(double));
+}
+
+/* TimeStep, src/equation.f90:791-837 (sequential MAX/MIN fold as written) */
+double orc_timestep(orc_t *o, double *lmx, double *lmy) {
+  double lx = 0.0, ly = 0.0, ts = 1.7976931348623157e308; /* HUGE(1.0) */
+  for (int jj = 1; jj <= o->ny; ++jj)
+    for (int ii = 1; ii <= o->nx; ++ii) {
+      double p[3];
+      cons_to_prim(&o->U[iU(o, 0, ii, jj)], p);
+      double fx = p[1] + sqrt(o->g * p[0]); /* WaveSpeeds2D, :902-903 */
+      double fy = p[2] + sqrt(o->g * p[0]);
+      lx = fmaxr(lx, fabs(fx));
+      ly = fmaxr(ly, fabs(fy));
+      ts = fminr(fminr(ts, o->dx / lx), o->dy / ly);
+    }
+  ts = o->cfl * ts;
+  if (ts < MIN_TIMESTEP) ts = MIN_TIMESTEP;
+  if (lmx) *lmx = lx;
+  if (lmy) *lmy = ly;
+  return ts;
+}
+
+/* PATANKAR_DIV, src/timediscretization.f90:913-924 */
+double orc_patankar_div(double prod, double h) {
+  if (h < MIN_DEPTH) return 0.;
+  return 2. * h * prod / (h * h + fmaxr(h * h, MIN_DEPTH));
+}
+
+void orc_set_jacobi_limits(orc_t *o, int max_iter, double tol) {
+  o->max_iter = max_iter;
+  o->tol = tol;
+}
+
+/* sparse_matmul + jacobi, src/jacobi_iteration.f90:25-87, on the CSR
+ * pattern of InitializeFiniteVolume (src/finitevolume.f90:219-267):
+ * row K holds columns S, W, self, E, N at 5(K-1)+0..4. */
+static int jacobi_impl(const orc_t *o, const double *mat, const double *diag,
+                       const double *rhs, double *sol, double *solp, double *prod) {
+  const int nx = o->nx, ny = o->ny;
+  const int n = nx * ny;
+  int k = 0;
+  for (int i = 0; i < n; ++i) { solp[i] = rhs[i]; sol[i] = 0.; }
+  double residual = 0.0;
+  for (int i = 0; i < n; ++i) residual = residual + fabs(sol[i] - solp[i]);
+  residual = residual / (double)n;
+  while ((o->tol < residual) && (k < o->max_iter)) {
+    k = k + 1;
+    for (int jj = 1; jj <= ny; ++jj)
+      for (int ii = 1; ii <= nx; ++ii) {
+        int row = global_elem(o, ii, jj) - 1;
+        int col[5] = {global_elem(o, ii, jj - 1), global_elem(o, ii - 1, jj),
+                      global_elem(o, ii, jj), global_elem(o, ii + 1, jj),
+                      global_elem(o, ii, jj + 1)};
+        double p = 0.;
+        for (int s = 0; s < 5; ++s) p = p + mat[5 * row + s] * solp[col[s] - 1];
+        prod[row] = p;
+      }
+    for (int i = 0; i < n; ++i) sol[i] = (rhs[i] - prod[i]) / diag[i];
+    residual = 0.0;
+    for (int i = 0; i < n; ++i) residual = residual + fabs(sol[i] - solp[i]);
+    residual = residual / (double)n;
+    for (int i = 0; i < n; ++i) solp[i] = sol[i];
+  }
+  return k;
+}
+
+int orc_jacobi(const orc_t *o, const double *Mass, const double *diag, const double *rhs,
+               double *sol) {
+  size_t n = (size_t)o->nx * o->ny;
+  double *solp = malloc(n * sizeof(double)), *prod = malloc(n * sizeof(double));
+  int k = jacobi_impl(o, Mass, diag, rhs, sol, solp, prod);
+  free(solp);
+  free(prod);
+  return k;
+}
+
+/* thetaDeC (column-major reshape) and betaDeC, src/timediscretization.f90:491-511 */
+static const double THETA[4][4] = {
+    /* THETA[m][r] = thetaDeC(m+1, r+1) */
+    {0.0000000000000000000, 0.0000000000000000000, 0.0000000000000000000, 0.0000000000000000000},
+    {0.1103005664791649049, 0.1896994335208351257, -0.0339073642291439076, 0.0103005664791649201},
+    {0.0730327668541684294, 0.4505740308958107176, 0.2269672331458314485, -0.0269672331458315692},
+    {0.0833333333333333287, 0.4166666666666666852, 0.4166666666666666852, 0.0833333333333333287}};
+
+/* MODIFIED_PATANKAR_MATRIX_INVERSION, src/timediscretization.f90:771-908.
+ * m is the 0-based node; hp = Up(:,1,:,:) read from o->Up; ha -> Ua(m,1). */
+static int patankar_solve(orc_t *o, int m, double dt) {
+  const int nx = o->nx, ny = o->ny, n = nx * ny;
+  for (int k = 0; k < 5 * n; ++k) o->Mass[k] = 0.;
+  for (int k = 0; k < n; ++k) o->Diag[k] = 1.;
+  /* neighbour offsets of slots S, W, E, N and their SparseIndexMat slot */
+  const int di[4] = {0, -1, +1, 0}, dj[4] = {-1, 0, 0, +1}, slot[4] = {0, 1, 3, 4};
+  for (int jj = 1; jj <= ny; ++jj)
+    for (int ii = 1; ii <= nx; ++ii) {
+      int K = global_elem(o, ii, jj) - 1;
+      double hK = o->Up[iN(o, m, 0, ii, jj)];
+      for (int s = 0; s < 4; ++s) {
+        int iiL = ((ii + di[s] - 1) % nx + nx) % nx + 1;
+        int jjL = ((jj + dj[s] - 1) % ny + ny) % ny + 1;
+        double hL = o->Up[iN(o, m, 0, iiL, jjL)];
+        size_t KL = (size_t)5 * K + slot[s];
+        for (int rr = 0; rr < 4; ++rr) {
+          double th = THETA[m][rr];
+          double pr = o->ProdUp[4 * KL + rr], de = o->DestUp[4 * KL + rr];
+          if (th > 0.) {
+            o->Mass[KL] = o->Mass[KL] - dt * th * (orc_patankar_div(pr, hL));
+            o->Diag[K] = o->Diag[K] + dt * th * (orc_patankar_div(de, hK));
+          } else {
+            o->Diag[K] = o->Diag[K] - dt * th * (orc_patankar_div(pr, hK));
+            o->Mass[KL] = o->Mass[KL] + dt * th * (orc_patankar_div(de, hL));
+          }
+        }
+      }
+      o->rhs[K] = o->Up[iN(o, 0, 0, ii, jj)];
+    }
+  int k = jacobi_impl(o, o->Mass, o->Diag, o->rhs, o->haVec, o->solp, o->prod);
+  for (int jj = 1; jj <= ny; ++jj)
+    for (int ii = 1; ii <= nx; ++ii) o->Ua[iN(o, m, 0, ii, jj)] = o->haVec[global_elem(o, ii, jj) - 1];
+  return k;
+}
+
+static void store_node_derivative(orc_t *o, int m) {
+  const int nx = o->nx, ny = o->ny;
+  for (int jj = 1; jj <= ny; ++jj)
+    for (int ii = 1; ii <= nx; ++ii)
+      for (int v = 1; v < NVAR; ++v) o->FUp[iN(o, m, v, ii, jj)] = o->Ut[iC(o, v, ii, jj)];
+  size_t nnz = (size_t)5 * nx * ny;
+  for (size_t k = 0; k < nnz; ++k) {
+    o->ProdUp[4 * k + m] = o->Prod[k];
+    o->DestUp[4 * k + m] = o->Dest[k];
+  }
+}
+
+static void momentum_update(orc_t *o, int m, double dt) {
+  /* Ua(m,2:3) = Ua(1,2:3); DO r: Ua(m) = Ua(m) + dt*theta(m,r)*FUp(r)  (:531-543) */
+  const int nx = o->nx, ny = o->ny;
+  for (int jj = 1; jj <= ny; ++jj)
+    for (int ii = 1; ii <= nx; ++ii)
+      for (int v = 1; v < NVAR; ++v) o->Ua[iN(o, m, v, ii, jj)] = o->Ua[iN(o, 0, v, ii, jj)];
+  for (int r = 0; r < 4; ++r)
+    for (int jj = 1; jj <= ny; ++jj)
+      for (int ii = 1; ii <= nx; ++ii)
+        for (int v = 1; v < NVAR; ++v)
+          o->Ua[iN(o, m, v, ii, jj)] =
+              o->Ua[iN(o, m, v, ii, jj)] + dt * THETA[m][r] * o->FUp[iN(o, r, v, ii, jj)];
+}
+
+/* TimeDiscretizationByMPDeC5, src/timediscretization.f90:458-566 */
+int orc_step_mpdec5(orc_t *o, double dt, int *jacobi_iters) {
+  const int nx = o->nx, ny = o->ny;
+  int nsolve = 0, nfail = 0;
+  for (int m = 0; m < 4; ++m)
+    for (int jj = 1; jj <= ny; ++jj)
+      for (int ii = 1; ii <= nx; ++ii)
+        for (int v = 0; v < NVAR; ++v) {
+          o->Ua[iN(o, m, v, ii, jj)] = o->U[iU(o, v, ii, jj)];
+          o->Up[iN(o, m, v, ii, jj)] = o->U[iU(o, v, ii, jj)];
+        }
+  fv_time_derivative(o);
+  for (int m = 0; m < 4; ++m) store_node_derivative(o, m);
+  for (int kk = 1; kk <= 5; ++kk) {
+    int m0 = (kk == 5) ? 3 : 1;
+    for (int m = m0; m < 4; ++m) {
+      momentum_update(o, m, dt);
+      int it = patankar_solve(o, m, dt);
+      if (jacobi_iters) jacobi_iters[nsolve] = it;
+      if (it >= o->max_iter) ++nfail;
+      ++nsolve;
+    }
+    if (kk != 5) {
+      for (int m = 1; m < 4; ++m) {
+        for (int jj = 1; jj <= ny; ++jj)
+          for (int ii = 1; ii <= nx; ++ii)
+            for (int v = 0; v < NVAR; ++v) o->U[iU(o, v, ii, jj)] = o->Ua[iN(o, m, v, ii, jj)];
+        fv_time_derivative(o);
+        store_node_derivative(o, m);
+      }
+    }
+    memcpy(o->Up, o->Ua, sizeof(double) * 4 * NVAR * (size_t)nx * ny);
+  }
+  for (int jj = 1; jj <= ny; ++jj)
+    for (int ii = 1; ii <= nx; ++ii)
+      for (int v = 0; v < NVAR; ++v) o->U[iU(o, v, ii, jj)] = o->Ua[iN(o, 3, v, ii, jj)];
+  return nfail;
+}

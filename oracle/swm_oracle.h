@@ -1,0 +1,72 @@
+/*
+ * swm_oracle.h -- CPU restatement of the sw-mpdec mPDeC5 / WENO5 PATANKAR time
+ * step, used ONLY as a test oracle (by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py).  The product path never links it.
+ *
+ * Arrays use the reference's Fortran layouts (column-major, first index
+ * fastest) so that dumps from the reference build (oracle/_ref) compare
+ * byte-for-byte:
+ *   U, Ut, S        (3, nx, ny)            interior only on the API
+ *   FX              (3, 0:nx, 1:ny)
+ *   FY              (3, 1:nx, 0:ny)
+ *   Prod/Dest       (5*nx*ny)              slots S,W,self,E,N per cell
+ *   bathymetry      dbdx/dbdy planes [(yg*4+xg)][j][i] (i fastest), the
+ *                   values of Bathymetry_X/_Y at MeshGP(:,i,j,xg,yg)
+ */
+#ifndef SWM_ORACLE_H
+#define SWM_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_state orc_t;
+
+/* nx == ny is not required.  wbnd: WeightsGPBnd(1:4) (src/mesh.f90:127).
+ * wellbalanced: the -DWELLBALANCED variant.  Returns NULL on bad input. */
+orc_t *orc_create(int nx, int ny, double dx, double dy, double gravity,
+                  double cfl, int wellbalanced, const double wbnd[4]);
+void orc_destroy(orc_t *o);
+
+/* BathymetryFlag > 0 switches the source term on (src/equation.f90:324);
+ * the tables hold d b/dx, d b/dy at the 4x4 Gauss points of every cell. */
+void orc_set_bathy(orc_t *o, const double *dbdx, const double *dbdy);
+
+/* Well-balanced reference state: FVTD(Uwb) stored as FXWB/FYWB/SWB
+ * (src/finitevolume.f90:373-392). */
+void orc_init_wb(orc_t *o, const double *Uwb);
+
+void orc_set_U(orc_t *o, const double *U);
+void orc_get_U(const orc_t *o, double *U);
+
+/* One FVTimeDerivative on the current U (src/finitevolume.f90:402-440).
+ * Any output pointer may be NULL. */
+void orc_fvtd(orc_t *o, double *Ut, double *FX, double *FY, double *S,
+              double *Prod, double *Dest);
+
+/* TimeStep (src/equation.f90:791-837): returns dt, stores LambdaMaxX/Y. */
+double orc_timestep(orc_t *o, double *lmx, double *lmy);
+
+/* One mPDeC5 step with the given dt (src/timediscretization.f90:458-566).
+ * jacobi_iters[13] (may be NULL) receives the sweep count of each solve in
+ * execution order.  Returns the number of solves that hit max_iter. */
+int orc_step_mpdec5(orc_t *o, double dt, int *jacobi_iters);
+
+/* Jacobi on the reference CSR pattern (src/jacobi_iteration.f90:52-87).
+ * Mass (5n), diag (n), rhs (n) -> sol (n).  Returns the sweep count. */
+int orc_jacobi(const orc_t *o, const double *Mass, const double *diag,
+               const double *rhs, double *sol);
+
+/* Overrides of the Jacobi stop rule (defaults 1000 and 1e-16). */
+void orc_set_jacobi_limits(orc_t *o, int max_iter, double tol);
+
+/* PATANKAR_DIV (src/timediscretization.f90:913-924). */
+double orc_patankar_div(double prod, double h);
+/* WENO5_FirstSweep / WENO5_SecondSweep on one 5-point stencil. */
+void orc_weno5_first(const double q[5], double *wm, double *wp);
+void orc_weno5_second(const double q[5], double w[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

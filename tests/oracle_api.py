@@ -1,0 +1,196 @@
+"""Test-side access to the two oracles:
+
+* ``Oracle``: ctypes binding of oracle/lib/liboracle.so, the C restatement of
+  the reference time step (oracle/swm_oracle.c);
+* ``run_swref``: the reference Fortran itself, built unchanged from its own
+  sources into oracle/_ref/ (oracle/Makefile), driven through
+  oracle/ref_driver.f90.
+
+Only tests, smoke() and bench.py's cpu_baseline leg use this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE_LIB = ROOT / "oracle" / "lib" / "liboracle.so"
+SWREF = {False: ROOT / "oracle" / "_ref" / "swref_p", True: ROOT / "oracle" / "_ref" / "swref_pwb"}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+_lib = None
+
+
+def oracle_lib():
+    global _lib
+    if _lib is None:
+        if not ORACLE_LIB.exists():
+            raise RuntimeError(f"{ORACLE_LIB} is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(str(ORACLE_LIB))
+        L.orc_create.restype = ctypes.c_void_p
+        L.orc_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp]
+        L.orc_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_set_bathy.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_init_wb.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_set_U.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_get_U.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_fvtd.argtypes = [ctypes.c_void_p] + [_dp] * 6
+        L.orc_timestep.restype = ctypes.c_double
+        L.orc_timestep.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_step_mpdec5.restype = ctypes.c_int
+        L.orc_step_mpdec5.argtypes = [ctypes.c_void_p, ctypes.c_double, _ip]
+        L.orc_jacobi.restype = ctypes.c_int
+        L.orc_jacobi.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_set_jacobi_limits.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        L.orc_patankar_div.restype = ctypes.c_double
+        L.orc_patankar_div.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.orc_weno5_first.argtypes = [_dp, _dp, _dp]
+        L.orc_weno5_second.argtypes = [_dp, _dp]
+        _lib = L
+    return _lib
+
+
+class Oracle:
+    """One problem instance on the C oracle."""
+
+    def __init__(self, problem):
+        L = oracle_lib()
+        self.p = problem
+        m = problem.mesh
+        self.n = problem.n
+        w = np.ascontiguousarray(m.wbnd, dtype=np.float64)
+        self.h = L.orc_create(self.n, self.n, m.dx, m.dx, problem.gravity, problem.cfl,
+                              1 if problem.wellbalanced else 0, _ptr(w))
+        if not self.h:
+            raise ValueError("orc_create failed")
+        if problem.dbdx is not None:
+            L.orc_set_bathy(self.h, _ptr(np.ascontiguousarray(problem.dbdx)),
+                            _ptr(np.ascontiguousarray(problem.dbdy)))
+        if problem.wellbalanced:
+            L.orc_init_wb(self.h, _ptr(np.ascontiguousarray(problem.Uwb)))
+        self.set_U(problem.U0)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            oracle_lib().orc_destroy(self.h)
+            self.h = None
+
+    def set_U(self, U):
+        oracle_lib().orc_set_U(self.h, _ptr(np.ascontiguousarray(U, dtype=np.float64)))
+
+    def get_U(self):
+        U = np.empty((self.n, self.n, 3))
+        oracle_lib().orc_get_U(self.h, _ptr(U))
+        return U
+
+    def fvtd(self):
+        n = self.n
+        out = dict(Ut=np.empty((n, n, 3)), FX=np.empty((n, n + 1, 3)), FY=np.empty((n + 1, n, 3)),
+                   S=np.empty((n, n, 3)), Prod=np.empty(5 * n * n), Dest=np.empty(5 * n * n))
+        oracle_lib().orc_fvtd(self.h, *(_ptr(out[k]) for k in ("Ut", "FX", "FY", "S", "Prod", "Dest")))
+        return out
+
+    def timestep(self):
+        lx, ly = ctypes.c_double(), ctypes.c_double()
+        dt = oracle_lib().orc_timestep(self.h, ctypes.byref(lx), ctypes.byref(ly))
+        return dt, lx.value, ly.value
+
+    def step(self, dt):
+        it = (ctypes.c_int * 13)()
+        nfail = oracle_lib().orc_step_mpdec5(self.h, dt, it)
+        return list(it), nfail
+
+    def run(self, steps):
+        """steps CFL steps; returns per-step (dt, lmx, lmy, iters) and final U."""
+        hist = []
+        for _ in range(steps):
+            dt, lx, ly = self.timestep()
+            it, _ = self.step(dt)
+            hist.append((dt, lx, ly, it))
+        return hist, self.get_U()
+
+    def set_jacobi_limits(self, max_iter, tol):
+        oracle_lib().orc_set_jacobi_limits(self.h, max_iter, tol)
+
+    def jacobi(self, Mass, diag, rhs):
+        sol = np.empty(self.n * self.n)
+        k = oracle_lib().orc_jacobi(self.h, _ptr(np.ascontiguousarray(Mass)),
+                                    _ptr(np.ascontiguousarray(diag)),
+                                    _ptr(np.ascontiguousarray(rhs)), _ptr(sol))
+        return sol, k
+
+
+def have_swref():
+    return SWREF[False].exists() and SWREF[True].exists()
+
+
+def run_swref(problem, mode: int, steps: int = 0, jac=None, timeout=600):
+    """Run the reference build on problem.  mode 1: one FVTimeDerivative,
+    2: `steps` CFL steps through TimeDiscretization, 3: TimeStep only,
+    4: jacobi on jac=(Mass, diag, rhs)."""
+    exe = SWREF[bool(problem.wellbalanced)]
+    if not exe.exists():
+        raise RuntimeError(f"{exe} not built")
+    n = problem.n
+    bflag = 1 if problem.dbdx is not None else 0
+    hdr = np.array([mode, n, steps, bflag, 1 if problem.wellbalanced else 0, 0], dtype="<i4")
+    fl = np.array([problem.x0, problem.x1, problem.gravity, problem.cfl], dtype="<f8")
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as td:
+        fin, fout = Path(td) / "in.bin", Path(td) / "out.bin"
+        with open(fin, "wb") as f:
+            f.write(hdr.tobytes())
+            f.write(fl.tobytes())
+            if mode == 4:
+                for a in jac:
+                    f.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
+            else:
+                f.write(np.ascontiguousarray(problem.U0, dtype="<f8").tobytes())
+                if problem.wellbalanced:
+                    f.write(np.ascontiguousarray(problem.Uwb, dtype="<f8").tobytes())
+        subprocess.run([str(exe), str(fin), str(fout)], check=True, cwd=td, timeout=timeout,
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE)
+        raw = np.fromfile(fout, dtype="<f8")
+    if mode == 4:
+        return raw
+    if mode == 3:
+        return tuple(raw[:3])
+    if mode == 1:
+        sizes = [("Ut", (n, n, 3)), ("FX", (n, n + 1, 3)), ("FY", (n + 1, n, 3)), ("S", (n, n, 3)),
+                 ("Prod", (5 * n * n,)), ("Dest", (5 * n * n,)), ("wbnd", (4,)), ("dx", (2,)),
+                 ("MeshGP", (4, 4, n, n, 2))]
+        out, off = {}, 0
+        for k, shp in sizes:
+            cnt = int(np.prod(shp))
+            out[k] = raw[off:off + cnt].reshape(shp)
+            off += cnt
+        assert off == raw.size
+        return out
+    # mode 2
+    # per step: dt, LambdaMaxX, LambdaMaxY, U; a non-finite dt ends the run
+    rec = 3 + 3 * n * n
+    hist = []
+    U = None
+    off = 0
+    while off < raw.size:
+        hist.append(tuple(raw[off:off + 3]))
+        if off + 3 == raw.size:
+            break
+        U = raw[off + 3:off + rec].reshape(n, n, 3)
+        off += rec
+    return hist, U
