@@ -1,0 +1,114 @@
+/*
+ * swm.h -- C ABI of the MI355X (gfx950) implementation of sw-mpdec's
+ * PATANKAR-build time step: timescheme=5 (mPDeC5) with Reconstruction=4
+ * (WENO5 at 4-point Gauss nodes), periodic boundaries, FP64.
+ *
+ * The reference (accdavlo/sw-mpdec, Fortran) has no FFI; its hot path sits
+ * behind module procedures that share module-global state.  Each entry point
+ * below names the reference code it replaces (fv-solver-sw/src/...).  A
+ * Fortran caller binds them with ISO_C_BINDING (REAL = c_double under
+ * -fdefault-real-8, INTEGER = c_int).
+ *
+ * Array layouts are the reference's, column-major, first index fastest:
+ *   U          (1:3, 1:nx, 1:ny)                     ghosts = 0
+ *              (1:3, -2:nx+3, -2:ny+3), the module array U itself, ghosts = 3
+ *   Ut, S      (1:3, 1:nx, 1:ny)
+ *   FX         (1:3, 0:nx, 1:ny)     FY (1:3, 1:nx, 0:ny)
+ *   Prod/Dest  (5*nx*ny)   CSR slots S, W, self, E, N of each row
+ *   dbdx/dbdy  (1:nx, 1:ny, 1:4, 1:4) = Bathymetry_X/_Y at MeshGP(:,i,j,xg,yg)
+ *
+ * All functions return SWM_OK (0) or a negative SWM_E* code; the message is
+ * available from swm_last_error().  Calls on one handle must come from one
+ * host thread at a time.  The library owns device memory and one HIP stream
+ * per handle; no host pointer is kept after a call returns.
+ */
+#ifndef SWM_H
+#define SWM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWM_OK 0
+#define SWM_E_ARG (-1)     /* invalid argument */
+#define SWM_E_HIP (-2)     /* HIP runtime failure (message names the call) */
+#define SWM_E_STATE (-3)   /* call out of order (e.g. WB state missing) */
+#define SWM_E_NOMEM (-4)   /* device memory exhausted */
+
+typedef struct swm_solver swm_solver_t;
+
+/* Replaces InitializeFiniteVolume's allocations (src/finitevolume.f90:44-271)
+ * and the geometry of BuildMesh (src/mesh.f90:28-146) for one grid.
+ * wgp_bnd = WeightsGPBnd(1:4) as computed by BuildMesh.  nx, ny >= 3.
+ * wellbalanced = 1 for the -DWELLBALANCED build. */
+int swm_create(int nx, int ny, double dx, double dy, double gravity, double cfl,
+               int wellbalanced, const double wgp_bnd[4], swm_solver_t **out);
+
+/* Frees the handle and its device memory (FinalizeFiniteVolume,
+ * src/finitevolume.f90:768-877). */
+void swm_destroy(swm_solver_t *h);
+
+/* Last error message of the handle (or of the failed swm_create when h is
+ * NULL; that one is per host thread). */
+const char *swm_last_error(const swm_solver_t *h);
+
+/* Device bytes a handle of this size allocates (for admission control). */
+long long swm_footprint_bytes(int nx, int ny, int wellbalanced, int bathymetry);
+
+/* Bathymetry gradient tables used by SourceFunc (src/equation.f90:398-413,
+ * Bathymetry_X/_Y :451-509), evaluated by the caller at the 4x4 Gauss points.
+ * NULL, NULL selects BathymetryFlag = 0 (S = 0, src/equation.f90:320-324). */
+int swm_set_bathy_grad(swm_solver_t *h, const double *dbdx, const double *dbdy);
+
+/* InitializeWBVariables (src/finitevolume.f90:339-396): FVTimeDerivative of
+ * the well-balanced state Uwb on the device, kept as FXWB/FYWB/SWB. */
+int swm_init_wb(swm_solver_t *h, const double *Uwb, int ghosts);
+
+/* Copy the state U into / out of the device (the module array U). */
+int swm_upload(swm_solver_t *h, const double *U, int ghosts);
+int swm_download(swm_solver_t *h, double *U);
+
+/* TimeStep (src/equation.f90:791-837): CFL time step of the device state,
+ * and LambdaMaxX/Y. */
+int swm_timestep(swm_solver_t *h, double *dt_min, double *lambda_max_x, double *lambda_max_y);
+
+/* TimeDiscretizationByMPDeC5 (src/timediscretization.f90:458-566) with its
+ * 13 FVTimeDerivative evaluations, 13 MODIFIED_PATANKAR_MATRIX_INVERSION
+ * assemblies (:771-924) and Jacobi solves (src/jacobi_iteration.f90:52-87).
+ * t is accepted for the signature's sake (periodic BCs and static bathymetry
+ * make the step independent of it).  jacobi_iters (may be NULL) receives the
+ * 13 sweep counts in execution order.  Returns the number (>= 0) of solves
+ * that stopped at the iteration limit with the residual above tolerance --
+ * the reference prints "Jacobi not converged" and continues -- or an error. */
+int swm_step_mpdec5(swm_solver_t *h, double t, double dt, int jacobi_iters[13]);
+
+/* Several steps without returning to the caller: before each step dt is the
+ * TimeStep value (a step-count stop instead of the tEnd stop of
+ * src/timediscretization.f90:57-95).  dts (may be NULL) receives nsteps dt
+ * values; stops early with SWM_E_STATE if dt is not finite. */
+int swm_run_steps(swm_solver_t *h, int nsteps, double *dts, int *jacobi_iters);
+
+/* One FVTimeDerivative of the device state (src/finitevolume.f90:402-440)
+ * with its module outputs in the reference layouts, for parity tests.  Any
+ * output may be NULL.  Ut(1,:,:) is 0 (unused with PATANKAR). */
+int swm_fvtd(swm_solver_t *h, double *Ut, double *FX, double *FY, double *S, double *Prod,
+             double *Dest);
+
+/* Test overrides of the Jacobi stop rule (maxIter = 1000, tolerance = 1e-16,
+ * src/jacobi_iteration.f90:63-64). */
+int swm_set_jacobi_limits(swm_solver_t *h, int max_iter, double tol);
+
+/* Number of Jacobi stop decisions so far that fell inside the rounding window
+ * of the parallel residual sum and were settled by the exact sequential sum. */
+long long swm_ambiguous_decisions(const swm_solver_t *h);
+
+/* Wait for the handle's stream. */
+int swm_sync(swm_solver_t *h);
+
+/* Self-check of the device's FP64 '/' and sqrt: q = a/b, r = sqrt(a). */
+int swm_debug_divsqrt(int n, const double *a, const double *b, double *q, double *r);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SWM_H */

@@ -1,0 +1,196 @@
+// swm_numerics.hpp -- per-point FP64 arithmetic of the mPDeC5 / WENO5 PATANKAR
+// step, shared by every kernel of the library.
+//
+// Each function evaluates the reference expression in the reference's order
+// (Fortran: left to right, parentheses honoured) and the whole library is
+// built with -ffp-contract=off, so no multiply-add is fused; '/' and sqrt are
+// the IEEE correctly rounded FP64 operations.  Together that makes the device
+// results bitwise equal to a contraction-free build of the reference.
+// MAX/MIN follow the reference compiler's lowering, which fixes NaN
+// behaviour: MAX(a,b) = (a > b) ? a : b, MIN(a,b) = (a < b) ? a : b.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace swm {
+
+constexpr double kWenoEps = 1.0E-24;     // src/finitevolume_vars.f90:97
+constexpr double kMinDepth = 1.0E-6;     // src/finitevolume_vars.f90:104
+constexpr double kMinTimestep = 1.0E-30; // src/finitevolume_vars.f90:106
+constexpr double kWLobatto = 1. / 12.;   // src/finitevolume_vars.f90:96
+constexpr double kHuge = 1.7976931348623157e308;  // HUGE(1.0), real(8)
+
+__device__ __forceinline__ double max_r(double a, double b) { return (a > b) ? a : b; }
+__device__ __forceinline__ double min_r(double a, double b) { return (a < b) ? a : b; }
+
+// ConsToPrim, src/equation.f90:912-945:  ht = h + MIN_DEPTH/h; u = hu/ht.
+__device__ __forceinline__ void cons_to_prim(double h, double hu, double hv, double& u,
+                                             double& v) {
+  double ht = h + kMinDepth / h;
+  u = hu / ht;
+  v = hv / ht;
+}
+
+// Smoothness indicators shared by both WENO5 sweeps,
+// src/reconstruction.f90:699-704 and :1024-1029, and (eps + beta)^2.
+__device__ __forceinline__ void weno5_den(double qm2, double qm1, double q0, double qp1,
+                                          double qp2, double& d1, double& d2, double& d3) {
+  double b1 = (1.0 / 3.0) * (4.0 * qm2 * qm2 - 19.0 * qm2 * qm1 + 25.0 * qm1 * qm1 +
+                             11.0 * qm2 * q0 - 31.0 * qm1 * q0 + 10.0 * q0 * q0);
+  double b2 = (1.0 / 3.0) * (4.0 * qm1 * qm1 - 13.0 * qm1 * q0 + 13.0 * q0 * q0 +
+                             5.0 * qm1 * qp1 - 13.0 * q0 * qp1 + 4.0 * qp1 * qp1);
+  double b3 = (1.0 / 3.0) * (10.0 * q0 * q0 - 31.0 * q0 * qp1 + 25.0 * qp1 * qp1 +
+                             11.0 * q0 * qp2 - 19.0 * qp1 * qp2 + 4.0 * qp2 * qp2);
+  d1 = (kWenoEps + b1) * (kWenoEps + b1);
+  d2 = (kWenoEps + b2) * (kWenoEps + b2);
+  d3 = (kWenoEps + b3) * (kWenoEps + b3);
+}
+
+// WENO5_FirstSweep, src/reconstruction.f90:673-755: cell average line values
+// at the left (wm) and right (wp) faces.
+__device__ __forceinline__ void weno5_first(double qm2, double qm1, double q0, double qp1,
+                                            double qp2, double& wm, double& wp) {
+  double d1, d2, d3;
+  weno5_den(qm2, qm1, q0, qp1, qp2, d1, d2, d3);
+  {
+    double a1 = (3.0 / 10.0) / d1;
+    double a2 = (3.0 / 5.0) / d2;
+    double a3 = (1.0 / 10.0) / d3;
+    double s = a1 + a2 + a3;
+    double o1 = a1 / s, o2 = a2 / s, o3 = a3 / s;
+    double w1 = (1.0 / 6.0) * (-qm2 + 5.0 * qm1 + 2.0 * q0);
+    double w2 = (1.0 / 6.0) * (2.0 * qm1 + 5.0 * q0 - qp1);
+    double w3 = (1.0 / 6.0) * (11.0 * q0 - 7.0 * qp1 + 2.0 * qp2);
+    wm = o1 * w1 + o2 * w2 + o3 * w3;
+  }
+  {
+    double a1 = (1.0 / 10.0) / d1;
+    double a2 = (3.0 / 5.0) / d2;
+    double a3 = (3.0 / 10.0) / d3;
+    double s = a1 + a2 + a3;
+    double o1 = a1 / s, o2 = a2 / s, o3 = a3 / s;
+    double w1 = (1.0 / 6.0) * (2.0 * qm2 - 7.0 * qm1 + 11.0 * q0);
+    double w2 = (1.0 / 6.0) * (-qm1 + 5.0 * q0 + 2.0 * qp1);
+    double w3 = (1.0 / 6.0) * (2.0 * q0 + 5.0 * qp1 - qp2);
+    wp = o1 * w1 + o2 * w2 + o3 * w3;
+  }
+}
+
+// One Gauss point of WENO5_SecondSweep, src/reconstruction.f90:1036-1133.
+// The literals are the reference's decimal strings; signed literals are folded
+// into the coefficients (a - c*q == a + (-c)*q exactly), and the 0.0*Q terms
+// are kept because they propagate NaN/Inf exactly as the reference does.
+template <int G>
+__device__ __forceinline__ double weno5_second_gp(double qm2, double qm1, double q0, double qp1,
+                                                  double qp2, double d1, double d2, double d3) {
+  double g1, g2, g3;
+  double c10, c11, c12, c13, c14, c20, c21, c22, c23, c24, c30, c31, c32, c33, c34;
+  if constexpr (G == 0) {
+    g1 = 0.2658420974778319; g2 = 0.6112504900322486; g3 = 0.1229074124899195;
+    c10 = -0.1642562761719537; c11 = 0.7590807081409336; c12 = 0.4051755680310201; c13 = 0.0000000000000000; c14 = 0.0000000000000000;
+    c20 = +0.0000000000000000; c21 = 0.2663118796250726; c22 = 0.8979443965468811; c23 = -0.1642562761719537; c24 = 0.0000000000000000;
+    c30 = +0.0000000000000000; c31 = 0.0000000000000000; c32 = 1.6968800354220990; c33 = -0.9631919150471715; c34 = 0.2663118796250726;
+  } else if constexpr (G == 1) {
+    g1 = 0.1281641584355059; g2 = 0.5219691498503563; g3 = 0.3498666917141379;
+    c10 = -0.1122135388132497; c11 = 0.3944175994189276; c12 = 0.7177959393943222; c13 = 0.0000000000000000; c14 = 0.0000000000000000;
+    c20 = +0.0000000000000000; c21 = 0.0577769829791784; c22 = 1.0544365558340714; c23 = -0.1122135388132497; c24 = 0.0000000000000000;
+    c30 = +0.0000000000000000; c31 = 0.0000000000000000; c32 = 1.2277675047716066; c33 = -0.2855444877507849; c34 = 0.0577769829791784;
+  } else if constexpr (G == 2) {
+    g1 = 0.3498666917141379; g2 = 0.5219691498503563; g3 = 0.1281641584355059;
+    c10 = +0.0577769829791784; c11 = -0.2855444877507849; c12 = 1.2277675047716066; c13 = 0.0000000000000000; c14 = 0.0000000000000000;
+    c20 = +0.0000000000000000; c21 = -0.1122135388132497; c22 = 1.0544365558340714; c23 = 0.0577769829791784; c24 = 0.0000000000000000;
+    c30 = +0.0000000000000000; c31 = 0.0000000000000000; c32 = 0.7177959393943222; c33 = 0.3944175994189276; c34 = -0.1122135388132497;
+  } else {
+    g1 = 0.1229074124899195; g2 = 0.6112504900322486; g3 = 0.2658420974778319;
+    c10 = +0.2663118796250726; c11 = -0.9631919150471715; c12 = 1.6968800354220990; c13 = 0.0000000000000000; c14 = 0.0000000000000000;
+    c20 = +0.0000000000000000; c21 = -0.1642562761719537; c22 = 0.8979443965468811; c23 = 0.2663118796250726; c24 = 0.0000000000000000;
+    c30 = +0.0000000000000000; c31 = 0.0000000000000000; c32 = 0.4051755680310201; c33 = 0.7590807081409336; c34 = -0.1642562761719537;
+  }
+  double a1 = g1 / d1;
+  double a2 = g2 / d2;
+  double a3 = g3 / d3;
+  double s = a1 + a2 + a3;
+  double o1 = a1 / s, o2 = a2 / s, o3 = a3 / s;
+  double w1 = c10 * qm2 + c11 * qm1 + c12 * q0 + c13 * qp1 + c14 * qp2;
+  double w2 = c20 * qm2 + c21 * qm1 + c22 * q0 + c23 * qp1 + c24 * qp2;
+  double w3 = c30 * qm2 + c31 * qm1 + c32 * q0 + c33 * qp1 + c34 * qp2;
+  return (o1 * w1 + o2 * w2 + o3 * w3);
+}
+
+// WENO5_SecondSweep, src/reconstruction.f90:999-1136: point values at the
+// four Gauss nodes of the cell.
+__device__ __forceinline__ void weno5_second(double qm2, double qm1, double q0, double qp1,
+                                             double qp2, double w[4]) {
+  double d1, d2, d3;
+  weno5_den(qm2, qm1, q0, qp1, qp2, d1, d2, d3);
+  w[0] = weno5_second_gp<0>(qm2, qm1, q0, qp1, qp2, d1, d2, d3);
+  w[1] = weno5_second_gp<1>(qm2, qm1, q0, qp1, qp2, d1, d2, d3);
+  w[2] = weno5_second_gp<2>(qm2, qm1, q0, qp1, qp2, d1, d2, d3);
+  w[3] = weno5_second_gp<3>(qm2, qm1, q0, qp1, qp2, d1, d2, d3);
+}
+
+// PositivityLimiterX/Y body, src/finitevolume.f90:671-691: theta-scaling of
+// the h values at the Gauss points of one cell towards the cell mean vh.
+__device__ __forceinline__ void positivity_limit(double vh, const double wbnd[4], double wm[4],
+                                                 double wp[4]) {
+  double alpha = 0., beta = 0.;
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    alpha = alpha + wbnd[g] * wm[g];
+    beta = beta + wbnd[g] * wp[g];
+  }
+  double csi = (vh - kWLobatto * alpha - kWLobatto * beta) / (1. - 2. * kWLobatto);
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    double mmin = min_r(min_r(csi, wm[g]), wp[g]);
+    double theta;
+    if (vh == mmin)
+      theta = 1.;
+    else
+      theta = min_r(1., fabs((vh - kMinDepth) / (vh - mmin)));
+    wp[g] = vh + theta * (wp[g] - vh);
+    wm[g] = vh + theta * (wm[g] - vh);
+  }
+}
+
+// EvaluateFlux1D, src/equation.f90:1014-1016
+__device__ __forceinline__ void flux_1d(double h, double vx, double vy, double g, double& f1,
+                                        double& f2, double& f3) {
+  f1 = h * vx;
+  f2 = h * (vx * vx) + (0.5 * g) * (h * h);
+  f3 = (h * vx) * vy;
+}
+
+// RiemannSolver + RiemannSolverByRusanov at one Gauss point,
+// src/equation.f90:1048-1069 and :1096-1103.  (n1,n2),(t1,t2) are the face's
+// NormVect/TangVect (src/mesh.f90:74-110); they are compile-time constants
+// here, and the products with 0 and 1 are kept as in the reference.
+template <int N1, int N2, int T1, int T2>
+__device__ __forceinline__ void rusanov(double hl, double ul, double vl, double hr, double ur,
+                                        double vr, double g, double& F1, double& F2, double& F3) {
+  const double n1 = N1, n2 = N2, t1 = T1, t2 = T2;
+  double pl1 = hl, pl2 = n1 * ul + n2 * vl, pl3 = t1 * ul + t2 * vl;
+  double pr1 = hr, pr2 = n1 * ur + n2 * vr, pr3 = t1 * ur + t2 * vr;
+  double cl2 = pl1 * pl2, cl3 = pl1 * pl3;  // PrimToCons
+  double cr2 = pr1 * pr2, cr3 = pr1 * pr3;
+  double fl1, fl2, fl3, fr1, fr2, fr3;
+  flux_1d(pl1, pl2, pl3, g, fl1, fl2, fl3);
+  flux_1d(pr1, pr2, pr3, g, fr1, fr2, fr3);
+  double fastl = pl2 + sqrt(g * pl1);
+  double fastr = pr2 + sqrt(g * pr1);
+  double lmax = max_r(fabs(fastl), fabs(fastr));
+  double f1 = 0.5 * ((fl1 + fr1) - lmax * (pr1 - pl1));
+  double f2 = 0.5 * ((fl2 + fr2) - lmax * (cr2 - cl2));
+  double f3 = 0.5 * ((fl3 + fr3) - lmax * (cr3 - cl3));
+  F1 = f1;
+  F2 = n1 * f2 + t1 * f3;
+  F3 = n2 * f2 + t2 * f3;
+}
+
+// PATANKAR_DIV, src/timediscretization.f90:913-924
+__device__ __forceinline__ double patankar_div(double prod, double h) {
+  if (h < kMinDepth) return 0.;
+  return 2. * h * prod / (h * h + max_r(h * h, kMinDepth));
+}
+
+}  // namespace swm
