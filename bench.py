@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Benchmark of the mPDeC5 / WENO5 PATANKAR time step on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): one 2048x2048
+periodic grid, FP64, PATANKAR + WELLBALANCED, lake at rest over 8 seeded
+Gaussian bumps plus a seeded 1e-3 Gaussian surface perturbation, CFL 0.5.
+A step is one full time step: the CFL reduction (TimeStep) and one mPDeC5
+step (13 FVTimeDerivative evaluations, 13 Patankar assemblies and Jacobi
+solves), with the state resident in HBM.
+
+    python bench.py [--gpus 1] [--steps 100] [--warmup 2] [--size 2048]
+
+Prints one JSON line.  value = cell-steps/s = size^2 * steps / seconds.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "sw-mpdec_amd"))
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+# Algorithmic bytes per cell of one launch of each kernel (8-byte words the
+# kernel must move at least once): fvtd_x/fvtd_y read the state (h, hu, hv)
+# and write 3 face fluxes per cell face; see DESIGN.md.
+ALGO_BYTES_PER_CELL = {"fvtd_x": 48.0, "fvtd_y": 48.0}
+# FP64 operations per cell of one fvtd_x / fvtd_y launch, counted from the
+# source (each +, -, *, /, sqrt = 1): ConsToPrim 4, three first sweeps 348,
+# six second sweeps 1320, limiter 70, Rusanov at 4 Gauss points 264.
+FP64_OPS_PER_CELL = {"fvtd_x": 2006.0, "fvtd_y": 2006.0}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(kernel: str, size: int):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy-kernel calibration), read from the committed summary."""
+    f = ROOT / "profiles" / f"traffic_{kernel}_{size}.md"
+    if not f.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read_text())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(budget_s: float = 20.0):
+    """The reference Fortran itself (oracle/_ref, built from its own sources)
+    on the host: one core, the reference's own lake-at-rest bathymetry
+    (BathymetryFlag=1) with the well-balanced build, 128x128, the number of
+    CFL steps that fits the time budget.  Release-style build if present."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import problems as P
+    from oracle_api import big_stack
+
+    exe = ROOT / "oracle" / "_ref" / "swref_pwb_fast"
+    kind_note = "amdflang -O3 -march=x86-64-v3"
+    if not exe.exists():
+        exe = ROOT / "oracle" / "_ref" / "swref_pwb"
+        kind_note = "amdflang -O2 -ffp-contract=off"
+    if not exe.exists():
+        return None
+    n = 128
+    p = P.lake_flag1(n, True)
+
+    def run(steps):
+        hdr = np.array([5, n, steps, 1, 1, 0], dtype="<i4")
+        fl = np.array([p.x0, p.x1, p.gravity, p.cfl], dtype="<f8")
+        with tempfile.TemporaryDirectory() as td:
+            fin, fout = Path(td) / "in.bin", Path(td) / "out.bin"
+            with open(fin, "wb") as f:
+                f.write(hdr.tobytes())
+                f.write(fl.tobytes())
+                f.write(np.ascontiguousarray(p.U0, dtype="<f8").tobytes())
+                f.write(np.ascontiguousarray(p.Uwb, dtype="<f8").tobytes())
+            cmd = [str(exe), str(fin), str(fout)]
+            if os.path.exists("/usr/bin/taskset"):
+                cmd = ["taskset", "-c", "1"] + cmd
+            subprocess.run(cmd, check=True, cwd=td, stdout=subprocess.DEVNULL,
+                           stderr=subprocess.DEVNULL, env=dict(os.environ, OMP_NUM_THREADS="1"),
+                           preexec_fn=big_stack, timeout=600)
+            raw = np.fromfile(fout, dtype="<f8")
+        return float(raw[0]), int(raw[1])
+
+    t1, k1 = run(1)
+    steps = max(1, min(50, int(budget_s / max(t1, 1e-3))))
+    t, k = run(steps)
+    rate = n * n * k / t
+    return {"value": rate, "unit": "cell-steps/s", "cores": 1, "kind": "reference",
+            "sample": (f"reference Fortran ({kind_note}), PATANKAR+WELLBALANCED, "
+                       f"BathymetryFlag=1 lake at rest + 1e-3 bump, {n}x{n}, {k} CFL steps, "
+                       f"{t:.1f} s, time loop only")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", type=int, default=2048)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "batched and served solves shard across GPUs without communication, see DESIGN.md",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import problems as P
+    import swm
+
+    n = args.size
+    t0 = time.time()
+    prob = P.lake_bumps(n, args.seed, steps=args.steps)
+    log(f"bench: problem {n}x{n} lake_bumps seed {args.seed} built in {time.time() - t0:.1f} s")
+    s = swm.Solver.from_problem(prob)
+    del prob
+    for _ in range(args.warmup):
+        dt, _, _ = s.timestep()
+        s.step(dt)
+    s.sync()
+    log(f"bench: {args.warmup} warmup steps done")
+
+    s.profile(True)
+    s.sync()
+    iters = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dt, _, _ = s.timestep()
+        it, _ = s.step(dt)
+        iters.append(it)
+    s.sync()
+    t1 = time.perf_counter()
+    prof = s.profile_read()
+    s.profile(False)
+    elapsed = t1 - t0
+    log(f"bench: {args.steps} timed steps in {elapsed:.3f} s")
+
+    value = n * n * args.steps / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+    kern = {k: {"ms_total": v[0], "launches": v[1],
+                "avg_us": (1e3 * v[0] / v[1]) if v[1] else 0.0} for k, v in prof.items() if v[1]}
+    dom = max(kern, key=lambda k: kern[k]["ms_total"])
+    avg_s = kern[dom]["avg_us"] * 1e-6
+    cells = n * n
+    algo = ALGO_BYTES_PER_CELL.get(dom, 0.0) * cells
+    achieved = algo / avg_s / 1e9 if avg_s > 0 else 0.0
+    roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic_from_profiles(dom, n),
+            "algo_bytes_per_launch": algo, "avg_launch_us": kern[dom]["avg_us"]}
+    if dom in FP64_OPS_PER_CELL:
+        tf = FP64_OPS_PER_CELL[dom] * cells / avg_s / 1e12
+        roof["fp64_valu"] = {"achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                             "frac": tf / FP64_PEAK_TFLOPS,
+                             "ops_per_launch": FP64_OPS_PER_CELL[dom] * cells}
+    jsum = int(np.sum(iters))
+    out = {
+        "metric": "mPDeC5 cell-steps/s FP64 at 2048\u00b2 vs CPU; batched jobs/s + p50/p99 latency",
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded lake at rest over Gaussian bumps + perturbation)",
+        "config": {"workload": f"mPDeC5/WENO5 PATANKAR+WELLBALANCED {n}x{n} periodic, "
+                               f"8 Gaussian bumps, CFL 0.5",
+                   "grid": [n, n], "cfl": 0.5, "wellbalanced": True, "seed": args.seed},
+        "jacobi_sweeps_per_step": jsum / args.steps,
+        "kernels": kern,
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline()
+        if cb:
+            out["cpu_baseline"] = cb
+            out["gpu_vs_cpu"] = value / cb["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

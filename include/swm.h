@@ -102,6 +102,16 @@ int swm_set_jacobi_limits(swm_solver_t *h, int max_iter, double tol);
  * of the parallel residual sum and were settled by the exact sequential sum. */
 long long swm_ambiguous_decisions(const swm_solver_t *h);
 
+/* Measurement (no counterpart in the reference, which has no timers): HIP
+ * events around every kernel launch of the handle, summed per category --
+ * 0 fvtd_x, 1 fvtd_y, 2 fvtd_update, 3 momentum, 4 patankar_assemble,
+ * 5 jacobi_sweep, 6 timestep, 7 other.  swm_profile_read waits for the
+ * stream, returns milliseconds and launch counts, and clears the record. */
+#define SWM_PROF_NCAT 8
+int swm_profile_enable(swm_solver_t *h, int on);
+int swm_profile_read(swm_solver_t *h, double ms[SWM_PROF_NCAT], long long launches[SWM_PROF_NCAT]);
+const char *swm_profile_name(int category);
+
 /* Wait for the handle's stream. */
 int swm_sync(swm_solver_t *h);
 

@@ -12,6 +12,8 @@
 !   TimeDiscretization(src/timediscretization.f90:24), one CFL step per call
 !                     (tEnd = dt_min, nOutputFiles = 1)  -> mode 2
 !   jacobi            (src/jacobi_iteration.f90:52) -> mode 4
+!   mode 5: like mode 2, but only the wall time of the step loop (SYSTEM_CLOCK)
+!           and the final state are written (CPU baseline timing)
 ! With WELLBALANCED builds the well-balanced reference state is installed the
 ! way InitializeWBVariables does it (src/finitevolume.f90:373-392).
 !
@@ -32,6 +34,7 @@ USE MOD_TimeDiscretization, ONLY: TimeDiscretization
 USE MOD_JacobiIteration,    ONLY: jacobi
 IMPLICIT NONE
 INTEGER            :: mode, nx, nsteps, bflag, haswb, reserved, k, nrow
+INTEGER(KIND=8)    :: c0, c1, crate
 REAL               :: x0, x1, grav, cflin, dtmin
 REAL, ALLOCATABLE  :: Uic(:,:,:), Uwb(:,:,:), Mass(:), Diag(:), rhs(:), sol(:)
 CHARACTER(LEN=1024):: fin, fout
@@ -128,6 +131,17 @@ SELECT CASE (mode)
   CASE (3)
     dtmin = TimeStep()
     WRITE(11) dtmin, LambdaMaxX, LambdaMaxY
+  CASE (5)
+    CALL SYSTEM_CLOCK(c0, crate)
+    DO k = 1, nsteps
+      dtmin = TimeStep()
+      IF (.NOT. (dtmin .LT. HUGE(1.0))) EXIT
+      tEnd = dtmin
+      CALL TimeDiscretization()
+    END DO
+    CALL SYSTEM_CLOCK(c1)
+    WRITE(11) REAL(c1 - c0) / REAL(crate), REAL(k - 1)
+    WRITE(11) U(1:3,1:nx,1:nx)
   CASE DEFAULT
     WRITE(*,*) "swref: unknown mode"
     STOP 2

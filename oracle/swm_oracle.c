@@ -681,6 +681,11 @@ static const double THETA[4][4] = {
     {0.0730327668541684294, 0.4505740308958107176, 0.2269672331458314485, -0.0269672331458315692},
     {0.0833333333333333287, 0.4166666666666666852, 0.4166666666666666852, 0.0833333333333333287}};
 
+void orc_theta(double out[16]) {
+  for (int m = 0; m < 4; ++m)
+    for (int r = 0; r < 4; ++r) out[4 * m + r] = THETA[m][r];
+}
+
 /* MODIFIED_PATANKAR_MATRIX_INVERSION, src/timediscretization.f90:771-908.
  * m is the 0-based node; hp = Up(:,1,:,:) read from o->Up; ha -> Ua(m,1). */
 static int patankar_solve(orc_t *o, int m, double dt) {

@@ -60,6 +60,9 @@ int orc_jacobi(const orc_t *o, const double *Mass, const double *diag,
 /* Overrides of the Jacobi stop rule (defaults 1000 and 1e-16). */
 void orc_set_jacobi_limits(orc_t *o, int max_iter, double tol);
 
+/* thetaDeC(m, r) of src/timediscretization.f90:491-506, out[4*(m-1)+(r-1)]. */
+void orc_theta(double out[16]);
+
 /* PATANKAR_DIV (src/timediscretization.f90:913-924). */
 double orc_patankar_div(double prod, double h);
 /* WENO5_FirstSweep / WENO5_SecondSweep on one 5-point stencil. */

@@ -28,6 +28,30 @@ const double THETA[4][4] = {
 
 size_t align_up(size_t b) { return (b + 255) & ~size_t(255); }
 
+const char* const PROF_NAMES[SWM_PROF_NCAT] = {
+    "fvtd_x", "fvtd_y", "fvtd_update", "momentum", "patankar_assemble", "jacobi_sweep",
+    "timestep", "other"};
+
+// HIP-event timing of every kernel launch of a handle, by category.
+struct Profiler {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<int> cats;  // category of event pair k = (pool[2k], pool[2k+1])
+
+  hipEvent_t next() {
+    if (used == pool.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      pool.push_back(e);
+    }
+    return pool[used++];
+  }
+  ~Profiler() {
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+  }
+};
+
 struct Layout {
   size_t n, nfx, nfy;
   int nb_jac, nb_ts;
@@ -106,6 +130,7 @@ struct swm_solver {
   double tol = 1.e-16;
   int pred[13];
   long long ambiguous = 0;
+  Profiler prof;
   std::string err;
 
   double* take(size_t cnt) {
@@ -140,6 +165,22 @@ struct swm_solver {
   } while (0)
 
 namespace {
+
+template <class F>
+void timed(swm_solver* s, int cat, F&& launch) {
+  if (!s->prof.on) {
+    launch();
+    return;
+  }
+  hipEvent_t a = s->prof.next(), b = s->prof.next();
+  if (a) (void)hipEventRecord(a, s->stream);
+  launch();
+  if (b) (void)hipEventRecord(b, s->stream);
+  if (a && b)
+    s->prof.cats.push_back(cat);
+  else
+    s->prof.used = 2 * s->prof.cats.size();
+}
 
 int allocate(swm_solver* s) {
   const Layout& L = s->L;
@@ -224,9 +265,9 @@ int fvtd(swm_solver* s, const double* h, const double* hu, const double* hv, int
     b.bx = s->bx;
     b.by = s->by;
   }
-  launch_fvtd_x(s->G, io, s->stream);
-  launch_fvtd_y(s->G, io, s->stream);
-  launch_fvtd_update(s->G, io, wb, b, false, s->stream);
+  timed(s, 0, [&] { launch_fvtd_x(s->G, io, s->stream); });
+  timed(s, 1, [&] { launch_fvtd_y(s->G, io, s->stream); });
+  timed(s, 2, [&] { launch_fvtd_update(s->G, io, wb, b, false, s->stream); });
   LAUNCHCK(s);
   return SWM_OK;
 }
@@ -274,12 +315,13 @@ int patankar_solve(swm_solver* s, double dt, const int* ms, int nsys, const int*
     nodes.qy[r] = qy_node[r];
   }
   JacCtl ctl = jac_ctl(s);
-  launch_patankar_assemble(s->G, nodes, sys, nsys, s->h, ctl, s->stream);
+  timed(s, 4, [&] { launch_patankar_assemble(s->G, nodes, sys, nsys, s->h, ctl, s->stream); });
   LAUNCHCK(s);
   int batch = std::max(1, pred - 1);
   long launched = 1;
   for (;;) {
-    for (int b = 0; b < batch; ++b) launch_jacobi_sweep(s->G, sys, nsys, s->h, ctl, s->stream);
+    for (int b = 0; b < batch; ++b)
+      timed(s, 5, [&] { launch_jacobi_sweep(s->G, sys, nsys, s->h, ctl, s->stream); });
     LAUNCHCK(s);
     launched += batch;
     HIPCK(s, hipMemcpyAsync(s->st_host, s->st, nsys * sizeof(JacState), hipMemcpyDeviceToHost,
@@ -477,7 +519,7 @@ int swm_timestep(swm_solver_t* s, double* dt_min, double* lmx, double* lmy) {
   c.counter = s->counters + 1;
   c.out = s->ts_out;
   c.cfl = s->cfl;
-  launch_timestep(s->G, s->h, s->hu, s->hv, c, s->stream);
+  timed(s, 6, [&] { launch_timestep(s->G, s->h, s->hu, s->hv, c, s->stream); });
   LAUNCHCK(s);
   HIPCK(s, hipMemcpyAsync(s->ts_host, s->ts_out, 3 * sizeof(double), hipMemcpyDeviceToHost,
                           s->stream));
@@ -514,7 +556,10 @@ int swm_step_mpdec5(swm_solver_t* s, double t, double dt, int jacobi_iters[13]) 
       const int m = ms[q];
       double cth[4];
       for (int r = 0; r < 4; ++r) cth[r] = dt * THETA[m][r];
-      launch_momentum((int)s->L.n, s->hu, s->hv, FU2p, FU3p, cth, s->hun[m], s->hvn[m], s->stream);
+      timed(s, 3, [&] {
+        launch_momentum((int)s->L.n, s->hu, s->hv, FU2p, FU3p, cth, s->hun[m], s->hvn[m],
+                        s->stream);
+      });
     }
     LAUNCHCK(s);
     int it3[3] = {0, 0, 0};
@@ -662,6 +707,34 @@ int swm_set_jacobi_limits(swm_solver_t* s, int max_iter, double tol) {
 }
 
 long long swm_ambiguous_decisions(const swm_solver_t* s) { return s ? s->ambiguous : -1; }
+
+int swm_profile_enable(swm_solver_t* s, int on) {
+  if (!s) return SWM_E_ARG;
+  s->prof.on = on != 0;
+  return SWM_OK;
+}
+
+int swm_profile_read(swm_solver_t* s, double ms[SWM_PROF_NCAT], long long launches[SWM_PROF_NCAT]) {
+  if (!s || !ms || !launches) return SWM_E_ARG;
+  HIPCK(s, hipStreamSynchronize(s->stream));
+  for (int c = 0; c < SWM_PROF_NCAT; ++c) {
+    ms[c] = 0.0;
+    launches[c] = 0;
+  }
+  for (size_t k = 0; k < s->prof.cats.size(); ++k) {
+    float t = 0.f;
+    HIPCK(s, hipEventElapsedTime(&t, s->prof.pool[2 * k], s->prof.pool[2 * k + 1]));
+    ms[s->prof.cats[k]] += t;
+    launches[s->prof.cats[k]] += 1;
+  }
+  s->prof.cats.clear();
+  s->prof.used = 0;
+  return SWM_OK;
+}
+
+const char* swm_profile_name(int cat) {
+  return (cat >= 0 && cat < SWM_PROF_NCAT) ? PROF_NAMES[cat] : "";
+}
 
 int swm_sync(swm_solver_t* s) {
   if (!s) return SWM_E_ARG;

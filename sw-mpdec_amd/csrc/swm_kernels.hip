@@ -9,8 +9,10 @@
 //   k_fvtd_update  source term (h-only WENO5 point values x -d b) and
 //                  UpdateTimeDerivative: Ut(2:3), face mass fluxes / dx
 // The Patankar matrix assembly fuses the first Jacobi sweep; every Jacobi
-// sweep reduces sum |x_new - x_old| per block and the last block to finish
-// takes the stop decision of src/jacobi_iteration.f90:71 on the device.
+// sweep writes per-block sums of |x_new - x_old| and a one-block kernel takes
+// the stop decision of src/jacobi_iteration.f90:71 on the device.  (No grid
+// fences: on a multi-XCD device an agent-scope fence writes back the L2, so
+// the kernel boundary is the cheap way to publish the partial sums.)
 #include <hip/hip_runtime.h>
 
 #include "swm_kernels.hpp"
@@ -377,14 +379,49 @@ __device__ __forceinline__ void patankar_slot(const PatSys& S, const double q[4]
   }
 }
 
-// MODIFIED_PATANKAR_MATRIX_INVERSION (src/timediscretization.f90:795-895)
-// for up to 3 nodes at once, fused with the first Jacobi sweep (x^0 = rhs).
+// One Patankar system: MODIFIED_PATANKAR_MATRIX_INVERSION's assembly
+// (src/timediscretization.f90:795-887) for cell k, fused with the first
+// Jacobi sweep from x^0 = rhs (src/jacobi_iteration.f90:67-76).  Returns
+// |x^1 - rhs| for the residual.
+template <int S_>
+__device__ __forceinline__ double assemble_cell(const PatArgs& A, int k, int kS, int kW, int kE,
+                                                int kN, const double* qS, const double* qW,
+                                                const double* qE, const double* qN, double b,
+                                                double bS, double bW, double bE, double bN) {
+  const PatSys& S = A.sys[S_];
+  const double hK = S.hp[k];
+  double D = 1.;
+  double mS, mW, mE, mN;
+  patankar_slot(S, qS, false, hK, S.hp[kS], mS, D);
+  patankar_slot(S, qW, false, hK, S.hp[kW], mW, D);
+  patankar_slot(S, qE, true, hK, S.hp[kE], mE, D);
+  patankar_slot(S, qN, true, hK, S.hp[kN], mN, D);
+  S.MS[k] = mS;
+  S.MW[k] = mW;
+  S.ME[k] = mE;
+  S.MN[k] = mN;
+  S.D[k] = D;
+  // sparse_matmul in CSR order S, W, self (value 0), E, N, then
+  // sol = (rhs - prod)/diag
+  double p = 0.;
+  p = p + mS * bS;
+  p = p + mW * bW;
+  p = p + 0. * b;
+  p = p + mE * bE;
+  p = p + mN * bN;
+  double x1 = (b - p) / D;
+  S.x[1][k] = x1;
+  return fabs(x1 - b);
+}
+
+// Assembly of up to 3 nodes' systems; per-block partial sums of |rhs| and of
+// |x^1 - rhs| go to ctl.partial, the stop decision is k_jacobi_decide's.
+template <int NSYS>
 __global__ __launch_bounds__(256) void k_patankar_assemble(PatArgs A) {
   __shared__ double red[4];
-  __shared__ int last;
   const int nx = A.G.nx, ny = A.G.ny, n = nx * ny;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  double dsum[kMaxSys + 1] = {0., 0., 0., 0.};
+  double d0 = 0., d1 = 0., d2 = 0., db = 0.;
   if (k < n) {
     const int i = k % nx, j = k / nx;
     const int kS = (j == 0 ? ny - 1 : j - 1) * nx + i;
@@ -403,97 +440,26 @@ __global__ __launch_bounds__(256) void k_patankar_assemble(PatArgs A) {
     }
     const double b = A.hn[k];
     const double bS = A.hn[kS], bW = A.hn[kW], bE = A.hn[kE], bN = A.hn[kN];
-    dsum[kMaxSys] = fabs(0.0 - b);
-    for (int s = 0; s < A.ctl.nsys; ++s) {
-      const PatSys& S = A.sys[s];
-      const double hK = S.hp[k];
-      double D = 1.;
-      double mS, mW, mE, mN;
-      patankar_slot(S, qS, false, hK, S.hp[kS], mS, D);
-      patankar_slot(S, qW, false, hK, S.hp[kW], mW, D);
-      patankar_slot(S, qE, true, hK, S.hp[kE], mE, D);
-      patankar_slot(S, qN, true, hK, S.hp[kN], mN, D);
-      S.MS[k] = mS;
-      S.MW[k] = mW;
-      S.ME[k] = mE;
-      S.MN[k] = mN;
-      S.D[k] = D;
-      // sparse_matmul in CSR order S, W, self (value 0), E, N, then
-      // sol = (rhs - prod)/diag  (src/jacobi_iteration.f90:35-40, 73-76)
-      double p = 0.;
-      p = p + mS * bS;
-      p = p + mW * bW;
-      p = p + 0. * b;
-      p = p + mE * bE;
-      p = p + mN * bN;
-      double x1 = (b - p) / D;
-      S.x[1][k] = x1;
-      dsum[s] = fabs(x1 - b);
-    }
+    db = fabs(0.0 - b);
+    d0 = assemble_cell<0>(A, k, kS, kW, kE, kN, qS, qW, qE, qN, b, bS, bW, bE, bN);
+    if constexpr (NSYS > 1)
+      d1 = assemble_cell<1>(A, k, kS, kW, kE, kN, qS, qW, qE, qN, b, bS, bW, bE, bN);
+    if constexpr (NSYS > 2)
+      d2 = assemble_cell<2>(A, k, kS, kW, kE, kN, qS, qW, qE, qN, b, bS, bW, bE, bN);
   }
-  __threadfence();
-  for (int s = 0; s <= kMaxSys; ++s) {
-    if (s < A.ctl.nsys || s == kMaxSys) {
-      double t = block_sum(dsum[s], red);
-      if (threadIdx.x == 0) A.ctl.partial[(size_t)s * gridDim.x + blockIdx.x] = t;
-    }
-  }
-  if (threadIdx.x == 0) {
-    __threadfence();
-    unsigned int done = atomicAdd(A.ctl.counter, 1u);
-    last = (done == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  // last block: sum the partials in block order and decide
   const int nb = gridDim.x;
-  for (int s = 0; s <= kMaxSys; ++s) {
-    if (!(s < A.ctl.nsys || s == kMaxSys)) continue;
-    double t = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) t += A.ctl.partial[(size_t)s * nb + b];
-    t = block_sum(t, red);
-    if (threadIdx.x == 0) red[0] = t;
-    __syncthreads();
-    dsum[s] = red[0];
-    __syncthreads();
+  double t = block_sum(d0, red);
+  if (threadIdx.x == 0) A.ctl.partial[blockIdx.x] = t;
+  if constexpr (NSYS > 1) {
+    t = block_sum(d1, red);
+    if (threadIdx.x == 0) A.ctl.partial[(size_t)nb + blockIdx.x] = t;
   }
-  if (threadIdx.x == 0) {
-    *A.ctl.counter = 0u;
-    // initial residual sum |0 - rhs| / n, shared by all systems
-    int c0 = certified_continue(dsum[kMaxSys], n, A.ctl.tol);
-    int amb0 = 0;
-    if (c0 < 0) {
-      double r = sequential_residual(n, A.hn, nullptr, 0);
-      c0 = (A.ctl.tol < r) ? 1 : 0;
-      amb0 = 1;
-    }
-    for (int s = 0; s < A.ctl.nsys; ++s) {
-      JacState& st = A.ctl.st[s];
-      st.ambiguous = amb0;
-      if (!c0 || A.ctl.max_iter <= 0) {
-        st.k = 0;
-        st.active = 0;
-        st.zero = 1;
-        st.cur = 1;
-        st.nonconv = 0;
-        st.residual = dsum[kMaxSys] / (double)n;
-        continue;
-      }
-      st.k = 1;
-      st.zero = 0;
-      st.cur = 1;
-      int c1 = certified_continue(dsum[s], n, A.ctl.tol);
-      if (c1 < 0) {
-        double r = sequential_residual(n, A.sys[s].x[1], A.hn, 1);
-        c1 = (A.ctl.tol < r) ? 1 : 0;
-        st.ambiguous += 1;
-      }
-      st.residual = dsum[s] / (double)n;
-      st.active = (c1 && st.k < A.ctl.max_iter) ? 1 : 0;
-      st.nonconv = (c1 && !st.active) ? 1 : 0;
-    }
+  if constexpr (NSYS > 2) {
+    t = block_sum(d2, red);
+    if (threadIdx.x == 0) A.ctl.partial[2 * (size_t)nb + blockIdx.x] = t;
   }
+  t = block_sum(db, red);
+  if (threadIdx.x == 0) A.ctl.partial[(size_t)kMaxSys * nb + blockIdx.x] = t;
 }
 
 struct JacArgs {
@@ -503,22 +469,25 @@ struct JacArgs {
   JacCtl ctl;
 };
 
-// One Jacobi sweep of every still-active system (src/jacobi_iteration.f90:72-80).
+// One Jacobi sweep of every still-active system (src/jacobi_iteration.f90:72-80);
+// per-block partial sums of |x_new - x_old|.
+template <int NSYS>
 __global__ __launch_bounds__(256) void k_jacobi_sweep(JacArgs A) {
   __shared__ double red[4];
-  __shared__ int last;
-  __shared__ int act[kMaxSys], cur[kMaxSys];
-  if (threadIdx.x < A.ctl.nsys) {
-    act[threadIdx.x] = A.ctl.st[threadIdx.x].active;
-    cur[threadIdx.x] = A.ctl.st[threadIdx.x].cur;
-  }
-  __syncthreads();
+  int act[NSYS], cur[NSYS];
   bool any = false;
-  for (int s = 0; s < A.ctl.nsys; ++s) any = any || act[s];
+#pragma unroll
+  for (int s = 0; s < NSYS; ++s) {
+    act[s] = A.ctl.st[s].active;
+    cur[s] = A.ctl.st[s].cur;
+    any = any || act[s];
+  }
   if (!any) return;  // every system has stopped: this launch is a no-op
   const int nx = A.nx, ny = A.ny, n = nx * ny;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  double dsum[kMaxSys] = {0., 0., 0.};
+  double dsum[NSYS];
+#pragma unroll
+  for (int s = 0; s < NSYS; ++s) dsum[s] = 0.;
   if (k < n) {
     const int i = k % nx, j = k / nx;
     const int kS = (j == 0 ? ny - 1 : j - 1) * nx + i;
@@ -526,7 +495,8 @@ __global__ __launch_bounds__(256) void k_jacobi_sweep(JacArgs A) {
     const int kW = j * nx + (i == 0 ? nx - 1 : i - 1);
     const int kE = j * nx + (i == nx - 1 ? 0 : i + 1);
     const double b = A.hn[k];
-    for (int s = 0; s < A.ctl.nsys; ++s) {
+#pragma unroll
+    for (int s = 0; s < NSYS; ++s) {
       if (!act[s]) continue;
       const PatSys& S = A.sys[s];
       const double* xo = S.x[cur[s]];
@@ -542,34 +512,77 @@ __global__ __launch_bounds__(256) void k_jacobi_sweep(JacArgs A) {
       dsum[s] = fabs(xv - xo[k]);
     }
   }
-  __threadfence();
-  for (int s = 0; s < A.ctl.nsys; ++s) {
+  const int nb = gridDim.x;
+#pragma unroll
+  for (int s = 0; s < NSYS; ++s) {
     if (!act[s]) continue;
     double t = block_sum(dsum[s], red);
-    if (threadIdx.x == 0) A.ctl.partial[(size_t)s * gridDim.x + blockIdx.x] = t;
+    if (threadIdx.x == 0) A.ctl.partial[(size_t)s * nb + blockIdx.x] = t;
   }
-  if (threadIdx.x == 0) {
-    __threadfence();
-    unsigned int done = atomicAdd(A.ctl.counter, 1u);
-    last = (done == gridDim.x - 1);
-  }
+}
+
+// Sum of the nb per-block partials of slot s, in block order (one block).
+__device__ double sum_partials(const double* part, int nb, double* red) {
+  double t = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) t += part[b];
+  t = block_sum(t, red);
+  __shared__ double bc;
+  if (threadIdx.x == 0) bc = t;
   __syncthreads();
-  if (!last) return;
-  __threadfence();
-  const int nb = gridDim.x;
-  double tot[kMaxSys] = {0., 0., 0.};
+  t = bc;
+  __syncthreads();
+  return t;
+}
+
+// Stop decision after the assembly (FIRST) or after a sweep: one block.
+template <bool FIRST>
+__global__ __launch_bounds__(256) void k_jacobi_decide(JacArgs A, int nb) {
+  __shared__ double red[4];
+  const int n = A.nx * A.ny;
+  double tot[kMaxSys + 1];
+  int act[kMaxSys], cur[kMaxSys];
   for (int s = 0; s < A.ctl.nsys; ++s) {
-    if (!act[s]) continue;
-    double t = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) t += A.ctl.partial[(size_t)s * nb + b];
-    t = block_sum(t, red);
-    if (threadIdx.x == 0) red[0] = t;
-    __syncthreads();
-    tot[s] = red[0];
-    __syncthreads();
+    act[s] = FIRST ? 1 : A.ctl.st[s].active;
+    cur[s] = FIRST ? 0 : A.ctl.st[s].cur;
   }
-  if (threadIdx.x == 0) {
-    *A.ctl.counter = 0u;
+  for (int s = 0; s < A.ctl.nsys; ++s)
+    tot[s] = act[s] ? sum_partials(A.ctl.partial + (size_t)s * nb, nb, red) : 0.0;
+  if (FIRST) tot[kMaxSys] = sum_partials(A.ctl.partial + (size_t)kMaxSys * nb, nb, red);
+  if (threadIdx.x != 0) return;
+  if (FIRST) {
+    // initial residual SUM(ABS(0 - rhs))/n, the same for every system
+    int c0 = certified_continue(tot[kMaxSys], n, A.ctl.tol);
+    int amb0 = 0;
+    if (c0 < 0) {
+      double r = sequential_residual(n, A.hn, nullptr, 0);
+      c0 = (A.ctl.tol < r) ? 1 : 0;
+      amb0 = 1;
+    }
+    for (int s = 0; s < A.ctl.nsys; ++s) {
+      JacState& st = A.ctl.st[s];
+      st.ambiguous = amb0;
+      st.cur = 1;
+      if (!c0 || A.ctl.max_iter <= 0) {
+        st.k = 0;
+        st.active = 0;
+        st.zero = 1;
+        st.nonconv = 0;
+        st.residual = tot[kMaxSys] / (double)n;
+        continue;
+      }
+      st.k = 1;
+      st.zero = 0;
+      int c1 = certified_continue(tot[s], n, A.ctl.tol);
+      if (c1 < 0) {
+        double r = sequential_residual(n, A.sys[s].x[1], A.hn, 1);
+        c1 = (A.ctl.tol < r) ? 1 : 0;
+        st.ambiguous += 1;
+      }
+      st.residual = tot[s] / (double)n;
+      st.active = (c1 && st.k < A.ctl.max_iter) ? 1 : 0;
+      st.nonconv = (c1 && !st.active) ? 1 : 0;
+    }
+  } else {
     for (int s = 0; s < A.ctl.nsys; ++s) {
       if (!act[s]) continue;
       JacState& st = A.ctl.st[s];
@@ -605,7 +618,7 @@ __global__ void k_jacobi_zero_fill(ZeroArgs a) {
 // TimeStep (src/equation.f90:791-837).  Without NaN the sequential MAX/MIN
 // fold equals dt = CFL*MIN(MIN(HUGE, dx/LX), dy/LY) with LX, LY the plain
 // maxima of |u + sqrt(g h)|, |v + sqrt(g h)| and 0; with a NaN anywhere the
-// fold is order dependent, so the last block replays it sequentially.
+// fold is order dependent, so k_timestep_final replays it sequentially.
 // ------------------------------------------------------------------------
 struct TsArgs {
   GridGeom G;
@@ -628,7 +641,6 @@ __device__ __forceinline__ void wave_speeds(const GridGeom& G, double h, double 
 __global__ __launch_bounds__(256) void k_timestep(TsArgs A) {
   __shared__ double sa[256], sb[256];
   __shared__ int snan[256];
-  __shared__ int last;
   const int n = A.G.nx * A.G.ny;
   double ma = 0.0, mb = 0.0;
   int isn = 0;
@@ -655,17 +667,15 @@ __global__ __launch_bounds__(256) void k_timestep(TsArgs A) {
     A.ctl.partial[4 * blockIdx.x + 0] = sa[0];
     A.ctl.partial[4 * blockIdx.x + 1] = sb[0];
     A.ctl.partial[4 * blockIdx.x + 2] = snan[0] ? 1.0 : 0.0;
-    __threadfence();
-    unsigned int done = atomicAdd(A.ctl.counter, 1u);
-    last = (done == gridDim.x - 1);
   }
-  __syncthreads();
-  if (!last || threadIdx.x != 0) return;
-  __threadfence();
-  *A.ctl.counter = 0u;
+}
+
+__global__ void k_timestep_final(TsArgs A, int nb) {
+  if (threadIdx.x != 0) return;
+  const int n = A.G.nx * A.G.ny;
   double LX = 0.0, LY = 0.0;
   int anynan = 0;
-  for (int b = 0; b < (int)gridDim.x; ++b) {
+  for (int b = 0; b < nb; ++b) {
     LX = max_r(LX, A.ctl.partial[4 * b + 0]);
     LY = max_r(LY, A.ctl.partial[4 * b + 1]);
     anynan |= (A.ctl.partial[4 * b + 2] != 0.0);
@@ -777,7 +787,20 @@ void launch_patankar_assemble(const GridGeom& G, const PatNodes& nodes, const Pa
   A.hn = hn;
   A.ctl = ctl;
   A.ctl.nsys = nsys;
-  hipLaunchKernelGGL(k_patankar_assemble, dim3(jacobi_nblocks(G)), dim3(256), 0, s, A);
+  const int nb = jacobi_nblocks(G);
+  if (nsys == 3)
+    hipLaunchKernelGGL(k_patankar_assemble<3>, dim3(nb), dim3(256), 0, s, A);
+  else if (nsys == 2)
+    hipLaunchKernelGGL(k_patankar_assemble<2>, dim3(nb), dim3(256), 0, s, A);
+  else
+    hipLaunchKernelGGL(k_patankar_assemble<1>, dim3(nb), dim3(256), 0, s, A);
+  JacArgs J;
+  J.nx = G.nx;
+  J.ny = G.ny;
+  for (int i = 0; i < nsys; ++i) J.sys[i] = sys[i];
+  J.hn = hn;
+  J.ctl = A.ctl;
+  hipLaunchKernelGGL(k_jacobi_decide<true>, dim3(1), dim3(256), 0, s, J, nb);
 }
 
 void launch_jacobi_sweep(const GridGeom& G, const PatSys* sys, int nsys, const double* hn,
@@ -789,7 +812,14 @@ void launch_jacobi_sweep(const GridGeom& G, const PatSys* sys, int nsys, const d
   A.hn = hn;
   A.ctl = ctl;
   A.ctl.nsys = nsys;
-  hipLaunchKernelGGL(k_jacobi_sweep, dim3(jacobi_nblocks(G)), dim3(256), 0, s, A);
+  const int nb = jacobi_nblocks(G);
+  if (nsys == 3)
+    hipLaunchKernelGGL(k_jacobi_sweep<3>, dim3(nb), dim3(256), 0, s, A);
+  else if (nsys == 2)
+    hipLaunchKernelGGL(k_jacobi_sweep<2>, dim3(nb), dim3(256), 0, s, A);
+  else
+    hipLaunchKernelGGL(k_jacobi_sweep<1>, dim3(nb), dim3(256), 0, s, A);
+  hipLaunchKernelGGL(k_jacobi_decide<false>, dim3(1), dim3(256), 0, s, A, nb);
 }
 
 void launch_jacobi_zero_fill(int n, const PatSys* sys, int nsys, const JacCtl& ctl,
@@ -818,7 +848,9 @@ void launch_timestep(const GridGeom& G, const double* h, const double* hu, const
   A.hu = hu;
   A.hv = hv;
   A.ctl = ctl;
-  hipLaunchKernelGGL(k_timestep, dim3(timestep_nblocks(G)), dim3(256), 0, s, A);
+  const int nb = timestep_nblocks(G);
+  hipLaunchKernelGGL(k_timestep, dim3(nb), dim3(256), 0, s, A);
+  hipLaunchKernelGGL(k_timestep_final, dim3(1), dim3(64), 0, s, A, nb);
 }
 
 void launch_aos_to_soa(int nx, int ny, int ghost, const double* aos, double* h, double* hu,

@@ -63,9 +63,10 @@ def cell_average(mesh: Mesh, f) -> np.ndarray:
     xs = mesh.gp  # (n, 4)
     for ig in range(4):
         for jg in range(4):
-            X = np.broadcast_to(xs[None, :, ig], (n, n))   # x varies along axis 1 (i)
-            Y = np.broadcast_to(xs[:, jg][:, None], (n, n))  # y varies along axis 0 (j)
-            val = f(X, Y)
+            # x varies along axis 1 (i), y along axis 0 (j); f broadcasts
+            X = xs[None, :, ig]
+            Y = xs[:, jg][:, None]
+            val = np.broadcast_to(f(X, Y), (n, n, 3))
             w = mesh.wbnd[ig] * mesh.wbnd[jg]
             out = w * val if out is None else out + w * val
     return out
@@ -102,7 +103,7 @@ class GaussBumps:
             gb = GaussBumps(A, cx, cy, s)
             # reject bathymetry reaching the free surface (would dry cells)
             xs = np.linspace(0.0, 1.0, 257)
-            if gb.value(xs[None, :], xs[:, None]).max() < max_b:
+            if np.max(gb.value(xs[None, :], xs[:, None])) < max_b:
                 return gb
 
     def _terms(self, x, y):
@@ -128,11 +129,11 @@ def bathy_tables(mesh: Mesh, grad) -> tuple[np.ndarray, np.ndarray]:
     by = np.empty((16, n, n))
     for yg in range(4):
         for xg in range(4):
-            X = np.broadcast_to(mesh.gp[None, :, xg], (n, n))
-            Y = np.broadcast_to(mesh.gp[:, yg][:, None], (n, n))
+            X = mesh.gp[None, :, xg]
+            Y = mesh.gp[:, yg][:, None]
             gx, gy = grad(X, Y)
-            bx[yg * 4 + xg] = gx
-            by[yg * 4 + xg] = gy
+            bx[yg * 4 + xg] = np.broadcast_to(gx, (n, n))
+            by[yg * 4 + xg] = np.broadcast_to(gy, (n, n))
     return bx, by
 
 
@@ -157,6 +158,7 @@ def bathy_tables_ref_flag1(mesh: Mesh) -> tuple[np.ndarray, np.ndarray]:
 
 
 def cons(h, u, v):
+    h, u, v = np.broadcast_arrays(h, u, v)
     U = np.empty(h.shape + (3,))
     U[..., 0] = h
     U[..., 1] = h * u
@@ -246,11 +248,32 @@ def dam_break(n: int, steps: int = 200, cfl: float = 0.5, radius: float = 0.25,
     def st(X, Y):
         r2 = (X - 0.5) ** 2 + (Y - 0.5) ** 2
         h = np.where(r2 < radius * radius, 1.0, dry)
-        z = np.zeros_like(X)
+        z = np.zeros_like(h)
         return cons(h, z, z)
 
     U0 = cell_average(mesh, st)
     return Problem(n, U0, cfl=cfl, steps=steps, kind="dam_break")
+
+
+def lake_flag1(n: int, wellbalanced: bool = True, pert: float = 1e-3) -> Problem:
+    """The reference's own lake at rest over BathymetryFlag=1 bathymetry,
+    b = 0.1 sin(2 pi x) cos(2 pi y) (src/equation.f90:135-140, :429-430), plus
+    a Gaussian surface perturbation of amplitude pert."""
+    m = Mesh(n)
+    pi = math.acos(-1.0)
+
+    def st(X, Y, base):
+        b = 0.1 * np.sin(2 * pi * X) * np.cos(2 * pi * Y)
+        h = 1.0 - b
+        if not base:
+            h = h + pert * np.exp(-((X - 0.3) ** 2 + (Y - 0.6) ** 2) / 0.01)
+        z = np.zeros_like(h)
+        return cons(h, z, z)
+
+    bx, by = bathy_tables_ref_flag1(m)
+    return Problem(n, cell_average(m, lambda X, Y: st(X, Y, False)), wellbalanced=wellbalanced,
+                   Uwb=cell_average(m, lambda X, Y: st(X, Y, True)) if wellbalanced else None,
+                   dbdx=bx, dbdy=by, kind="lake_flag1")
 
 
 def mixed_batch(count: int, seed: int, nmin: int = 32, nmax: int = 512,

@@ -18,6 +18,7 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 
 SWM_OK = 0
+PROF_NCAT = 8
 
 
 class SwmError(RuntimeError):
@@ -67,6 +68,10 @@ def lib():
     L.swm_ambiguous_decisions.argtypes = [vp]
     L.swm_sync.argtypes = [vp]
     L.swm_debug_divsqrt.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
+    L.swm_profile_enable.argtypes = [vp, ctypes.c_int]
+    L.swm_profile_read.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
+    L.swm_profile_name.restype = ctypes.c_char_p
+    L.swm_profile_name.argtypes = [ctypes.c_int]
     _lib = L
     return L
 
@@ -174,6 +179,16 @@ class Solver:
 
     def sync(self):
         self._check(lib().swm_sync(self._h), "swm_sync")
+
+    def profile(self, on=True):
+        self._check(lib().swm_profile_enable(self._h, int(bool(on))), "swm_profile_enable")
+
+    def profile_read(self):
+        """{kernel category: (milliseconds, launches)} since the last read."""
+        ms = (ctypes.c_double * PROF_NCAT)()
+        cnt = (ctypes.c_longlong * PROF_NCAT)()
+        self._check(lib().swm_profile_read(self._h, ms, cnt), "swm_profile_read")
+        return {lib().swm_profile_name(c).decode(): (ms[c], int(cnt[c])) for c in range(PROF_NCAT)}
 
 
 def debug_divsqrt(a, b):

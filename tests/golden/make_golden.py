@@ -1,0 +1,79 @@
+"""Writes the golden fixtures of tests/golden/ from the reference Fortran itself.
+
+The reference (accdavlo/sw-mpdec) ships no tests or fixtures, so these are
+produced by running the reference's own sources, built unchanged by
+oracle/Makefile into oracle/_ref/ and driven by oracle/ref_driver.f90, on
+small seeded inputs.  Each .npz holds the inputs and the reference outputs
+(data only).  Run from the repository root after __graft_entry__.build():
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parents[1]
+sys.path.insert(0, str(ROOT / "sw-mpdec_amd"))
+sys.path.insert(0, str(ROOT / "tests"))
+
+import problems as P  # noqa: E402
+from oracle_api import run_swref  # noqa: E402
+
+
+def problem_arrays(p):
+    d = dict(n=p.n, wellbalanced=int(p.wellbalanced), gravity=p.gravity, cfl=p.cfl,
+             x0=p.x0, x1=p.x1, U0=p.U0)
+    if p.Uwb is not None:
+        d["Uwb"] = p.Uwb
+    if p.dbdx is not None:
+        d["dbdx"] = p.dbdx
+        d["dbdy"] = p.dbdy
+    return d
+
+
+CASES = {
+    "vortex12": (lambda: P.vortex(12, 101), 4),
+    "lakeflag1_wb10": (lambda: P.lake_flag1(10, True), 3),
+    "lakeflag1_nowb9": (lambda: P.lake_flag1(9, False), 2),
+    # the state of a Gaussian-bump lake, run with a flat bottom (the reference
+    # has no Gaussian-bump bathymetry)
+    "bumpstate_flat11": (lambda: P.Problem(11, P.lake_bumps(11, 102).U0, kind="flat"), 2),
+    "dambreak10": (lambda: P.dam_break(10), 2),
+    "shallowdam12": (lambda: P.dam_break(12, dry=0.1), 3),
+}
+
+
+def main():
+    for name, (mk, steps) in CASES.items():
+        p = mk()
+        out = problem_arrays(p)
+        f = run_swref(p, 1)
+        for k in ("Ut", "FX", "FY", "S", "Prod", "Dest", "wbnd", "MeshGP"):
+            out["fvtd_" + k] = f[k]
+        if steps:
+            hist, U = run_swref(p, 2, steps=steps)
+            out["steps_hist"] = np.array(hist)
+            out["steps_U"] = U if U is not None else np.zeros(0)
+        out["ts"] = np.array(run_swref(p, 3))
+        np.savez_compressed(HERE / f"{name}.npz", **out)
+        print(f"wrote {name}.npz")
+    # Jacobi on a random diagonally dominant M-matrix in the reference CSR pattern
+    rng = np.random.default_rng(7)
+    n = 8
+    N = n * n
+    mass = -rng.uniform(0, 0.2, 5 * N)
+    mass[2::5] = 0.0  # self slot
+    diag = 1.0 + rng.uniform(0, 1, N)
+    rhs = rng.uniform(0, 1, N)
+    p = P.Problem(n, np.zeros((n, n, 3)))
+    sol = run_swref(p, 4, jac=(mass, diag, rhs))
+    np.savez_compressed(HERE / "jacobi8.npz", n=n, mass=mass, diag=diag, rhs=rhs, sol=sol)
+    print("wrote jacobi8.npz")
+
+
+if __name__ == "__main__":
+    main()

@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import resource
 import subprocess
 import tempfile
 from pathlib import Path
@@ -62,6 +63,7 @@ def oracle_lib():
         L.orc_patankar_div.argtypes = [ctypes.c_double, ctypes.c_double]
         L.orc_weno5_first.argtypes = [_dp, _dp, _dp]
         L.orc_weno5_second.argtypes = [_dp, _dp]
+        L.orc_theta.argtypes = [_dp]
         _lib = L
     return _lib
 
@@ -136,6 +138,13 @@ class Oracle:
         return sol, k
 
 
+def big_stack():
+    """preexec_fn raising the stack limit: the reference keeps O(n) automatic
+    arrays on the stack (src/equation.f90:313-315, src/timediscretization.f90:786-788)."""
+    soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
+    resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+
+
 def have_swref():
     return SWREF[False].exists() and SWREF[True].exists()
 
@@ -164,7 +173,7 @@ def run_swref(problem, mode: int, steps: int = 0, jac=None, timeout=600):
                 if problem.wellbalanced:
                     f.write(np.ascontiguousarray(problem.Uwb, dtype="<f8").tobytes())
         subprocess.run([str(exe), str(fin), str(fout)], check=True, cwd=td, timeout=timeout,
-                       stdout=subprocess.PIPE, stderr=subprocess.PIPE)
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, preexec_fn=big_stack)
         raw = np.fromfile(fout, dtype="<f8")
     if mode == 4:
         return raw

@@ -17,24 +17,8 @@ from parity_util import FVTD_KEYS, describe_diff, fvtd_view, rel_l1, same_bits
 pytestmark = pytest.mark.gpu
 
 
-def flag1_problem(n, wb, pert=1e-3):
-    """The reference's own BathymetryFlag=1 lake at rest (src/equation.f90:135-140,
-    :461-494) with a Gaussian surface perturbation."""
-    m = P.Mesh(n)
-    pi = math.acos(-1.0)
-
-    def st(X, Y, base):
-        b = 0.1 * np.sin(2 * pi * X) * np.cos(2 * pi * Y)
-        h = 1.0 - b
-        if not base:
-            h = h + pert * np.exp(-((X - 0.3) ** 2 + (Y - 0.6) ** 2) / 0.01)
-        z = np.zeros_like(X)
-        return P.cons(h, z, z)
-
-    bx, by = P.bathy_tables_ref_flag1(m)
-    return P.Problem(n, P.cell_average(m, lambda X, Y: st(X, Y, False)), wellbalanced=wb,
-                     Uwb=P.cell_average(m, lambda X, Y: st(X, Y, True)) if wb else None,
-                     dbdx=bx, dbdy=by, kind="flag1")
+def flag1_problem(n, wb):
+    return P.lake_flag1(n, wb)
 
 
 CASES = {

@@ -1,0 +1,230 @@
+"""Checks that pin the oracle by mathematics rather than by the reference:
+each is chosen so that a plausible transcription slip (a wrong literal, a
+dropped term, a swapped index) fails it."""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import problems as P
+from oracle_api import Oracle, oracle_lib
+from parity_util import same_bits
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def weno_first(q):
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    wm, wp = ctypes.c_double(), ctypes.c_double()
+    oracle_lib().orc_weno5_first(q.ctypes.data_as(_dp), ctypes.byref(wm), ctypes.byref(wp))
+    return wm.value, wp.value
+
+
+def weno_second(q):
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    w = np.empty(4)
+    oracle_lib().orc_weno5_second(q.ctypes.data_as(_dp), w.ctypes.data_as(_dp))
+    return w
+
+
+def quad_cell_avg(a, b, c, i):
+    """Average of a + b x + c x^2 over the cell [i - 1/2, i + 1/2]."""
+    return a + b * i + c * (i * i + 1.0 / 12.0)
+
+
+@pytest.mark.parametrize("seed", range(20))
+def test_weno5_first_sweep_exact_for_quadratics(seed):
+    # every candidate stencil reproduces quadratics, so WM/WP are the exact
+    # face values whatever the nonlinear weights are
+    rng = np.random.default_rng(seed)
+    a, b, c = rng.normal(size=3)
+    q = [quad_cell_avg(a, b, c, i) for i in range(-2, 3)]
+    wm, wp = weno_first(q)
+    f = lambda x: a + b * x + c * x * x  # noqa: E731
+    assert wm == pytest.approx(f(-0.5), abs=1e-12)
+    assert wp == pytest.approx(f(0.5), abs=1e-12)
+
+
+@pytest.mark.parametrize("seed", range(20))
+def test_weno5_second_sweep_exact_for_quadratics_at_gauss_nodes(seed):
+    rng = np.random.default_rng(100 + seed)
+    a, b, c = rng.normal(size=3)
+    q = [quad_cell_avg(a, b, c, i) for i in range(-2, 3)]
+    w = weno_second(q)
+    nodes, _ = P.gauss4()
+    for g in range(4):
+        x = nodes[g]
+        assert w[g] == pytest.approx(a + b * x + c * x * x, abs=1e-12)
+
+
+def test_weno5_second_sweep_linear_weights_are_optimal():
+    # for a smooth quartic the nonlinear weights approach the linear ones and
+    # the 5-point reconstruction is 5th order: the error of point values on a
+    # refined stencil drops by ~2^5
+    errs = []
+    nodes, _ = P.gauss4()
+    for h in (0.2, 0.1):
+        # cell averages of sin over cells of width h centred at i*h
+        q = [(math.cos((i - 0.5) * h) - math.cos((i + 0.5) * h)) / h for i in range(-2, 3)]
+        w = weno_second(q)
+        errs.append(max(abs(w[g] - math.sin(nodes[g] * h)) for g in range(4)))
+    assert errs[0] / errs[1] > 20
+
+
+def test_weno_symmetry():
+    # mirror symmetry: reversing the stencil swaps WM and WP and reverses the
+    # Gauss-point order (the literal tables are mirror images of each other)
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        q = rng.uniform(0.5, 1.5, 5)
+        wm, wp = weno_first(q)
+        wm2, wp2 = weno_first(q[::-1])
+        assert wm == pytest.approx(wp2, rel=1e-13) and wp == pytest.approx(wm2, rel=1e-13)
+        w = weno_second(q)
+        w2 = weno_second(q[::-1])
+        assert np.allclose(w, w2[::-1], rtol=1e-13)
+
+
+def test_gauss_rule():
+    nodes, w = P.gauss4()
+    assert sum(w) == pytest.approx(1.0, abs=1e-15)
+    # nodes of P4 on [-1/2, 1/2]: P4(t) = (35 t^4 - 30 t^2 + 3)/8 at t = 2x
+    for x in nodes:
+        t = 2 * x
+        assert abs(35 * t ** 4 - 30 * t ** 2 + 3) < 1e-13
+    # exact for degree 7
+    for d in range(8):
+        exact = 0.0 if d % 2 else (0.5 ** (d + 1) * 2) / (d + 1)
+        assert sum(wi * xi ** d for wi, xi in zip(w, nodes)) == pytest.approx(exact, abs=1e-15)
+
+
+def test_patankar_div_closed_form():
+    L = oracle_lib()
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        p = rng.uniform(-1, 1)
+        h = 10 ** rng.uniform(-8, 1)
+        got = L.orc_patankar_div(p, h)
+        if h < 1e-6:
+            assert got == 0.0
+        else:
+            assert got == pytest.approx(2 * h * p / (h * h + max(h * h, 1e-6)), rel=1e-15)
+    # for h^2 >= MIN_DEPTH it reduces to p/h
+    assert L.orc_patankar_div(0.3, 0.5) == pytest.approx(0.3 / 0.5, rel=1e-15)
+
+
+def test_dec_theta_is_lobatto_integration_matrix():
+    # thetaDeC(m, r) = integral_0^{t_m} l_r(t) dt with l_r the Lagrange basis on
+    # the Gauss-Lobatto nodes t = (0, (5-sqrt5)/10, (5+sqrt5)/10, 1)
+    # (src/timediscretization.f90:491-511); pins every literal of the table
+    th = np.empty(16)
+    oracle_lib().orc_theta(th.ctypes.data_as(_dp))
+    th = th.reshape(4, 4)
+    t = np.array([0.0, (5 - math.sqrt(5)) / 10, (5 + math.sqrt(5)) / 10, 1.0])
+    for r in range(4):
+        others = [t[k] for k in range(4) if k != r]
+        coef = np.poly(others) / np.prod([t[r] - o for o in others])
+        icoef = np.polyint(coef)
+        for m in range(4):
+            # the literals carry ~16 significant digits
+            assert th[m, r] == pytest.approx(np.polyval(icoef, t[m]), abs=1e-15)
+    for m in range(4):
+        assert th[m].sum() == pytest.approx(t[m], abs=2e-15)
+
+
+def uniform_problem(n=9, h=1.3, u=0.4, v=-0.7):
+    U = np.zeros((n, n, 3))
+    U[..., 0] = h
+    U[..., 1] = h * u
+    U[..., 2] = h * v
+    return P.Problem(n, U)
+
+
+def test_constant_state_has_zero_time_derivative():
+    o = Oracle(uniform_problem())
+    f = o.fvtd()
+    assert np.all(f["Ut"][..., 1:] == 0.0)
+    # PDS: the mass flux is the same on every face, so production of a cell
+    # from its west neighbour equals its destruction towards the east
+    prod = f["Prod"].reshape(-1, 5)
+    dest = f["Dest"].reshape(-1, 5)
+    assert np.all(prod[:, 1] == dest[:, 3])
+
+
+def test_constant_state_step_is_exact():
+    p = uniform_problem()
+    o = Oracle(p)
+    dt, _, _ = o.timestep()
+    o.step(dt)
+    U = o.get_U()
+    assert np.allclose(U, p.U0, rtol=0, atol=1e-14)
+
+
+def test_lake_at_rest_wellbalanced_is_exact():
+    p = P.lake_bumps(16, 9, amp=0.0)
+    o = Oracle(p)
+    f = o.fvtd()
+    assert np.all(f["Ut"][..., 1:] == 0.0)
+    assert np.all(f["Prod"] == 0.0) and np.all(f["Dest"] == 0.0)
+    hist, U = o.run(2)
+    assert same_bits(U, p.U0)
+
+
+def test_lake_at_rest_without_wb_is_only_approximate():
+    p = P.lake_flag1(16, False, pert=0.0)
+    f = Oracle(p).fvtd()
+    assert np.abs(f["Ut"][..., 1:]).max() > 0  # the WB subtraction matters
+
+
+def test_jacobi_converges_to_dense_solution():
+    rng = np.random.default_rng(11)
+    n = 6
+    N = n * n
+    o = Oracle(P.Problem(n, np.zeros((n, n, 3))))
+    mass = -rng.uniform(0, 0.15, 5 * N)
+    mass[2::5] = 0.0
+    diag = 1.0 + rng.uniform(0, 0.5, N)
+    rhs = rng.uniform(0.5, 1.5, N)
+    sol, k = o.jacobi(mass, diag, rhs)
+    A = np.diag(diag)
+    for row in range(N):
+        i, j = row % n, row // n
+        cols = [((j - 1) % n) * n + i, j * n + (i - 1) % n, row, j * n + (i + 1) % n,
+                ((j + 1) % n) * n + i]
+        for s in range(5):
+            A[row, cols[s]] += mass[5 * row + s]
+    x = np.linalg.solve(A, rhs)
+    assert np.allclose(sol, x, rtol=1e-13, atol=1e-14)
+    assert 2 < k < 1000
+
+
+def test_jacobi_stops_without_sweeps_on_zero_rhs():
+    n = 4
+    o = Oracle(P.Problem(n, np.zeros((n, n, 3))))
+    N = n * n
+    sol, k = o.jacobi(np.zeros(5 * N), np.ones(N), np.zeros(N))
+    assert k == 0 and np.all(sol == 0.0)
+
+
+def test_mass_conservation_and_positivity():
+    p = P.vortex(24, 5)
+    o = Oracle(p)
+    m0 = p.U0[..., 0].sum()
+    hist, U = o.run(4)
+    assert abs(U[..., 0].sum() - m0) <= 1e-12 * m0
+    assert (U[..., 0] > 0).all()
+
+
+def test_shallow_dam_break_positivity():
+    p = P.dam_break(20, dry=0.1)
+    o = Oracle(p)
+    m0 = p.U0[..., 0].sum()
+    for _ in range(4):
+        dt, _, _ = o.timestep()
+        o.step(dt)
+        U = o.get_U()
+        assert (U[..., 0] >= 0).all()
+    assert abs(U[..., 0].sum() - m0) <= 1e-12 * m0

@@ -1,0 +1,77 @@
+"""The C oracle against the reference Fortran itself, built unchanged from its
+own sources into oracle/_ref/ (skipped where that build is absent)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import problems as P
+from oracle_api import Oracle, have_swref, run_swref
+from parity_util import FVTD_KEYS, describe_diff, same_bits
+
+pytestmark = pytest.mark.skipif(not have_swref(), reason="oracle/_ref not built")
+
+
+def random_state(n, seed, amp=0.2):
+    """A rough but positive random state: exercises the nonlinear WENO weights
+    and the limiter far from the smooth regime."""
+    rng = np.random.default_rng(seed)
+    U = np.empty((n, n, 3))
+    U[..., 0] = 1.0 + amp * rng.uniform(-1, 1, (n, n))
+    U[..., 1] = U[..., 0] * rng.normal(0, 0.3, (n, n))
+    U[..., 2] = U[..., 0] * rng.normal(0, 0.3, (n, n))
+    return P.Problem(n, U, kind="random")
+
+
+CASES = {
+    "vortex21": lambda: P.vortex(21, 31),
+    "random13": lambda: random_state(13, 1),
+    "random16_rough": lambda: random_state(16, 2, amp=0.9),
+    "lakeflag1_wb14": lambda: P.lake_flag1(14, True),
+    "dam15": lambda: P.dam_break(15),
+    "dam15_mindepth": lambda: P.dam_break(15, dry=1e-6),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_fvtd_bitwise(case):
+    p = CASES[case]()
+    ref = run_swref(p, 1)
+    got = Oracle(p).fvtd()
+    for k in FVTD_KEYS:
+        a, b = got[k], ref[k]
+        if k in ("Ut", "S"):
+            a, b = a[..., 1:], b[..., 1:]
+        assert same_bits(a, b), f"{case} {k}: {describe_diff(a, b)}"
+
+
+@pytest.mark.parametrize("case,steps", [("vortex21", 5), ("random13", 3), ("lakeflag1_wb14", 4),
+                                        ("dam15_mindepth", 3), ("dam15", 2)])
+def test_steps_bitwise(case, steps):
+    p = CASES[case]()
+    hist_r, U_r = run_swref(p, 2, steps=steps)
+    o = Oracle(p)
+    U = None
+    for k, row in enumerate(hist_r):
+        dt, lx, ly = o.timestep()
+        assert same_bits([dt, lx, ly], row), f"step {k}"
+        if not np.isfinite(dt):
+            break
+        o.step(dt)
+        U = o.get_U()
+    assert same_bits(U, U_r), describe_diff(U, U_r)
+
+
+def test_jacobi_bitwise_random_matrices():
+    rng = np.random.default_rng(3)
+    for n in (5, 9):
+        N = n * n
+        mass = -rng.uniform(0, 0.24, 5 * N)
+        mass[2::5] = 0.0
+        diag = 1.0 + rng.uniform(0, 0.1, N)
+        rhs = rng.uniform(0, 2, N)
+        p = P.Problem(n, np.zeros((n, n, 3)))
+        ref = run_swref(p, 4, jac=(mass, diag, rhs))
+        sol, k = Oracle(p).jacobi(mass, diag, rhs)
+        assert k > 5
+        assert same_bits(sol, ref)
