@@ -34,6 +34,7 @@ extern "C" {
 #define SWM_E_HIP (-2)     /* HIP runtime failure (message names the call) */
 #define SWM_E_STATE (-3)   /* call out of order (e.g. WB state missing) */
 #define SWM_E_NOMEM (-4)   /* device memory exhausted */
+#define SWM_E_BUDGET (-5)  /* the job does not fit the engine's memory budget now */
 
 typedef struct swm_solver swm_solver_t;
 
@@ -114,6 +115,63 @@ const char *swm_profile_name(int category);
 
 /* Wait for the handle's stream. */
 int swm_sync(swm_solver_t *h);
+
+/* ------------------------------------------------------------------------
+ * Batched and served solves (no counterpart in the reference, which runs one
+ * grid per process): an engine holds any number of independent grids on one
+ * GPU and advances every running grid by one mPDeC5 step per
+ * swm_engine_step, each with its own CFL time step, in shared kernel
+ * launches.  Grids can be submitted and released between steps (continuous
+ * batching); admission is checked against a device-memory budget.  Each
+ * grid's results are bitwise those of the plain solver above.
+ * ------------------------------------------------------------------------ */
+typedef struct swm_engine swm_engine_t;
+
+typedef struct {
+  int nx, ny;
+  double dx, dy, gravity, cfl;
+  int wellbalanced;          /* 1: Uwb required */
+  double wgp_bnd[4];         /* WeightsGPBnd(1:4) */
+  const double *U0;          /* (1:3, 1:nx, 1:ny) */
+  const double *Uwb;         /* (1:3, 1:nx, 1:ny) or NULL */
+  const double *dbdx;        /* (1:nx, 1:ny, 1:4, 1:4) or NULL (flat bottom) */
+  const double *dbdy;
+  int nsteps;                /* CFL steps to run; < 0: until released */
+} swm_job_desc;
+
+#define SWM_JOB_RUNNING 0
+#define SWM_JOB_DONE 1
+#define SWM_JOB_FAILED 2     /* TimeStep returned a non-finite dt */
+
+typedef struct {
+  int state, steps_done, steps_left;
+  double t;                  /* simulated time reached */
+  long long jacobi_sweeps;   /* total over all solves */
+  int nonconverged;          /* solves stopped at the iteration limit */
+  long long bytes;           /* device memory held */
+} swm_job_info;
+
+/* budget_bytes <= 0: no limit beyond the device's memory. */
+int swm_engine_create(long long budget_bytes, swm_engine_t **out);
+void swm_engine_destroy(swm_engine_t *e);
+const char *swm_engine_last_error(const swm_engine_t *e);
+long long swm_job_footprint_bytes(int nx, int ny, int wellbalanced, int bathymetry);
+long long swm_engine_bytes_in_use(const swm_engine_t *e);
+/* Copies the inputs to the device (and builds the well-balanced state);
+ * returns SWM_E_BUDGET without side effects when the job does not fit. */
+int swm_engine_submit(swm_engine_t *e, const swm_job_desc *job, int *job_id);
+/* One step of every running job; *n_running (may be NULL) receives the
+ * number still running afterwards. */
+int swm_engine_step(swm_engine_t *e, int *n_running);
+int swm_engine_job_info(swm_engine_t *e, int job_id, swm_job_info *info);
+/* Current state of a job, (1:3, 1:nx, 1:ny). */
+int swm_engine_fetch(swm_engine_t *e, int job_id, double *U);
+/* Frees the job's device memory; its id may be reused. */
+int swm_engine_release(swm_engine_t *e, int job_id);
+int swm_engine_set_jacobi_limits(swm_engine_t *e, int max_iter, double tol);
+int swm_engine_profile_enable(swm_engine_t *e, int on);
+int swm_engine_profile_read(swm_engine_t *e, double ms[SWM_PROF_NCAT],
+                            long long launches[SWM_PROF_NCAT]);
 
 /* Self-check of the device's FP64 '/' and sqrt: q = a/b, r = sqrt(a). */
 int swm_debug_divsqrt(int n, const double *a, const double *b, double *q, double *r);

@@ -21,7 +21,7 @@ CSRC = PKG / "csrc"
 LIBDIR = PKG / "lib"
 LIB = LIBDIR / "libswm.so"
 
-SOURCES = ["swm_kernels.hip", "swm_api.cpp"]
+SOURCES = ["swm_kernels.hip", "swm_engine.cpp"]
 
 HIPFLAGS = [
     "--offload-arch=gfx950",
@@ -59,5 +59,37 @@ def build_library(verbose: bool = False, extra: list[str] | None = None) -> Path
     return LIB
 
 
+FORTRAN = PKG / "fortran"
+FDRIVER = LIBDIR / "swm_fdriver"
+
+
+def flang() -> str | None:
+    for c in (os.environ.get("FC"), "/opt/rocm/bin/amdflang", shutil.which("amdflang"),
+              shutil.which("flang")):
+        if c and Path(c).exists():
+            return c
+    return None
+
+
+def build_fortran_driver(verbose: bool = False) -> Path | None:
+    """The Fortran caller (fortran/swm_bridge.f90 + swm_fdriver.f90) linked
+    against libswm.so with an $ORIGIN rpath; None when no Fortran compiler."""
+    fc = flang()
+    if fc is None:
+        return None
+    moddir = LIBDIR / "fmod"
+    moddir.mkdir(parents=True, exist_ok=True)
+    cmd = [fc, "-O2", "-ffp-contract=off", "-fdefault-real-8", "-fdefault-double-8",
+           f"-J{moddir}", str(FORTRAN / "swm_bridge.f90"), str(FORTRAN / "swm_fdriver.f90"),
+           f"-L{LIBDIR}", "-lswm", "-Wl,-rpath,$ORIGIN", "-o", str(FDRIVER)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=str(moddir))
+    if r.returncode != 0:
+        raise RuntimeError(f"flang failed ({r.returncode}):\n{r.stdout}\n{r.stderr}")
+    return FDRIVER
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_fortran_driver(verbose=True))

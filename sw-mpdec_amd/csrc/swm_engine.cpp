@@ -1,0 +1,1093 @@
+// swm_engine.cpp -- host side of the solve engine and of the C ABI
+// (include/swm.h): per-job device memory, tile tables, the lockstep mPDeC5
+// step over all running jobs, the device-decided Jacobi loop, admission
+// against a device-memory budget, and HIP-event kernel timing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "swm.h"
+#include "swm_engine.hpp"
+
+using namespace swm;
+
+namespace {
+
+thread_local std::string g_error;
+
+size_t align_up(size_t b) { return (b + 255) & ~size_t(255); }
+int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+const char* const PROF_NAMES[SWM_PROF_NCAT] = {
+    "fvtd_x", "fvtd_y", "fvtd_update", "momentum", "patankar_assemble", "jacobi_sweep",
+    "timestep", "other"};
+
+// HIP-event timing of kernel launches by category.
+struct Profiler {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<int> cats;
+
+  hipEvent_t next() {
+    if (used == pool.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      pool.push_back(e);
+    }
+    return pool[used++];
+  }
+  ~Profiler() {
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+  }
+};
+
+// Bytes of every device array of one job, in allocation order.
+struct JobLayout {
+  size_t n, nfx, nfy;
+  int nblk;
+  size_t bytes;
+};
+
+JobLayout job_layout(int nx, int ny, int wb, int bathy) {
+  JobLayout L;
+  L.n = (size_t)nx * ny;
+  L.nfx = (size_t)(nx + 1) * ny;
+  L.nfy = (size_t)nx * (ny + 1);
+  L.nblk = cdiv((long)L.n, kCellBlock);
+  size_t b = 0;
+  auto add = [&](size_t cnt) { b += align_up(cnt * sizeof(double)); };
+  for (int k = 0; k < 3; ++k) add(L.n);                  // U^n
+  for (int k = 0; k < 3 * 5; ++k) add(L.n);              // nodes 1..3: hres, hun, hvn, 2 x ping-pong
+  for (int k = 0; k < 4 * 2; ++k) add(L.n);              // FU(2:3) per node
+  for (int k = 0; k < 4; ++k) { add(L.nfx); add(L.nfy); }  // qx, qy per node
+  for (int k = 0; k < kSlots * 3; ++k) { add(L.nfx); add(L.nfy); }  // face flux scratch
+  for (int k = 0; k < kMaxSys * 5; ++k) add(L.n);        // Patankar matrices
+  if (wb) {
+    for (int k = 0; k < 3; ++k) { add(L.nfx); add(L.nfy); }
+    add(L.n);
+    add(L.n);
+  }
+  add((size_t)(kMaxSys + 1) * L.nblk);                   // Jacobi partial sums
+  add(3 * (size_t)L.nblk);                               // TimeStep partials
+  L.bytes = b;
+  (void)bathy;  // the bathymetry tables are a separate allocation (bathy_bytes)
+  return L;
+}
+
+size_t bathy_bytes(size_t n) { return 32 * n * sizeof(double); }
+
+// A growable device array (owning, move-only).
+template <class T>
+struct DevVec {
+  T* p = nullptr;
+  size_t cap = 0;
+  DevVec() = default;
+  DevVec(const DevVec&) = delete;
+  DevVec& operator=(const DevVec&) = delete;
+  DevVec(DevVec&& o) noexcept : p(o.p), cap(o.cap) {
+    o.p = nullptr;
+    o.cap = 0;
+  }
+  DevVec& operator=(DevVec&& o) noexcept {
+    if (this != &o) {
+      if (p) (void)hipFree(p);
+      p = o.p;
+      cap = o.cap;
+      o.p = nullptr;
+      o.cap = 0;
+    }
+    return *this;
+  }
+  hipError_t reserve(size_t cnt) {
+    if (cnt <= cap) return hipSuccess;
+    size_t c = std::max(cnt, cap * 2 + 16);
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    hipError_t e = hipMalloc((void**)&p, c * sizeof(T));
+    if (e == hipSuccess) cap = c;
+    return e;
+  }
+  ~DevVec() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+struct Tables {
+  std::vector<TileRef> xt, yt, ct, cb;
+  std::vector<int> jl;
+  DevVec<TileRef> dxt, dyt, dct, dcb;
+  DevVec<int> djl;
+  int nxt = 0, nyt = 0, nct = 0, ncb = 0, njobs = 0;
+};
+
+enum JobState { JOB_RUNNING = 0, JOB_DONE = 1, JOB_FAILED = 2, JOB_FREE = 3 };
+
+struct Job {
+  JobDev dev{};        // host copy of the device descriptor
+  JobLayout lay{};
+  char* slab = nullptr;
+  double* bathy_buf = nullptr;  // 32 n doubles, allocated when tables are set
+  int state = JOB_FREE;
+  int steps_left = 0;   // < 0: run until told otherwise (plain solver)
+  int steps_done = 0;
+  double t = 0.0;
+  long long jacobi_sweeps = 0;
+  int nonconv = 0;
+  int last_iters[13] = {0};
+  int pred[13];
+  bool wb_ready = false;
+  Tables own;          // tables covering this job alone
+};
+
+}  // namespace
+
+struct swm_engine {
+  long long budget = 0;     // <= 0: no limit
+  long long in_use = 0;
+  hipStream_t stream = nullptr;
+  std::vector<std::unique_ptr<Job>> jobs;   // index = job id
+  DevVec<JobDev> djobs;
+  DevVec<JobStatus> dstatus;
+  JobStatus* hstatus = nullptr;  // pinned mirror
+  size_t hstatus_cap = 0;
+  double* hdt = nullptr;         // pinned: host-provided dt of a step
+  bool dirty = true;             // device descriptors / tables out of date
+  Tables all;
+  int max_iter = 1000;
+  double tol = 1.e-16;
+  long long ambiguous = 0;
+  Profiler prof;
+  std::string err;
+};
+
+struct swm_solver {
+  swm_engine* e = nullptr;
+  int job = -1;
+};
+
+#define FAIL(E, code, ...)                                                     \
+  do {                                                                         \
+    char _b[512];                                                              \
+    snprintf(_b, sizeof(_b), __VA_ARGS__);                                     \
+    if (E) (E)->err = _b;                                                      \
+    g_error = _b;                                                              \
+    return code;                                                               \
+  } while (0)
+
+#define HIPCK(E, call)                                                         \
+  do {                                                                         \
+    hipError_t _e = (call);                                                    \
+    if (_e != hipSuccess)                                                      \
+      FAIL(E, _e == hipErrorOutOfMemory ? SWM_E_NOMEM : SWM_E_HIP, "%s failed: %s", #call, \
+           hipGetErrorString(_e));                                             \
+  } while (0)
+
+#define LAUNCHCK(E)                                                            \
+  do {                                                                         \
+    hipError_t _e = hipGetLastError();                                         \
+    if (_e != hipSuccess) FAIL(E, SWM_E_HIP, "kernel launch failed: %s", hipGetErrorString(_e)); \
+  } while (0)
+
+namespace {
+
+template <class F>
+void timed(swm_engine* e, int cat, F&& launch) {
+  if (!e->prof.on) {
+    launch();
+    return;
+  }
+  hipEvent_t a = e->prof.next(), b = e->prof.next();
+  if (a) (void)hipEventRecord(a, e->stream);
+  launch();
+  if (b) (void)hipEventRecord(b, e->stream);
+  if (a && b)
+    e->prof.cats.push_back(cat);
+  else
+    e->prof.used = 2 * e->prof.cats.size();
+}
+
+void add_tiles(Tables& T, int id, const JobDev& d) {
+  const int nxt = cdiv(d.nx + 1, kFaceTile) * cdiv(d.ny, kTile);
+  const int nyt = cdiv(d.nx, kTile) * cdiv(d.ny + 1, kFaceTile);
+  const int nct = cdiv(d.nx, kTile) * cdiv(d.ny, kTile);
+  for (int t = 0; t < nxt; ++t) T.xt.push_back({id, t});
+  for (int t = 0; t < nyt; ++t) T.yt.push_back({id, t});
+  for (int t = 0; t < nct; ++t) T.ct.push_back({id, t});
+  for (int t = 0; t < d.nblk; ++t) T.cb.push_back({id, t});
+  T.jl.push_back(id);
+}
+
+int upload_tables(swm_engine* e, Tables& T) {
+  T.nxt = (int)T.xt.size();
+  T.nyt = (int)T.yt.size();
+  T.nct = (int)T.ct.size();
+  T.ncb = (int)T.cb.size();
+  T.njobs = (int)T.jl.size();
+  HIPCK(e, T.dxt.reserve(T.xt.size()));
+  HIPCK(e, T.dyt.reserve(T.yt.size()));
+  HIPCK(e, T.dct.reserve(T.ct.size()));
+  HIPCK(e, T.dcb.reserve(T.cb.size()));
+  HIPCK(e, T.djl.reserve(T.jl.size()));
+  if (T.nxt) HIPCK(e, hipMemcpy(T.dxt.p, T.xt.data(), T.nxt * sizeof(TileRef), hipMemcpyHostToDevice));
+  if (T.nyt) HIPCK(e, hipMemcpy(T.dyt.p, T.yt.data(), T.nyt * sizeof(TileRef), hipMemcpyHostToDevice));
+  if (T.nct) HIPCK(e, hipMemcpy(T.dct.p, T.ct.data(), T.nct * sizeof(TileRef), hipMemcpyHostToDevice));
+  if (T.ncb) HIPCK(e, hipMemcpy(T.dcb.p, T.cb.data(), T.ncb * sizeof(TileRef), hipMemcpyHostToDevice));
+  if (T.njobs) HIPCK(e, hipMemcpy(T.djl.p, T.jl.data(), T.njobs * sizeof(int), hipMemcpyHostToDevice));
+  return SWM_OK;
+}
+
+// Device job descriptors and the all-running-jobs tables.
+int refresh(swm_engine* e) {
+  if (!e->dirty) return SWM_OK;
+  HIPCK(e, hipStreamSynchronize(e->stream));
+  const size_t nj = e->jobs.size();
+  HIPCK(e, e->djobs.reserve(std::max<size_t>(nj, 1)));
+  std::vector<JobDev> host(nj);
+  for (size_t i = 0; i < nj; ++i) host[i] = e->jobs[i]->dev;
+  if (nj) HIPCK(e, hipMemcpy(e->djobs.p, host.data(), nj * sizeof(JobDev), hipMemcpyHostToDevice));
+  Tables& T = e->all;
+  T.xt.clear();
+  T.yt.clear();
+  T.ct.clear();
+  T.cb.clear();
+  T.jl.clear();
+  for (size_t i = 0; i < nj; ++i)
+    if (e->jobs[i]->state == JOB_RUNNING) add_tiles(T, (int)i, e->jobs[i]->dev);
+  int rc = upload_tables(e, T);
+  if (rc) return rc;
+  e->dirty = false;
+  return SWM_OK;
+}
+
+EngineArgs eargs(swm_engine* e) {
+  EngineArgs a;
+  a.jobs = e->djobs.p;
+  a.max_iter = e->max_iter;
+  a.tol = e->tol;
+  return a;
+}
+
+int fvtd_launch(swm_engine* e, Tables& T, const NodeList& L) {
+  EngineArgs a = eargs(e);
+  hipStream_t s = e->stream;
+  timed(e, 0, [&] { launch_fvtd(a, T.dxt.p, T.nxt, nullptr, 0, nullptr, 0, L, s); });
+  timed(e, 1, [&] { launch_fvtd(a, nullptr, 0, T.dyt.p, T.nyt, nullptr, 0, L, s); });
+  timed(e, 2, [&] { launch_fvtd(a, nullptr, 0, nullptr, 0, T.dct.p, T.nct, L, s); });
+  LAUNCHCK(e);
+  return SWM_OK;
+}
+
+int read_status(swm_engine* e) {
+  const size_t nj = e->jobs.size();
+  HIPCK(e, hipMemcpyAsync(e->hstatus, e->dstatus.p, nj * sizeof(JobStatus),
+                          hipMemcpyDeviceToHost, e->stream));
+  HIPCK(e, hipStreamSynchronize(e->stream));
+  return SWM_OK;
+}
+
+// One mPDeC5 step (src/timediscretization.f90:458-566) of every job in T.
+// host_dt: if not NULL the step uses this dt (written to the jobs' status),
+// otherwise each job's TimeStep (src/equation.f90:791) is computed first.
+int engine_step(swm_engine* e, Tables& T, const double* host_dt, int iters[13]) {
+  if (T.njobs == 0) return SWM_OK;
+  EngineArgs a = eargs(e);
+  hipStream_t s = e->stream;
+  if (host_dt) {
+    *e->hdt = *host_dt;
+    for (int q = 0; q < T.njobs; ++q) {
+      JobStatus* st = e->jobs[T.jl[q]]->dev.status;
+      HIPCK(e, hipMemcpyAsync(&st->dt, e->hdt, sizeof(double), hipMemcpyHostToDevice, s));
+    }
+  } else {
+    timed(e, 6, [&] { launch_timestep(a, T.dcb.p, T.ncb, T.djl.p, T.njobs, s); });
+    LAUNCHCK(e);
+  }
+  // FVTimeDerivative(U^n): node 0, shared by all four nodes at kk = 1
+  NodeList L0{};
+  L0.count = 1;
+  L0.m[0] = 0;
+  L0.slot[0] = 0;
+  int rc = fvtd_launch(e, T, L0);
+  if (rc) return rc;
+  SysList S{};
+  for (int r = 0; r < 4; ++r) S.src[r] = 0;
+  S.hp_is_un = 1;
+  int solve = 0;
+  for (int kk = 1; kk <= 5; ++kk) {
+    S.count = 0;
+    if (kk == 5)
+      S.m[S.count++] = 3;
+    else
+      for (int m = 1; m < 4; ++m) S.m[S.count++] = m;
+    timed(e, 3, [&] { launch_momentum(a, T.dcb.p, T.ncb, S, s); });
+    timed(e, 4, [&] { launch_assemble(a, T.dcb.p, T.ncb, T.djl.p, T.njobs, S, s); });
+    LAUNCHCK(e);
+    int pred = 1;
+    for (int q = 0; q < T.njobs; ++q)
+      for (int k = 0; k < S.count; ++k)
+        pred = std::max(pred, e->jobs[T.jl[q]]->pred[solve + k]);
+    int batch = std::max(0, pred - 1);
+    long launched = 1;
+    for (;;) {
+      for (int b = 0; b < batch; ++b)
+        timed(e, 5, [&] { launch_sweep(a, T.dcb.p, T.ncb, T.djl.p, T.njobs, S, s); });
+      LAUNCHCK(e);
+      launched += batch;
+      rc = read_status(e);
+      if (rc) return rc;
+      int kmin = e->max_iter;
+      bool any = false;
+      for (int q = 0; q < T.njobs; ++q) {
+        const JobStatus& js = e->hstatus[T.jl[q]];
+        for (int k = 0; k < S.count; ++k)
+          if (js.st[k].active) {
+            any = true;
+            kmin = std::min(kmin, js.st[k].k);
+          }
+      }
+      if (!any) break;
+      if (launched > (long)e->max_iter + 2)
+        FAIL(e, SWM_E_STATE, "internal: Jacobi did not stop after %ld sweeps", launched);
+      batch = std::max(2, kmin / 4);
+    }
+    for (int q = 0; q < T.njobs; ++q) {
+      Job& J = *e->jobs[T.jl[q]];
+      const JobStatus& js = e->hstatus[T.jl[q]];
+      for (int k = 0; k < S.count; ++k) {
+        J.last_iters[solve + k] = js.st[k].k;
+        J.pred[solve + k] = js.st[k].k;
+        J.jacobi_sweeps += js.st[k].k;
+        J.nonconv += js.st[k].nonconv;
+        e->ambiguous += js.st[k].ambiguous;
+      }
+    }
+    if (iters && T.njobs == 1) {
+      const JobStatus& js = e->hstatus[T.jl[0]];
+      for (int k = 0; k < S.count; ++k) iters[solve + k] = js.st[k].k;
+    }
+    timed(e, 7, [&] { launch_finalize(a, T.dcb.p, T.ncb, S, s); });
+    LAUNCHCK(e);
+    solve += S.count;
+    if (kk != 5) {
+      NodeList L{};
+      L.count = S.count;
+      for (int k = 0; k < S.count; ++k) {
+        L.m[k] = S.m[k];
+        L.slot[k] = k;
+      }
+      rc = fvtd_launch(e, T, L);
+      if (rc) return rc;
+      for (int r = 0; r < 4; ++r) S.src[r] = r;
+      S.hp_is_un = 0;
+    }
+  }
+  timed(e, 7, [&] { launch_commit(a, T.dcb.p, T.ncb, s); });
+  LAUNCHCK(e);
+  return SWM_OK;
+}
+
+int job_from_id(swm_engine* e, int id, Job** out) {
+  if (id < 0 || id >= (int)e->jobs.size() || e->jobs[id]->state == JOB_FREE)
+    FAIL(e, SWM_E_ARG, "no job with id %d", id);
+  *out = e->jobs[id].get();
+  return SWM_OK;
+}
+
+int upload_state(swm_engine* e, Job& J, const double* U, int ghosts, double* h, double* hu,
+                 double* hv) {
+  const int nx = J.dev.nx, ny = J.dev.ny;
+  size_t cnt = 3 * (size_t)(nx + 2 * ghosts) * (ny + 2 * ghosts);
+  double* tmp = nullptr;
+  HIPCK(e, hipMalloc((void**)&tmp, cnt * sizeof(double)));
+  hipError_t er = hipMemcpyAsync(tmp, U, cnt * sizeof(double), hipMemcpyHostToDevice, e->stream);
+  if (er == hipSuccess) {
+    launch_aos_to_soa(nx, ny, ghosts, tmp, h, hu, hv, e->stream);
+    er = hipGetLastError();
+  }
+  if (er == hipSuccess) er = hipStreamSynchronize(e->stream);
+  (void)hipFree(tmp);
+  if (er != hipSuccess) FAIL(e, SWM_E_HIP, "upload failed: %s", hipGetErrorString(er));
+  return SWM_OK;
+}
+
+// FVTimeDerivative of the well-balanced state into FXWB/FYWB/SWB
+// (InitializeWBVariables, src/finitevolume.f90:373-392).
+int init_wb(swm_engine* e, int id, const double* Uwb, int ghosts) {
+  Job& J = *e->jobs[id];
+  int rc = upload_state(e, J, Uwb, ghosts, J.dev.hres[1], J.dev.hun[1], J.dev.hvn[1]);
+  if (rc) return rc;
+  rc = refresh(e);
+  if (rc) return rc;
+  NodeList L{};
+  L.count = 1;
+  L.m[0] = 1;
+  L.slot[0] = 0;
+  L.wb_init = 1;
+  rc = fvtd_launch(e, J.own, L);
+  if (rc) return rc;
+  launch_copy_wb(eargs(e), J.own.dcb.p, J.own.ncb, 0, e->stream);
+  LAUNCHCK(e);
+  HIPCK(e, hipStreamSynchronize(e->stream));
+  J.wb_ready = true;
+  return SWM_OK;
+}
+
+int status_capacity(swm_engine* e, size_t nj) {
+  if (nj <= e->hstatus_cap) return SWM_OK;
+  size_t cap = std::max<size_t>(nj, 2 * e->hstatus_cap + 8);
+  // the device status array is reallocated: jobs point into it
+  HIPCK(e, hipStreamSynchronize(e->stream));
+  JobStatus* oldp = e->dstatus.p;
+  size_t oldcap = e->dstatus.cap;
+  JobStatus* np = nullptr;
+  HIPCK(e, hipMalloc((void**)&np, cap * sizeof(JobStatus)));
+  HIPCK(e, hipMemset(np, 0, cap * sizeof(JobStatus)));
+  if (oldp) HIPCK(e, hipMemcpy(np, oldp, oldcap * sizeof(JobStatus), hipMemcpyDeviceToDevice));
+  if (oldp) (void)hipFree(oldp);
+  e->dstatus.p = np;
+  e->dstatus.cap = cap;
+  for (size_t i = 0; i < e->jobs.size(); ++i) e->jobs[i]->dev.status = np + i;
+  JobStatus* h = nullptr;
+  HIPCK(e, hipHostMalloc((void**)&h, cap * sizeof(JobStatus), hipHostMallocDefault));
+  if (e->hstatus) (void)hipHostFree(e->hstatus);
+  e->hstatus = h;
+  e->hstatus_cap = cap;
+  e->dirty = true;
+  return SWM_OK;
+}
+
+// Allocates a job's device memory and descriptors (no data yet).
+int alloc_job(swm_engine* e, int nx, int ny, double dx, double dy, double g, double cfl,
+              int wb, const double wbnd[4], int nsteps, long long extra_bytes, int* id_out) {
+  *id_out = -1;
+  if (nx < 3 || ny < 3) FAIL(e, SWM_E_ARG, "nx, ny must be >= 3 (got %d, %d)", nx, ny);
+  if ((long long)nx * ny > (1LL << 30)) FAIL(e, SWM_E_ARG, "grid too large (%d x %d)", nx, ny);
+  if (!(dx > 0) || !(dy > 0)) FAIL(e, SWM_E_ARG, "dx, dy must be > 0");
+  if (!wbnd) FAIL(e, SWM_E_ARG, "wgp_bnd is NULL");
+  JobLayout lay = job_layout(nx, ny, wb, 0);
+  if (e->budget > 0 && e->in_use + (long long)lay.bytes + extra_bytes > e->budget)
+    FAIL(e, SWM_E_BUDGET, "job of %lld bytes does not fit (%lld of %lld in use)",
+         (long long)lay.bytes + extra_bytes, e->in_use, e->budget);
+  int id = -1;
+  for (size_t i = 0; i < e->jobs.size(); ++i)
+    if (e->jobs[i]->state == JOB_FREE) {
+      id = (int)i;
+      break;
+    }
+  if (id < 0) {
+    int rc = status_capacity(e, e->jobs.size() + 1);
+    if (rc) return rc;
+    e->jobs.emplace_back(new Job());
+    id = (int)e->jobs.size() - 1;
+    e->jobs[id]->dev.status = e->dstatus.p + id;
+  }
+  Job& J = *e->jobs[id];
+  JobStatus* keep_status = J.dev.status;
+  J.dev = JobDev{};
+  J.dev.status = keep_status;
+  J.own = Tables{};
+  J.lay = lay;
+  hipError_t er = hipMalloc((void**)&J.slab, lay.bytes);
+  if (er != hipSuccess) {
+    J.slab = nullptr;
+    FAIL(e, er == hipErrorOutOfMemory ? SWM_E_NOMEM : SWM_E_HIP, "hipMalloc(%zu) failed: %s",
+         lay.bytes, hipGetErrorString(er));
+  }
+  size_t off = 0;
+  auto take = [&](size_t cnt) {
+    double* p = reinterpret_cast<double*>(J.slab + off);
+    off += align_up(cnt * sizeof(double));
+    return p;
+  };
+  JobDev& D = J.dev;
+  D.nx = nx;
+  D.ny = ny;
+  D.n = (int)lay.n;
+  D.wb = wb;
+  D.bathy = 0;
+  D.nblk = lay.nblk;
+  D.dx = dx;
+  D.dy = dy;
+  D.g = g;
+  D.cfl = cfl;
+  for (int i = 0; i < 4; ++i) D.wbnd[i] = wbnd[i];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) D.wgp[i][j] = wbnd[i] * wbnd[j];  // src/mesh.f90:136
+  D.h = take(lay.n);
+  D.hu = take(lay.n);
+  D.hv = take(lay.n);
+  for (int m = 1; m < 4; ++m) {
+    D.hres[m] = take(lay.n);
+    D.hun[m] = take(lay.n);
+    D.hvn[m] = take(lay.n);
+    D.xj[m][0] = take(lay.n);
+    D.xj[m][1] = take(lay.n);
+  }
+  for (int r = 0; r < 4; ++r) {
+    D.FU2[r] = take(lay.n);
+    D.FU3[r] = take(lay.n);
+  }
+  for (int r = 0; r < 4; ++r) {
+    D.qx[r] = take(lay.nfx);
+    D.qy[r] = take(lay.nfy);
+  }
+  for (int sl = 0; sl < kSlots; ++sl)
+    for (int v = 0; v < 3; ++v) {
+      D.FX[sl][v] = take(lay.nfx);
+      D.FY[sl][v] = take(lay.nfy);
+    }
+  for (int q = 0; q < kMaxSys; ++q)
+    for (int k = 0; k < 5; ++k) D.M[q][k] = take(lay.n);
+  if (wb) {
+    for (int v = 0; v < 3; ++v) {
+      D.FXWB[v] = take(lay.nfx);
+      D.FYWB[v] = take(lay.nfy);
+    }
+    D.SWB2 = take(lay.n);
+    D.SWB3 = take(lay.n);
+  }
+  D.part = take((size_t)(kMaxSys + 1) * lay.nblk);
+  D.tspart = take(3 * (size_t)lay.nblk);
+  if (off > lay.bytes) FAIL(e, SWM_E_STATE, "internal: job slab overflow");
+  e->in_use += (long long)lay.bytes;
+  J.state = JOB_RUNNING;
+  J.steps_left = nsteps;
+  J.steps_done = 0;
+  J.t = 0.0;
+  J.jacobi_sweeps = 0;
+  J.nonconv = 0;
+  J.wb_ready = false;
+  for (int i = 0; i < 13; ++i) J.pred[i] = 4;
+  e->dirty = true;
+  add_tiles(J.own, id, D);
+  int rc = upload_tables(e, J.own);
+  if (rc) return rc;
+  HIPCK(e, hipMemsetAsync(e->dstatus.p + id, 0, sizeof(JobStatus), e->stream));
+  HIPCK(e, hipStreamSynchronize(e->stream));
+  *id_out = id;
+  return SWM_OK;
+}
+
+int job_set_bathy(swm_engine* e, int id, const double* bx, const double* by) {
+  Job& J = *e->jobs[id];
+  if (!bx != !by) FAIL(e, SWM_E_ARG, "dbdx and dbdy must both be set or both NULL");
+  J.wb_ready = false;  // the well-balanced reference depends on the source term
+  if (!bx) {
+    J.dev.bathy = 0;
+    e->dirty = true;
+    return SWM_OK;
+  }
+  const size_t n = J.lay.n;
+  if (!J.bathy_buf) {
+    const long long b = (long long)bathy_bytes(n);
+    if (e->budget > 0 && e->in_use + b > e->budget)
+      FAIL(e, SWM_E_BUDGET, "bathymetry tables of %lld bytes do not fit", b);
+    HIPCK(e, hipMalloc((void**)&J.bathy_buf, (size_t)b));
+    e->in_use += b;
+  }
+  J.dev.bx = J.bathy_buf;
+  J.dev.by = J.bathy_buf + 16 * n;
+  J.dev.bathy = 1;
+  HIPCK(e, hipMemcpyAsync(J.dev.bx, bx, 16 * n * sizeof(double), hipMemcpyHostToDevice, e->stream));
+  HIPCK(e, hipMemcpyAsync(J.dev.by, by, 16 * n * sizeof(double), hipMemcpyHostToDevice, e->stream));
+  HIPCK(e, hipStreamSynchronize(e->stream));
+  e->dirty = true;
+  return SWM_OK;
+}
+
+int engine_add(swm_engine* e, const swm_job_desc* d, int* id_out) {
+  *id_out = -1;
+  if (!d) FAIL(e, SWM_E_ARG, "job description is NULL");
+  if (!d->U0) FAIL(e, SWM_E_ARG, "U0 is NULL");
+  if (d->wellbalanced && !d->Uwb) FAIL(e, SWM_E_ARG, "wellbalanced job needs Uwb");
+  if (!d->dbdx != !d->dbdy) FAIL(e, SWM_E_ARG, "dbdx and dbdy must both be set or both NULL");
+  const long long extra = d->dbdx ? (long long)bathy_bytes((size_t)d->nx * d->ny) : 0;
+  int id;
+  int rc = alloc_job(e, d->nx, d->ny, d->dx, d->dy, d->gravity, d->cfl, d->wellbalanced ? 1 : 0,
+                     d->wgp_bnd, d->nsteps, extra, &id);
+  if (rc) return rc;
+  *id_out = id;
+  Job& J = *e->jobs[id];
+  if (d->dbdx) {
+    rc = job_set_bathy(e, id, d->dbdx, d->dbdy);
+    if (rc) return rc;
+  }
+  if (d->wellbalanced) {
+    rc = init_wb(e, id, d->Uwb, 0);
+    if (rc) return rc;
+  }
+  return upload_state(e, J, d->U0, 0, J.dev.h, J.dev.hu, J.dev.hv);
+}
+
+void engine_free_job(swm_engine* e, int id) {
+  Job& J = *e->jobs[id];
+  if (J.state == JOB_FREE) return;
+  (void)hipStreamSynchronize(e->stream);
+  if (J.slab) (void)hipFree(J.slab);
+  J.slab = nullptr;
+  e->in_use -= (long long)J.lay.bytes;
+  if (J.bathy_buf) {
+    (void)hipFree(J.bathy_buf);
+    J.bathy_buf = nullptr;
+    e->in_use -= (long long)bathy_bytes(J.lay.n);
+  }
+  J.state = JOB_FREE;
+  J.own = Tables{};
+  e->dirty = true;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// engine
+// ---------------------------------------------------------------------------
+
+int swm_engine_create(long long budget_bytes, swm_engine_t** out) {
+  if (!out) FAIL((swm_engine*)nullptr, SWM_E_ARG, "out is NULL");
+  *out = nullptr;
+  swm_engine* e = new swm_engine();
+  e->budget = budget_bytes;
+  hipError_t er = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+  if (er != hipSuccess) {
+    g_error = std::string("hipStreamCreate failed: ") + hipGetErrorString(er);
+    delete e;
+    return SWM_E_HIP;
+  }
+  int rc = status_capacity(e, 8);
+  if (rc == SWM_OK && hipHostMalloc((void**)&e->hdt, sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    e->hdt = nullptr;
+    g_error = "hipHostMalloc failed";
+    rc = SWM_E_HIP;
+  }
+  if (rc) {
+    swm_engine_destroy(e);
+    return rc;
+  }
+  *out = e;
+  return SWM_OK;
+}
+
+void swm_engine_destroy(swm_engine_t* e) {
+  if (!e) return;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  for (size_t i = 0; i < e->jobs.size(); ++i) engine_free_job(e, (int)i);
+  e->jobs.clear();
+  if (e->hstatus) (void)hipHostFree(e->hstatus);
+  if (e->hdt) (void)hipHostFree(e->hdt);
+  if (e->dstatus.p) (void)hipFree(e->dstatus.p);
+  e->dstatus.p = nullptr;
+  if (e->djobs.p) (void)hipFree(e->djobs.p);
+  e->djobs.p = nullptr;
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+const char* swm_engine_last_error(const swm_engine_t* e) { return e ? e->err.c_str() : g_error.c_str(); }
+
+long long swm_job_footprint_bytes(int nx, int ny, int wellbalanced, int bathymetry) {
+  if (nx < 3 || ny < 3) return -1;
+  const JobLayout L = job_layout(nx, ny, wellbalanced ? 1 : 0, 0);
+  return (long long)(L.bytes + (bathymetry ? bathy_bytes(L.n) : 0));
+}
+
+long long swm_footprint_bytes(int nx, int ny, int wellbalanced, int bathymetry) {
+  return swm_job_footprint_bytes(nx, ny, wellbalanced, bathymetry);
+}
+
+long long swm_engine_bytes_in_use(const swm_engine_t* e) { return e ? e->in_use : -1; }
+
+int swm_engine_submit(swm_engine_t* e, const swm_job_desc* d, int* job_id) {
+  if (!e || !job_id) return SWM_E_ARG;
+  *job_id = -1;
+  int id = -1;
+  int rc = engine_add(e, d, &id);
+  if (rc) {
+    if (id >= 0) engine_free_job(e, id);  // a partially built job keeps no memory
+    return rc;
+  }
+  *job_id = id;
+  return SWM_OK;
+}
+
+int swm_engine_step(swm_engine_t* e, int* n_running) {
+  if (!e) return SWM_E_ARG;
+  int rc = refresh(e);
+  if (rc) return rc;
+  rc = engine_step(e, e->all, nullptr, nullptr);
+  if (rc) return rc;
+  HIPCK(e, hipStreamSynchronize(e->stream));
+  rc = read_status(e);
+  if (rc) return rc;
+  int running = 0;
+  for (int q = 0; q < e->all.njobs; ++q) {
+    const int id = e->all.jl[q];
+    Job& J = *e->jobs[id];
+    const JobStatus& js = e->hstatus[id];
+    J.steps_done += 1;
+    J.t += js.dt;
+    if (js.dt_bad) {
+      J.state = JOB_FAILED;
+      e->dirty = true;
+    } else if (J.steps_left > 0 && --J.steps_left == 0) {
+      J.state = JOB_DONE;
+      e->dirty = true;
+    } else {
+      ++running;
+    }
+  }
+  if (n_running) *n_running = running;
+  return SWM_OK;
+}
+
+int swm_engine_job_info(swm_engine_t* e, int job_id, swm_job_info* info) {
+  if (!e || !info) return SWM_E_ARG;
+  Job* J;
+  int rc = job_from_id(e, job_id, &J);
+  if (rc) return rc;
+  info->state = J->state;
+  info->steps_done = J->steps_done;
+  info->steps_left = J->steps_left;
+  info->t = J->t;
+  info->jacobi_sweeps = J->jacobi_sweeps;
+  info->nonconverged = J->nonconv;
+  info->bytes = (long long)J->lay.bytes;
+  return SWM_OK;
+}
+
+int swm_engine_fetch(swm_engine_t* e, int job_id, double* U) {
+  if (!e || !U) return SWM_E_ARG;
+  Job* J;
+  int rc = job_from_id(e, job_id, &J);
+  if (rc) return rc;
+  const size_t n = J->lay.n;
+  double* tmp = nullptr;
+  HIPCK(e, hipMalloc((void**)&tmp, 3 * n * sizeof(double)));
+  launch_soa_to_aos(J->dev.nx, J->dev.ny, J->dev.h, J->dev.hu, J->dev.hv, tmp, e->stream);
+  hipError_t er = hipGetLastError();
+  if (er == hipSuccess)
+    er = hipMemcpyAsync(U, tmp, 3 * n * sizeof(double), hipMemcpyDeviceToHost, e->stream);
+  if (er == hipSuccess) er = hipStreamSynchronize(e->stream);
+  (void)hipFree(tmp);
+  if (er != hipSuccess) FAIL(e, SWM_E_HIP, "fetch failed: %s", hipGetErrorString(er));
+  return SWM_OK;
+}
+
+int swm_engine_release(swm_engine_t* e, int job_id) {
+  if (!e) return SWM_E_ARG;
+  Job* J;
+  int rc = job_from_id(e, job_id, &J);
+  if (rc) return rc;
+  engine_free_job(e, job_id);
+  return SWM_OK;
+}
+
+int swm_engine_set_jacobi_limits(swm_engine_t* e, int max_iter, double tol) {
+  if (!e) return SWM_E_ARG;
+  if (max_iter < 0) FAIL(e, SWM_E_ARG, "max_iter < 0");
+  e->max_iter = max_iter;
+  e->tol = tol;
+  return SWM_OK;
+}
+
+int swm_engine_profile_enable(swm_engine_t* e, int on) {
+  if (!e) return SWM_E_ARG;
+  e->prof.on = on != 0;
+  return SWM_OK;
+}
+
+int swm_engine_profile_read(swm_engine_t* e, double ms[SWM_PROF_NCAT],
+                            long long launches[SWM_PROF_NCAT]) {
+  if (!e || !ms || !launches) return SWM_E_ARG;
+  HIPCK(e, hipStreamSynchronize(e->stream));
+  for (int c = 0; c < SWM_PROF_NCAT; ++c) {
+    ms[c] = 0.0;
+    launches[c] = 0;
+  }
+  for (size_t k = 0; k < e->prof.cats.size(); ++k) {
+    float t = 0.f;
+    HIPCK(e, hipEventElapsedTime(&t, e->prof.pool[2 * k], e->prof.pool[2 * k + 1]));
+    ms[e->prof.cats[k]] += t;
+    launches[e->prof.cats[k]] += 1;
+  }
+  e->prof.cats.clear();
+  e->prof.used = 0;
+  return SWM_OK;
+}
+
+const char* swm_profile_name(int cat) {
+  return (cat >= 0 && cat < SWM_PROF_NCAT) ? PROF_NAMES[cat] : "";
+}
+
+// ---------------------------------------------------------------------------
+// plain solver: a one-job engine with the reference's call structure
+// ---------------------------------------------------------------------------
+
+static Job* plain_job(swm_solver_t* s) { return s->e->jobs[s->job].get(); }
+
+int swm_create(int nx, int ny, double dx, double dy, double gravity, double cfl, int wellbalanced,
+               const double wgp_bnd[4], swm_solver_t** out) {
+  if (!out) FAIL((swm_engine*)nullptr, SWM_E_ARG, "out is NULL");
+  *out = nullptr;
+  if (nx < 3 || ny < 3)
+    FAIL((swm_engine*)nullptr, SWM_E_ARG, "nx, ny must be >= 3 (got %d, %d)", nx, ny);
+  if (!wgp_bnd) FAIL((swm_engine*)nullptr, SWM_E_ARG, "wgp_bnd is NULL");
+  if (!(dx > 0) || !(dy > 0)) FAIL((swm_engine*)nullptr, SWM_E_ARG, "dx, dy must be > 0");
+  swm_solver* s = new swm_solver();
+  int rc = swm_engine_create(0, &s->e);
+  if (rc) {
+    delete s;
+    return rc;
+  }
+  rc = alloc_job(s->e, nx, ny, dx, dy, gravity, cfl, wellbalanced ? 1 : 0, wgp_bnd, -1, 0, &s->job);
+  if (rc) {
+    swm_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return SWM_OK;
+}
+
+void swm_destroy(swm_solver_t* s) {
+  if (!s) return;
+  swm_engine_destroy(s->e);
+  delete s;
+}
+
+const char* swm_last_error(const swm_solver_t* s) {
+  return s ? s->e->err.c_str() : g_error.c_str();
+}
+
+int swm_set_bathy_grad(swm_solver_t* s, const double* dbdx, const double* dbdy) {
+  if (!s) return SWM_E_ARG;
+  return job_set_bathy(s->e, s->job, dbdx, dbdy);
+}
+
+int swm_init_wb(swm_solver_t* s, const double* Uwb, int ghosts) {
+  if (!s) return SWM_E_ARG;
+  swm_engine* e = s->e;
+  if (!Uwb) FAIL(e, SWM_E_ARG, "Uwb is NULL");
+  if (ghosts != 0 && ghosts != 3) FAIL(e, SWM_E_ARG, "ghosts must be 0 or 3 (got %d)", ghosts);
+  if (!plain_job(s)->dev.wb) FAIL(e, SWM_E_STATE, "handle was created without wellbalanced");
+  return init_wb(e, s->job, Uwb, ghosts);
+}
+
+int swm_upload(swm_solver_t* s, const double* U, int ghosts) {
+  if (!s) return SWM_E_ARG;
+  swm_engine* e = s->e;
+  if (!U) FAIL(e, SWM_E_ARG, "U is NULL");
+  if (ghosts != 0 && ghosts != 3) FAIL(e, SWM_E_ARG, "ghosts must be 0 or 3 (got %d)", ghosts);
+  Job& J = *plain_job(s);
+  return upload_state(e, J, U, ghosts, J.dev.h, J.dev.hu, J.dev.hv);
+}
+
+int swm_download(swm_solver_t* s, double* U) {
+  if (!s) return SWM_E_ARG;
+  return swm_engine_fetch(s->e, s->job, U);
+}
+
+int swm_timestep(swm_solver_t* s, double* dt_min, double* lmx, double* lmy) {
+  if (!s) return SWM_E_ARG;
+  swm_engine* e = s->e;
+  int rc = refresh(e);
+  if (rc) return rc;
+  Tables& T = plain_job(s)->own;
+  timed(e, 6, [&] { launch_timestep(eargs(e), T.dcb.p, T.ncb, T.djl.p, T.njobs, e->stream); });
+  LAUNCHCK(e);
+  rc = read_status(e);
+  if (rc) return rc;
+  const JobStatus& js = e->hstatus[s->job];
+  if (dt_min) *dt_min = js.dt;
+  if (lmx) *lmx = js.lmx;
+  if (lmy) *lmy = js.lmy;
+  return SWM_OK;
+}
+
+int swm_step_mpdec5(swm_solver_t* s, double t, double dt, int jacobi_iters[13]) {
+  (void)t;
+  if (!s) return SWM_E_ARG;
+  swm_engine* e = s->e;
+  Job& J = *plain_job(s);
+  if (J.dev.wb && !J.wb_ready)
+    FAIL(e, SWM_E_STATE, "well-balanced state not initialised (swm_init_wb)");
+  int rc = refresh(e);
+  if (rc) return rc;
+  const int before = J.nonconv;
+  int iters[13] = {0};
+  rc = engine_step(e, J.own, &dt, iters);
+  if (rc) return rc;
+  if (jacobi_iters)
+    for (int i = 0; i < 13; ++i) jacobi_iters[i] = iters[i];
+  J.steps_done += 1;
+  return J.nonconv - before;
+}
+
+int swm_run_steps(swm_solver_t* s, int nsteps, double* dts, int* jacobi_iters) {
+  if (!s) return SWM_E_ARG;
+  swm_engine* e = s->e;
+  if (nsteps < 0) FAIL(e, SWM_E_ARG, "nsteps < 0");
+  Job& J = *plain_job(s);
+  if (J.dev.wb && !J.wb_ready)
+    FAIL(e, SWM_E_STATE, "well-balanced state not initialised (swm_init_wb)");
+  int rc = refresh(e);
+  if (rc) return rc;
+  const int before = J.nonconv;
+  for (int k = 0; k < nsteps; ++k) {
+    // dt from TimeStep on the device; the first Jacobi status read of the step
+    // returns it to the host, so a non-finite dt stops the loop one step late
+    // at most -- check it before the step instead
+    double dt;
+    rc = swm_timestep(s, &dt, nullptr, nullptr);
+    if (rc) return rc;
+    if (dts) dts[k] = dt;
+    if (!std::isfinite(dt)) FAIL(e, SWM_E_STATE, "time step %d: dt is not finite (%g)", k, dt);
+    int iters[13] = {0};
+    rc = engine_step(e, J.own, &dt, iters);
+    if (rc) return rc;
+    if (jacobi_iters)
+      for (int i = 0; i < 13; ++i) jacobi_iters[13 * k + i] = iters[i];
+    J.steps_done += 1;
+  }
+  HIPCK(e, hipStreamSynchronize(e->stream));
+  return J.nonconv - before;
+}
+
+int swm_fvtd(swm_solver_t* s, double* Ut, double* FX, double* FY, double* S, double* Prod,
+             double* Dest) {
+  if (!s) return SWM_E_ARG;
+  swm_engine* e = s->e;
+  Job& J = *plain_job(s);
+  if (J.dev.wb && !J.wb_ready) FAIL(e, SWM_E_STATE, "well-balanced state not initialised");
+  int rc = refresh(e);
+  if (rc) return rc;
+  NodeList L{};
+  L.count = 1;
+  L.m[0] = 0;
+  L.slot[0] = 0;
+  L.debug_S = 1;
+  rc = fvtd_launch(e, J.own, L);
+  if (rc) return rc;
+  const int nx = J.dev.nx, ny = J.dev.ny;
+  const size_t n = J.lay.n, nfx = J.lay.nfx, nfy = J.lay.nfy;
+  const JobDev& D = J.dev;
+  std::vector<double> ut2(n), ut3(n), s2(n), s3(n), qx(nfx), qy(nfy);
+  std::vector<double> fx[3], fy[3], fxwb[3], fywb[3];
+  auto get = [&](std::vector<double>& v, const double* p, size_t cnt) {
+    v.resize(cnt);
+    return hipMemcpyAsync(v.data(), p, cnt * sizeof(double), hipMemcpyDeviceToHost, e->stream);
+  };
+  for (int v = 0; v < 3; ++v) {
+    HIPCK(e, get(fx[v], D.FX[0][v], nfx));
+    HIPCK(e, get(fy[v], D.FY[0][v], nfy));
+    if (D.wb) {
+      HIPCK(e, get(fxwb[v], D.FXWB[v], nfx));
+      HIPCK(e, get(fywb[v], D.FYWB[v], nfy));
+    }
+  }
+  HIPCK(e, get(ut2, D.FU2[0], n));
+  HIPCK(e, get(ut3, D.FU3[0], n));
+  HIPCK(e, get(s2, D.M[0][0], n));
+  HIPCK(e, get(s3, D.M[0][1], n));
+  HIPCK(e, get(qx, D.qx[0], nfx));
+  HIPCK(e, get(qy, D.qy[0], nfy));
+  HIPCK(e, hipStreamSynchronize(e->stream));
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      size_t k = (size_t)j * nx + i;
+      if (Ut) {
+        Ut[3 * k] = 0.0;
+        Ut[3 * k + 1] = ut2[k];
+        Ut[3 * k + 2] = ut3[k];
+      }
+      if (S) {
+        S[3 * k] = 0.0;  // SourceFunc's S(1) is identically 0
+        S[3 * k + 1] = s2[k];
+        S[3 * k + 2] = s3[k];
+      }
+    }
+  // the reference's module FX/FY after UpdateTimeDerivative (WB subtracted)
+  if (FX)
+    for (size_t k = 0; k < nfx; ++k)
+      for (int v = 0; v < 3; ++v) FX[3 * k + v] = D.wb ? fx[v][k] - fxwb[v][k] : fx[v][k];
+  if (FY)
+    for (size_t k = 0; k < nfy; ++k)
+      for (int v = 0; v < 3; ++v) FY[3 * k + v] = D.wb ? fy[v][k] - fywb[v][k] : fy[v][k];
+  if (Prod || Dest)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        size_t k = (size_t)j * nx + i;
+        double q[5] = {qy[(size_t)j * nx + i], qx[(size_t)j * (nx + 1) + i], 0.0,
+                       qx[(size_t)j * (nx + 1) + i + 1], qy[(size_t)(j + 1) * nx + i]};
+        for (int sl = 0; sl < 5; ++sl) {
+          double P, Dd;
+          if (sl == 2) {
+            P = 0.;
+            Dd = 0.;
+          } else if (sl < 2) {
+            P = (q[sl] > 0.) ? q[sl] : 0.;
+            Dd = (q[sl] < 0.) ? -q[sl] : -0.;
+          } else {
+            P = (q[sl] < 0.) ? -q[sl] : 0.;
+            Dd = (q[sl] > 0.) ? q[sl] : -0.;
+          }
+          if (Prod) Prod[5 * k + sl] = P;
+          if (Dest) Dest[5 * k + sl] = Dd;
+        }
+      }
+  return SWM_OK;
+}
+
+int swm_set_jacobi_limits(swm_solver_t* s, int max_iter, double tol) {
+  if (!s) return SWM_E_ARG;
+  return swm_engine_set_jacobi_limits(s->e, max_iter, tol);
+}
+
+long long swm_ambiguous_decisions(const swm_solver_t* s) { return s ? s->e->ambiguous : -1; }
+
+int swm_profile_enable(swm_solver_t* s, int on) {
+  if (!s) return SWM_E_ARG;
+  return swm_engine_profile_enable(s->e, on);
+}
+
+int swm_profile_read(swm_solver_t* s, double ms[SWM_PROF_NCAT], long long launches[SWM_PROF_NCAT]) {
+  if (!s) return SWM_E_ARG;
+  return swm_engine_profile_read(s->e, ms, launches);
+}
+
+int swm_sync(swm_solver_t* s) {
+  if (!s) return SWM_E_ARG;
+  HIPCK(s->e, hipStreamSynchronize(s->e->stream));
+  return SWM_OK;
+}
+
+int swm_debug_divsqrt(int n, const double* a, const double* b, double* q, double* r) {
+  if (n <= 0 || !a || !b || !q || !r) return SWM_E_ARG;
+  double* da = nullptr;
+  size_t bytes = (size_t)n * sizeof(double);
+  if (hipMalloc((void**)&da, 4 * bytes) != hipSuccess) return SWM_E_NOMEM;
+  double *db = da + n, *dq = db + n, *dr = dq + n;
+  int rc = SWM_OK;
+  if (hipMemcpy(da, a, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(db, b, bytes, hipMemcpyHostToDevice) != hipSuccess)
+    rc = SWM_E_HIP;
+  if (rc == SWM_OK) {
+    launch_divsqrt(n, da, db, dq, dr, nullptr);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(q, dq, bytes, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(r, dr, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = SWM_E_HIP;
+  }
+  (void)hipFree(da);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,131 @@
+// swm_engine.hpp -- device data structures of the solve engine and the kernel
+// launch interface shared by swm_kernels.hip and swm_engine.cpp.
+//
+// The engine advances any number of independent periodic grids ("jobs") in
+// lockstep, one mPDeC5 step per engine step.  Every kernel launch covers all
+// active jobs: a block looks up (job, tile) in a tile table built on the host
+// whenever the job set changes.  A single plain solve is a one-job engine.
+//
+// Device layout of one job (FP64, structure of arrays, x fastest):
+//   cell field    [j][i]         j = 0..ny-1, i = 0..nx-1   (n = nx*ny)
+//   x-face field  [j][f]         f = 0..nx: face between Fortran cells f and
+//                                f+1; f = 0 and f = nx are the two values of the
+//                                periodic seam (src/finitevolume.f90:670)
+//   y-face field  [g][i]         g = 0..ny
+//   bathymetry    [yg*4+xg][j][i]
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace swm {
+
+constexpr int kTile = 16;             // FVTD tiles: 16x16 cells, 256 threads
+constexpr int kFaceTile = kTile - 1;  // faces a tile produces along its sweep
+constexpr int kCellBlock = 256;       // cells per block of the pointwise kernels
+constexpr int kMaxSys = 3;            // Patankar systems solved together (nodes 2..4)
+constexpr int kSlots = 3;             // concurrent FVTD evaluations per job
+
+// Jacobi state of one system, written by the decide kernel.
+struct JacState {
+  int k;          // sweeps done
+  int active;     // 1 while the reference loop would continue
+  int cur;        // 0/1: which ping-pong buffer holds the latest iterate
+  int zero;       // 1 when no sweep ran (the reference returns sol = 0)
+  int ambiguous;  // stop decisions settled by the exact sequential sum
+  int nonconv;    // stopped at max_iter with the residual above tolerance
+  int pad[2];
+  double residual;
+};
+
+// Per-job status the host reads back after every Jacobi batch.
+struct JobStatus {
+  JacState st[kMaxSys];
+  double dt, lmx, lmy;
+  int dt_bad;     // dt not finite
+  int pad;
+};
+
+struct JobDev {
+  // geometry and physics
+  int nx, ny, n;
+  int wb, bathy;
+  int nblk;       // cell blocks (kCellBlock cells) of this job
+  double dx, dy, g, cfl;
+  double wbnd[4];
+  double wgp[4][4];
+  // state U^n
+  double *h, *hu, *hv;
+  // DeC nodes m = 1..3 (index 0 unused): current depth, momenta, Jacobi ping-pong
+  double* hres[4];
+  double *hun[4], *hvn[4];
+  double* xj[4][2];
+  // per DeC node r = 0..3: FVTD outputs
+  double *FU2[4], *FU3[4];
+  double *qx[4], *qy[4];
+  // FVTD face-flux scratch per concurrent slot
+  double* FX[kSlots][3];
+  double* FY[kSlots][3];
+  // Patankar matrices per system: S, W, E, N off-diagonals and diagonal
+  double* M[kMaxSys][5];
+  // well-balanced reference (FXWB, FYWB, SWB) and bathymetry tables
+  double *FXWB[3], *FYWB[3];
+  double *SWB2, *SWB3;
+  double *bx, *by;
+  // reductions
+  double* part;     // [kMaxSys + 1][nblk] Jacobi partial sums
+  double* tspart;   // [nblk][3] TimeStep partial maxima
+  JobStatus* status;  // this job's status record
+};
+
+struct TileRef {
+  int job;
+  int tile;
+};
+
+// Arguments shared by every launch of one engine step.
+struct EngineArgs {
+  const JobDev* jobs;
+  int max_iter;
+  double tol;
+};
+
+// Which DeC nodes an FVTD launch evaluates (blockIdx.y picks one).
+struct NodeList {
+  int count;
+  int m[kSlots];      // node evaluated (0 = U^n)
+  int slot[kSlots];   // scratch slot used
+  int wb_init;        // 1: building the well-balanced state (outputs to FXWB...)
+  int debug_S;        // 1: also store S(2:3) to the slot's FX[.][0..1] scratch tail
+};
+
+// Patankar/Jacobi launch parameters (the same for every job of a step).
+struct SysList {
+  int count;
+  int m[kMaxSys];       // DeC node of each system
+  int src[4];           // buffer index r' holding node r's FU/q (0 at kk = 1, else r)
+  int hp_is_un;         // 1: the denominators use h^n (first correction)
+};
+
+// ---- launches (swm_kernels.hip) -------------------------------------------
+void launch_fvtd(const EngineArgs& E, const TileRef* xt, int nxt, const TileRef* yt, int nyt,
+                 const TileRef* ct, int nct, const NodeList& L, hipStream_t s);
+void launch_momentum(const EngineArgs& E, const TileRef* cb, int ncb, const SysList& S,
+                     hipStream_t s);
+void launch_assemble(const EngineArgs& E, const TileRef* cb, int ncb, const int* jl, int njobs,
+                     const SysList& S, hipStream_t s);
+void launch_sweep(const EngineArgs& E, const TileRef* cb, int ncb, const int* jl, int njobs,
+                  const SysList& S, hipStream_t s);
+void launch_finalize(const EngineArgs& E, const TileRef* cb, int ncb, const SysList& S,
+                     hipStream_t s);
+void launch_timestep(const EngineArgs& E, const TileRef* cb, int ncb, const int* jl, int njobs,
+                     hipStream_t s);
+void launch_commit(const EngineArgs& E, const TileRef* cb, int ncb, hipStream_t s);
+void launch_copy_wb(const EngineArgs& E, const TileRef* cb, int ncb, int slot, hipStream_t s);
+void launch_aos_to_soa(int nx, int ny, int ghost, const double* aos, double* h, double* hu,
+                       double* hv, hipStream_t s);
+void launch_soa_to_aos(int nx, int ny, const double* h, const double* hu, const double* hv,
+                       double* aos, hipStream_t s);
+void launch_divsqrt(int n, const double* a, const double* b, double* q, double* r,
+                    hipStream_t s);
+
+}  // namespace swm

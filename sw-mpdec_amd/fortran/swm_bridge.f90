@@ -1,0 +1,119 @@
+!===============================================================================!
+! swm_bridge: ISO_C_BINDING interfaces to libswm (include/swm.h), the MI355X
+! implementation of the PATANKAR-build mPDeC5 / WENO5 time step of sw-mpdec.
+!
+! Built with the reference's -fdefault-real-8, so REAL is C double; default
+! INTEGER is C int.  Array arguments are passed by reference in the
+! reference's own layouts (U(1:3,-2:nx+3,-2:ny+3) with ghosts = 3, or
+! U(1:3,1:nx,1:ny) with ghosts = 0).
+!===============================================================================!
+MODULE swm_bridge
+USE ISO_C_BINDING
+IMPLICIT NONE
+PRIVATE
+
+INTEGER(C_INT), PARAMETER, PUBLIC :: SWM_OK = 0
+
+PUBLIC :: swm_create, swm_destroy, swm_set_bathy_grad, swm_init_wb
+PUBLIC :: swm_upload, swm_download, swm_timestep, swm_step_mpdec5
+PUBLIC :: swm_error_message, swm_check
+
+INTERFACE
+  ! replaces the allocations of InitializeFiniteVolume (src/finitevolume.f90:44)
+  INTEGER(C_INT) FUNCTION swm_create(nx, ny, dx, dy, g, cfl, wb, wgp, h) &
+      BIND(C, NAME='swm_create')
+    IMPORT :: C_INT, C_DOUBLE, C_PTR
+    INTEGER(C_INT), VALUE :: nx, ny, wb
+    REAL(C_DOUBLE), VALUE :: dx, dy, g, cfl
+    REAL(C_DOUBLE), INTENT(IN) :: wgp(4)
+    TYPE(C_PTR), INTENT(OUT) :: h
+  END FUNCTION
+
+  SUBROUTINE swm_destroy(h) BIND(C, NAME='swm_destroy')
+    IMPORT :: C_PTR
+    TYPE(C_PTR), VALUE :: h
+  END SUBROUTINE
+
+  ! Bathymetry_X/_Y at MeshGP, arrays (1:nx,1:ny,1:4,1:4) (src/equation.f90:451-509)
+  INTEGER(C_INT) FUNCTION swm_set_bathy_grad(h, bx, by) BIND(C, NAME='swm_set_bathy_grad')
+    IMPORT :: C_INT, C_DOUBLE, C_PTR
+    TYPE(C_PTR), VALUE :: h
+    REAL(C_DOUBLE), INTENT(IN) :: bx(*), by(*)
+  END FUNCTION
+
+  ! InitializeWBVariables (src/finitevolume.f90:339-396)
+  INTEGER(C_INT) FUNCTION swm_init_wb(h, Uwb, ghosts) BIND(C, NAME='swm_init_wb')
+    IMPORT :: C_INT, C_DOUBLE, C_PTR
+    TYPE(C_PTR), VALUE :: h
+    REAL(C_DOUBLE), INTENT(IN) :: Uwb(*)
+    INTEGER(C_INT), VALUE :: ghosts
+  END FUNCTION
+
+  INTEGER(C_INT) FUNCTION swm_upload(h, U, ghosts) BIND(C, NAME='swm_upload')
+    IMPORT :: C_INT, C_DOUBLE, C_PTR
+    TYPE(C_PTR), VALUE :: h
+    REAL(C_DOUBLE), INTENT(IN) :: U(*)
+    INTEGER(C_INT), VALUE :: ghosts
+  END FUNCTION
+
+  INTEGER(C_INT) FUNCTION swm_download(h, U) BIND(C, NAME='swm_download')
+    IMPORT :: C_INT, C_DOUBLE, C_PTR
+    TYPE(C_PTR), VALUE :: h
+    REAL(C_DOUBLE), INTENT(OUT) :: U(*)
+  END FUNCTION
+
+  ! TimeStep (src/equation.f90:791-837)
+  INTEGER(C_INT) FUNCTION swm_timestep(h, dt_min, lmx, lmy) BIND(C, NAME='swm_timestep')
+    IMPORT :: C_INT, C_DOUBLE, C_PTR
+    TYPE(C_PTR), VALUE :: h
+    REAL(C_DOUBLE), INTENT(OUT) :: dt_min, lmx, lmy
+  END FUNCTION
+
+  ! TimeDiscretizationByMPDeC5 (src/timediscretization.f90:458-566)
+  INTEGER(C_INT) FUNCTION swm_step_mpdec5(h, t, dt, iters) BIND(C, NAME='swm_step_mpdec5')
+    IMPORT :: C_INT, C_DOUBLE, C_PTR
+    TYPE(C_PTR), VALUE :: h
+    REAL(C_DOUBLE), VALUE :: t, dt
+    INTEGER(C_INT), INTENT(OUT) :: iters(13)
+  END FUNCTION
+
+  FUNCTION swm_last_error_c(h) BIND(C, NAME='swm_last_error')
+    IMPORT :: C_PTR
+    TYPE(C_PTR), VALUE :: h
+    TYPE(C_PTR) :: swm_last_error_c
+  END FUNCTION
+END INTERFACE
+
+CONTAINS
+
+! The handle's last error message as a Fortran string.
+FUNCTION swm_error_message(h) RESULT(msg)
+  TYPE(C_PTR), INTENT(IN) :: h
+  CHARACTER(LEN=512) :: msg
+  CHARACTER(KIND=C_CHAR), POINTER :: c(:)
+  TYPE(C_PTR) :: p
+  INTEGER :: k
+  msg = ''
+  p = swm_last_error_c(h)
+  IF (.NOT. C_ASSOCIATED(p)) RETURN
+  CALL C_F_POINTER(p, c, (/512/))
+  DO k = 1, 512
+    IF (c(k) == C_NULL_CHAR) EXIT
+    msg(k:k) = c(k)
+  END DO
+END FUNCTION
+
+! The reference's error convention for hard errors: print and STOP
+! (src/timediscretization.f90:78-80).
+SUBROUTINE swm_check(h, rc, what)
+  TYPE(C_PTR), INTENT(IN) :: h
+  INTEGER(C_INT), INTENT(IN) :: rc
+  CHARACTER(LEN=*), INTENT(IN) :: what
+  IF (rc < 0) THEN
+    WRITE(*,*) "swm: ", TRIM(what), " failed: ", TRIM(swm_error_message(h))
+    STOP 1
+  END IF
+END SUBROUTINE
+
+END MODULE swm_bridge
+!===============================================================================!

@@ -1,0 +1,83 @@
+!===============================================================================!
+! swm_fdriver: a Fortran caller of libswm through swm_bridge, shaped like the
+! reference's TimeDiscretization loop (src/timediscretization.f90:57-95) with
+! timescheme = 5: dt = TimeStep(), one TimeDiscretizationByMPDeC5 step, repeat.
+!
+! Reads a raw input file
+!   int32  nx, ny, nsteps, wellbalanced, has_bathy, reserved
+!   real64 dx, dy, gravity, cfl, wgp_bnd(4)
+!   real64 U(3,nx,ny) [, Uwb(3,nx,ny) if wellbalanced] [, dbdx, dbdy(nx,ny,4,4)]
+! and writes the output file
+!   int32  nsteps, nonconverged_total
+!   real64 dt(nsteps), U(3,nx,ny)
+! Usage: swm_fdriver <input> <output>
+!===============================================================================!
+PROGRAM swm_fdriver
+USE ISO_C_BINDING
+USE swm_bridge
+IMPLICIT NONE
+INTEGER(C_INT) :: hdr(6), nx, ny, nsteps, wb, hasb, rc, k, nonconv
+INTEGER(C_INT) :: iters(13)
+REAL(C_DOUBLE) :: par(8), dx, dy, g, cfl, wgp(4), t, dt, lmx, lmy
+REAL(C_DOUBLE), ALLOCATABLE :: U(:,:,:), Uwb(:,:,:), bx(:,:,:,:), by(:,:,:,:), dts(:)
+TYPE(C_PTR) :: h
+CHARACTER(LEN=1024) :: fin, fout
+INTEGER :: uin, uout
+
+CALL GET_COMMAND_ARGUMENT(1, fin)
+CALL GET_COMMAND_ARGUMENT(2, fout)
+OPEN(NEWUNIT=uin, FILE=TRIM(fin), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
+READ(uin) hdr
+nx = hdr(1); ny = hdr(2); nsteps = hdr(3); wb = hdr(4); hasb = hdr(5)
+READ(uin) par
+dx = par(1); dy = par(2); g = par(3); cfl = par(4); wgp = par(5:8)
+ALLOCATE(U(3,nx,ny), dts(MAX(nsteps,1)))
+READ(uin) U
+IF (wb .NE. 0) THEN
+  ALLOCATE(Uwb(3,nx,ny))
+  READ(uin) Uwb
+END IF
+IF (hasb .NE. 0) THEN
+  ALLOCATE(bx(nx,ny,4,4), by(nx,ny,4,4))
+  READ(uin) bx
+  READ(uin) by
+END IF
+CLOSE(uin)
+
+rc = swm_create(nx, ny, dx, dy, g, cfl, wb, wgp, h)
+IF (rc .NE. SWM_OK) THEN
+  WRITE(*,*) "swm_create failed: ", rc
+  STOP 1
+END IF
+IF (hasb .NE. 0) THEN
+  CALL swm_check(h, swm_set_bathy_grad(h, bx, by), "swm_set_bathy_grad")
+END IF
+IF (wb .NE. 0) THEN
+  CALL swm_check(h, swm_init_wb(h, Uwb, 0_C_INT), "swm_init_wb")
+END IF
+CALL swm_check(h, swm_upload(h, U, 0_C_INT), "swm_upload")
+
+t = 0.0
+nonconv = 0
+DO k = 1, nsteps
+  CALL swm_check(h, swm_timestep(h, dt, lmx, lmy), "swm_timestep")
+  IF (.NOT. (dt .LT. HUGE(dt))) THEN
+    WRITE(*,*) "swm_fdriver: non-finite time step at step ", k
+    STOP 2
+  END IF
+  rc = swm_step_mpdec5(h, t, dt, iters)
+  CALL swm_check(h, rc, "swm_step_mpdec5")
+  nonconv = nonconv + rc
+  dts(k) = dt
+  t = t + dt
+END DO
+CALL swm_check(h, swm_download(h, U), "swm_download")
+CALL swm_destroy(h)
+
+OPEN(NEWUNIT=uout, FILE=TRIM(fout), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+WRITE(uout) nsteps, nonconv
+WRITE(uout) dts(1:nsteps)
+WRITE(uout) U
+CLOSE(uout)
+END PROGRAM swm_fdriver
+!===============================================================================!

@@ -200,3 +200,158 @@ def debug_divsqrt(a, b):
     if rc != SWM_OK:
         raise SwmError(f"swm_debug_divsqrt rc={rc}")
     return q, r
+
+
+# ---------------------------------------------------------------------------
+# batched / served solves
+# ---------------------------------------------------------------------------
+SWM_E_BUDGET = -5
+JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2
+
+
+class JobDesc(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+                ("gravity", ctypes.c_double), ("cfl", ctypes.c_double),
+                ("wellbalanced", ctypes.c_int), ("wgp_bnd", ctypes.c_double * 4),
+                ("U0", _dp), ("Uwb", _dp), ("dbdx", _dp), ("dbdy", _dp),
+                ("nsteps", ctypes.c_int)]
+
+
+class JobInfo(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int), ("steps_done", ctypes.c_int),
+                ("steps_left", ctypes.c_int), ("t", ctypes.c_double),
+                ("jacobi_sweeps", ctypes.c_longlong), ("nonconverged", ctypes.c_int),
+                ("bytes", ctypes.c_longlong)]
+
+
+def _engine_lib():
+    L = lib()
+    if getattr(L, "_engine_bound", False):
+        return L
+    vp = ctypes.c_void_p
+    L.swm_engine_create.argtypes = [ctypes.c_longlong, ctypes.POINTER(vp)]
+    L.swm_engine_destroy.argtypes = [vp]
+    L.swm_engine_destroy.restype = None
+    L.swm_engine_last_error.restype = ctypes.c_char_p
+    L.swm_engine_last_error.argtypes = [vp]
+    L.swm_job_footprint_bytes.restype = ctypes.c_longlong
+    L.swm_job_footprint_bytes.argtypes = [ctypes.c_int] * 4
+    L.swm_engine_bytes_in_use.restype = ctypes.c_longlong
+    L.swm_engine_bytes_in_use.argtypes = [vp]
+    L.swm_engine_submit.argtypes = [vp, ctypes.POINTER(JobDesc), ctypes.POINTER(ctypes.c_int)]
+    L.swm_engine_step.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    L.swm_engine_job_info.argtypes = [vp, ctypes.c_int, ctypes.POINTER(JobInfo)]
+    L.swm_engine_fetch.argtypes = [vp, ctypes.c_int, _dp]
+    L.swm_engine_release.argtypes = [vp, ctypes.c_int]
+    L.swm_engine_set_jacobi_limits.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+    L.swm_engine_profile_enable.argtypes = [vp, ctypes.c_int]
+    L.swm_engine_profile_read.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
+    L._engine_bound = True
+    return L
+
+
+def job_footprint_bytes(nx, ny, wellbalanced, bathymetry) -> int:
+    return int(_engine_lib().swm_job_footprint_bytes(nx, ny, int(bool(wellbalanced)),
+                                                     int(bool(bathymetry))))
+
+
+class Engine:
+    """Many independent grids on one GPU, advanced in lockstep (one mPDeC5
+    step of every running grid per step()); grids can join and leave between
+    steps."""
+
+    def __init__(self, budget_bytes: int = 0):
+        L = _engine_lib()
+        self._e = ctypes.c_void_p()
+        rc = L.swm_engine_create(int(budget_bytes), ctypes.byref(self._e))
+        if rc != SWM_OK:
+            raise SwmError(f"swm_engine_create: {L.swm_engine_last_error(None).decode()}")
+        self._keep = {}
+
+    def _check(self, rc, what):
+        if rc < 0:
+            raise SwmError(f"{what}: {_engine_lib().swm_engine_last_error(self._e).decode()} "
+                           f"(rc={rc})")
+        return rc
+
+    def close(self):
+        if self._e:
+            _engine_lib().swm_engine_destroy(self._e)
+            self._e = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, p, nsteps):
+        """Copies problem p to the device; returns its job id, or None when it
+        does not fit the memory budget right now."""
+        m = p.mesh
+        arrs = [np.ascontiguousarray(p.U0, dtype=np.float64)]
+        d = JobDesc()
+        d.nx = d.ny = p.n
+        d.dx = d.dy = m.dx
+        d.gravity = p.gravity
+        d.cfl = p.cfl
+        d.wellbalanced = int(bool(p.wellbalanced))
+        for i in range(4):
+            d.wgp_bnd[i] = m.wbnd[i]
+        d.U0 = _ptr(arrs[0])
+        if p.wellbalanced:
+            arrs.append(np.ascontiguousarray(p.Uwb, dtype=np.float64))
+            d.Uwb = _ptr(arrs[-1])
+        if p.dbdx is not None:
+            arrs.append(np.ascontiguousarray(p.dbdx, dtype=np.float64))
+            d.dbdx = _ptr(arrs[-1])
+            arrs.append(np.ascontiguousarray(p.dbdy, dtype=np.float64))
+            d.dbdy = _ptr(arrs[-1])
+        d.nsteps = int(nsteps)
+        jid = ctypes.c_int(-1)
+        rc = _engine_lib().swm_engine_submit(self._e, ctypes.byref(d), ctypes.byref(jid))
+        if rc == SWM_E_BUDGET:
+            return None
+        self._check(rc, "swm_engine_submit")
+        self._keep[jid.value] = p.n
+        return jid.value
+
+    def step(self):
+        n = ctypes.c_int(0)
+        self._check(_engine_lib().swm_engine_step(self._e, ctypes.byref(n)), "swm_engine_step")
+        return n.value
+
+    def info(self, jid):
+        i = JobInfo()
+        self._check(_engine_lib().swm_engine_job_info(self._e, jid, ctypes.byref(i)),
+                    "swm_engine_job_info")
+        return {k: getattr(i, k) for k, _ in JobInfo._fields_}
+
+    def fetch(self, jid):
+        n = self._keep[jid]
+        U = np.empty((n, n, 3))
+        self._check(_engine_lib().swm_engine_fetch(self._e, jid, _ptr(U)), "swm_engine_fetch")
+        return U
+
+    def release(self, jid):
+        self._check(_engine_lib().swm_engine_release(self._e, jid), "swm_engine_release")
+        self._keep.pop(jid, None)
+
+    def bytes_in_use(self):
+        return int(_engine_lib().swm_engine_bytes_in_use(self._e))
+
+    def set_jacobi_limits(self, max_iter, tol):
+        self._check(_engine_lib().swm_engine_set_jacobi_limits(self._e, max_iter, tol),
+                    "swm_engine_set_jacobi_limits")
+
+    def profile(self, on=True):
+        self._check(_engine_lib().swm_engine_profile_enable(self._e, int(bool(on))),
+                    "swm_engine_profile_enable")
+
+    def profile_read(self):
+        ms = (ctypes.c_double * PROF_NCAT)()
+        cnt = (ctypes.c_longlong * PROF_NCAT)()
+        self._check(_engine_lib().swm_engine_profile_read(self._e, ms, cnt),
+                    "swm_engine_profile_read")
+        return {lib().swm_profile_name(c).decode(): (ms[c], int(cnt[c])) for c in range(PROF_NCAT)}
