@@ -32,14 +32,32 @@ sys.path.insert(0, str(ROOT / "sw-mpdec_amd"))
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
-# Algorithmic bytes per cell of one launch of each kernel (8-byte words the
-# kernel must move at least once): fvtd_x/fvtd_y read the state (h, hu, hv)
-# and write 3 face fluxes per cell face; see DESIGN.md.
-ALGO_BYTES_PER_CELL = {"fvtd_x": 48.0, "fvtd_y": 48.0}
-# FP64 operations per cell of one fvtd_x / fvtd_y launch, counted from the
+# Algorithmic bytes per unit of work of each kernel (8-byte words the kernel
+# must move at least once), and the units of one step (cells = nx*ny):
+#   fvtd_x / fvtd_y   per face of one node: read h, hu, hv (24), write F1..F3 (24)
+#   fvtd_update       per cell of one node: read one x and one y face flux (48),
+#                     write Ut(2:3) (16) and the east/north mass flux (16)
+#   momentum          per cell-node: read Ua(1,2:3) and FUp(1:4,2:3) (80), write 16
+#   patankar_assemble per row-system: face mass fluxes of 4 nodes (64), h (8),
+#                     write 4 off-diagonals, the diagonal and x1 (48)
+#   jacobi_sweep      per row-sweep: 5 matrix planes, x_old, b (56), write x_new (8)
+#   timestep          per cell: read h, hu, hv (24)
+# The 13 FVTDs / momentum updates / Patankar systems of a step cover 13
+# node-evaluations of every cell; Jacobi covers J_step sweeps of every row.
+ALGO_BYTES = {"fvtd_x": 48.0, "fvtd_y": 48.0, "fvtd_update": 80.0, "momentum": 96.0,
+              "patankar_assemble": 120.0, "jacobi_sweep": 64.0, "timestep": 24.0}
+# FP64 operations per face of one node in fvtd_x / fvtd_y, counted from the
 # source (each +, -, *, /, sqrt = 1): ConsToPrim 4, three first sweeps 348,
 # six second sweeps 1320, limiter 70, Rusanov at 4 Gauss points 264.
-FP64_OPS_PER_CELL = {"fvtd_x": 2006.0, "fvtd_y": 2006.0}
+FP64_OPS_PER_FACE = 2006.0
+
+
+def units_per_step(kernel: str, cells: int, sweeps_per_step: float) -> float:
+    if kernel == "jacobi_sweep":
+        return cells * sweeps_per_step
+    if kernel == "timestep":
+        return float(cells)
+    return 13.0 * cells
 
 
 def log(msg):
@@ -104,6 +122,81 @@ def cpu_baseline(budget_s: float = 20.0):
                        f"{t:.1f} s, time loop only")}
 
 
+def batched_section(count: int, seed: int) -> dict:
+    """`count` uneven solves (N = round(exp(U[ln 32, ln 512])), 20..200 CFL
+    steps, half vortices, half well-balanced lakes over bumps) run together
+    in one engine, against the same solves run one after another with the
+    plain solver.  Both timings exclude host-side setup and upload.  Every
+    job's final state must be bitwise the same both ways."""
+    import problems as P
+    import serve
+    import swm
+
+    t0 = time.time()
+    probs = P.mixed_batch(count, seed)
+    work = sum(p.n * p.n * p.steps for p in probs)
+    log(f"bench: batch of {count} problems built in {time.time() - t0:.1f} s "
+        f"({work / 1e9:.2f} G cell-steps)")
+    res, secs, esteps, _, states = serve.run_batch(probs, [p.steps for p in probs])
+    log(f"bench: batched run {secs:.2f} s, {esteps} engine steps")
+    seq = 0.0
+    same = True
+    for p, r in zip(probs, res):
+        sv = swm.Solver.from_problem(p)
+        sv.sync()
+        ts = time.perf_counter()
+        sv.run_steps(p.steps)
+        sv.sync()
+        seq += time.perf_counter() - ts
+        same = same and bool(np.array_equal(sv.download().view(np.uint64), r.view(np.uint64)))
+        sv.close()
+    log(f"bench: sequential plain solver {seq:.2f} s, identical={same}")
+    return {"workload": f"{count} jobs, N~logU[32,512], steps~U{{20..200}}, 50% vortex "
+                        f"(PATANKAR) / 50% WB lake over bumps, seed {seed}",
+            "jobs": count, "cell_steps": work,
+            "jobs_per_s": count / secs, "cell_steps_per_s": work / secs, "seconds": secs,
+            "sequential_jobs_per_s": count / seq, "sequential_seconds": seq,
+            "speedup_vs_sequential": seq / secs, "engine_steps": esteps,
+            "failed": int(sum(st == swm.JOB_FAILED for st in states)),
+            "bitwise_identical_to_sequential": same}
+
+
+def served_section(count: int, seed: int, rate: float) -> dict:
+    """Continuous batching of `count` requests arriving as a seeded Poisson
+    stream at `rate` per second (N ~ logU[64, 1024], 50..500 CFL steps,
+    vortex / lake over bumps / shallow dam break), against the sequential
+    plain solver on the same stream.  Latency = completion - arrival."""
+    import serve
+
+    t0 = time.time()
+    reqs = serve.request_mix(count, seed, rate, nmin=64, nmax=1024, smin=50, smax=500)
+    work = sum(r.problem.n ** 2 * r.nsteps for r in reqs)
+    log(f"bench: {count} requests built in {time.time() - t0:.1f} s "
+        f"({work / 1e9:.2f} G cell-steps)")
+    sv = serve.serve(reqs, budget_bytes=0, keep_results=True, log=log)
+    served = [r.result for r in reqs]
+    log(f"bench: served makespan {sv['makespan_s']:.2f} s, p50 {sv['p50_latency_s']:.2f} s, "
+        f"p99 {sv['p99_latency_s']:.2f} s")
+    for r in reqs:
+        r.result, r.done, r.admitted = None, float("nan"), float("nan")
+    sq = serve.sequential(reqs, keep_results=True)
+    same = all(np.array_equal(a.view(np.uint64), r.result.view(np.uint64))
+               for a, r in zip(served, reqs))
+    log(f"bench: sequential makespan {sq['makespan_s']:.2f} s, p50 {sq['p50_latency_s']:.2f} s")
+    return {"workload": f"{count} requests, Poisson {rate}/s, N~logU[64,1024], "
+                        f"steps~U{{50..500}}, seed {seed}",
+            "requests": count, "cell_steps": work, "arrival_rate_per_s": rate,
+            "jobs_per_s": sv["jobs_per_s"], "p50_latency_s": sv["p50_latency_s"],
+            "p99_latency_s": sv["p99_latency_s"], "makespan_s": sv["makespan_s"],
+            "peak_running": sv["peak_running"], "peak_bytes": sv["peak_bytes"],
+            "failed": sv["failed"],
+            "sequential_jobs_per_s": sq["jobs_per_s"],
+            "sequential_p50_latency_s": sq["p50_latency_s"],
+            "sequential_p99_latency_s": sq["p99_latency_s"],
+            "speedup_vs_sequential": sv["jobs_per_s"] / sq["jobs_per_s"],
+            "bitwise_identical_to_sequential": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +205,12 @@ def main():
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--batch-jobs", type=int, default=256,
+                    help="jobs of the batched run (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=48,
+                    help="requests of the served run (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=20.0,
+                    help="Poisson arrival rate of the served run, per second")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -153,21 +252,26 @@ def main():
     ms_per_step = 1e3 * elapsed / args.steps
     kern = {k: {"ms_total": v[0], "launches": v[1],
                 "avg_us": (1e3 * v[0] / v[1]) if v[1] else 0.0} for k, v in prof.items() if v[1]}
+    jsum = int(np.sum(iters))
+    cells = n * n
+    for k, v in kern.items():
+        if k in ALGO_BYTES:
+            u = units_per_step(k, cells, jsum / args.steps) * args.steps / v["launches"]
+            v["algo_bytes_per_launch"] = ALGO_BYTES[k] * u
+            v["algo_GBps"] = ALGO_BYTES[k] * u / (v["avg_us"] * 1e-6) / 1e9
     dom = max(kern, key=lambda k: kern[k]["ms_total"])
     avg_s = kern[dom]["avg_us"] * 1e-6
-    cells = n * n
-    algo = ALGO_BYTES_PER_CELL.get(dom, 0.0) * cells
+    algo = kern[dom].get("algo_bytes_per_launch", 0.0)
     achieved = algo / avg_s / 1e9 if avg_s > 0 else 0.0
     roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic_from_profiles(dom, n),
             "algo_bytes_per_launch": algo, "avg_launch_us": kern[dom]["avg_us"]}
-    if dom in FP64_OPS_PER_CELL:
-        tf = FP64_OPS_PER_CELL[dom] * cells / avg_s / 1e12
+    if dom in ("fvtd_x", "fvtd_y"):
+        ops = FP64_OPS_PER_FACE * units_per_step(dom, cells, 0) * args.steps / kern[dom]["launches"]
+        tf = ops / avg_s / 1e12
         roof["fp64_valu"] = {"achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                             "frac": tf / FP64_PEAK_TFLOPS,
-                             "ops_per_launch": FP64_OPS_PER_CELL[dom] * cells}
-    jsum = int(np.sum(iters))
+                             "frac": tf / FP64_PEAK_TFLOPS, "ops_per_launch": ops}
     out = {
         "metric": "mPDeC5 cell-steps/s FP64 at 2048\u00b2 vs CPU; batched jobs/s + p50/p99 latency",
         "value": value,
@@ -188,6 +292,11 @@ def main():
         "kernels": kern,
         "roofline": roof,
     }
+    del s
+    if args.batch_jobs > 0:
+        out["batched"] = batched_section(args.batch_jobs, args.seed)
+    if args.serve_requests > 0:
+        out["served"] = served_section(args.serve_requests, args.seed, args.serve_rate)
     if not args.no_cpu:
         cb = cpu_baseline()
         if cb:

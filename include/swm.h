@@ -176,6 +176,11 @@ int swm_engine_profile_read(swm_engine_t *e, double ms[SWM_PROF_NCAT],
 /* Self-check of the device's FP64 '/' and sqrt: q = a/b, r = sqrt(a). */
 int swm_debug_divsqrt(int n, const double *a, const double *b, double *q, double *r);
 
+/* Self-check of the division the WENO weights use when all operands lie in
+ * [2^-400, 2^400]: q = a/b through one refined reciprocal of b shared by
+ * every quotient with that denominator (no range check here). */
+int swm_debug_shared_div(int n, const double *a, const double *b, double *q);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1090,4 +1090,24 @@ int swm_debug_divsqrt(int n, const double* a, const double* b, double* q, double
   return rc;
 }
 
+int swm_debug_shared_div(int n, const double* a, const double* b, double* q) {
+  if (n <= 0 || !a || !b || !q) return SWM_E_ARG;
+  double* da = nullptr;
+  size_t bytes = (size_t)n * sizeof(double);
+  if (hipMalloc((void**)&da, 3 * bytes) != hipSuccess) return SWM_E_NOMEM;
+  double *db = da + n, *dq = db + n;
+  int rc = SWM_OK;
+  if (hipMemcpy(da, a, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(db, b, bytes, hipMemcpyHostToDevice) != hipSuccess)
+    rc = SWM_E_HIP;
+  if (rc == SWM_OK) {
+    launch_shared_div(n, da, db, dq, nullptr);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(q, dq, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = SWM_E_HIP;
+  }
+  (void)hipFree(da);
+  return rc;
+}
+
 }  // extern "C"

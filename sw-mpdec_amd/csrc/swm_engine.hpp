@@ -127,5 +127,6 @@ void launch_soa_to_aos(int nx, int ny, const double* h, const double* hu, const 
                        double* aos, hipStream_t s);
 void launch_divsqrt(int n, const double* a, const double* b, double* q, double* r,
                     hipStream_t s);
+void launch_shared_div(int n, const double* a, const double* b, double* q, hipStream_t s);
 
 }  // namespace swm

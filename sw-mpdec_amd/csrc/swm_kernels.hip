@@ -405,7 +405,9 @@ __device__ __forceinline__ void patankar_slot(int m, double dt, const double q[4
                                               double hK, double hL, double& M, double& D) {
   // production / destruction of the slot at the 4 DeC nodes, from the face
   // mass flux q = F/dx: west/south slots take max(F,0)/dx, -min(F,0)/dx; the
-  // east/north slots the same of -F (src/finitevolume.f90:496-523)
+  // east/north slots the same of -F (src/finitevolume.f90:496-523).
+  // theta > 0: M = M - c*PD(P, hL), D = D + c*PD(De, hK);
+  // otherwise: D = D - c*PD(P, hK), M = M + c*PD(De, hL)  (:800-884)
   M = 0.;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -419,13 +421,11 @@ __device__ __forceinline__ void patankar_slot(int m, double dt, const double q[4
     }
     const double th = THETA[m][r];
     const double c = dt * th;
-    if (th > 0.) {
-      M = M - c * (patankar_div(P, hL));
-      D = D + c * (patankar_div(De, hK));
-    } else {
-      D = D - c * (patankar_div(P, hK));
-      M = M + c * (patankar_div(De, hL));
-    }
+    const bool pos = th > 0.;
+    const double tL = c * patankar_div(pos ? P : De, hL);
+    const double tK = c * patankar_div(pos ? De : P, hK);
+    M = pos ? (M - tL) : (M + tL);
+    D = pos ? (D + tK) : (D - tK);
   }
 }
 
@@ -436,18 +436,20 @@ __global__ __launch_bounds__(256) void k_assemble(EngineArgs E, const TileRef* b
   const JobDev& J = E.jobs[T.job];
   const int nx = J.nx, ny = J.ny, n = J.n;
   const int k = T.tile * kCellBlock + threadIdx.x;
+  const bool in = k < n;
   const double dt = J.status->dt;
-  double d[kMaxSys] = {0., 0., 0.};
-  double db = 0.;
-  if (k < n) {
+  int kS = 0, kN = 0, kW = 0, kE = 0;
+  double qS[4] = {0., 0., 0., 0.}, qW[4] = {0., 0., 0., 0.};
+  double qE[4] = {0., 0., 0., 0.}, qN[4] = {0., 0., 0., 0.};
+  double b = 0., bS = 0., bW = 0., bE = 0., bN = 0.;
+  if (in) {
     const int i = k % nx, j = k / nx;
-    const int kS = (j == 0 ? ny - 1 : j - 1) * nx + i;
-    const int kN = (j == ny - 1 ? 0 : j + 1) * nx + i;
-    const int kW = j * nx + (i == 0 ? nx - 1 : i - 1);
-    const int kE = j * nx + (i == nx - 1 ? 0 : i + 1);
+    kS = (j == 0 ? ny - 1 : j - 1) * nx + i;
+    kN = (j == ny - 1 ? 0 : j + 1) * nx + i;
+    kW = j * nx + (i == 0 ? nx - 1 : i - 1);
+    kE = j * nx + (i == nx - 1 ? 0 : i + 1);
     const size_t fW = (size_t)j * (nx + 1) + i, fE = fW + 1;
     const size_t gS = (size_t)j * nx + i, gN = gS + nx;
-    double qS[4], qW[4], qE[4], qN[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       qS[r] = J.qy[S.src[r]][gS];
@@ -455,24 +457,31 @@ __global__ __launch_bounds__(256) void k_assemble(EngineArgs E, const TileRef* b
       qE[r] = J.qx[S.src[r]][fE];
       qN[r] = J.qy[S.src[r]][gN];
     }
-    const double b = J.h[k];
-    const double bS = J.h[kS], bW = J.h[kW], bE = J.h[kE], bN = J.h[kN];
-    db = fabs(0.0 - b);
-    for (int s = 0; s < S.count; ++s) {
+    b = J.h[k];
+    bS = J.h[kS];
+    bW = J.h[kW];
+    bE = J.h[kE];
+    bN = J.h[kN];
+  }
+  // one system at a time; its block sum is taken before the next one starts
+#pragma unroll 1
+  for (int s = 0; s < S.count; ++s) {
+    double d = 0.;
+    if (in) {
       const int m = S.m[s];
       const double* hp = S.hp_is_un ? J.h : J.hres[m];
-      const double hK = hp[k];
-      double D = 1.;
-      double mS, mW, mE, mN;
-      patankar_slot(m, dt, qS, false, hK, hp[kS], mS, D);
-      patankar_slot(m, dt, qW, false, hK, hp[kW], mW, D);
-      patankar_slot(m, dt, qE, true, hK, hp[kE], mE, D);
-      patankar_slot(m, dt, qN, true, hK, hp[kN], mN, D);
-      J.M[s][0][k] = mS;
-      J.M[s][1][k] = mW;
-      J.M[s][2][k] = mE;
-      J.M[s][3][k] = mN;
-      J.M[s][4][k] = D;
+      const double hK = hp[k], hS = hp[kS], hW = hp[kW], hE = hp[kE], hN = hp[kN];
+      double D = 1., mS, mW, mE, mN;
+      patankar_slot(m, dt, qS, false, hK, hS, mS, D);
+      patankar_slot(m, dt, qW, false, hK, hW, mW, D);
+      patankar_slot(m, dt, qE, true, hK, hE, mE, D);
+      patankar_slot(m, dt, qN, true, hK, hN, mN, D);
+      double* const* Ms = J.M[s];
+      Ms[0][k] = mS;
+      Ms[1][k] = mW;
+      Ms[2][k] = mE;
+      Ms[3][k] = mN;
+      Ms[4][k] = D;
       // sparse_matmul in CSR order S, W, self (value 0), E, N; then
       // sol = (rhs - prod)/diag
       double p = 0.;
@@ -483,14 +492,12 @@ __global__ __launch_bounds__(256) void k_assemble(EngineArgs E, const TileRef* b
       p = p + mN * bN;
       double x1 = (b - p) / D;
       J.xj[m][1][k] = x1;
-      d[s] = fabs(x1 - b);
+      d = fabs(x1 - b);
     }
-  }
-  for (int s = 0; s < S.count; ++s) {
-    double t = block_sum_all(d[s], red);
+    double t = block_sum_all(d, red);
     if (threadIdx.x == 0) J.part[(size_t)s * J.nblk + T.tile] = t;
   }
-  double t = block_sum_all(db, red);
+  double t = block_sum_all(in ? fabs(0.0 - b) : 0., red);
   if (threadIdx.x == 0) J.part[(size_t)kMaxSys * J.nblk + T.tile] = t;
 }
 
@@ -786,6 +793,14 @@ __global__ void k_divsqrt(int n, const double* a, const double* b, double* q, do
   r[k] = sqrt(a[k]);
 }
 
+// a/b through the shared-reciprocal sequence (recip_refined + div_by),
+// without the range check, for comparison with '/'.
+__global__ void k_shared_div(int n, const double* a, const double* b, double* q) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  q[k] = div_by(a[k], b[k], recip_refined(b[k]));
+}
+
 // ------------------------------------------------------------------------
 // launch wrappers
 // ------------------------------------------------------------------------
@@ -852,6 +867,10 @@ void launch_soa_to_aos(int nx, int ny, const double* h, const double* hu, const 
 void launch_divsqrt(int n, const double* a, const double* b, double* q, double* r,
                     hipStream_t s) {
   hipLaunchKernelGGL(k_divsqrt, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, b, q, r);
+}
+
+void launch_shared_div(int n, const double* a, const double* b, double* q, hipStream_t s) {
+  hipLaunchKernelGGL(k_shared_div, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, b, q);
 }
 
 }  // namespace swm

@@ -23,6 +23,44 @@ constexpr double kHuge = 1.7976931348623157e308;  // HUGE(1.0), real(8)
 __device__ __forceinline__ double max_r(double a, double b) { return (a > b) ? a : b; }
 __device__ __forceinline__ double min_r(double a, double b) { return (a < b) ? a : b; }
 
+// ------------------------------------------------------------------------
+// Division with a reciprocal shared between quotients of one denominator.
+//
+// gfx950 evaluates the IEEE quotient a/b as
+//   sb = div_scale(b), sa = div_scale(a), r = rcp(sb),
+//   r = fma(r, fma(-sb, r, 1), r)  (twice),  q = sa*r,
+//   div_fixup(div_fmas(fma(-sb, q, sa), r, q), b, a).
+// When a and b are normal, a/b is normal and their exponents are far from
+// the ranges where the hardware rescales (|a|, |b| in [2^-500, 2^500] with
+// |a/b| >= 2^-900 is far inside them), div_scale returns its operand,
+// div_fmas is a plain fma and div_fixup returns its value, so recip_refined
+// once per denominator followed by div_by per numerator yields the same bits
+// as '/', with 5 + 3 instructions instead of 11 per quotient.  Callers check
+// the range first and use '/' otherwise.
+// ------------------------------------------------------------------------
+#ifndef SWM_SHARED_RECIP
+#define SWM_SHARED_RECIP 1
+#endif
+constexpr double kRecipLo = 0x1p-400;
+constexpr double kRecipHi = 0x1p+400;
+
+__device__ __forceinline__ bool recip_range(double b) { return (b >= kRecipLo) & (b <= kRecipHi); }
+
+__device__ __forceinline__ double recip_refined(double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  return r;
+}
+
+__device__ __forceinline__ double div_by(double a, double b, double rb) {
+  double q = a * rb;
+  double e = fma(-b, q, a);
+  return fma(e, rb, q);
+}
+
 // ConsToPrim, src/equation.f90:912-945:  ht = h + MIN_DEPTH/h; u = hu/ht.
 __device__ __forceinline__ void cons_to_prim(double h, double hu, double hv, double& u,
                                              double& v) {
@@ -52,6 +90,38 @@ __device__ __forceinline__ void weno5_first(double qm2, double qm1, double q0, d
                                             double qp2, double& wm, double& wp) {
   double d1, d2, d3;
   weno5_den(qm2, qm1, q0, qp1, qp2, d1, d2, d3);
+#if SWM_SHARED_RECIP
+  // With the (eps+beta)^2 in [2^-400, 2^400] the alphas lie in
+  // [2^-404, 2^400], their sum in [2^-404, 2^402] and the weights above 2^-806.
+  if (recip_range(d1) & recip_range(d2) & recip_range(d3)) {
+    const double r1 = recip_refined(d1), r2 = recip_refined(d2), r3 = recip_refined(d3);
+    {
+      double a1 = div_by(3.0 / 10.0, d1, r1);
+      double a2 = div_by(3.0 / 5.0, d2, r2);
+      double a3 = div_by(1.0 / 10.0, d3, r3);
+      double s = a1 + a2 + a3;
+      double rs = recip_refined(s);
+      double o1 = div_by(a1, s, rs), o2 = div_by(a2, s, rs), o3 = div_by(a3, s, rs);
+      double w1 = (1.0 / 6.0) * (-qm2 + 5.0 * qm1 + 2.0 * q0);
+      double w2 = (1.0 / 6.0) * (2.0 * qm1 + 5.0 * q0 - qp1);
+      double w3 = (1.0 / 6.0) * (11.0 * q0 - 7.0 * qp1 + 2.0 * qp2);
+      wm = o1 * w1 + o2 * w2 + o3 * w3;
+    }
+    {
+      double a1 = div_by(1.0 / 10.0, d1, r1);
+      double a2 = div_by(3.0 / 5.0, d2, r2);
+      double a3 = div_by(3.0 / 10.0, d3, r3);
+      double s = a1 + a2 + a3;
+      double rs = recip_refined(s);
+      double o1 = div_by(a1, s, rs), o2 = div_by(a2, s, rs), o3 = div_by(a3, s, rs);
+      double w1 = (1.0 / 6.0) * (2.0 * qm2 - 7.0 * qm1 + 11.0 * q0);
+      double w2 = (1.0 / 6.0) * (-qm1 + 5.0 * q0 + 2.0 * qp1);
+      double w3 = (1.0 / 6.0) * (2.0 * q0 + 5.0 * qp1 - qp2);
+      wp = o1 * w1 + o2 * w2 + o3 * w3;
+    }
+    return;
+  }
+#endif
   {
     double a1 = (3.0 / 10.0) / d1;
     double a2 = (3.0 / 5.0) / d2;
@@ -80,9 +150,10 @@ __device__ __forceinline__ void weno5_first(double qm2, double qm1, double q0, d
 // The literals are the reference's decimal strings; signed literals are folded
 // into the coefficients (a - c*q == a + (-c)*q exactly), and the 0.0*Q terms
 // are kept because they propagate NaN/Inf exactly as the reference does.
-template <int G>
+template <int G, bool SHARED>
 __device__ __forceinline__ double weno5_second_gp(double qm2, double qm1, double q0, double qp1,
-                                                  double qp2, double d1, double d2, double d3) {
+                                                  double qp2, double d1, double d2, double d3,
+                                                  double r1, double r2, double r3) {
   double g1, g2, g3;
   double c10, c11, c12, c13, c14, c20, c21, c22, c23, c24, c30, c31, c32, c33, c34;
   if constexpr (G == 0) {
@@ -106,11 +177,25 @@ __device__ __forceinline__ double weno5_second_gp(double qm2, double qm1, double
     c20 = +0.0000000000000000; c21 = -0.1642562761719537; c22 = 0.8979443965468811; c23 = 0.2663118796250726; c24 = 0.0000000000000000;
     c30 = +0.0000000000000000; c31 = 0.0000000000000000; c32 = 0.4051755680310201; c33 = 0.7590807081409336; c34 = -0.1642562761719537;
   }
-  double a1 = g1 / d1;
-  double a2 = g2 / d2;
-  double a3 = g3 / d3;
-  double s = a1 + a2 + a3;
-  double o1 = a1 / s, o2 = a2 / s, o3 = a3 / s;
+  double o1, o2, o3;
+  if constexpr (SHARED) {
+    double a1 = div_by(g1, d1, r1);
+    double a2 = div_by(g2, d2, r2);
+    double a3 = div_by(g3, d3, r3);
+    double s = a1 + a2 + a3;
+    double rs = recip_refined(s);
+    o1 = div_by(a1, s, rs);
+    o2 = div_by(a2, s, rs);
+    o3 = div_by(a3, s, rs);
+  } else {
+    double a1 = g1 / d1;
+    double a2 = g2 / d2;
+    double a3 = g3 / d3;
+    double s = a1 + a2 + a3;
+    o1 = a1 / s;
+    o2 = a2 / s;
+    o3 = a3 / s;
+  }
   double w1 = c10 * qm2 + c11 * qm1 + c12 * q0 + c13 * qp1 + c14 * qp2;
   double w2 = c20 * qm2 + c21 * qm1 + c22 * q0 + c23 * qp1 + c24 * qp2;
   double w3 = c30 * qm2 + c31 * qm1 + c32 * q0 + c33 * qp1 + c34 * qp2;
@@ -123,10 +208,21 @@ __device__ __forceinline__ void weno5_second(double qm2, double qm1, double q0, 
                                              double qp2, double w[4]) {
   double d1, d2, d3;
   weno5_den(qm2, qm1, q0, qp1, qp2, d1, d2, d3);
-  w[0] = weno5_second_gp<0>(qm2, qm1, q0, qp1, qp2, d1, d2, d3);
-  w[1] = weno5_second_gp<1>(qm2, qm1, q0, qp1, qp2, d1, d2, d3);
-  w[2] = weno5_second_gp<2>(qm2, qm1, q0, qp1, qp2, d1, d2, d3);
-  w[3] = weno5_second_gp<3>(qm2, qm1, q0, qp1, qp2, d1, d2, d3);
+#if SWM_SHARED_RECIP
+  // the gammas lie in [0.12, 0.62]: same ranges as in weno5_first
+  if (recip_range(d1) & recip_range(d2) & recip_range(d3)) {
+    const double r1 = recip_refined(d1), r2 = recip_refined(d2), r3 = recip_refined(d3);
+    w[0] = weno5_second_gp<0, true>(qm2, qm1, q0, qp1, qp2, d1, d2, d3, r1, r2, r3);
+    w[1] = weno5_second_gp<1, true>(qm2, qm1, q0, qp1, qp2, d1, d2, d3, r1, r2, r3);
+    w[2] = weno5_second_gp<2, true>(qm2, qm1, q0, qp1, qp2, d1, d2, d3, r1, r2, r3);
+    w[3] = weno5_second_gp<3, true>(qm2, qm1, q0, qp1, qp2, d1, d2, d3, r1, r2, r3);
+    return;
+  }
+#endif
+  w[0] = weno5_second_gp<0, false>(qm2, qm1, q0, qp1, qp2, d1, d2, d3, 0., 0., 0.);
+  w[1] = weno5_second_gp<1, false>(qm2, qm1, q0, qp1, qp2, d1, d2, d3, 0., 0., 0.);
+  w[2] = weno5_second_gp<2, false>(qm2, qm1, q0, qp1, qp2, d1, d2, d3, 0., 0., 0.);
+  w[3] = weno5_second_gp<3, false>(qm2, qm1, q0, qp1, qp2, d1, d2, d3, 0., 0., 0.);
 }
 
 // PositivityLimiterX/Y body, src/finitevolume.f90:671-691: theta-scaling of

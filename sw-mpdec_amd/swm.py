@@ -7,12 +7,15 @@ creating a solver without a GPU, raises.
 from __future__ import annotations
 
 import ctypes
+import os
 from pathlib import Path
 
 import numpy as np
 
 PKG = Path(__file__).resolve().parent
-LIB_PATH = PKG / "lib" / "libswm.so"
+# SWM_LIBRARY selects another build of the same library (e.g. one compiled
+# with different -D options); the default is the in-tree build.
+LIB_PATH = Path(os.environ.get("SWM_LIBRARY") or PKG / "lib" / "libswm.so")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -68,6 +71,7 @@ def lib():
     L.swm_ambiguous_decisions.argtypes = [vp]
     L.swm_sync.argtypes = [vp]
     L.swm_debug_divsqrt.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
+    L.swm_debug_shared_div.argtypes = [ctypes.c_int, _dp, _dp, _dp]
     L.swm_profile_enable.argtypes = [vp, ctypes.c_int]
     L.swm_profile_read.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
     L.swm_profile_name.restype = ctypes.c_char_p
@@ -200,6 +204,16 @@ def debug_divsqrt(a, b):
     if rc != SWM_OK:
         raise SwmError(f"swm_debug_divsqrt rc={rc}")
     return q, r
+
+
+def debug_shared_div(a, b):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    q = np.empty_like(a)
+    rc = lib().swm_debug_shared_div(a.size, _ptr(a), _ptr(b), _ptr(q))
+    if rc != SWM_OK:
+        raise SwmError(f"swm_debug_shared_div rc={rc}")
+    return q
 
 
 # ---------------------------------------------------------------------------

@@ -49,6 +49,32 @@ def test_device_div_sqrt_are_ieee():
         assert same_bits(r, np.sqrt(a)), describe_diff(r, np.sqrt(a))
 
 
+def test_shared_reciprocal_division_is_ieee_in_range():
+    """The WENO weights divide through one refined reciprocal per denominator
+    when every operand lies in [2^-400, 2^400]; over that range (and the
+    weight-shaped operands it produces) the quotient must equal a/b."""
+    rng = np.random.default_rng(1)
+    n = 1 << 22
+    sign = np.where(rng.uniform(size=n) < 0.5, -1.0, 1.0)
+    a = sign * np.ldexp(rng.uniform(1, 2, n), rng.integers(-500, 501, n))
+    b = np.ldexp(rng.uniform(1, 2, n), rng.integers(-400, 400, n))
+    # the range edges, mantissas of all ones, and powers of two
+    b[:4096] = np.ldexp(1.0, rng.integers(-400, 401, 4096))
+    b[4096:8192] = np.ldexp(np.nextafter(2.0, 0.0), rng.integers(-401, 400, 4096))
+    a[8192:12288] = np.ldexp(np.nextafter(2.0, 0.0), rng.integers(-500, 500, 4096))
+    # gamma / (eps + beta)^2 and alpha / sum(alpha), as the weights form them
+    m = n // 4
+    d = (1e-24 + np.abs(rng.standard_normal(m)) * 10.0 ** rng.uniform(-30, 30, m)) ** 2
+    d = d[(d >= 2.0 ** -400) & (d <= 2.0 ** 400)]
+    g = rng.choice([0.1, 0.3, 0.6, 0.2658420974778319, 0.1229074124899195], d.size)
+    a[n - 2 * d.size:n - d.size], b[n - 2 * d.size:n - d.size] = g, d
+    al = g / d
+    s = al + np.roll(al, 1) + np.roll(al, 2)
+    a[n - d.size:], b[n - d.size:] = al, s
+    q = swm.debug_shared_div(a, b)
+    assert same_bits(q, a / b), describe_diff(q, a / b)
+
+
 @pytest.mark.parametrize("case", sorted(CASES))
 def test_fvtd_parity(case):
     p = CASES[case]()
