@@ -2,7 +2,8 @@
 // PATANKAR time step of sw-mpdec, over all active jobs of an engine.
 //
 // One FVTimeDerivative (src/finitevolume.f90:402-440) is three launches, each
-// covering every job and up to three DeC nodes (blockIdx.y):
+// covering every job and up to three DeC nodes (blockIdx.y in the face
+// kernels, XCD-grouped block numbers in k_fvtd_update):
 //   k_fvtd_x       x faces: halo tile -> ConsToPrim -> WENO5 first sweep along
 //                  x (shared through LDS) -> second sweep along y at the 4 Gauss
 //                  points -> positivity limiter on h -> Rusanov -> Gauss sum
@@ -28,6 +29,14 @@ __constant__ double THETA[4][4] = {
     {0.0730327668541684294, 0.4505740308958107176, 0.2269672331458314485, -0.0269672331458315692},
     {0.0833333333333333287, 0.4166666666666666852, 0.4166666666666666852, 0.0833333333333333287}};
 
+// Global-address-space view of a device array.  The arrays' pointers are read
+// from the job table in memory, so the compiler cannot infer their address
+// space and would use flat loads and stores, which also occupy the LDS wait
+// counter; through G() they become global loads and stores.
+typedef __attribute__((address_space(1))) double gdouble;
+__device__ __forceinline__ const gdouble* G(const double* p) { return (const gdouble*)p; }
+__device__ __forceinline__ gdouble* G(double* p) { return (gdouble*)p; }
+
 // 0-based index of Fortran cell c (1..n or a periodic ghost) on an axis of n.
 __device__ __forceinline__ int wrap1(int c, int n) {
   int m = (c - 1) % n;
@@ -37,16 +46,16 @@ __device__ __forceinline__ int wrap1(int c, int n) {
 __device__ __forceinline__ int cdiv_d(int a, int b) { return (a + b - 1) / b; }
 
 // State of DeC node m of a job: m = 0 is U^n.
-__device__ __forceinline__ void node_state(const JobDev& J, int m, const double*& h,
-                                           const double*& hu, const double*& hv) {
+__device__ __forceinline__ void node_state(const JobDev& J, int m, const gdouble*& h,
+                                           const gdouble*& hu, const gdouble*& hv) {
   if (m == 0) {
-    h = J.h;
-    hu = J.hu;
-    hv = J.hv;
+    h = G(J.h);
+    hu = G(J.hu);
+    hv = G(J.hv);
   } else {
-    h = J.hres[m];
-    hu = J.hun[m];
-    hv = J.hvn[m];
+    h = G(J.hres[m]);
+    hu = G(J.hun[m]);
+    hv = G(J.hvn[m]);
   }
 }
 
@@ -73,7 +82,7 @@ __global__ __launch_bounds__(256) void k_fvtd_x(EngineArgs E, const TileRef* til
   const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
   const int c0 = (T.tile % ntx) * kFaceTile;
   const int j0 = (T.tile / ntx) * kTile + 1;
-  const double *h, *hu, *hv;
+  const gdouble *h, *hu, *hv;
   node_state(J, m, h, hu, hv);
 
   for (int t = tid; t < 400; t += 256) {
@@ -125,9 +134,9 @@ __global__ __launch_bounds__(256) void k_fvtd_x(EngineArgs E, const TileRef* til
       acc3 = acc3 + J.wbnd[g] * F3;
     }
     size_t o = (size_t)(j - 1) * (nx + 1) + c;
-    J.FX[slot][0][o] = acc1;
-    J.FX[slot][1][o] = acc2;
-    J.FX[slot][2][o] = acc3;
+    G(J.FX[slot][0])[o] = acc1;
+    G(J.FX[slot][1])[o] = acc2;
+    G(J.FX[slot][2])[o] = acc3;
   }
 }
 
@@ -149,7 +158,7 @@ __global__ __launch_bounds__(256) void k_fvtd_y(EngineArgs E, const TileRef* til
   const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
   const int i0 = (T.tile % ntx) * kTile + 1;
   const int r0 = (T.tile / ntx) * kFaceTile;
-  const double *h, *hu, *hv;
+  const gdouble *h, *hu, *hv;
   node_state(J, m, h, hu, hv);
 
   for (int t = tid; t < 400; t += 256) {
@@ -201,9 +210,9 @@ __global__ __launch_bounds__(256) void k_fvtd_y(EngineArgs E, const TileRef* til
       acc3 = acc3 + J.wbnd[g] * F3;
     }
     size_t o = (size_t)r * nx + (i - 1);
-    J.FY[slot][0][o] = acc1;
-    J.FY[slot][1][o] = acc2;
-    J.FY[slot][2][o] = acc3;
+    G(J.FY[slot][0])[o] = acc1;
+    G(J.FY[slot][1])[o] = acc2;
+    G(J.FY[slot][2])[o] = acc3;
   }
 }
 
@@ -214,100 +223,110 @@ __global__ __launch_bounds__(256) void k_fvtd_y(EngineArgs E, const TileRef* til
 // WENO5 second sweep along y (columns i-2..i+2) followed by one along x per
 // y Gauss point; SW = -g*h*db; S = sum over x-GP (outer), y-GP (inner) of W*SW.
 // ------------------------------------------------------------------------
+// Blocks are numbered so that the L.count nodes of one tile share an XCD:
+// workgroups are dealt round-robin to the 8 XCDs, so block
+// ((t / 8) * count + node) * 8 + t % 8 runs tile t on XCD t % 8 for every
+// node, close together in time, and the tile's bathymetry and well-balanced
+// tables are read from HBM once and from that XCD's L2 for the other nodes.
 __global__ __launch_bounds__(256) void k_fvtd_update(EngineArgs E, const TileRef* tiles,
-                                                     NodeList L) {
+                                                     int ntiles, NodeList L) {
   __shared__ double sH[400];           // [20 rows][20 cols]
   __shared__ double sVt[4 * 16 * 20];  // [yg][16 rows][20 cols]
-  const TileRef T = tiles[blockIdx.x];
+  const int grp = blockIdx.x / (8 * L.count), rem = blockIdx.x % (8 * L.count);
+  const int node = rem / 8, tile = grp * 8 + (rem % 8);
+  if (tile >= ntiles) return;
+  const TileRef T = tiles[tile];
   const JobDev& J = E.jobs[T.job];
-  const int m = L.m[blockIdx.y], slot = L.slot[blockIdx.y];
   const int nx = J.nx, ny = J.ny;
   const int ntx = cdiv_d(nx, kTile);
   const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
   const int i0 = (T.tile % ntx) * kTile + 1, j0 = (T.tile / ntx) * kTile + 1;
   const int i = i0 + tx, j = j0 + ty;
-  const double *h, *hu, *hv;
-  node_state(J, m, h, hu, hv);
-  double S2 = 0.0, S3 = 0.0;
-  if (J.bathy) {  // uniform per block
-    for (int t = tid; t < 400; t += 256) {
-      int lx = t % 20, ly = t / 20;
-      sH[t] = h[(size_t)wrap1(j0 - 2 + ly, ny) * nx + wrap1(i0 - 2 + lx, nx)];
-    }
-    __syncthreads();
-    for (int t = tid; t < 320; t += 256) {
-      int ly = t / 20, lx = t % 20;
-      const double* q = sH + ly * 20 + lx;
-      double w[4];
-      weno5_second(q[0], q[20], q[40], q[60], q[80], w);
-#pragma unroll
-      for (int yg = 0; yg < 4; ++yg) sVt[yg * 320 + t] = w[yg];
-    }
-    __syncthreads();
-    if (i <= nx && j <= ny) {
-      double hgp[4][4];  // [yg][xg]
-#pragma unroll
-      for (int yg = 0; yg < 4; ++yg) {
-        const double* q = sVt + yg * 320 + ty * 20 + tx;
-        weno5_second(q[0], q[1], q[2], q[3], q[4], hgp[yg]);
+  const bool in = i <= nx && j <= ny;
+  const size_t n = (size_t)J.n;
+  const size_t k = in ? (size_t)(j - 1) * nx + (i - 1) : 0;
+  {
+    const int m = L.m[node], slot = L.slot[node];
+    const gdouble *h, *hu, *hv;
+    node_state(J, m, h, hu, hv);
+    double S2 = 0.0, S3 = 0.0;
+    if (J.bathy) {  // uniform per block
+      for (int t = tid; t < 400; t += 256) {
+        int lx = t % 20, ly = t / 20;
+        sH[t] = h[(size_t)wrap1(j0 - 2 + ly, ny) * nx + wrap1(i0 - 2 + lx, nx)];
       }
-      const size_t n = (size_t)J.n;
-      const size_t k = (size_t)(j - 1) * nx + (i - 1);
+      __syncthreads();
+      for (int t = tid; t < 320; t += 256) {
+        int ly = t / 20, lx = t % 20;
+        const double* qh = sH + ly * 20 + lx;
+        double w[4];
+        weno5_second(qh[0], qh[20], qh[40], qh[60], qh[80], w);
 #pragma unroll
-      for (int xg = 0; xg < 4; ++xg)
+        for (int yg = 0; yg < 4; ++yg) sVt[yg * 320 + t] = w[yg];
+      }
+      __syncthreads();
+      if (in) {
+        double hgp[4][4];  // [yg][xg]
 #pragma unroll
         for (int yg = 0; yg < 4; ++yg) {
-          double hh = hgp[yg][xg];
-          double bx = J.bx[(size_t)(yg * 4 + xg) * n + k];
-          double by = J.by[(size_t)(yg * 4 + xg) * n + k];
-          double sw2 = -J.g * hh * bx;
-          double sw3 = -J.g * hh * by;
-          S2 = S2 + J.wgp[xg][yg] * sw2;
-          S3 = S3 + J.wgp[xg][yg] * sw3;
+          const double* qv = sVt + yg * 320 + ty * 20 + tx;
+          weno5_second(qv[0], qv[1], qv[2], qv[3], qv[4], hgp[yg]);
         }
-    }
-  }
-  if (i > nx || j > ny) return;
-  const size_t k = (size_t)(j - 1) * nx + (i - 1);
-  if (L.wb_init) {  // building the well-balanced reference: keep S as SWB
-    J.SWB2[k] = S2;
-    J.SWB3[k] = S3;
-    return;
-  }
-  const size_t fe = (size_t)(j - 1) * (nx + 1) + i, fw = fe - 1;  // x faces i, i-1
-  const size_t gn = (size_t)j * nx + (i - 1), gs = gn - nx;        // y faces j, j-1
-  const bool use_wb = J.wb != 0;
-  if (use_wb) {
-    S2 = S2 - J.SWB2[k];
-    S3 = S3 - J.SWB3[k];
-  }
-  if (L.debug_S) {  // parity tests read S(2:3) from the first matrix scratch
-    J.M[0][0][k] = S2;
-    J.M[0][1][k] = S3;
-  }
-  double fxe[3], fxw[3], fyn[3], fys[3];
 #pragma unroll
-  for (int v = 0; v < 3; ++v) {
-    fxe[v] = J.FX[slot][v][fe];
-    fxw[v] = J.FX[slot][v][fw];
-    fyn[v] = J.FY[slot][v][gn];
-    fys[v] = J.FY[slot][v][gs];
-    if (use_wb) {  // FX = FX - FXWB; FY = FY - FYWB (src/finitevolume.f90:485-488)
-      fxe[v] = fxe[v] - J.FXWB[v][fe];
-      fxw[v] = fxw[v] - J.FXWB[v][fw];
-      fyn[v] = fyn[v] - J.FYWB[v][gn];
-      fys[v] = fys[v] - J.FYWB[v][gs];
+        for (int xg = 0; xg < 4; ++xg)
+#pragma unroll
+          for (int yg = 0; yg < 4; ++yg) {
+            double hh = hgp[yg][xg];
+            double bx = G(J.bx)[(size_t)(yg * 4 + xg) * n + k];
+            double by = G(J.by)[(size_t)(yg * 4 + xg) * n + k];
+            double sw2 = -J.g * hh * bx;
+            double sw3 = -J.g * hh * by;
+            S2 = S2 + J.wgp[xg][yg] * sw2;
+            S3 = S3 + J.wgp[xg][yg] * sw3;
+          }
+      }
     }
+    if (!in) return;
+    if (L.wb_init) {  // building the well-balanced reference: keep S as SWB
+      G(J.SWB2)[k] = S2;
+      G(J.SWB3)[k] = S3;
+      return;
+    }
+    const size_t fe = (size_t)(j - 1) * (nx + 1) + i, fw = fe - 1;  // x faces i, i-1
+    const size_t gn = (size_t)j * nx + (i - 1), gs = gn - nx;        // y faces j, j-1
+    const bool use_wb = J.wb != 0;
+    if (use_wb) {
+      S2 = S2 - G(J.SWB2)[k];
+      S3 = S3 - G(J.SWB3)[k];
+    }
+    if (L.debug_S) {  // parity tests read S(2:3) from the first matrix scratch
+      G(J.M[0][0])[k] = S2;
+      G(J.M[0][1])[k] = S3;
+    }
+    double fxe[3], fxw[3], fyn[3], fys[3];
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+      fxe[v] = G(J.FX[slot][v])[fe];
+      fxw[v] = G(J.FX[slot][v])[fw];
+      fyn[v] = G(J.FY[slot][v])[gn];
+      fys[v] = G(J.FY[slot][v])[gs];
+      if (use_wb) {  // FX = FX - FXWB; FY = FY - FYWB (src/finitevolume.f90:485-488)
+        fxe[v] = fxe[v] - G(J.FXWB[v])[fe];
+        fxw[v] = fxw[v] - G(J.FXWB[v])[fw];
+        fyn[v] = fyn[v] - G(J.FYWB[v])[gn];
+        fys[v] = fys[v] - G(J.FYWB[v])[gs];
+      }
+    }
+    // Ut(2:3) = S - (FX(i)-FX(i-1))/dx - (FY(j)-FY(j-1))/dy  (:525-527)
+    G(J.FU2[m])[k] = S2 - (fxe[1] - fxw[1]) / J.dx - (fyn[1] - fys[1]) / J.dy;
+    G(J.FU3[m])[k] = S3 - (fxe[2] - fxw[2]) / J.dx - (fyn[2] - fys[2]) / J.dy;
+    // mass flux per face / dx: the production/destruction entries of
+    // src/finitevolume.f90:496-523 are max(+-q, 0) and -min(+-q, 0) of these.
+    G(J.qx[m])[fe] = fxe[0] / J.dx;
+    G(J.qy[m])[gn] = fyn[0] / J.dy;
+    if (i == 1) G(J.qx[m])[fw] = fxw[0] / J.dx;
+    if (j == 1) G(J.qy[m])[gs] = fys[0] / J.dy;
   }
-  // Ut(2:3) = S - (FX(i)-FX(i-1))/dx - (FY(j)-FY(j-1))/dy  (:525-527)
-  J.FU2[m][k] = S2 - (fxe[1] - fxw[1]) / J.dx - (fyn[1] - fys[1]) / J.dy;
-  J.FU3[m][k] = S3 - (fxe[2] - fxw[2]) / J.dx - (fyn[2] - fys[2]) / J.dy;
-  // mass flux per face / dx: the production/destruction entries of
-  // src/finitevolume.f90:496-523 are max(+-q, 0) and -min(+-q, 0) of these.
-  J.qx[m][fe] = fxe[0] / J.dx;
-  J.qy[m][gn] = fyn[0] / J.dy;
-  if (i == 1) J.qx[m][fw] = fxw[0] / J.dx;
-  if (j == 1) J.qy[m][gs] = fys[0] / J.dy;
 }
 
 // ------------------------------------------------------------------------
@@ -323,15 +342,15 @@ __global__ __launch_bounds__(256) void k_momentum(EngineArgs E, const TileRef* b
   const double dt = J.status->dt;
   for (int s = 0; s < S.count; ++s) {
     const int m = S.m[s];
-    double u = J.hu[k], v = J.hv[k];
+    double u = G(J.hu)[k], v = G(J.hv)[k];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const double c = dt * THETA[m][r];
-      u = u + c * J.FU2[S.src[r]][k];
-      v = v + c * J.FU3[S.src[r]][k];
+      u = u + c * G(J.FU2[S.src[r]])[k];
+      v = v + c * G(J.FU3[S.src[r]])[k];
     }
-    J.hun[m][k] = u;
-    J.hvn[m][k] = v;
+    G(J.hun[m])[k] = u;
+    G(J.hvn[m])[k] = v;
   }
 }
 
@@ -401,8 +420,19 @@ __device__ double sequential_residual(int n, const double* a, const double* b, i
 // src/timediscretization.f90:795-887) fused with the first Jacobi sweep from
 // x^0 = rhs (src/jacobi_iteration.f90:67-76).
 // ------------------------------------------------------------------------
+// An empty asm the compiler must assume changes x: work that uses the result
+// cannot move above it.  The assembly passes each slot's depths through it so
+// that the 32 divisions of a row are not all hoisted to the top at once (which
+// costs registers and occupancy for no gain).
+__device__ __forceinline__ double order_point(double x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
 __device__ __forceinline__ void patankar_slot(int m, double dt, const double q[4], bool east,
                                               double hK, double hL, double& M, double& D) {
+  hK = order_point(hK);
+  hL = order_point(hL);
   // production / destruction of the slot at the 4 DeC nodes, from the face
   // mass flux q = F/dx: west/south slots take max(F,0)/dx, -min(F,0)/dx; the
   // east/north slots the same of -F (src/finitevolume.f90:496-523).
@@ -411,13 +441,16 @@ __device__ __forceinline__ void patankar_slot(int m, double dt, const double q[4
   M = 0.;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
+    const double qr = order_point(q[r]);
+    hK = order_point(hK);
+    hL = order_point(hL);
     double P, De;
     if (!east) {
-      P = (q[r] > 0.) ? q[r] : 0.;
-      De = (q[r] < 0.) ? -q[r] : -0.;
+      P = (qr > 0.) ? qr : 0.;
+      De = (qr < 0.) ? -qr : -0.;
     } else {
-      P = (q[r] < 0.) ? -q[r] : 0.;
-      De = (q[r] > 0.) ? q[r] : -0.;
+      P = (qr < 0.) ? -qr : 0.;
+      De = (qr > 0.) ? qr : -0.;
     }
     const double th = THETA[m][r];
     const double c = dt * th;
@@ -452,16 +485,16 @@ __global__ __launch_bounds__(256) void k_assemble(EngineArgs E, const TileRef* b
     const size_t gS = (size_t)j * nx + i, gN = gS + nx;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      qS[r] = J.qy[S.src[r]][gS];
-      qW[r] = J.qx[S.src[r]][fW];
-      qE[r] = J.qx[S.src[r]][fE];
-      qN[r] = J.qy[S.src[r]][gN];
+      qS[r] = G(J.qy[S.src[r]])[gS];
+      qW[r] = G(J.qx[S.src[r]])[fW];
+      qE[r] = G(J.qx[S.src[r]])[fE];
+      qN[r] = G(J.qy[S.src[r]])[gN];
     }
-    b = J.h[k];
-    bS = J.h[kS];
-    bW = J.h[kW];
-    bE = J.h[kE];
-    bN = J.h[kN];
+    b = G(J.h)[k];
+    bS = G(J.h)[kS];
+    bW = G(J.h)[kW];
+    bE = G(J.h)[kE];
+    bN = G(J.h)[kN];
   }
   // one system at a time; its block sum is taken before the next one starts
 #pragma unroll 1
@@ -469,7 +502,7 @@ __global__ __launch_bounds__(256) void k_assemble(EngineArgs E, const TileRef* b
     double d = 0.;
     if (in) {
       const int m = S.m[s];
-      const double* hp = S.hp_is_un ? J.h : J.hres[m];
+      const gdouble* hp = G(S.hp_is_un ? J.h : J.hres[m]);
       const double hK = hp[k], hS = hp[kS], hW = hp[kW], hE = hp[kE], hN = hp[kN];
       double D = 1., mS, mW, mE, mN;
       patankar_slot(m, dt, qS, false, hK, hS, mS, D);
@@ -477,11 +510,11 @@ __global__ __launch_bounds__(256) void k_assemble(EngineArgs E, const TileRef* b
       patankar_slot(m, dt, qE, true, hK, hE, mE, D);
       patankar_slot(m, dt, qN, true, hK, hN, mN, D);
       double* const* Ms = J.M[s];
-      Ms[0][k] = mS;
-      Ms[1][k] = mW;
-      Ms[2][k] = mE;
-      Ms[3][k] = mN;
-      Ms[4][k] = D;
+      G(Ms[0])[k] = mS;
+      G(Ms[1])[k] = mW;
+      G(Ms[2])[k] = mE;
+      G(Ms[3])[k] = mN;
+      G(Ms[4])[k] = D;
       // sparse_matmul in CSR order S, W, self (value 0), E, N; then
       // sol = (rhs - prod)/diag
       double p = 0.;
@@ -491,7 +524,7 @@ __global__ __launch_bounds__(256) void k_assemble(EngineArgs E, const TileRef* b
       p = p + mE * bE;
       p = p + mN * bN;
       double x1 = (b - p) / D;
-      J.xj[m][1][k] = x1;
+      G(J.xj[m][1])[k] = x1;
       d = fabs(x1 - b);
     }
     double t = block_sum_all(d, red);
@@ -523,19 +556,19 @@ __global__ __launch_bounds__(256) void k_sweep(EngineArgs E, const TileRef* bloc
     const int kN = (j == ny - 1 ? 0 : j + 1) * nx + i;
     const int kW = j * nx + (i == 0 ? nx - 1 : i - 1);
     const int kE = j * nx + (i == nx - 1 ? 0 : i + 1);
-    const double b = J.h[k];
+    const double b = G(J.h)[k];
     for (int s = 0; s < S.count; ++s) {
       if (!act[s]) continue;
       const int m = S.m[s];
-      const double* xo = J.xj[m][cur[s]];
-      double* xn = J.xj[m][1 - cur[s]];
+      const gdouble* xo = G(J.xj[m][cur[s]]);
+      gdouble* xn = G(J.xj[m][1 - cur[s]]);
       double p = 0.;
-      p = p + J.M[s][0][k] * xo[kS];
-      p = p + J.M[s][1][k] * xo[kW];
+      p = p + G(J.M[s][0])[k] * xo[kS];
+      p = p + G(J.M[s][1])[k] * xo[kW];
       p = p + 0. * xo[k];
-      p = p + J.M[s][2][k] * xo[kE];
-      p = p + J.M[s][3][k] * xo[kN];
-      double xv = (b - p) / J.M[s][4][k];
+      p = p + G(J.M[s][2])[k] * xo[kE];
+      p = p + G(J.M[s][3])[k] * xo[kN];
+      double xv = (b - p) / G(J.M[s][4])[k];
       xn[k] = xv;
       d[s] = fabs(xv - xo[k]);
     }
@@ -628,7 +661,7 @@ __global__ __launch_bounds__(256) void k_finalize(EngineArgs E, const TileRef* b
   for (int s = 0; s < S.count; ++s) {
     const int m = S.m[s];
     const JacState& q = J.status->st[s];
-    J.hres[m][k] = q.zero ? 0. : J.xj[m][q.cur][k];
+    G(J.hres[m])[k] = q.zero ? 0. : G(J.xj[m][q.cur])[k];
   }
 }
 
@@ -638,9 +671,9 @@ __global__ __launch_bounds__(256) void k_commit(EngineArgs E, const TileRef* blo
   const JobDev& J = E.jobs[T.job];
   const int k = T.tile * kCellBlock + threadIdx.x;
   if (k >= J.n) return;
-  J.h[k] = J.hres[3][k];
-  J.hu[k] = J.hun[3][k];
-  J.hv[k] = J.hvn[3][k];
+  G(J.h)[k] = G(J.hres[3])[k];
+  G(J.hu)[k] = G(J.hun[3])[k];
+  G(J.hv)[k] = G(J.hvn[3])[k];
 }
 
 // FXWB/FYWB from the slot scratch after the well-balanced FVTD.  The job's
@@ -652,8 +685,8 @@ __global__ __launch_bounds__(256) void k_copy_wb(EngineArgs E, const TileRef* bl
   for (int k = T.tile * kCellBlock + threadIdx.x; k < nfx || k < nfy;
        k += J.nblk * kCellBlock) {
     for (int v = 0; v < 3; ++v) {
-      if (k < nfx) J.FXWB[v][k] = J.FX[slot][v][k];
-      if (k < nfy) J.FYWB[v][k] = J.FY[slot][v][k];
+      if (k < nfx) G(J.FXWB[v])[k] = G(J.FX[slot][v])[k];
+      if (k < nfy) G(J.FYWB[v])[k] = G(J.FY[slot][v])[k];
     }
   }
 }
@@ -687,7 +720,7 @@ __global__ __launch_bounds__(256) void k_timestep(EngineArgs E, const TileRef* b
   const int k = T.tile * kCellBlock + threadIdx.x;
   double a = 0.0, b = 0.0, nan = 0.0;
   if (k < J.n) {
-    wave_speeds(J, J.h[k], J.hu[k], J.hv[k], a, b);
+    wave_speeds(J, G(J.h)[k], G(J.hu)[k], G(J.hv)[k], a, b);
     if (isnan(a) || isnan(b)) {
       nan = 1.0;
       a = 0.0;
@@ -748,7 +781,7 @@ __global__ __launch_bounds__(256) void k_timestep_final(EngineArgs E, const int*
     ts = kHuge;
     for (int k = 0; k < J.n; ++k) {
       double aa, bb;
-      wave_speeds(J, J.h[k], J.hu[k], J.hv[k], aa, bb);
+      wave_speeds(J, G(J.h)[k], G(J.hu)[k], G(J.hv)[k], aa, bb);
       LX = max_r(LX, aa);
       LY = max_r(LY, bb);
       ts = min_r(min_r(ts, J.dx / LX), J.dy / LY);
@@ -810,7 +843,9 @@ void launch_fvtd(const EngineArgs& E, const TileRef* xt, int nxt, const TileRef*
                  const TileRef* ct, int nct, const NodeList& L, hipStream_t s) {
   if (nxt) hipLaunchKernelGGL(k_fvtd_x, dim3(nxt, L.count), dim3(256), 0, s, E, xt, L);
   if (nyt) hipLaunchKernelGGL(k_fvtd_y, dim3(nyt, L.count), dim3(256), 0, s, E, yt, L);
-  if (nct) hipLaunchKernelGGL(k_fvtd_update, dim3(nct, L.count), dim3(256), 0, s, E, ct, L);
+  if (nct)
+    hipLaunchKernelGGL(k_fvtd_update, dim3(cdiv(nct, 8) * 8 * L.count), dim3(256), 0, s, E, ct,
+                       nct, L);
 }
 
 void launch_momentum(const EngineArgs& E, const TileRef* cb, int ncb, const SysList& S,

@@ -44,7 +44,11 @@ __device__ __forceinline__ double min_r(double a, double b) { return (a < b) ? a
 constexpr double kRecipLo = 0x1p-400;
 constexpr double kRecipHi = 0x1p+400;
 
-__device__ __forceinline__ bool recip_range(double b) { return (b >= kRecipLo) & (b <= kRecipHi); }
+// 1 when kRecipLo <= b <= kRecipHi (0 for NaN); combined with '&' so that the
+// three checks of a reconstruction need no branches.
+__device__ __forceinline__ int recip_range(double b) {
+  return (int)(b >= kRecipLo) & (int)(b <= kRecipHi);
+}
 
 __device__ __forceinline__ double recip_refined(double b) {
   double r = __builtin_amdgcn_rcp(b);
