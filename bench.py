@@ -64,13 +64,14 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles(kernel: str, size: int):
-    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
-    scaled by the copy-kernel calibration), read from the committed summary."""
-    f = ROOT / "profiles" / f"traffic_{kernel}_{size}.md"
+def traffic_per_cell_node(kernel: str, size: int):
+    """Measured HBM bytes per cell-node of an FVTD kernel (rocprofv3 --pmc
+    FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration), read from
+    the committed summary profiles/traffic_<size>.md."""
+    f = ROOT / "profiles" / f"traffic_{size}.md"
     if not f.exists():
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read_text())
+    m = re.search(rf"traffic_bytes_per_cell_node\[{kernel}\]:\s*([0-9.eE+]+)", f.read_text())
     return float(m.group(1)) if m else None
 
 
@@ -265,8 +266,11 @@ def main():
     achieved = algo / avg_s / 1e9 if avg_s > 0 else 0.0
     roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles(dom, n),
+            "traffic": None,
             "algo_bytes_per_launch": algo, "avg_launch_us": kern[dom]["avg_us"]}
+    tpc = traffic_per_cell_node(dom, n)
+    if tpc is not None:
+        roof["traffic"] = tpc * units_per_step(dom, cells, 0) * args.steps / kern[dom]["launches"]
     if dom in ("fvtd_x", "fvtd_y"):
         ops = FP64_OPS_PER_FACE * units_per_step(dom, cells, 0) * args.steps / kern[dom]["launches"]
         tf = ops / avg_s / 1e12
