@@ -181,6 +181,12 @@ int swm_debug_divsqrt(int n, const double *a, const double *b, double *q, double
  * every quotient with that denominator (no range check here). */
 int swm_debug_shared_div(int n, const double *a, const double *b, double *q);
 
+/* Self-check of the WENO5 arithmetic as the kernels evaluate it
+ * (WENO5_FirstSweep, src/reconstruction.f90:673-755, and WENO5_SecondSweep,
+ * :999-1136) on n five-point stencils q[5i..5i+4]: first[2i..2i+1] = (wm, wp),
+ * second[4i..4i+3] = the 4 Gauss-point values. */
+int swm_debug_weno5(int n, const double *q, double *first, double *second);
+
 #ifdef __cplusplus
 }
 #endif

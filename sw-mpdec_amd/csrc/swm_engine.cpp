@@ -1110,4 +1110,23 @@ int swm_debug_shared_div(int n, const double* a, const double* b, double* q) {
   return rc;
 }
 
+int swm_debug_weno5(int n, const double* q, double* first, double* second) {
+  if (n <= 0 || !q || !first || !second) return SWM_E_ARG;
+  double* dq = nullptr;
+  const size_t nq = 5 * (size_t)n, nf = 2 * (size_t)n, ns = 4 * (size_t)n;
+  if (hipMalloc((void**)&dq, (nq + nf + ns) * sizeof(double)) != hipSuccess) return SWM_E_NOMEM;
+  double *df = dq + nq, *ds = df + nf;
+  int rc = SWM_OK;
+  if (hipMemcpy(dq, q, nq * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) rc = SWM_E_HIP;
+  if (rc == SWM_OK) {
+    launch_weno5(n, dq, df, ds, nullptr);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(first, df, nf * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(second, ds, ns * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = SWM_E_HIP;
+  }
+  (void)hipFree(dq);
+  return rc;
+}
+
 }  // extern "C"

@@ -128,5 +128,6 @@ void launch_soa_to_aos(int nx, int ny, const double* h, const double* hu, const 
 void launch_divsqrt(int n, const double* a, const double* b, double* q, double* r,
                     hipStream_t s);
 void launch_shared_div(int n, const double* a, const double* b, double* q, hipStream_t s);
+void launch_weno5(int n, const double* q, double* first, double* second, hipStream_t s);
 
 }  // namespace swm

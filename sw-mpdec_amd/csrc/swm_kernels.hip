@@ -37,10 +37,37 @@ typedef __attribute__((address_space(1))) double gdouble;
 __device__ __forceinline__ const gdouble* G(const double* p) { return (const gdouble*)p; }
 __device__ __forceinline__ gdouble* G(double* p) { return (gdouble*)p; }
 
+// Minimum waves per SIMD the face kernels are compiled for (caps their VGPRs).
+#ifndef SWM_FACE_MIN_WAVES
+#define SWM_FACE_MIN_WAVES 1
+#endif
+
 // 0-based index of Fortran cell c (1..n or a periodic ghost) on an axis of n.
+// The tiles' halos keep c within [-2, n + 17], so one add or subtract of n
+// suffices for n >= 17; smaller axes take the modulo.
 __device__ __forceinline__ int wrap1(int c, int n) {
-  int m = (c - 1) % n;
-  return m < 0 ? m + n : m;
+  int m = c - 1;
+  if (m < 0)
+    m += n;
+  else if (m >= n)
+    m -= n;
+  if ((unsigned)m >= (unsigned)n) {
+    m = (c - 1) % n;
+    if (m < 0) m += n;
+  }
+  return m;
+}
+
+// Row-major cell k = k0 + t of a grid nx wide, with (i0, j0) the cell of k0
+// (computed once per block): i = k % nx, j = k / nx without a per-thread
+// integer division.
+__device__ __forceinline__ void cell_ij(int i0, int j0, int t, int nx, int& i, int& j) {
+  i = i0 + t;
+  j = j0;
+  while (i >= nx) {
+    i -= nx;
+    ++j;
+  }
 }
 
 __device__ __forceinline__ int cdiv_d(int a, int b) { return (a + b - 1) / b; }
@@ -68,7 +95,7 @@ __device__ __forceinline__ void node_state(const JobDev& J, int m, const gdouble
 // face f = nx uses the unlimited left value of cell nx+1 while f = 0 uses the
 // limited one of cell 1: the two values of the periodic seam.
 // ------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_fvtd_x(EngineArgs E, const TileRef* tiles,
+__global__ __launch_bounds__(256, SWM_FACE_MIN_WAVES) void k_fvtd_x(EngineArgs E, const TileRef* tiles,
                                                 NodeList L) {
   __shared__ double smem[3 * 400 + 2 * 3 * 320];
   double* sV = smem;          // [3][20 rows][20 cols]
@@ -144,7 +171,7 @@ __global__ __launch_bounds__(256) void k_fvtd_x(EngineArgs E, const TileRef* til
 // y faces: the transpose of k_fvtd_x.  Tile (bx, by): columns
 // i = 16*bx + 1 + tx, Fortran rows r = 15*by + ty (0..ny+1); face g = r.
 // ------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_fvtd_y(EngineArgs E, const TileRef* tiles,
+__global__ __launch_bounds__(256, SWM_FACE_MIN_WAVES) void k_fvtd_y(EngineArgs E, const TileRef* tiles,
                                                 NodeList L) {
   __shared__ double smem[3 * 400 + 2 * 3 * 320];
   double* sV = smem;          // [3][20 rows][20 cols]
@@ -420,30 +447,27 @@ __device__ double sequential_residual(int n, const double* a, const double* b, i
 // src/timediscretization.f90:795-887) fused with the first Jacobi sweep from
 // x^0 = rhs (src/jacobi_iteration.f90:67-76).
 // ------------------------------------------------------------------------
-// An empty asm the compiler must assume changes x: work that uses the result
-// cannot move above it.  The assembly passes each slot's depths through it so
-// that the 32 divisions of a row are not all hoisted to the top at once (which
-// costs registers and occupancy for no gain).
-__device__ __forceinline__ double order_point(double x) {
-  asm volatile("" : "+v"(x));
-  return x;
+// PATANKAR_DIV of a zero numerator, without the division: the reference's
+// 2*h*z/(h*h + MAX(h*h, 1e-6)) for z = +0 or -0 is +0 when h < MIN_DEPTH, z
+// itself when h is finite (the quotient keeps the sign of z), and NaN when h
+// is +Inf or NaN (2*h*z is then NaN) -- exactly what this returns.
+__device__ __forceinline__ double patankar_div_zero(double z, double h) {
+  return (h < kMinDepth) ? 0. : z * (h * 0. + 1.);
 }
 
 __device__ __forceinline__ void patankar_slot(int m, double dt, const double q[4], bool east,
                                               double hK, double hL, double& M, double& D) {
-  hK = order_point(hK);
-  hL = order_point(hL);
   // production / destruction of the slot at the 4 DeC nodes, from the face
   // mass flux q = F/dx: west/south slots take max(F,0)/dx, -min(F,0)/dx; the
   // east/north slots the same of -F (src/finitevolume.f90:496-523).
   // theta > 0: M = M - c*PD(P, hL), D = D + c*PD(De, hK);
-  // otherwise: D = D - c*PD(P, hK), M = M + c*PD(De, hL)  (:800-884)
+  // otherwise: D = D - c*PD(P, hK), M = M + c*PD(De, hL)  (:800-884).
+  // At most one of P and De is nonzero, so each node needs one division; the
+  // zero one goes through patankar_div_zero, which gives the same value.
   M = 0.;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const double qr = order_point(q[r]);
-    hK = order_point(hK);
-    hL = order_point(hL);
+    const double qr = q[r];
     double P, De;
     if (!east) {
       P = (qr > 0.) ? qr : 0.;
@@ -455,8 +479,11 @@ __device__ __forceinline__ void patankar_slot(int m, double dt, const double q[4
     const double th = THETA[m][r];
     const double c = dt * th;
     const bool pos = th > 0.;
-    const double tL = c * patankar_div(pos ? P : De, hL);
-    const double tK = c * patankar_div(pos ? De : P, hK);
+    const double numL = pos ? P : De, numK = pos ? De : P;
+    const bool l_nz = numL != 0.;
+    const double dv = patankar_div(l_nz ? numL : numK, l_nz ? hL : hK);
+    const double tL = c * (l_nz ? dv : patankar_div_zero(numL, hL));
+    const double tK = c * (l_nz ? patankar_div_zero(numK, hK) : dv);
     M = pos ? (M - tL) : (M + tL);
     D = pos ? (D + tK) : (D - tK);
   }
@@ -465,30 +492,32 @@ __device__ __forceinline__ void patankar_slot(int m, double dt, const double q[4
 __global__ __launch_bounds__(256) void k_assemble(EngineArgs E, const TileRef* blocks,
                                                   SysList S) {
   __shared__ double red[4];
+  __shared__ double sq[16 * 256];  // [slot*4 + node][thread]: face mass fluxes
   const TileRef T = blocks[blockIdx.x];
   const JobDev& J = E.jobs[T.job];
   const int nx = J.nx, ny = J.ny, n = J.n;
-  const int k = T.tile * kCellBlock + threadIdx.x;
+  const int tid = threadIdx.x;
+  const int k = T.tile * kCellBlock + tid;
   const bool in = k < n;
   const double dt = J.status->dt;
-  int kS = 0, kN = 0, kW = 0, kE = 0;
-  double qS[4] = {0., 0., 0., 0.}, qW[4] = {0., 0., 0., 0.};
-  double qE[4] = {0., 0., 0., 0.}, qN[4] = {0., 0., 0., 0.};
+  int kS = 0, kW = 0, kE = 0, kN = 0;  // neighbours
   double b = 0., bS = 0., bW = 0., bE = 0., bN = 0.;
   if (in) {
-    const int i = k % nx, j = k / nx;
+    const int k0 = T.tile * kCellBlock;
+    int i, j;
+    cell_ij(k0 % nx, k0 / nx, tid, nx, i, j);
     kS = (j == 0 ? ny - 1 : j - 1) * nx + i;
-    kN = (j == ny - 1 ? 0 : j + 1) * nx + i;
     kW = j * nx + (i == 0 ? nx - 1 : i - 1);
     kE = j * nx + (i == nx - 1 ? 0 : i + 1);
+    kN = (j == ny - 1 ? 0 : j + 1) * nx + i;
     const size_t fW = (size_t)j * (nx + 1) + i, fE = fW + 1;
     const size_t gS = (size_t)j * nx + i, gN = gS + nx;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      qS[r] = G(J.qy[S.src[r]])[gS];
-      qW[r] = G(J.qx[S.src[r]])[fW];
-      qE[r] = G(J.qx[S.src[r]])[fE];
-      qN[r] = G(J.qy[S.src[r]])[gN];
+      sq[(0 * 4 + r) * 256 + tid] = G(J.qy[S.src[r]])[gS];
+      sq[(1 * 4 + r) * 256 + tid] = G(J.qx[S.src[r]])[fW];
+      sq[(2 * 4 + r) * 256 + tid] = G(J.qx[S.src[r]])[fE];
+      sq[(3 * 4 + r) * 256 + tid] = G(J.qy[S.src[r]])[gN];
     }
     b = G(J.h)[k];
     bS = G(J.h)[kS];
@@ -496,7 +525,9 @@ __global__ __launch_bounds__(256) void k_assemble(EngineArgs E, const TileRef* b
     bE = G(J.h)[kE];
     bN = G(J.h)[kN];
   }
-  // one system at a time; its block sum is taken before the next one starts
+  // One system at a time, and within it one stencil slot at a time (S, W, E,
+  // N: the CSR order), so that only one slot's divisions are live.  Each
+  // thread reads back only its own column of sq, so no barrier is needed.
 #pragma unroll 1
   for (int s = 0; s < S.count; ++s) {
     double d = 0.;
@@ -504,34 +535,34 @@ __global__ __launch_bounds__(256) void k_assemble(EngineArgs E, const TileRef* b
       const int m = S.m[s];
       const gdouble* hp = G(S.hp_is_un ? J.h : J.hres[m]);
       const double hK = hp[k], hS = hp[kS], hW = hp[kW], hE = hp[kE], hN = hp[kN];
-      double D = 1., mS, mW, mE, mN;
-      patankar_slot(m, dt, qS, false, hK, hS, mS, D);
-      patankar_slot(m, dt, qW, false, hK, hW, mW, D);
-      patankar_slot(m, dt, qE, true, hK, hE, mE, D);
-      patankar_slot(m, dt, qN, true, hK, hN, mN, D);
-      double* const* Ms = J.M[s];
-      G(Ms[0])[k] = mS;
-      G(Ms[1])[k] = mW;
-      G(Ms[2])[k] = mE;
-      G(Ms[3])[k] = mN;
-      G(Ms[4])[k] = D;
+      double D = 1.;
       // sparse_matmul in CSR order S, W, self (value 0), E, N; then
       // sol = (rhs - prod)/diag
       double p = 0.;
-      p = p + mS * bS;
-      p = p + mW * bW;
-      p = p + 0. * b;
-      p = p + mE * bE;
-      p = p + mN * bN;
+#pragma unroll 1
+      for (int sl = 0; sl < 4; ++sl) {
+        // the slot's neighbour values, selected from the ones loaded above
+        const double hL = sl == 0 ? hS : sl == 1 ? hW : sl == 2 ? hE : hN;
+        const double bL = sl == 0 ? bS : sl == 1 ? bW : sl == 2 ? bE : bN;
+        double q[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) q[r] = sq[(sl * 4 + r) * 256 + tid];
+        double M;
+        patankar_slot(m, dt, q, sl >= 2, hK, hL, M, D);
+        G(J.M[s][sl])[k] = M;
+        p = p + M * bL;
+        if (sl == 1) p = p + 0. * b;
+      }
+      G(J.M[s][4])[k] = D;
       double x1 = (b - p) / D;
       G(J.xj[m][1])[k] = x1;
       d = fabs(x1 - b);
     }
     double t = block_sum_all(d, red);
-    if (threadIdx.x == 0) J.part[(size_t)s * J.nblk + T.tile] = t;
+    if (tid == 0) J.part[(size_t)s * J.nblk + T.tile] = t;
   }
   double t = block_sum_all(in ? fabs(0.0 - b) : 0., red);
-  if (threadIdx.x == 0) J.part[(size_t)kMaxSys * J.nblk + T.tile] = t;
+  if (tid == 0) J.part[(size_t)kMaxSys * J.nblk + T.tile] = t;
 }
 
 // One Jacobi sweep of every still-active system (src/jacobi_iteration.f90:72-80).
@@ -551,7 +582,9 @@ __global__ __launch_bounds__(256) void k_sweep(EngineArgs E, const TileRef* bloc
   const int k = T.tile * kCellBlock + threadIdx.x;
   double d[kMaxSys] = {0., 0., 0.};
   if (k < n) {
-    const int i = k % nx, j = k / nx;
+    const int k0 = T.tile * kCellBlock;
+    int i, j;
+    cell_ij(k0 % nx, k0 / nx, threadIdx.x, nx, i, j);
     const int kS = (j == 0 ? ny - 1 : j - 1) * nx + i;
     const int kN = (j == ny - 1 ? 0 : j + 1) * nx + i;
     const int kW = j * nx + (i == 0 ? nx - 1 : i - 1);
@@ -834,6 +867,20 @@ __global__ void k_shared_div(int n, const double* a, const double* b, double* q)
   q[k] = div_by(a[k], b[k], recip_refined(b[k]));
 }
 
+// WENO5 first sweep (out: wm, wp) and second sweep (out: 4 Gauss-point
+// values) of n five-point stencils q[5*i .. 5*i+4], as the kernels compute them.
+__global__ void k_weno5(int n, const double* q, double* first, double* second) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double* s = q + 5 * (size_t)k;
+  double wm, wp, w[4];
+  weno5_first(s[0], s[1], s[2], s[3], s[4], wm, wp);
+  weno5_second(s[0], s[1], s[2], s[3], s[4], w);
+  first[2 * (size_t)k] = wm;
+  first[2 * (size_t)k + 1] = wp;
+  for (int g = 0; g < 4; ++g) second[4 * (size_t)k + g] = w[g];
+}
+
 // ------------------------------------------------------------------------
 // launch wrappers
 // ------------------------------------------------------------------------
@@ -902,6 +949,10 @@ void launch_soa_to_aos(int nx, int ny, const double* h, const double* hu, const 
 void launch_divsqrt(int n, const double* a, const double* b, double* q, double* r,
                     hipStream_t s) {
   hipLaunchKernelGGL(k_divsqrt, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, b, q, r);
+}
+
+void launch_weno5(int n, const double* q, double* first, double* second, hipStream_t s) {
+  hipLaunchKernelGGL(k_weno5, dim3(cdiv(n, 256)), dim3(256), 0, s, n, q, first, second);
 }
 
 void launch_shared_div(int n, const double* a, const double* b, double* q, hipStream_t s) {

@@ -150,59 +150,86 @@ __device__ __forceinline__ void weno5_first(double qm2, double qm1, double q0, d
   }
 }
 
-// One Gauss point of WENO5_SecondSweep, src/reconstruction.f90:1036-1133.
-// The literals are the reference's decimal strings; signed literals are folded
-// into the coefficients (a - c*q == a + (-c)*q exactly), and the 0.0*Q terms
-// are kept because they propagate NaN/Inf exactly as the reference does.
-template <int G, bool SHARED>
-__device__ __forceinline__ double weno5_second_gp(double qm2, double qm1, double q0, double qp1,
-                                                  double qp2, double d1, double d2, double d3,
-                                                  double r1, double r2, double r3) {
+// Coefficients of one Gauss point of WENO5_SecondSweep,
+// src/reconstruction.f90:1036-1133.  The literals are the reference's decimal
+// strings; signed literals are folded into the coefficients (a - c*q ==
+// a + (-c)*q exactly).  c13, c14, c20, c24, c30, c31 are the reference's
+// 0.0*Q terms.
+struct Weno2Coef {
   double g1, g2, g3;
   double c10, c11, c12, c13, c14, c20, c21, c22, c23, c24, c30, c31, c32, c33, c34;
-  if constexpr (G == 0) {
-    g1 = 0.2658420974778319; g2 = 0.6112504900322486; g3 = 0.1229074124899195;
-    c10 = -0.1642562761719537; c11 = 0.7590807081409336; c12 = 0.4051755680310201; c13 = 0.0000000000000000; c14 = 0.0000000000000000;
-    c20 = +0.0000000000000000; c21 = 0.2663118796250726; c22 = 0.8979443965468811; c23 = -0.1642562761719537; c24 = 0.0000000000000000;
-    c30 = +0.0000000000000000; c31 = 0.0000000000000000; c32 = 1.6968800354220990; c33 = -0.9631919150471715; c34 = 0.2663118796250726;
-  } else if constexpr (G == 1) {
-    g1 = 0.1281641584355059; g2 = 0.5219691498503563; g3 = 0.3498666917141379;
-    c10 = -0.1122135388132497; c11 = 0.3944175994189276; c12 = 0.7177959393943222; c13 = 0.0000000000000000; c14 = 0.0000000000000000;
-    c20 = +0.0000000000000000; c21 = 0.0577769829791784; c22 = 1.0544365558340714; c23 = -0.1122135388132497; c24 = 0.0000000000000000;
-    c30 = +0.0000000000000000; c31 = 0.0000000000000000; c32 = 1.2277675047716066; c33 = -0.2855444877507849; c34 = 0.0577769829791784;
-  } else if constexpr (G == 2) {
-    g1 = 0.3498666917141379; g2 = 0.5219691498503563; g3 = 0.1281641584355059;
-    c10 = +0.0577769829791784; c11 = -0.2855444877507849; c12 = 1.2277675047716066; c13 = 0.0000000000000000; c14 = 0.0000000000000000;
-    c20 = +0.0000000000000000; c21 = -0.1122135388132497; c22 = 1.0544365558340714; c23 = 0.0577769829791784; c24 = 0.0000000000000000;
-    c30 = +0.0000000000000000; c31 = 0.0000000000000000; c32 = 0.7177959393943222; c33 = 0.3944175994189276; c34 = -0.1122135388132497;
-  } else {
-    g1 = 0.1229074124899195; g2 = 0.6112504900322486; g3 = 0.2658420974778319;
-    c10 = +0.2663118796250726; c11 = -0.9631919150471715; c12 = 1.6968800354220990; c13 = 0.0000000000000000; c14 = 0.0000000000000000;
-    c20 = +0.0000000000000000; c21 = -0.1642562761719537; c22 = 0.8979443965468811; c23 = 0.2663118796250726; c24 = 0.0000000000000000;
-    c30 = +0.0000000000000000; c31 = 0.0000000000000000; c32 = 0.4051755680310201; c33 = 0.7590807081409336; c34 = -0.1642562761719537;
-  }
-  double o1, o2, o3;
+};
+
+template <int G>
+__device__ constexpr Weno2Coef weno2_coef() {
+  if constexpr (G == 0)
+    return {0.2658420974778319, 0.6112504900322486, 0.1229074124899195,
+            -0.1642562761719537, 0.7590807081409336, 0.4051755680310201, 0.0000000000000000, 0.0000000000000000,
+            +0.0000000000000000, 0.2663118796250726, 0.8979443965468811, -0.1642562761719537, 0.0000000000000000,
+            +0.0000000000000000, 0.0000000000000000, 1.6968800354220990, -0.9631919150471715, 0.2663118796250726};
+  else if constexpr (G == 1)
+    return {0.1281641584355059, 0.5219691498503563, 0.3498666917141379,
+            -0.1122135388132497, 0.3944175994189276, 0.7177959393943222, 0.0000000000000000, 0.0000000000000000,
+            +0.0000000000000000, 0.0577769829791784, 1.0544365558340714, -0.1122135388132497, 0.0000000000000000,
+            +0.0000000000000000, 0.0000000000000000, 1.2277675047716066, -0.2855444877507849, 0.0577769829791784};
+  else if constexpr (G == 2)
+    return {0.3498666917141379, 0.5219691498503563, 0.1281641584355059,
+            +0.0577769829791784, -0.2855444877507849, 1.2277675047716066, 0.0000000000000000, 0.0000000000000000,
+            +0.0000000000000000, -0.1122135388132497, 1.0544365558340714, 0.0577769829791784, 0.0000000000000000,
+            +0.0000000000000000, 0.0000000000000000, 0.7177959393943222, 0.3944175994189276, -0.1122135388132497};
+  else
+    return {0.1229074124899195, 0.6112504900322486, 0.2658420974778319,
+            +0.2663118796250726, -0.9631919150471715, 1.6968800354220990, 0.0000000000000000, 0.0000000000000000,
+            +0.0000000000000000, -0.1642562761719537, 0.8979443965468811, 0.2663118796250726, 0.0000000000000000,
+            +0.0000000000000000, 0.0000000000000000, 0.4051755680310201, 0.7590807081409336, -0.1642562761719537};
+}
+
+// The three candidate values, every term kept: the 0.0*Q terms propagate
+// NaN/Inf and fix the sign of zero exactly as the reference does (the
+// compiler shares the four distinct 0.0*q products between the Gauss points).
+template <int G>
+__device__ __forceinline__ void weno2_cand(double qm2, double qm1, double q0, double qp1,
+                                           double qp2, double& w1, double& w2, double& w3) {
+  constexpr Weno2Coef C = weno2_coef<G>();
+  w1 = C.c10 * qm2 + C.c11 * qm1 + C.c12 * q0 + C.c13 * qp1 + C.c14 * qp2;
+  w2 = C.c20 * qm2 + C.c21 * qm1 + C.c22 * q0 + C.c23 * qp1 + C.c24 * qp2;
+  w3 = C.c30 * qm2 + C.c31 * qm1 + C.c32 * q0 + C.c33 * qp1 + C.c34 * qp2;
+}
+
+// omega_k = alpha_k / sum(alpha), alpha_k = gamma_k / (eps + beta_k)^2
+// (src/reconstruction.f90:1030-1035), through shared reciprocals (SHARED) or '/'.
+template <int G, bool SHARED>
+__device__ __forceinline__ void weno2_weights(double d1, double d2, double d3, double r1,
+                                              double r2, double r3, double& o1, double& o2,
+                                              double& o3) {
+  constexpr Weno2Coef C = weno2_coef<G>();
   if constexpr (SHARED) {
-    double a1 = div_by(g1, d1, r1);
-    double a2 = div_by(g2, d2, r2);
-    double a3 = div_by(g3, d3, r3);
+    double a1 = div_by(C.g1, d1, r1);
+    double a2 = div_by(C.g2, d2, r2);
+    double a3 = div_by(C.g3, d3, r3);
     double s = a1 + a2 + a3;
     double rs = recip_refined(s);
     o1 = div_by(a1, s, rs);
     o2 = div_by(a2, s, rs);
     o3 = div_by(a3, s, rs);
   } else {
-    double a1 = g1 / d1;
-    double a2 = g2 / d2;
-    double a3 = g3 / d3;
+    double a1 = C.g1 / d1;
+    double a2 = C.g2 / d2;
+    double a3 = C.g3 / d3;
     double s = a1 + a2 + a3;
     o1 = a1 / s;
     o2 = a2 / s;
     o3 = a3 / s;
   }
-  double w1 = c10 * qm2 + c11 * qm1 + c12 * q0 + c13 * qp1 + c14 * qp2;
-  double w2 = c20 * qm2 + c21 * qm1 + c22 * q0 + c23 * qp1 + c24 * qp2;
-  double w3 = c30 * qm2 + c31 * qm1 + c32 * q0 + c33 * qp1 + c34 * qp2;
+}
+
+template <int G, bool SHARED>
+__device__ __forceinline__ double weno5_second_gp(double qm2, double qm1, double q0, double qp1,
+                                                  double qp2, double d1, double d2, double d3,
+                                                  double r1, double r2, double r3) {
+  double o1, o2, o3, w1, w2, w3;
+  weno2_weights<G, SHARED>(d1, d2, d3, r1, r2, r3, o1, o2, o3);
+  weno2_cand<G>(qm2, qm1, q0, qp1, qp2, w1, w2, w3);
   return (o1 * w1 + o2 * w2 + o3 * w3);
 }
 

@@ -72,6 +72,7 @@ def lib():
     L.swm_sync.argtypes = [vp]
     L.swm_debug_divsqrt.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
     L.swm_debug_shared_div.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+    L.swm_debug_weno5.argtypes = [ctypes.c_int, _dp, _dp, _dp]
     L.swm_profile_enable.argtypes = [vp, ctypes.c_int]
     L.swm_profile_read.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
     L.swm_profile_name.restype = ctypes.c_char_p
@@ -204,6 +205,17 @@ def debug_divsqrt(a, b):
     if rc != SWM_OK:
         raise SwmError(f"swm_debug_divsqrt rc={rc}")
     return q, r
+
+
+def debug_weno5(q):
+    """Device WENO5 first/second sweeps of stencils q (n, 5) -> (n, 2), (n, 4)."""
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 5)
+    first = np.empty((q.shape[0], 2))
+    second = np.empty((q.shape[0], 4))
+    rc = lib().swm_debug_weno5(q.shape[0], _ptr(q), _ptr(first), _ptr(second))
+    if rc != SWM_OK:
+        raise SwmError(f"swm_debug_weno5 rc={rc}")
+    return first, second
 
 
 def debug_shared_div(a, b):

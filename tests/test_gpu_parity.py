@@ -75,6 +75,45 @@ def test_shared_reciprocal_division_is_ieee_in_range():
     assert same_bits(q, a / b), describe_diff(q, a / b)
 
 
+def _weno_stencils(rng, n):
+    """Five-point stencils covering the kernels' fast paths and every reason
+    to leave them: smooth data, constant and exactly-zero data with both signs
+    of zero (zero partial sums), subnormal and huge values (beta outside the
+    shared-reciprocal range), Inf and NaN."""
+    smooth = 1.0 + 0.1 * np.cumsum(rng.standard_normal((n, 5)) * 1e-2, axis=1)
+    rough = rng.standard_normal((n, 5)) * 10.0 ** rng.uniform(-12, 3, (n, 1))
+    zeros = rng.choice([0.0, -0.0], (n, 5))
+    mixed = np.where(rng.uniform(size=(n, 5)) < 0.5, zeros, rough)
+    special = rng.choice([0.0, -0.0, 1.0, -1.0, 5e-324, -5e-324, 1e-310, 1e150, -1e150, 1e300,
+                          np.inf, -np.inf, np.nan], (n, 5))
+    const = np.repeat(rng.choice([0.0, -0.0, 1.0, -2.5, 1e-200], (n, 1)), 5, axis=1)
+    return np.concatenate([smooth, rough, zeros, mixed, special, const])
+
+
+def test_weno5_device_arithmetic_matches_oracle():
+    """The device WENO5 sweeps (shared reciprocals, zero terms dropped where
+    exact) against the oracle's plain evaluation, bitwise, on adversarial
+    stencils."""
+    from oracle_api import oracle_lib
+    import ctypes
+
+    rng = np.random.default_rng(5)
+    q = _weno_stencils(rng, 4000)
+    first, second = swm.debug_weno5(q)
+    L = oracle_lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    ef = np.empty_like(first)
+    es = np.empty_like(second)
+    wm, wp = ctypes.c_double(), ctypes.c_double()
+    for i in range(q.shape[0]):
+        qi = np.ascontiguousarray(q[i])
+        L.orc_weno5_first(qi.ctypes.data_as(dp), ctypes.byref(wm), ctypes.byref(wp))
+        ef[i] = (wm.value, wp.value)
+        L.orc_weno5_second(qi.ctypes.data_as(dp), es[i].ctypes.data_as(dp))
+    assert same_bits(first, ef), describe_diff(first, ef)
+    assert same_bits(second, es), describe_diff(second, es)
+
+
 @pytest.mark.parametrize("case", sorted(CASES))
 def test_fvtd_parity(case):
     p = CASES[case]()
