@@ -99,6 +99,14 @@ int swm_fvtd(swm_solver_t *h, double *Ut, double *FX, double *FY, double *S, dou
  * src/jacobi_iteration.f90:63-64). */
 int swm_set_jacobi_limits(swm_solver_t *h, int max_iter, double tol);
 
+/* Stage positivity check (no counterpart in the reference): while on, the
+ * device folds the minimum depth of every DeC stage solution (the 13 Jacobi
+ * results of each step) into a per-handle record; NaN if any depth was NaN.
+ * Enabling resets the record.  swm_stage_min_h returns SWM_E_STATE when no
+ * stage has run since. */
+int swm_track_stage_min(swm_solver_t *h, int on);
+int swm_stage_min_h(swm_solver_t *h, double *hmin);
+
 /* Number of Jacobi stop decisions so far that fell inside the rounding window
  * of the parallel residual sum and were settled by the exact sequential sum. */
 long long swm_ambiguous_decisions(const swm_solver_t *h);
@@ -169,6 +177,9 @@ int swm_engine_fetch(swm_engine_t *e, int job_id, double *U);
 /* Frees the job's device memory; its id may be reused. */
 int swm_engine_release(swm_engine_t *e, int job_id);
 int swm_engine_set_jacobi_limits(swm_engine_t *e, int max_iter, double tol);
+/* The stage positivity check of swm_track_stage_min, per job of an engine. */
+int swm_engine_track_stage_min(swm_engine_t *e, int on);
+int swm_engine_stage_min_h(swm_engine_t *e, int job_id, double *hmin);
 int swm_engine_profile_enable(swm_engine_t *e, int on);
 int swm_engine_profile_read(swm_engine_t *e, double ms[SWM_PROF_NCAT],
                             long long launches[SWM_PROF_NCAT]);

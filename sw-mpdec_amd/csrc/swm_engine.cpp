@@ -164,6 +164,7 @@ struct swm_engine {
   int max_iter = 1000;
   double tol = 1.e-16;
   long long ambiguous = 0;
+  bool track_min = false;        // fold every stage depth into JobStatus::hmin
   Profiler prof;
   std::string err;
 };
@@ -320,6 +321,7 @@ int engine_step(swm_engine* e, Tables& T, const double* host_dt, int iters[13]) 
   SysList S{};
   for (int r = 0; r < 4; ++r) S.src[r] = 0;
   S.hp_is_un = 1;
+  S.track_min = e->track_min ? 1 : 0;
   int solve = 0;
   for (int kk = 1; kk <= 5; ++kk) {
     S.count = 0;
@@ -373,7 +375,7 @@ int engine_step(swm_engine* e, Tables& T, const double* host_dt, int iters[13]) 
       const JobStatus& js = e->hstatus[T.jl[0]];
       for (int k = 0; k < S.count; ++k) iters[solve + k] = js.st[k].k;
     }
-    timed(e, 7, [&] { launch_finalize(a, T.dcb.p, T.ncb, S, s); });
+    timed(e, 7, [&] { launch_finalize(a, T.dcb.p, T.ncb, T.djl.p, T.njobs, S, s); });
     LAUNCHCK(e);
     solve += S.count;
     if (kk != 5) {
@@ -764,6 +766,29 @@ int swm_engine_job_info(swm_engine_t* e, int job_id, swm_job_info* info) {
   return SWM_OK;
 }
 
+int swm_engine_track_stage_min(swm_engine_t* e, int on) {
+  if (!e) return SWM_E_ARG;
+  e->track_min = on != 0;
+  for (size_t id = 0; id < e->jobs.size(); ++id)
+    if (e->jobs[id]->state != JOB_FREE)
+      HIPCK(e, hipMemsetAsync(&e->dstatus.p[id].hmin_set, 0, sizeof(int), e->stream));
+  HIPCK(e, hipStreamSynchronize(e->stream));
+  return SWM_OK;
+}
+
+int swm_engine_stage_min_h(swm_engine_t* e, int job_id, double* hmin) {
+  if (!e || !hmin) return SWM_E_ARG;
+  Job* J;
+  int rc = job_from_id(e, job_id, &J);
+  if (rc) return rc;
+  JobStatus st;
+  HIPCK(e, hipStreamSynchronize(e->stream));
+  HIPCK(e, hipMemcpy(&st, e->dstatus.p + job_id, sizeof(JobStatus), hipMemcpyDeviceToHost));
+  if (!st.hmin_set) FAIL(e, SWM_E_STATE, "no DeC stage recorded since tracking was enabled");
+  *hmin = st.hmin;
+  return SWM_OK;
+}
+
 int swm_engine_fetch(swm_engine_t* e, int job_id, double* U) {
   if (!e || !U) return SWM_E_ARG;
   Job* J;
@@ -1044,6 +1069,16 @@ int swm_fvtd(swm_solver_t* s, double* Ut, double* FX, double* FY, double* S, dou
         }
       }
   return SWM_OK;
+}
+
+int swm_track_stage_min(swm_solver_t* s, int on) {
+  if (!s) return SWM_E_ARG;
+  return swm_engine_track_stage_min(s->e, on);
+}
+
+int swm_stage_min_h(swm_solver_t* s, double* hmin) {
+  if (!s) return SWM_E_ARG;
+  return swm_engine_stage_min_h(s->e, s->job, hmin);
 }
 
 int swm_set_jacobi_limits(swm_solver_t* s, int max_iter, double tol) {

@@ -42,7 +42,9 @@ struct JobStatus {
   JacState st[kMaxSys];
   double dt, lmx, lmy;
   int dt_bad;     // dt not finite
-  int pad;
+  int hmin_set;   // 1 once hmin holds a value
+  double hmin;    // minimum depth over every DeC stage solution since the last
+                  // reset (NaN if any was NaN); kept only while tracking is on
 };
 
 struct JobDev {
@@ -104,6 +106,7 @@ struct SysList {
   int m[kMaxSys];       // DeC node of each system
   int src[4];           // buffer index r' holding node r's FU/q (0 at kk = 1, else r)
   int hp_is_un;         // 1: the denominators use h^n (first correction)
+  int track_min;        // 1: fold the new stage depths into JobStatus::hmin
 };
 
 // ---- launches (swm_kernels.hip) -------------------------------------------
@@ -115,8 +118,8 @@ void launch_assemble(const EngineArgs& E, const TileRef* cb, int ncb, const int*
                      const SysList& S, hipStream_t s);
 void launch_sweep(const EngineArgs& E, const TileRef* cb, int ncb, const int* jl, int njobs,
                   const SysList& S, hipStream_t s);
-void launch_finalize(const EngineArgs& E, const TileRef* cb, int ncb, const SysList& S,
-                     hipStream_t s);
+void launch_finalize(const EngineArgs& E, const TileRef* cb, int ncb, const int* jl, int njobs,
+                     const SysList& S, hipStream_t s);
 void launch_timestep(const EngineArgs& E, const TileRef* cb, int ncb, const int* jl, int njobs,
                      hipStream_t s);
 void launch_commit(const EngineArgs& E, const TileRef* cb, int ncb, hipStream_t s);

@@ -683,18 +683,59 @@ __global__ __launch_bounds__(256) void k_decide(EngineArgs E, const int* jobs, S
   }
 }
 
+// min that propagates NaN (the stage check must see a NaN depth)
+__device__ __forceinline__ double nan_min(double a, double b) {
+  return (a != a || b != b) ? __builtin_nan("") : (a < b ? a : b);
+}
+
+__device__ __forceinline__ double block_min_all(double v, double* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = nan_min(v, __shfl_xor(v, off, 64));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double r = nan_min(nan_min(nan_min(red[0], red[1]), red[2]), red[3]);
+  __syncthreads();
+  return r;
+}
+
 // Result of each solve into the node's depth: x[cur], or 0 when no sweep ran
-// (src/jacobi_iteration.f90:68, src/timediscretization.f90:898-903).
+// (src/jacobi_iteration.f90:68, src/timediscretization.f90:898-903).  With
+// stage tracking on, each block also leaves the minimum of the depths it
+// wrote in the (by now unused) Jacobi partial-sum slot 0.
 __global__ __launch_bounds__(256) void k_finalize(EngineArgs E, const TileRef* blocks,
                                                   SysList S) {
+  __shared__ double red[4];
   const TileRef T = blocks[blockIdx.x];
   const JobDev& J = E.jobs[T.job];
   const int k = T.tile * kCellBlock + threadIdx.x;
-  if (k >= J.n) return;
-  for (int s = 0; s < S.count; ++s) {
-    const int m = S.m[s];
-    const JacState& q = J.status->st[s];
-    G(J.hres[m])[k] = q.zero ? 0. : G(J.xj[m][q.cur])[k];
+  double mn = __builtin_inf();
+  if (k < J.n) {
+    for (int s = 0; s < S.count; ++s) {
+      const int m = S.m[s];
+      const JacState& q = J.status->st[s];
+      const double v = q.zero ? 0. : G(J.xj[m][q.cur])[k];
+      G(J.hres[m])[k] = v;
+      mn = nan_min(mn, v);
+    }
+  }
+  if (!S.track_min) return;  // uniform
+  mn = block_min_all(mn, red);
+  if (threadIdx.x == 0) J.part[T.tile] = mn;
+}
+
+// One block per job: fold the finalize blocks' minima into JobStatus::hmin.
+__global__ __launch_bounds__(256) void k_stage_min(EngineArgs E, const int* jobs) {
+  __shared__ double red[4];
+  const JobDev& J = E.jobs[jobs[blockIdx.x]];
+  double mn = __builtin_inf();
+  for (int t = threadIdx.x; t < J.nblk; t += blockDim.x) mn = nan_min(mn, J.part[t]);
+  mn = block_min_all(mn, red);
+  if (threadIdx.x == 0) {
+    JobStatus* st = J.status;
+    st->hmin = st->hmin_set ? nan_min(st->hmin, mn) : mn;
+    st->hmin_set = 1;
   }
 }
 
@@ -914,9 +955,11 @@ void launch_sweep(const EngineArgs& E, const TileRef* cb, int ncb, const int* jl
   hipLaunchKernelGGL(k_decide<false>, dim3(njobs), dim3(256), 0, s, E, jl, S);
 }
 
-void launch_finalize(const EngineArgs& E, const TileRef* cb, int ncb, const SysList& S,
-                     hipStream_t s) {
-  if (ncb) hipLaunchKernelGGL(k_finalize, dim3(ncb), dim3(256), 0, s, E, cb, S);
+void launch_finalize(const EngineArgs& E, const TileRef* cb, int ncb, const int* jl, int njobs,
+                     const SysList& S, hipStream_t s) {
+  if (!ncb) return;
+  hipLaunchKernelGGL(k_finalize, dim3(ncb), dim3(256), 0, s, E, cb, S);
+  if (S.track_min) hipLaunchKernelGGL(k_stage_min, dim3(njobs), dim3(256), 0, s, E, jl);
 }
 
 void launch_timestep(const EngineArgs& E, const TileRef* cb, int ncb, const int* jl, int njobs,

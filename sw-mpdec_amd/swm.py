@@ -72,6 +72,8 @@ def lib():
     L.swm_sync.argtypes = [vp]
     L.swm_debug_divsqrt.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
     L.swm_debug_shared_div.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+    L.swm_track_stage_min.argtypes = [vp, ctypes.c_int]
+    L.swm_stage_min_h.argtypes = [vp, _dp]
     L.swm_debug_weno5.argtypes = [ctypes.c_int, _dp, _dp, _dp]
     L.swm_profile_enable.argtypes = [vp, ctypes.c_int]
     L.swm_profile_read.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
@@ -167,6 +169,15 @@ class Solver:
         nonconv = self._check(lib().swm_run_steps(self._h, steps, _ptr(dts),
                                                   its.ctypes.data_as(_ip)), "swm_run_steps")
         return dts, its.reshape(steps, 13), nonconv
+
+    def track_stage_min(self, on=True):
+        """Record the minimum depth over every DeC stage solution from now on."""
+        self._check(lib().swm_track_stage_min(self._h, int(bool(on))), "swm_track_stage_min")
+
+    def stage_min_h(self):
+        v = ctypes.c_double()
+        self._check(lib().swm_stage_min_h(self._h, ctypes.byref(v)), "swm_stage_min_h")
+        return v.value
 
     def fvtd(self):
         n, m = self.nx, self.ny
@@ -273,6 +284,8 @@ def _engine_lib():
     L.swm_engine_set_jacobi_limits.argtypes = [vp, ctypes.c_int, ctypes.c_double]
     L.swm_engine_profile_enable.argtypes = [vp, ctypes.c_int]
     L.swm_engine_profile_read.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
+    L.swm_engine_track_stage_min.argtypes = [vp, ctypes.c_int]
+    L.swm_engine_stage_min_h.argtypes = [vp, ctypes.c_int, _dp]
     L._engine_bound = True
     return L
 
@@ -370,6 +383,16 @@ class Engine:
     def set_jacobi_limits(self, max_iter, tol):
         self._check(_engine_lib().swm_engine_set_jacobi_limits(self._e, max_iter, tol),
                     "swm_engine_set_jacobi_limits")
+
+    def track_stage_min(self, on=True):
+        self._check(_engine_lib().swm_engine_track_stage_min(self._e, int(bool(on))),
+                    "swm_engine_track_stage_min")
+
+    def stage_min_h(self, job_id):
+        v = ctypes.c_double()
+        self._check(_engine_lib().swm_engine_stage_min_h(self._e, job_id, ctypes.byref(v)),
+                    "swm_engine_stage_min_h")
+        return v.value
 
     def profile(self, on=True):
         self._check(_engine_lib().swm_engine_profile_enable(self._e, int(bool(on))),

@@ -1,0 +1,104 @@
+"""GPU tests at the sizes of the benchmark configurations.
+
+* 64x64 vortex, CFL 0.5, 50 steps (the CPU-runnable case): dt sequence and
+  final state bitwise equal to the oracle, hence relative L1 = 0 <= 1e-12.
+* 2048x2048 well-balanced lake over 8 bumps + 1e-3 perturbation (the headline
+  case), a few steps: depth >= 0 at every DeC stage, finite state, total mass
+  conserved to roundoff; without the perturbation the lake at rest stays at
+  rest to 1e-13.
+* 1024x1024 circular dam break: over a dry bed (h = 0) the first step keeps
+  h >= 0 at every stage and TimeStep then turns non-finite, where the
+  reference's time loop never ends; over a 1e-3 film, 200 steps with h >= 0
+  at every stage, bounded Jacobi counts and conserved mass.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+import problems as P
+import swm
+from oracle_api import Oracle
+from parity_util import describe_diff, rel_l1, same_bits
+
+pytestmark = pytest.mark.gpu
+
+
+def mass(U):
+    return math.fsum(np.asarray(U)[..., 0].ravel().tolist())
+
+
+def test_vortex_64_50_steps_matches_oracle():
+    p = P.vortex(64, 1, steps=50)
+    s = swm.Solver.from_problem(p)
+    dts, iters, nonconv = s.run_steps(50)
+    U = s.download()
+    o = Oracle(p)
+    dts_r = []
+    for _ in range(50):
+        dt, _, _ = o.timestep()
+        o.step(dt)
+        dts_r.append(dt)
+    U_r = o.get_U()
+    assert same_bits(dts, np.array(dts_r))
+    assert same_bits(U, U_r), describe_diff(U, U_r)
+    for v in range(3):
+        assert rel_l1(U, U_r, v) <= 1e-12
+    assert nonconv == 0
+
+
+def test_lake_bumps_2048_stages_positive_and_mass_conserved():
+    p = P.lake_bumps(2048, 2024, steps=4)
+    m0 = mass(p.U0)
+    s = swm.Solver.from_problem(p)
+    del p
+    s.track_stage_min(True)
+    dts, iters, nonconv = s.run_steps(4)
+    hmin = s.stage_min_h()
+    U = s.download()
+    assert np.isfinite(U).all()
+    assert hmin >= 0.0, hmin
+    assert (U[..., 0] >= 0).all()
+    rel = abs(mass(U) - m0) / m0
+    assert rel <= 1e-12, rel
+    assert nonconv == 0 and (iters >= 1).all() and (iters < 1000).all()
+
+
+def test_lake_at_rest_2048_preserved():
+    p = P.lake_bumps(2048, 7, steps=3, amp=0.0)
+    U0 = p.U0.copy()
+    s = swm.Solver.from_problem(p)
+    del p
+    s.run_steps(3)
+    U = s.download()
+    assert np.max(np.abs(U[..., 0] - U0[..., 0])) <= 1e-13
+    assert np.max(np.abs(U[..., 1:])) <= 1e-13
+
+
+def test_dam_break_1024_dry_bed():
+    p = P.dam_break(1024)
+    s = swm.Solver.from_problem(p)
+    s.track_stage_min(True)
+    s.run_steps(1)
+    hmin = s.stage_min_h()
+    assert hmin >= 0.0, hmin
+    assert (s.download()[..., 0] >= 0).all()
+    with pytest.raises(swm.SwmError):
+        s.run_steps(1)  # TimeStep is no longer finite
+
+
+def test_dam_break_1024_thin_film_200_steps():
+    p = P.dam_break(1024, dry=1e-3)
+    m0 = mass(p.U0)
+    s = swm.Solver.from_problem(p)
+    s.track_stage_min(True)
+    dts, iters, nonconv = s.run_steps(200)
+    hmin = s.stage_min_h()
+    U = s.download()
+    assert np.isfinite(U).all()
+    assert hmin >= 0.0, hmin
+    assert nonconv == 0 and iters.max() < 1000
+    rel = abs(mass(U) - m0) / m0
+    assert rel <= 1e-11, rel
