@@ -107,6 +107,10 @@ int swm_set_jacobi_limits(swm_solver_t *h, int max_iter, double tol);
 int swm_track_stage_min(swm_solver_t *h, int on);
 int swm_stage_min_h(swm_solver_t *h, double *hmin);
 
+/* swm_engine_set_execution / swm_engine_execution_stats for a plain handle. */
+int swm_set_execution(swm_solver_t *h, int use_graph, int margin);
+int swm_execution_stats(const swm_solver_t *h, long long stats[6]);
+
 /* Number of Jacobi stop decisions so far that fell inside the rounding window
  * of the parallel residual sum and were settled by the exact sequential sum. */
 long long swm_ambiguous_decisions(const swm_solver_t *h);
@@ -177,6 +181,19 @@ int swm_engine_fetch(swm_engine_t *e, int job_id, double *U);
 /* Frees the job's device memory; its id may be reused. */
 int swm_engine_release(swm_engine_t *e, int job_id);
 int swm_engine_set_jacobi_limits(swm_engine_t *e, int max_iter, double tol);
+/* Step execution (no counterpart in the reference; results are bitwise the
+ * same in every mode).  Once every job has run a step, a step launches each
+ * correction's predicted Jacobi sweeps (last step's counts + margin) without
+ * reading back, replayed from a captured HIP graph when use_graph is 1, and
+ * checks once before committing; a step whose solves needed more sweeps is
+ * redone with read-backs.  -1 lets the built-in autotuner time the
+ * candidates (graph 0/1, margin 1/2) on the first steps and keep the fastest;
+ * margin = -2 turns speculation off (every step reads back).
+ * stats: speculative steps, redone steps, graph launches, graph captures,
+ * graph and margin in use. */
+int swm_engine_set_execution(swm_engine_t *e, int use_graph, int margin);
+int swm_engine_execution_stats(const swm_engine_t *e, long long stats[6]);
+
 /* The stage positivity check of swm_track_stage_min, per job of an engine. */
 int swm_engine_track_stage_min(swm_engine_t *e, int on);
 int swm_engine_stage_min_h(swm_engine_t *e, int job_id, double *hmin);

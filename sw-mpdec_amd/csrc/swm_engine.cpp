@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -143,6 +144,7 @@ struct Job {
   int nonconv = 0;
   int last_iters[13] = {0};
   int pred[13];
+  bool have_pred = false;  // pred holds the sweep counts of an accepted step
   bool wb_ready = false;
   Tables own;          // tables covering this job alone
 };
@@ -159,12 +161,30 @@ struct swm_engine {
   JobStatus* hstatus = nullptr;  // pinned mirror
   size_t hstatus_cap = 0;
   double* hdt = nullptr;         // pinned: host-provided dt of a step
+  int* hzero = nullptr;          // pinned: 0, for clearing dt_bad
   bool dirty = true;             // device descriptors / tables out of date
   Tables all;
   int max_iter = 1000;
   double tol = 1.e-16;
   long long ambiguous = 0;
   bool track_min = false;        // fold every stage depth into JobStatus::hmin
+  // Step execution (results are identical either way; only speed differs):
+  // speculative steps launch each correction's predicted sweeps without host
+  // reads and check once at the end; they may be replayed from a captured
+  // HIP graph.  -1 = chosen by the autotuner.
+  int use_graph = -1, margin = -1;
+  int tuned_graph = 1, tuned_margin = 1;
+  int tune_phase = 0;            // autotuner: candidates timed so far
+  double tune_ms[4] = {0, 0, 0, 0};
+  long long graph_epoch = 0;     // bumped whenever captured arguments change
+  long long spec_steps = 0, redone_steps = 0, graph_launches = 0, graph_captures = 0;
+  struct Captured {
+    long long epoch = -1;
+    int device_dt = 0, track = 0;
+    int sweeps[5] = {0, 0, 0, 0, 0};
+    const void* tables = nullptr;
+    hipGraphExec_t exec = nullptr;
+  } cap;
   Profiler prof;
   std::string err;
 };
@@ -265,6 +285,7 @@ int refresh(swm_engine* e) {
   int rc = upload_tables(e, T);
   if (rc) return rc;
   e->dirty = false;
+  ++e->graph_epoch;  // job descriptors and tables may have moved
   return SWM_OK;
 }
 
@@ -294,20 +315,35 @@ int read_status(swm_engine* e) {
   return SWM_OK;
 }
 
+// ---------------------------------------------------------------------------
 // One mPDeC5 step (src/timediscretization.f90:458-566) of every job in T.
-// host_dt: if not NULL the step uses this dt (written to the jobs' status),
-// otherwise each job's TimeStep (src/equation.f90:791) is computed first.
-int engine_step(swm_engine* e, Tables& T, const double* host_dt, int iters[13]) {
-  if (T.njobs == 0) return SWM_OK;
+//
+// The launch sequence: [TimeStep] -> FVTD(U^n) -> 5 corrections, each
+// momentum -> assembly (+ first sweep) -> Jacobi sweeps -> record -> finalize
+// -> FVTD of the new nodes (corrections 1..4) -> (after the host check)
+// commit.  Each sweep is followed by the device-side stop decision, and a
+// sweep of a stopped system is a no-op, so launching more sweeps than a solve
+// needs changes nothing.
+//
+// Polling mode reads the status back during each correction until every
+// system has stopped.  Speculative mode launches a fixed number of sweeps per
+// correction (the last step's counts plus a margin) and reads the status once,
+// before the commit; if any solve needed more sweeps ("overrun") U^n is still
+// intact and the step is run again in polling mode.  A speculative launch
+// sequence depends only on the job set and the sweep counts, so it is captured
+// once as a HIP graph and replayed.
+// ---------------------------------------------------------------------------
+struct StepPlan {
+  bool device_dt;
+  bool speculative;
+  int sweeps[5];  // speculative: sweeps per correction, counting the fused one
+};
+
+// The step's launches; returns after the last finalize / FVTD (no commit).
+int step_body(swm_engine* e, Tables& T, const StepPlan& P) {
   EngineArgs a = eargs(e);
   hipStream_t s = e->stream;
-  if (host_dt) {
-    *e->hdt = *host_dt;
-    for (int q = 0; q < T.njobs; ++q) {
-      JobStatus* st = e->jobs[T.jl[q]]->dev.status;
-      HIPCK(e, hipMemcpyAsync(&st->dt, e->hdt, sizeof(double), hipMemcpyHostToDevice, s));
-    }
-  } else {
+  if (P.device_dt) {
     timed(e, 6, [&] { launch_timestep(a, T.dcb.p, T.ncb, T.djl.p, T.njobs, s); });
     LAUNCHCK(e);
   }
@@ -332,49 +368,41 @@ int engine_step(swm_engine* e, Tables& T, const double* host_dt, int iters[13]) 
     timed(e, 3, [&] { launch_momentum(a, T.dcb.p, T.ncb, S, s); });
     timed(e, 4, [&] { launch_assemble(a, T.dcb.p, T.ncb, T.djl.p, T.njobs, S, s); });
     LAUNCHCK(e);
-    int pred = 1;
-    for (int q = 0; q < T.njobs; ++q)
-      for (int k = 0; k < S.count; ++k)
-        pred = std::max(pred, e->jobs[T.jl[q]]->pred[solve + k]);
-    int batch = std::max(0, pred - 1);
-    long launched = 1;
-    for (;;) {
-      for (int b = 0; b < batch; ++b)
+    if (P.speculative) {
+      for (int b = 1; b < P.sweeps[kk - 1]; ++b)
         timed(e, 5, [&] { launch_sweep(a, T.dcb.p, T.ncb, T.djl.p, T.njobs, S, s); });
       LAUNCHCK(e);
-      launched += batch;
-      rc = read_status(e);
-      if (rc) return rc;
-      int kmin = e->max_iter;
-      bool any = false;
-      for (int q = 0; q < T.njobs; ++q) {
-        const JobStatus& js = e->hstatus[T.jl[q]];
+    } else {
+      int pred = 1;
+      for (int q = 0; q < T.njobs; ++q)
         for (int k = 0; k < S.count; ++k)
-          if (js.st[k].active) {
-            any = true;
-            kmin = std::min(kmin, js.st[k].k);
-          }
+          pred = std::max(pred, e->jobs[T.jl[q]]->pred[solve + k]);
+      int batch = std::max(0, pred - 1);
+      long launched = 1;
+      for (;;) {
+        for (int b = 0; b < batch; ++b)
+          timed(e, 5, [&] { launch_sweep(a, T.dcb.p, T.ncb, T.djl.p, T.njobs, S, s); });
+        LAUNCHCK(e);
+        launched += batch;
+        rc = read_status(e);
+        if (rc) return rc;
+        int kmin = e->max_iter;
+        bool any = false;
+        for (int q = 0; q < T.njobs; ++q) {
+          const JobStatus& js = e->hstatus[T.jl[q]];
+          for (int k = 0; k < S.count; ++k)
+            if (js.st[k].active) {
+              any = true;
+              kmin = std::min(kmin, js.st[k].k);
+            }
+        }
+        if (!any) break;
+        if (launched > (long)e->max_iter + 2)
+          FAIL(e, SWM_E_STATE, "internal: Jacobi did not stop after %ld sweeps", launched);
+        batch = std::max(2, kmin / 4);
       }
-      if (!any) break;
-      if (launched > (long)e->max_iter + 2)
-        FAIL(e, SWM_E_STATE, "internal: Jacobi did not stop after %ld sweeps", launched);
-      batch = std::max(2, kmin / 4);
     }
-    for (int q = 0; q < T.njobs; ++q) {
-      Job& J = *e->jobs[T.jl[q]];
-      const JobStatus& js = e->hstatus[T.jl[q]];
-      for (int k = 0; k < S.count; ++k) {
-        J.last_iters[solve + k] = js.st[k].k;
-        J.pred[solve + k] = js.st[k].k;
-        J.jacobi_sweeps += js.st[k].k;
-        J.nonconv += js.st[k].nonconv;
-        e->ambiguous += js.st[k].ambiguous;
-      }
-    }
-    if (iters && T.njobs == 1) {
-      const JobStatus& js = e->hstatus[T.jl[0]];
-      for (int k = 0; k < S.count; ++k) iters[solve + k] = js.st[k].k;
-    }
+    timed(e, 7, [&] { launch_solve_done(a, T.djl.p, T.njobs, S, solve, s); });
     timed(e, 7, [&] { launch_finalize(a, T.dcb.p, T.ncb, T.djl.p, T.njobs, S, s); });
     LAUNCHCK(e);
     solve += S.count;
@@ -391,8 +419,148 @@ int engine_step(swm_engine* e, Tables& T, const double* host_dt, int iters[13]) 
       S.hp_is_un = 0;
     }
   }
+  return SWM_OK;
+}
+
+void drop_graph(swm_engine* e) {
+  if (e->cap.exec) (void)hipGraphExecDestroy(e->cap.exec);
+  e->cap = swm_engine::Captured{};
+}
+
+// Speculative body through the captured graph (capturing it when the job set
+// or the sweep counts changed).
+int step_body_graph(swm_engine* e, Tables& T, const StepPlan& P) {
+  auto& c = e->cap;
+  bool same = c.exec && c.epoch == e->graph_epoch && c.device_dt == (int)P.device_dt &&
+              c.track == (int)e->track_min && c.tables == (const void*)&T;
+  for (int i = 0; same && i < 5; ++i) same = c.sweeps[i] == P.sweeps[i];
+  if (!same) {
+    drop_graph(e);
+    HIPCK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    int rc = step_body(e, T, P);
+    hipGraph_t g = nullptr;
+    hipError_t er = hipStreamEndCapture(e->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (er != hipSuccess) FAIL(e, SWM_E_HIP, "graph capture failed: %s", hipGetErrorString(er));
+    hipGraphExec_t ge = nullptr;
+    er = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (er != hipSuccess) FAIL(e, SWM_E_HIP, "graph instantiate failed: %s", hipGetErrorString(er));
+    c.exec = ge;
+    c.epoch = e->graph_epoch;
+    c.device_dt = P.device_dt;
+    c.track = e->track_min;
+    c.tables = &T;
+    for (int i = 0; i < 5; ++i) c.sweeps[i] = P.sweeps[i];
+    ++e->graph_captures;
+  }
+  HIPCK(e, hipGraphLaunch(c.exec, e->stream));
+  ++e->graph_launches;
+  return SWM_OK;
+}
+
+// Autotuner over (graph, margin): the first speculative steps of an engine
+// run the candidates (1,1), (0,1), (1,2), (0,2) one step each and keep the
+// fastest; explicit settings (swm_engine_set_execution) bypass it.
+void tune_choice(swm_engine* e, int& graph, int& margin) {
+  static const int cg[4] = {1, 0, 1, 0}, cm[4] = {1, 1, 2, 2};
+  if (e->tune_phase < 4 && (e->use_graph < 0 || e->margin < 0)) {
+    graph = cg[e->tune_phase];
+    margin = cm[e->tune_phase];
+  } else {
+    graph = e->tuned_graph;
+    margin = e->tuned_margin;
+  }
+  if (e->use_graph >= 0) graph = e->use_graph;
+  if (e->margin >= 0) margin = e->margin;
+}
+
+void tune_record(swm_engine* e, double ms) {
+  if (e->tune_phase >= 4 || (e->use_graph >= 0 && e->margin >= 0)) return;
+  static const int cg[4] = {1, 0, 1, 0}, cm[4] = {1, 1, 2, 2};
+  e->tune_ms[e->tune_phase++] = ms;
+  if (e->tune_phase == 4) {
+    int best = 0;
+    for (int i = 1; i < 4; ++i)
+      if (e->tune_ms[i] < e->tune_ms[best]) best = i;
+    e->tuned_graph = cg[best];
+    e->tuned_margin = cm[best];
+  }
+}
+
+// Runs one step of the jobs in T and leaves their status in e->hstatus.
+// host_dt: the step uses this dt for every job (the plain solver's
+// swm_step_mpdec5); otherwise each job's TimeStep is computed on the device.
+int engine_step(swm_engine* e, Tables& T, const double* host_dt, int iters[13]) {
+  if (T.njobs == 0) return SWM_OK;
+  hipStream_t s = e->stream;
+  if (host_dt) {
+    *e->hdt = *host_dt;
+    for (int q = 0; q < T.njobs; ++q) {
+      JobStatus* st = e->jobs[T.jl[q]]->dev.status;
+      HIPCK(e, hipMemcpyAsync(&st->dt, e->hdt, sizeof(double), hipMemcpyHostToDevice, s));
+      HIPCK(e, hipMemcpyAsync(&st->dt_bad, e->hzero, sizeof(int), hipMemcpyHostToDevice, s));
+    }
+  }
+  StepPlan P{};
+  P.device_dt = host_dt == nullptr;
+  bool spec = e->margin != -2;
+  for (int q = 0; q < T.njobs; ++q) spec = spec && e->jobs[T.jl[q]]->have_pred;
+  bool accepted = false;
+  if (spec) {
+    int graph, margin;
+    tune_choice(e, graph, margin);
+    P.speculative = true;
+    static const int first[5] = {0, 3, 6, 9, 12};
+    for (int kk = 0; kk < 5; ++kk) {
+      int mx = 1;
+      const int ns = kk == 4 ? 1 : 3;
+      for (int q = 0; q < T.njobs; ++q)
+        for (int k = 0; k < ns; ++k) mx = std::max(mx, e->jobs[T.jl[q]]->pred[first[kk] + k]);
+      P.sweeps[kk] = std::min(mx + margin, e->max_iter);
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc = (graph && !e->prof.on) ? step_body_graph(e, T, P) : step_body(e, T, P);
+    if (rc) return rc;
+    rc = read_status(e);
+    if (rc) return rc;
+    const double ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    ++e->spec_steps;
+    accepted = true;
+    for (int q = 0; q < T.njobs; ++q) accepted = accepted && !e->hstatus[T.jl[q]].overrun;
+    if (accepted)
+      tune_record(e, ms);
+    else
+      ++e->redone_steps;
+  }
+  if (!accepted) {
+    P.speculative = false;
+    int rc = step_body(e, T, P);
+    if (rc) return rc;
+    rc = read_status(e);
+    if (rc) return rc;
+  }
+  EngineArgs a = eargs(e);
   timed(e, 7, [&] { launch_commit(a, T.dcb.p, T.ncb, s); });
   LAUNCHCK(e);
+  for (int q = 0; q < T.njobs; ++q) {
+    Job& J = *e->jobs[T.jl[q]];
+    const JobStatus& js = e->hstatus[T.jl[q]];
+    for (int i = 0; i < 13; ++i) {
+      J.last_iters[i] = js.iters[i];
+      J.pred[i] = js.iters[i];
+      J.jacobi_sweeps += js.iters[i];
+    }
+    J.nonconv += js.nonconv;
+    e->ambiguous += js.ambiguous;
+    J.have_pred = true;
+  }
+  if (iters && T.njobs == 1)
+    for (int i = 0; i < 13; ++i) iters[i] = e->hstatus[T.jl[0]].iters[i];
   return SWM_OK;
 }
 
@@ -570,6 +738,7 @@ int alloc_job(swm_engine* e, int nx, int ny, double dx, double dy, double g, dou
   for (int i = 0; i < 13; ++i) J.pred[i] = 4;
   e->dirty = true;
   add_tiles(J.own, id, D);
+  ++e->graph_epoch;
   int rc = upload_tables(e, J.own);
   if (rc) return rc;
   HIPCK(e, hipMemsetAsync(e->dstatus.p + id, 0, sizeof(JobStatus), e->stream));
@@ -671,6 +840,12 @@ int swm_engine_create(long long budget_bytes, swm_engine_t** out) {
     g_error = "hipHostMalloc failed";
     rc = SWM_E_HIP;
   }
+  if (rc == SWM_OK && hipHostMalloc((void**)&e->hzero, sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    e->hzero = nullptr;
+    g_error = "hipHostMalloc failed";
+    rc = SWM_E_HIP;
+  }
+  if (rc == SWM_OK) *e->hzero = 0;
   if (rc) {
     swm_engine_destroy(e);
     return rc;
@@ -682,10 +857,12 @@ int swm_engine_create(long long budget_bytes, swm_engine_t** out) {
 void swm_engine_destroy(swm_engine_t* e) {
   if (!e) return;
   if (e->stream) (void)hipStreamSynchronize(e->stream);
+  drop_graph(e);
   for (size_t i = 0; i < e->jobs.size(); ++i) engine_free_job(e, (int)i);
   e->jobs.clear();
   if (e->hstatus) (void)hipHostFree(e->hstatus);
   if (e->hdt) (void)hipHostFree(e->hdt);
+  if (e->hzero) (void)hipHostFree(e->hzero);
   if (e->dstatus.p) (void)hipFree(e->dstatus.p);
   e->dstatus.p = nullptr;
   if (e->djobs.p) (void)hipFree(e->djobs.p);
@@ -728,19 +905,19 @@ int swm_engine_step(swm_engine_t* e, int* n_running) {
   rc = engine_step(e, e->all, nullptr, nullptr);
   if (rc) return rc;
   HIPCK(e, hipStreamSynchronize(e->stream));
-  rc = read_status(e);
-  if (rc) return rc;
   int running = 0;
   for (int q = 0; q < e->all.njobs; ++q) {
     const int id = e->all.jl[q];
     Job& J = *e->jobs[id];
     const JobStatus& js = e->hstatus[id];
-    J.steps_done += 1;
-    J.t += js.dt;
-    if (js.dt_bad) {
+    if (js.dt_bad) {  // the step was not committed: the job keeps its last state
       J.state = JOB_FAILED;
       e->dirty = true;
-    } else if (J.steps_left > 0 && --J.steps_left == 0) {
+      continue;
+    }
+    J.steps_done += 1;
+    J.t += js.dt;
+    if (J.steps_left > 0 && --J.steps_left == 0) {
       J.state = JOB_DONE;
       e->dirty = true;
     } else {
@@ -821,6 +998,26 @@ int swm_engine_set_jacobi_limits(swm_engine_t* e, int max_iter, double tol) {
   if (max_iter < 0) FAIL(e, SWM_E_ARG, "max_iter < 0");
   e->max_iter = max_iter;
   e->tol = tol;
+  ++e->graph_epoch;  // captured launches carry the limits
+  return SWM_OK;
+}
+
+int swm_engine_set_execution(swm_engine_t* e, int use_graph, int margin) {
+  if (!e) return SWM_E_ARG;
+  if (margin > 100 || margin < -2) FAIL(e, SWM_E_ARG, "margin must be in [-2, 100]");
+  e->use_graph = use_graph < 0 ? -1 : (use_graph != 0);
+  e->margin = margin;
+  return SWM_OK;
+}
+
+int swm_engine_execution_stats(const swm_engine_t* e, long long out[6]) {
+  if (!e || !out) return SWM_E_ARG;
+  out[0] = e->spec_steps;
+  out[1] = e->redone_steps;
+  out[2] = e->graph_launches;
+  out[3] = e->graph_captures;
+  out[4] = e->use_graph >= 0 ? e->use_graph : e->tuned_graph;
+  out[5] = e->margin != -1 ? e->margin : e->tuned_margin;
   return SWM_OK;
 }
 
@@ -967,17 +1164,17 @@ int swm_run_steps(swm_solver_t* s, int nsteps, double* dts, int* jacobi_iters) {
   if (rc) return rc;
   const int before = J.nonconv;
   for (int k = 0; k < nsteps; ++k) {
-    // dt from TimeStep on the device; the first Jacobi status read of the step
-    // returns it to the host, so a non-finite dt stops the loop one step late
-    // at most -- check it before the step instead
-    double dt;
-    rc = swm_timestep(s, &dt, nullptr, nullptr);
-    if (rc) return rc;
-    if (dts) dts[k] = dt;
-    if (!std::isfinite(dt)) FAIL(e, SWM_E_STATE, "time step %d: dt is not finite (%g)", k, dt);
+    // dt from TimeStep on the device, within the step; a step whose dt is
+    // not finite is not committed, so the state stays that of step k
     int iters[13] = {0};
-    rc = engine_step(e, J.own, &dt, iters);
+    rc = engine_step(e, J.own, nullptr, iters);
     if (rc) return rc;
+    const JobStatus& js = e->hstatus[s->job];
+    if (dts) dts[k] = js.dt;
+    if (js.dt_bad) {
+      HIPCK(e, hipStreamSynchronize(e->stream));
+      FAIL(e, SWM_E_STATE, "time step %d: dt is not finite (%g)", k, js.dt);
+    }
     if (jacobi_iters)
       for (int i = 0; i < 13; ++i) jacobi_iters[13 * k + i] = iters[i];
     J.steps_done += 1;
@@ -1079,6 +1276,16 @@ int swm_track_stage_min(swm_solver_t* s, int on) {
 int swm_stage_min_h(swm_solver_t* s, double* hmin) {
   if (!s) return SWM_E_ARG;
   return swm_engine_stage_min_h(s->e, s->job, hmin);
+}
+
+int swm_set_execution(swm_solver_t* s, int use_graph, int margin) {
+  if (!s) return SWM_E_ARG;
+  return swm_engine_set_execution(s->e, use_graph, margin);
+}
+
+int swm_execution_stats(const swm_solver_t* s, long long stats[6]) {
+  if (!s) return SWM_E_ARG;
+  return swm_engine_execution_stats(s->e, stats);
 }
 
 int swm_set_jacobi_limits(swm_solver_t* s, int max_iter, double tol) {

@@ -45,6 +45,13 @@ struct JobStatus {
   int hmin_set;   // 1 once hmin holds a value
   double hmin;    // minimum depth over every DeC stage solution since the last
                   // reset (NaN if any was NaN); kept only while tracking is on
+  // the current step's solves, recorded after each correction's Jacobi phase
+  int iters[13];  // sweeps of each solve, in execution order
+  int nonconv;    // solves stopped at the iteration limit
+  int ambiguous;  // stop decisions settled by the exact sequential sum
+  int overrun;    // a solve was still running when its phase ended (the
+                  // step ran with too few sweeps launched and must be redone)
+  int pad2;
 };
 
 struct JobDev {
@@ -123,6 +130,8 @@ void launch_finalize(const EngineArgs& E, const TileRef* cb, int ncb, const int*
 void launch_timestep(const EngineArgs& E, const TileRef* cb, int ncb, const int* jl, int njobs,
                      hipStream_t s);
 void launch_commit(const EngineArgs& E, const TileRef* cb, int ncb, hipStream_t s);
+void launch_solve_done(const EngineArgs& E, const int* jl, int njobs, const SysList& S,
+                       int first_solve, hipStream_t s);
 void launch_copy_wb(const EngineArgs& E, const TileRef* cb, int ncb, int slot, hipStream_t s);
 void launch_aos_to_soa(int nx, int ny, int ghost, const double* aos, double* h, double* hu,
                        double* hv, hipStream_t s);

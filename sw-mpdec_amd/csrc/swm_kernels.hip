@@ -739,12 +739,33 @@ __global__ __launch_bounds__(256) void k_stage_min(EngineArgs E, const int* jobs
   }
 }
 
-// U^{n+1} = Ua(4) (src/timediscretization.f90:564)
+// After a correction's Jacobi phase, one thread per job records the solves
+// (sweep counts, non-converged and ambiguous stops) and whether any system was
+// still running, i.e. needed more sweeps than were launched.
+__global__ void k_solve_done(EngineArgs E, const int* jobs, SysList S, int first_solve) {
+  if (threadIdx.x != 0) return;
+  JobStatus* st = E.jobs[jobs[blockIdx.x]].status;
+  if (first_solve == 0) {
+    st->nonconv = 0;
+    st->ambiguous = 0;
+    st->overrun = 0;
+  }
+  for (int s = 0; s < S.count; ++s) {
+    const JacState& q = st->st[s];
+    st->iters[first_solve + s] = q.k;
+    st->nonconv += q.nonconv;
+    st->ambiguous += q.ambiguous;
+    st->overrun |= q.active;
+  }
+}
+
+// U^{n+1} = Ua(4) (src/timediscretization.f90:564); a job whose dt was not
+// finite keeps U^n.
 __global__ __launch_bounds__(256) void k_commit(EngineArgs E, const TileRef* blocks) {
   const TileRef T = blocks[blockIdx.x];
   const JobDev& J = E.jobs[T.job];
   const int k = T.tile * kCellBlock + threadIdx.x;
-  if (k >= J.n) return;
+  if (k >= J.n || J.status->dt_bad) return;
   G(J.h)[k] = G(J.hres[3])[k];
   G(J.hu)[k] = G(J.hun[3])[k];
   G(J.hv)[k] = G(J.hvn[3])[k];
@@ -971,6 +992,11 @@ void launch_timestep(const EngineArgs& E, const TileRef* cb, int ncb, const int*
 
 void launch_commit(const EngineArgs& E, const TileRef* cb, int ncb, hipStream_t s) {
   if (ncb) hipLaunchKernelGGL(k_commit, dim3(ncb), dim3(256), 0, s, E, cb);
+}
+
+void launch_solve_done(const EngineArgs& E, const int* jl, int njobs, const SysList& S,
+                       int first_solve, hipStream_t s) {
+  if (njobs) hipLaunchKernelGGL(k_solve_done, dim3(njobs), dim3(64), 0, s, E, jl, S, first_solve);
 }
 
 void launch_copy_wb(const EngineArgs& E, const TileRef* cb, int ncb, int slot, hipStream_t s) {

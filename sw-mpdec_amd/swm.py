@@ -18,6 +18,8 @@ PKG = Path(__file__).resolve().parent
 LIB_PATH = Path(os.environ.get("SWM_LIBRARY") or PKG / "lib" / "libswm.so")
 
 _dp = ctypes.POINTER(ctypes.c_double)
+EXEC_STATS = ("speculative_steps", "redone_steps", "graph_launches", "graph_captures", "graph",
+              "margin")
 _ip = ctypes.POINTER(ctypes.c_int)
 
 SWM_OK = 0
@@ -73,6 +75,8 @@ def lib():
     L.swm_debug_divsqrt.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
     L.swm_debug_shared_div.argtypes = [ctypes.c_int, _dp, _dp, _dp]
     L.swm_track_stage_min.argtypes = [vp, ctypes.c_int]
+    L.swm_set_execution.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.swm_execution_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
     L.swm_stage_min_h.argtypes = [vp, _dp]
     L.swm_debug_weno5.argtypes = [ctypes.c_int, _dp, _dp, _dp]
     L.swm_profile_enable.argtypes = [vp, ctypes.c_int]
@@ -169,6 +173,17 @@ class Solver:
         nonconv = self._check(lib().swm_run_steps(self._h, steps, _ptr(dts),
                                                   its.ctypes.data_as(_ip)), "swm_run_steps")
         return dts, its.reshape(steps, 13), nonconv
+
+    def set_execution(self, use_graph=-1, margin=-1):
+        """Step execution mode (results are identical in every mode): use_graph
+        0/1, margin >= 0 extra speculative sweeps; -1 = autotuned; margin -2 =
+        no speculation."""
+        self._check(lib().swm_set_execution(self._h, use_graph, margin), "swm_set_execution")
+
+    def execution_stats(self):
+        out = (ctypes.c_longlong * 6)()
+        self._check(lib().swm_execution_stats(self._h, out), "swm_execution_stats")
+        return dict(zip(EXEC_STATS, list(out)))
 
     def track_stage_min(self, on=True):
         """Record the minimum depth over every DeC stage solution from now on."""
@@ -285,6 +300,8 @@ def _engine_lib():
     L.swm_engine_profile_enable.argtypes = [vp, ctypes.c_int]
     L.swm_engine_profile_read.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
     L.swm_engine_track_stage_min.argtypes = [vp, ctypes.c_int]
+    L.swm_engine_set_execution.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.swm_engine_execution_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
     L.swm_engine_stage_min_h.argtypes = [vp, ctypes.c_int, _dp]
     L._engine_bound = True
     return L
@@ -383,6 +400,16 @@ class Engine:
     def set_jacobi_limits(self, max_iter, tol):
         self._check(_engine_lib().swm_engine_set_jacobi_limits(self._e, max_iter, tol),
                     "swm_engine_set_jacobi_limits")
+
+    def set_execution(self, use_graph=-1, margin=-1):
+        self._check(_engine_lib().swm_engine_set_execution(self._e, use_graph, margin),
+                    "swm_engine_set_execution")
+
+    def execution_stats(self):
+        out = (ctypes.c_longlong * 6)()
+        self._check(_engine_lib().swm_engine_execution_stats(self._e, out),
+                    "swm_engine_execution_stats")
+        return dict(zip(EXEC_STATS, list(out)))
 
     def track_stage_min(self, on=True):
         self._check(_engine_lib().swm_engine_track_stage_min(self._e, int(bool(on))),
