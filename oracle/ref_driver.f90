@@ -19,10 +19,10 @@
 !
 ! Usage: swref <input.bin> <output.bin>
 ! Input (stream, little endian):
-!   int32 mode, nx, nsteps, bathy_flag, has_wb, reserved
-!   real64 x0, x1, gravity, cfl
-!   real64 U(3,nx,nx)              [mode 1-3]   or  Mass(5nx^2), Diag(nx^2), rhs(nx^2) [mode 4]
-!   real64 Uwb(3,nx,nx)            [if has_wb]
+!   int32 mode, nx, nsteps, bathy_flag, has_wb, ny (0: ny = nx)
+!   real64 x0, x1, gravity, cfl    (the domain is [x0,x1]^2; dy = dx*nx/ny)
+!   real64 U(3,nx,ny)              [mode 1-3]   or  Mass(5nx*ny), Diag(nx*ny), rhs(nx*ny) [mode 4]
+!   real64 Uwb(3,nx,ny)            [if has_wb]
 !===============================================================================!
 PROGRAM swref
 USE MOD_FiniteVolume2D_vars
@@ -33,7 +33,7 @@ USE MOD_Equation,           ONLY: TimeStep
 USE MOD_TimeDiscretization, ONLY: TimeDiscretization
 USE MOD_JacobiIteration,    ONLY: jacobi
 IMPLICIT NONE
-INTEGER            :: mode, nx, nsteps, bflag, haswb, reserved, k, nrow
+INTEGER            :: mode, nx, ny, nsteps, bflag, haswb, k, nrow
 INTEGER(KIND=8)    :: c0, c1, crate
 REAL               :: x0, x1, grav, cflin, dtmin
 REAL, ALLOCATABLE  :: Uic(:,:,:), Uwb(:,:,:), Mass(:), Diag(:), rhs(:), sol(:)
@@ -43,7 +43,8 @@ CALL get_command_argument(1, fin)
 CALL get_command_argument(2, fout)
 
 OPEN(10, FILE=TRIM(fin), ACCESS='stream', FORM='unformatted', STATUS='old')
-READ(10) mode, nx, nsteps, bflag, haswb, reserved
+READ(10) mode, nx, nsteps, bflag, haswb, ny
+IF (ny .LE. 0) ny = nx
 READ(10) x0, x1, grav, cflin
 
 #ifdef WELLBALANCED
@@ -60,7 +61,7 @@ END IF
 
 ! Module globals as InitializeParameters (src/parameters.f90:24-171) sets them.
 nElemsX = nx
-nElemsY = nx
+nElemsY = ny
 MESH_X0 = (/x0, x0/)
 MESH_X1 = (/x1, x1/)
 Gravity = grav
@@ -80,7 +81,7 @@ CALL BuildMesh()
 OPEN(11, FILE=TRIM(fout), ACCESS='stream', FORM='unformatted', STATUS='replace')
 
 IF (mode .EQ. 4) THEN
-  nrow = nx*nx
+  nrow = nx*ny
   ALLOCATE(Mass(5*nrow), Diag(nrow), rhs(nrow), sol(nrow))
   READ(10) Mass, Diag, rhs
   sol = 0.
@@ -91,17 +92,17 @@ IF (mode .EQ. 4) THEN
   STOP
 END IF
 
-ALLOCATE(Uic(3,nx,nx))
+ALLOCATE(Uic(3,nx,ny))
 READ(10) Uic
 IF (haswb .EQ. 1) THEN
-  ALLOCATE(Uwb(3,nx,nx))
+  ALLOCATE(Uwb(3,nx,ny))
   READ(10) Uwb
 END IF
 CLOSE(10)
 
 #ifdef WELLBALANCED
 U = 0.
-U(1:3,1:nx,1:nx) = Uwb
+U(1:3,1:nx,1:ny) = Uwb
 CALL FVTimeDerivative(0.)
 UtWB = Ut
 FXWB = FX
@@ -110,7 +111,7 @@ SWB  = S
 #endif
 
 U = 0.
-U(1:3,1:nx,1:nx) = Uic
+U(1:3,1:nx,1:ny) = Uic
 
 SELECT CASE (mode)
   CASE (1)
@@ -126,7 +127,7 @@ SELECT CASE (mode)
       IF (.NOT. (dtmin .LT. HUGE(1.0))) EXIT
       tEnd = dtmin
       CALL TimeDiscretization()
-      WRITE(11) U(1:3,1:nx,1:nx)
+      WRITE(11) U(1:3,1:nx,1:ny)
     END DO
   CASE (3)
     dtmin = TimeStep()
@@ -141,7 +142,7 @@ SELECT CASE (mode)
     END DO
     CALL SYSTEM_CLOCK(c1)
     WRITE(11) REAL(c1 - c0) / REAL(crate), REAL(k - 1)
-    WRITE(11) U(1:3,1:nx,1:nx)
+    WRITE(11) U(1:3,1:nx,1:ny)
   CASE DEFAULT
     WRITE(*,*) "swref: unknown mode"
     STOP 2

@@ -55,18 +55,18 @@ class Mesh:
         self.gp = self.bary[:, None] + nodes[None, :] * self.dx
 
 
-def cell_average(mesh: Mesh, f) -> np.ndarray:
-    """16-point Gauss cell averages of f(x, y) -> (..., ) arrays, accumulated
-    in the order of FillInitialConditions (iGP outer, jGP inner)."""
-    n = mesh.n
+def cell_average(mesh: Mesh, f, mesh_y: Mesh | None = None) -> np.ndarray:
+    """16-point Gauss cell averages of f(x, y) -> (ny, nx, 3), accumulated in
+    the order of FillInitialConditions (iGP outer, jGP inner).  mesh is the x
+    axis, mesh_y the y axis (default: the same)."""
+    my = mesh_y or mesh
     out = None
-    xs = mesh.gp  # (n, 4)
     for ig in range(4):
         for jg in range(4):
             # x varies along axis 1 (i), y along axis 0 (j); f broadcasts
-            X = xs[None, :, ig]
-            Y = xs[:, jg][:, None]
-            val = np.broadcast_to(f(X, Y), (n, n, 3))
+            X = mesh.gp[None, :, ig]
+            Y = my.gp[:, jg][:, None]
+            val = np.broadcast_to(f(X, Y), (my.n, mesh.n, 3))
             w = mesh.wbnd[ig] * mesh.wbnd[jg]
             out = w * val if out is None else out + w * val
     return out
@@ -121,36 +121,36 @@ class GaussBumps:
         return bx, by
 
 
-def bathy_tables(mesh: Mesh, grad) -> tuple[np.ndarray, np.ndarray]:
-    """d b/dx, d b/dy at the 4x4 Gauss points of every cell, planes yg*4+xg.
-    grad(x, y) -> (bx, by) evaluated on broadcast coordinate arrays."""
-    n = mesh.n
-    bx = np.empty((16, n, n))
-    by = np.empty((16, n, n))
+def bathy_tables(mesh: Mesh, grad, mesh_y: Mesh | None = None) -> tuple[np.ndarray, np.ndarray]:
+    """d b/dx, d b/dy at the 4x4 Gauss points of every cell, planes yg*4+xg,
+    (16, ny, nx).  grad(x, y) -> (bx, by) on broadcast coordinate arrays."""
+    my = mesh_y or mesh
+    bx = np.empty((16, my.n, mesh.n))
+    by = np.empty((16, my.n, mesh.n))
     for yg in range(4):
         for xg in range(4):
             X = mesh.gp[None, :, xg]
-            Y = mesh.gp[:, yg][:, None]
+            Y = my.gp[:, yg][:, None]
             gx, gy = grad(X, Y)
-            bx[yg * 4 + xg] = np.broadcast_to(gx, (n, n))
-            by[yg * 4 + xg] = np.broadcast_to(gy, (n, n))
+            bx[yg * 4 + xg] = np.broadcast_to(gx, (my.n, mesh.n))
+            by[yg * 4 + xg] = np.broadcast_to(gy, (my.n, mesh.n))
     return bx, by
 
 
-def bathy_tables_ref_flag1(mesh: Mesh) -> tuple[np.ndarray, np.ndarray]:
+def bathy_tables_ref_flag1(mesh: Mesh, mesh_y: Mesh | None = None) -> tuple[np.ndarray, np.ndarray]:
     """Tables of the reference's own BathymetryFlag=1 (src/equation.f90:461-494),
     0.1*2*PI*COS(2PI x)*COS(2PI y) and -0.1*2*PI*SIN(2PI x)*SIN(2PI y), with
     libm cos/sin and the reference's operation order, point by point."""
-    n = mesh.n
+    my = mesh_y or mesh
     pi = math.acos(-1.0)
-    bx = np.empty((16, n, n))
-    by = np.empty((16, n, n))
+    bx = np.empty((16, my.n, mesh.n))
+    by = np.empty((16, my.n, mesh.n))
     for yg in range(4):
         for xg in range(4):
-            for j in range(n):
-                y = float(mesh.gp[j, yg])
+            for j in range(my.n):
+                y = float(my.gp[j, yg])
                 cy, sy = math.cos(2. * pi * y), math.sin(2. * pi * y)
-                for i in range(n):
+                for i in range(mesh.n):
                     x = float(mesh.gp[i, xg])
                     bx[yg * 4 + xg, j, i] = 0.1 * 2. * pi * math.cos(2. * pi * x) * cy
                     by[yg * 4 + xg, j, i] = -(0.1 * 2. * pi * math.sin(2. * pi * x) * sy)
@@ -170,12 +170,13 @@ def cons(h, u, v):
 
 @dataclass
 class Problem:
-    """One solve: initial state, optional bathymetry and well-balanced state."""
+    """One solve: initial state, optional bathymetry and well-balanced state,
+    on nx = n by ny = rows cells of the periodic square [x0, x1]^2."""
     n: int
-    U0: np.ndarray                  # (n, n, 3)
+    U0: np.ndarray                  # (rows, n, 3)
     wellbalanced: bool = False
-    Uwb: np.ndarray | None = None   # (n, n, 3)
-    dbdx: np.ndarray | None = None  # (16, n, n)
+    Uwb: np.ndarray | None = None   # (rows, n, 3)
+    dbdx: np.ndarray | None = None  # (16, rows, n)
     dbdy: np.ndarray | None = None
     gravity: float = GRAVITY
     cfl: float = 0.5
@@ -183,19 +184,30 @@ class Problem:
     kind: str = ""
     x0: float = 0.0
     x1: float = 1.0
+    ny: int = 0                     # 0: square (ny = n)
+
+    @property
+    def rows(self) -> int:
+        return self.ny or self.n
 
     @property
     def mesh(self) -> Mesh:
+        """x axis"""
         return Mesh(self.n, self.x0, self.x1)
 
+    @property
+    def mesh_y(self) -> Mesh:
+        return Mesh(self.rows, self.x0, self.x1)
 
-def vortex(n: int, seed: int, steps: int = 50, cfl: float = 0.5) -> Problem:
+
+def vortex(n: int, seed: int, steps: int = 50, cfl: float = 0.5, ny: int = 0) -> Problem:
     """Smooth vortex on the periodic unit square, flat bottom:
     h = 1 - eps^2/(2g) exp(1 - r^2), (u, v) = eps exp((1 - r^2)/2) (-y', x'),
-    x' = (x - 1/2)/0.1, eps ~ U[0.5, 1.5] from the seed."""
+    x' = (x - 1/2)/0.1, eps ~ U[0.5, 1.5] from the seed; n x ny cells."""
     rng = np.random.default_rng(seed)
     eps = rng.uniform(0.5, 1.5)
     mesh = Mesh(n)
+    mesh_y = Mesh(ny or n)
     g = GRAVITY
 
     def h_u_v(X, Y):
@@ -206,8 +218,8 @@ def vortex(n: int, seed: int, steps: int = 50, cfl: float = 0.5) -> Problem:
         s = eps * np.exp(0.5 * (1.0 - r2))
         return h, -s * yp, s * xp
 
-    U0 = cell_average(mesh, lambda X, Y: cons(*h_u_v(X, Y)))
-    return Problem(n, U0, cfl=cfl, steps=steps, kind="vortex")
+    U0 = cell_average(mesh, lambda X, Y: cons(*h_u_v(X, Y)), mesh_y)
+    return Problem(n, U0, cfl=cfl, steps=steps, kind="vortex", ny=ny)
 
 
 def lake_bumps(n: int, seed: int, steps: int = 100, cfl: float = 0.5,
@@ -255,11 +267,12 @@ def dam_break(n: int, steps: int = 200, cfl: float = 0.5, radius: float = 0.25,
     return Problem(n, U0, cfl=cfl, steps=steps, kind="dam_break")
 
 
-def lake_flag1(n: int, wellbalanced: bool = True, pert: float = 1e-3) -> Problem:
+def lake_flag1(n: int, wellbalanced: bool = True, pert: float = 1e-3, ny: int = 0) -> Problem:
     """The reference's own lake at rest over BathymetryFlag=1 bathymetry,
     b = 0.1 sin(2 pi x) cos(2 pi y) (src/equation.f90:135-140, :429-430), plus
-    a Gaussian surface perturbation of amplitude pert."""
+    a Gaussian surface perturbation of amplitude pert; n x ny cells."""
     m = Mesh(n)
+    my = Mesh(ny or n)
     pi = math.acos(-1.0)
 
     def st(X, Y, base):
@@ -270,10 +283,10 @@ def lake_flag1(n: int, wellbalanced: bool = True, pert: float = 1e-3) -> Problem
         z = np.zeros_like(h)
         return cons(h, z, z)
 
-    bx, by = bathy_tables_ref_flag1(m)
-    return Problem(n, cell_average(m, lambda X, Y: st(X, Y, False)), wellbalanced=wellbalanced,
-                   Uwb=cell_average(m, lambda X, Y: st(X, Y, True)) if wellbalanced else None,
-                   dbdx=bx, dbdy=by, kind="lake_flag1")
+    bx, by = bathy_tables_ref_flag1(m, my)
+    return Problem(n, cell_average(m, lambda X, Y: st(X, Y, False), my), wellbalanced=wellbalanced,
+                   Uwb=cell_average(m, lambda X, Y: st(X, Y, True), my) if wellbalanced else None,
+                   dbdx=bx, dbdy=by, kind="lake_flag1", ny=ny)
 
 
 def mixed_batch(count: int, seed: int, nmin: int = 32, nmax: int = 512,

@@ -107,7 +107,7 @@ class Solver:
     @classmethod
     def from_problem(cls, p):
         m = p.mesh
-        s = cls(p.n, p.n, m.dx, m.dx, p.gravity, p.cfl, p.wellbalanced, m.wbnd)
+        s = cls(p.n, p.rows, m.dx, p.mesh_y.dx, p.gravity, p.cfl, p.wellbalanced, m.wbnd)
         if p.dbdx is not None:
             s.set_bathy_grad(p.dbdx, p.dbdy)
         if p.wellbalanced:
@@ -348,8 +348,8 @@ class Engine:
         m = p.mesh
         arrs = [np.ascontiguousarray(p.U0, dtype=np.float64)]
         d = JobDesc()
-        d.nx = d.ny = p.n
-        d.dx = d.dy = m.dx
+        d.nx, d.ny = p.n, p.rows
+        d.dx, d.dy = m.dx, p.mesh_y.dx
         d.gravity = p.gravity
         d.cfl = p.cfl
         d.wellbalanced = int(bool(p.wellbalanced))
@@ -370,7 +370,7 @@ class Engine:
         if rc == SWM_E_BUDGET:
             return None
         self._check(rc, "swm_engine_submit")
-        self._keep[jid.value] = p.n
+        self._keep[jid.value] = (p.rows, p.n)
         return jid.value
 
     def step(self):
@@ -385,8 +385,8 @@ class Engine:
         return {k: getattr(i, k) for k, _ in JobInfo._fields_}
 
     def fetch(self, jid):
-        n = self._keep[jid]
-        U = np.empty((n, n, 3))
+        rows, n = self._keep[jid]
+        U = np.empty((rows, n, 3))
         self._check(_engine_lib().swm_engine_fetch(self._e, jid, _ptr(U)), "swm_engine_fetch")
         return U
 

@@ -6,7 +6,7 @@ oracle/Makefile into oracle/_ref/ and driven by oracle/ref_driver.f90, on
 small seeded inputs.  Each .npz holds the inputs and the reference outputs
 (data only).  Run from the repository root after __graft_entry__.build():
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]      # default: every case
 """
 from __future__ import annotations
 
@@ -25,7 +25,7 @@ from oracle_api import run_swref  # noqa: E402
 
 
 def problem_arrays(p):
-    d = dict(n=p.n, wellbalanced=int(p.wellbalanced), gravity=p.gravity, cfl=p.cfl,
+    d = dict(n=p.n, ny=p.ny, wellbalanced=int(p.wellbalanced), gravity=p.gravity, cfl=p.cfl,
              x0=p.x0, x1=p.x1, U0=p.U0)
     if p.Uwb is not None:
         d["Uwb"] = p.Uwb
@@ -44,11 +44,16 @@ CASES = {
     "bumpstate_flat11": (lambda: P.Problem(11, P.lake_bumps(11, 102).U0, kind="flat"), 2),
     "dambreak10": (lambda: P.dam_break(10), 2),
     "shallowdam12": (lambda: P.dam_break(12, dry=0.1), 3),
+    # rectangular grids (nx != ny, hence dx != dy on the unit square)
+    "vortex_rect13x20": (lambda: P.vortex(13, 103, ny=20), 3),
+    "lakeflag1_wb_rect11x8": (lambda: P.lake_flag1(11, True, ny=8), 2),
 }
 
 
-def main():
+def main(only=None):
     for name, (mk, steps) in CASES.items():
+        if only and name not in only:
+            continue
         p = mk()
         out = problem_arrays(p)
         f = run_swref(p, 1)
@@ -61,6 +66,8 @@ def main():
         out["ts"] = np.array(run_swref(p, 3))
         np.savez_compressed(HERE / f"{name}.npz", **out)
         print(f"wrote {name}.npz")
+    if only:
+        return
     # Jacobi on a random diagonally dominant M-matrix in the reference CSR pattern
     rng = np.random.default_rng(7)
     n = 8
@@ -76,4 +83,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -76,8 +76,10 @@ class Oracle:
         self.p = problem
         m = problem.mesh
         self.n = problem.n
+        self.rows = problem.rows
         w = np.ascontiguousarray(m.wbnd, dtype=np.float64)
-        self.h = L.orc_create(self.n, self.n, m.dx, m.dx, problem.gravity, problem.cfl,
+        self.h = L.orc_create(self.n, self.rows, m.dx, problem.mesh_y.dx, problem.gravity,
+                              problem.cfl,
                               1 if problem.wellbalanced else 0, _ptr(w))
         if not self.h:
             raise ValueError("orc_create failed")
@@ -97,14 +99,14 @@ class Oracle:
         oracle_lib().orc_set_U(self.h, _ptr(np.ascontiguousarray(U, dtype=np.float64)))
 
     def get_U(self):
-        U = np.empty((self.n, self.n, 3))
+        U = np.empty((self.rows, self.n, 3))
         oracle_lib().orc_get_U(self.h, _ptr(U))
         return U
 
     def fvtd(self):
-        n = self.n
-        out = dict(Ut=np.empty((n, n, 3)), FX=np.empty((n, n + 1, 3)), FY=np.empty((n + 1, n, 3)),
-                   S=np.empty((n, n, 3)), Prod=np.empty(5 * n * n), Dest=np.empty(5 * n * n))
+        n, r = self.n, self.rows
+        out = dict(Ut=np.empty((r, n, 3)), FX=np.empty((r, n + 1, 3)), FY=np.empty((r + 1, n, 3)),
+                   S=np.empty((r, n, 3)), Prod=np.empty(5 * n * r), Dest=np.empty(5 * n * r))
         oracle_lib().orc_fvtd(self.h, *(_ptr(out[k]) for k in ("Ut", "FX", "FY", "S", "Prod", "Dest")))
         return out
 
@@ -131,7 +133,7 @@ class Oracle:
         oracle_lib().orc_set_jacobi_limits(self.h, max_iter, tol)
 
     def jacobi(self, Mass, diag, rhs):
-        sol = np.empty(self.n * self.n)
+        sol = np.empty(self.n * self.rows)
         k = oracle_lib().orc_jacobi(self.h, _ptr(np.ascontiguousarray(Mass)),
                                     _ptr(np.ascontiguousarray(diag)),
                                     _ptr(np.ascontiguousarray(rhs)), _ptr(sol))
@@ -156,9 +158,10 @@ def run_swref(problem, mode: int, steps: int = 0, jac=None, timeout=600):
     exe = SWREF[bool(problem.wellbalanced)]
     if not exe.exists():
         raise RuntimeError(f"{exe} not built")
-    n = problem.n
+    n, r = problem.n, problem.rows
     bflag = 1 if problem.dbdx is not None else 0
-    hdr = np.array([mode, n, steps, bflag, 1 if problem.wellbalanced else 0, 0], dtype="<i4")
+    hdr = np.array([mode, n, steps, bflag, 1 if problem.wellbalanced else 0, r if r != n else 0],
+                   dtype="<i4")
     fl = np.array([problem.x0, problem.x1, problem.gravity, problem.cfl], dtype="<f8")
     with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as td:
         fin, fout = Path(td) / "in.bin", Path(td) / "out.bin"
@@ -180,9 +183,9 @@ def run_swref(problem, mode: int, steps: int = 0, jac=None, timeout=600):
     if mode == 3:
         return tuple(raw[:3])
     if mode == 1:
-        sizes = [("Ut", (n, n, 3)), ("FX", (n, n + 1, 3)), ("FY", (n + 1, n, 3)), ("S", (n, n, 3)),
-                 ("Prod", (5 * n * n,)), ("Dest", (5 * n * n,)), ("wbnd", (4,)), ("dx", (2,)),
-                 ("MeshGP", (4, 4, n, n, 2))]
+        sizes = [("Ut", (r, n, 3)), ("FX", (r, n + 1, 3)), ("FY", (r + 1, n, 3)), ("S", (r, n, 3)),
+                 ("Prod", (5 * n * r,)), ("Dest", (5 * n * r,)), ("wbnd", (4,)), ("dx", (2,)),
+                 ("MeshGP", (4, 4, r, n, 2))]
         out, off = {}, 0
         for k, shp in sizes:
             cnt = int(np.prod(shp))
@@ -192,7 +195,7 @@ def run_swref(problem, mode: int, steps: int = 0, jac=None, timeout=600):
         return out
     # mode 2
     # per step: dt, LambdaMaxX, LambdaMaxY, U; a non-finite dt ends the run
-    rec = 3 + 3 * n * n
+    rec = 3 + 3 * n * r
     hist = []
     U = None
     off = 0
@@ -200,6 +203,6 @@ def run_swref(problem, mode: int, steps: int = 0, jac=None, timeout=600):
         hist.append(tuple(raw[off:off + 3]))
         if off + 3 == raw.size:
             break
-        U = raw[off + 3:off + rec].reshape(n, n, 3)
+        U = raw[off + 3:off + rec].reshape(r, n, 3)
         off += rec
     return hist, U

@@ -12,15 +12,16 @@ from parity_util import FVTD_KEYS, describe_diff, same_bits
 pytestmark = pytest.mark.skipif(not have_swref(), reason="oracle/_ref not built")
 
 
-def random_state(n, seed, amp=0.2):
+def random_state(n, seed, amp=0.2, ny=0):
     """A rough but positive random state: exercises the nonlinear WENO weights
-    and the limiter far from the smooth regime."""
+    and the limiter far from the smooth regime (n x ny cells)."""
     rng = np.random.default_rng(seed)
-    U = np.empty((n, n, 3))
-    U[..., 0] = 1.0 + amp * rng.uniform(-1, 1, (n, n))
-    U[..., 1] = U[..., 0] * rng.normal(0, 0.3, (n, n))
-    U[..., 2] = U[..., 0] * rng.normal(0, 0.3, (n, n))
-    return P.Problem(n, U, kind="random")
+    r = ny or n
+    U = np.empty((r, n, 3))
+    U[..., 0] = 1.0 + amp * rng.uniform(-1, 1, (r, n))
+    U[..., 1] = U[..., 0] * rng.normal(0, 0.3, (r, n))
+    U[..., 2] = U[..., 0] * rng.normal(0, 0.3, (r, n))
+    return P.Problem(n, U, kind="random", ny=ny)
 
 
 CASES = {
@@ -30,6 +31,9 @@ CASES = {
     "lakeflag1_wb14": lambda: P.lake_flag1(14, True),
     "dam15": lambda: P.dam_break(15),
     "dam15_mindepth": lambda: P.dam_break(15, dry=1e-6),
+    "vortex_rect21x34": lambda: P.vortex(21, 5, ny=34),
+    "random_rect17x11": lambda: random_state(17, 3, ny=11),
+    "lakeflag1_wb_rect19x26": lambda: P.lake_flag1(19, True, ny=26),
 }
 
 
@@ -46,7 +50,9 @@ def test_fvtd_bitwise(case):
 
 
 @pytest.mark.parametrize("case,steps", [("vortex21", 5), ("random13", 3), ("lakeflag1_wb14", 4),
-                                        ("dam15_mindepth", 3), ("dam15", 2)])
+                                        ("dam15_mindepth", 3), ("dam15", 2),
+                                        ("vortex_rect21x34", 3), ("random_rect17x11", 2),
+                                        ("lakeflag1_wb_rect19x26", 2)])
 def test_steps_bitwise(case, steps):
     p = CASES[case]()
     hist_r, U_r = run_swref(p, 2, steps=steps)
