@@ -212,6 +212,10 @@ def main():
                     help="requests of the served run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0,
                     help="Poisson arrival rate of the served run, per second")
+    ap.add_argument("--graph", type=int, default=-1,
+                    help="step execution: 1 replay a captured HIP graph, 0 no graph, -1 autotuned")
+    ap.add_argument("--margin", type=int, default=-1,
+                    help="extra speculative Jacobi sweeps per correction (-1 autotuned, -2 none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -227,6 +231,7 @@ def main():
     prob = P.lake_bumps(n, args.seed, steps=args.steps)
     log(f"bench: problem {n}x{n} lake_bumps seed {args.seed} built in {time.time() - t0:.1f} s")
     s = swm.Solver.from_problem(prob)
+    s.set_execution(args.graph, args.margin)
     del prob
     for _ in range(args.warmup):
         dt, _, _ = s.timestep()
@@ -293,6 +298,7 @@ def main():
                                f"8 Gaussian bumps, CFL 0.5",
                    "grid": [n, n], "cfl": 0.5, "wellbalanced": True, "seed": args.seed},
         "jacobi_sweeps_per_step": jsum / args.steps,
+        "execution": s.execution_stats(),
         "kernels": kern,
         "roofline": roof,
     }
