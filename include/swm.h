@@ -89,6 +89,17 @@ int swm_step_mpdec5(swm_solver_t *h, double t, double dt, int jacobi_iters[13]);
  * values; stops early with SWM_E_STATE if dt is not finite. */
 int swm_run_steps(swm_solver_t *h, int nsteps, double *dts, int *jacobi_iters);
 
+/* The reference's time loop on the device (src/timediscretization.f90:57-95)
+ * up to the next output time: each step's dt is
+ * MIN(MIN(dt_min, t_analyze - t), MIN(dt_min, t_end - t)) with dt_min the
+ * TimeStep value, then t = t + dt; returns when |t_analyze - t| < 1e-10 or
+ * |t - t_end| < 1e-10 or after max_steps steps.  *t is updated in place (the
+ * same double arithmetic as the reference); steps_done and dts (max_steps
+ * values) may be NULL.  Returns the number of non-converged Jacobi solves, or
+ * SWM_E_STATE if a dt is not finite (the state is then that before the step). */
+int swm_advance(swm_solver_t *h, double *t, double t_analyze, double t_end, int max_steps,
+                int *steps_done, double *dts);
+
 /* One FVTimeDerivative of the device state (src/finitevolume.f90:402-440)
  * with its module outputs in the reference layouts, for parity tests.  Any
  * output may be NULL.  Ut(1,:,:) is 0 (unused with PATANKAR). */

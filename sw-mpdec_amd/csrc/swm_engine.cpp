@@ -145,6 +145,7 @@ struct Job {
   int last_iters[13] = {0};
   int pred[13];
   bool have_pred = false;  // pred holds the sweep counts of an accepted step
+  bool caps_set = false;   // the device status holds time-loop caps
   bool wb_ready = false;
   Tables own;          // tables covering this job alone
 };
@@ -162,6 +163,12 @@ struct swm_engine {
   size_t hstatus_cap = 0;
   double* hdt = nullptr;         // pinned: host-provided dt of a step
   int* hzero = nullptr;          // pinned: 0, for clearing dt_bad
+  struct Caps {
+    int use_cap;
+    int pad;
+    double cap_a, cap_e;
+  };
+  Caps* hcaps = nullptr;         // pinned: the time-loop clipping of one step
   bool dirty = true;             // device descriptors / tables out of date
   Tables all;
   int max_iter = 1000;
@@ -494,9 +501,21 @@ void tune_record(swm_engine* e, double ms) {
 // Runs one step of the jobs in T and leaves their status in e->hstatus.
 // host_dt: the step uses this dt for every job (the plain solver's
 // swm_step_mpdec5); otherwise each job's TimeStep is computed on the device.
-int engine_step(swm_engine* e, Tables& T, const double* host_dt, int iters[13]) {
+int engine_step(swm_engine* e, Tables& T, const double* host_dt, int iters[13],
+                const double* caps = nullptr) {
   if (T.njobs == 0) return SWM_OK;
   hipStream_t s = e->stream;
+  // the time-loop clipping (caps = {tAnalyze - t, tEnd - t}) or none
+  *e->hcaps = caps ? swm_engine::Caps{1, 0, caps[0], caps[1]} : swm_engine::Caps{0, 0, 0.0, 0.0};
+  for (int q = 0; q < T.njobs; ++q) {
+    Job& J = *e->jobs[T.jl[q]];
+    if (!caps && !J.caps_set) continue;
+    JobStatus* st = J.dev.status;
+    HIPCK(e, hipMemcpyAsync(&st->use_cap, &e->hcaps->use_cap, sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCK(e, hipMemcpyAsync(&st->cap_a, &e->hcaps->cap_a, 2 * sizeof(double),
+                            hipMemcpyHostToDevice, s));
+    J.caps_set = caps != nullptr;
+  }
   if (host_dt) {
     *e->hdt = *host_dt;
     for (int q = 0; q < T.njobs; ++q) {
@@ -845,7 +864,16 @@ int swm_engine_create(long long budget_bytes, swm_engine_t** out) {
     g_error = "hipHostMalloc failed";
     rc = SWM_E_HIP;
   }
-  if (rc == SWM_OK) *e->hzero = 0;
+  if (rc == SWM_OK &&
+      hipHostMalloc((void**)&e->hcaps, sizeof(swm_engine::Caps), hipHostMallocDefault) != hipSuccess) {
+    e->hcaps = nullptr;
+    g_error = "hipHostMalloc failed";
+    rc = SWM_E_HIP;
+  }
+  if (rc == SWM_OK) {
+    *e->hzero = 0;
+    *e->hcaps = swm_engine::Caps{0, 0, 0.0, 0.0};
+  }
   if (rc) {
     swm_engine_destroy(e);
     return rc;
@@ -863,6 +891,7 @@ void swm_engine_destroy(swm_engine_t* e) {
   if (e->hstatus) (void)hipHostFree(e->hstatus);
   if (e->hdt) (void)hipHostFree(e->hdt);
   if (e->hzero) (void)hipHostFree(e->hzero);
+  if (e->hcaps) (void)hipHostFree(e->hcaps);
   if (e->dstatus.p) (void)hipFree(e->dstatus.p);
   e->dstatus.p = nullptr;
   if (e->djobs.p) (void)hipFree(e->djobs.p);
@@ -1178,6 +1207,40 @@ int swm_run_steps(swm_solver_t* s, int nsteps, double* dts, int* jacobi_iters) {
     if (jacobi_iters)
       for (int i = 0; i < 13; ++i) jacobi_iters[13 * k + i] = iters[i];
     J.steps_done += 1;
+  }
+  HIPCK(e, hipStreamSynchronize(e->stream));
+  return J.nonconv - before;
+}
+
+int swm_advance(swm_solver_t* s, double* t, double t_analyze, double t_end, int max_steps,
+                int* steps_done, double* dts) {
+  if (!s || !t) return SWM_E_ARG;
+  swm_engine* e = s->e;
+  if (max_steps < 0) FAIL(e, SWM_E_ARG, "max_steps < 0");
+  Job& J = *plain_job(s);
+  if (J.dev.wb && !J.wb_ready)
+    FAIL(e, SWM_E_STATE, "well-balanced state not initialised (swm_init_wb)");
+  int rc = refresh(e);
+  if (rc) return rc;
+  const int before = J.nonconv;
+  int k = 0;
+  if (steps_done) *steps_done = 0;
+  while (k < max_steps) {
+    // src/timediscretization.f90:58-59 and :83-94
+    const double caps[2] = {t_analyze - *t, t_end - *t};
+    rc = engine_step(e, J.own, nullptr, nullptr, caps);
+    if (rc) return rc;
+    const JobStatus& js = e->hstatus[s->job];
+    if (js.dt_bad) {
+      HIPCK(e, hipStreamSynchronize(e->stream));
+      FAIL(e, SWM_E_STATE, "dt is not finite (%g) at t = %.17g", js.dt, *t);
+    }
+    if (dts) dts[k] = js.dt;
+    *t = *t + js.dt;
+    ++k;
+    J.steps_done += 1;
+    if (steps_done) *steps_done = k;
+    if (std::fabs(t_analyze - *t) < 1.0E-10 || std::fabs(*t - t_end) < 1.0E-10) break;
   }
   HIPCK(e, hipStreamSynchronize(e->stream));
   return J.nonconv - before;

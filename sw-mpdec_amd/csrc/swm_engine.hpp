@@ -51,7 +51,10 @@ struct JobStatus {
   int ambiguous;  // stop decisions settled by the exact sequential sum
   int overrun;    // a solve was still running when its phase ended (the
                   // step ran with too few sweeps launched and must be redone)
-  int pad2;
+  int use_cap;    // 1: dt = MIN(MIN(dt_min, cap_a), MIN(dt_min, cap_e)), the
+                  // clipping of the reference's time loop
+  double cap_a, cap_e;  // tAnalyze - t, tEnd - t (set by the host per step)
+  double dt_min;  // TimeStep's value before the clipping
 };
 
 struct JobDev {

@@ -884,10 +884,15 @@ __global__ __launch_bounds__(256) void k_timestep_final(EngineArgs E, const int*
   }
   ts = J.cfl * ts;
   if (ts < kMinTimestep) ts = kMinTimestep;
-  J.status->dt = ts;
-  J.status->lmx = LX;
-  J.status->lmy = LY;
-  J.status->dt_bad = isfinite(ts) ? 0 : 1;
+  JobStatus* st = J.status;
+  st->dt_min = ts;
+  // the time loop's dt = MIN(MIN(dt_min, tAnalyze-t), MIN(dt_min, tEnd-t))
+  // (src/timediscretization.f90:58-59), when the host set the caps
+  const double dt = st->use_cap ? min_r(min_r(ts, st->cap_a), min_r(ts, st->cap_e)) : ts;
+  st->dt = dt;
+  st->lmx = LX;
+  st->lmy = LY;
+  st->dt_bad = isfinite(dt) ? 0 : 1;
 }
 
 // ------------------------------------------------------------------------

@@ -15,7 +15,7 @@ PRIVATE
 INTEGER(C_INT), PARAMETER, PUBLIC :: SWM_OK = 0
 
 PUBLIC :: swm_create, swm_destroy, swm_set_bathy_grad, swm_init_wb
-PUBLIC :: swm_upload, swm_download, swm_timestep, swm_step_mpdec5
+PUBLIC :: swm_upload, swm_download, swm_timestep, swm_step_mpdec5, swm_advance
 PUBLIC :: swm_error_message, swm_check
 
 INTERFACE
@@ -75,6 +75,19 @@ INTERFACE
     TYPE(C_PTR), VALUE :: h
     REAL(C_DOUBLE), VALUE :: t, dt
     INTEGER(C_INT), INTENT(OUT) :: iters(13)
+  END FUNCTION
+
+  ! the time loop of TimeDiscretization to the next output time
+  ! (src/timediscretization.f90:57-95), dt clipped to t_analyze and t_end
+  INTEGER(C_INT) FUNCTION swm_advance(h, t, t_analyze, t_end, max_steps, steps_done, dts) &
+      BIND(C, NAME='swm_advance')
+    IMPORT :: C_INT, C_DOUBLE, C_PTR
+    TYPE(C_PTR), VALUE :: h
+    REAL(C_DOUBLE), INTENT(INOUT) :: t
+    REAL(C_DOUBLE), VALUE :: t_analyze, t_end
+    INTEGER(C_INT), VALUE :: max_steps
+    INTEGER(C_INT), INTENT(OUT) :: steps_done
+    TYPE(C_PTR), VALUE :: dts
   END FUNCTION
 
   FUNCTION swm_last_error_c(h) BIND(C, NAME='swm_last_error')

@@ -75,6 +75,7 @@ def lib():
     L.swm_debug_divsqrt.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
     L.swm_debug_shared_div.argtypes = [ctypes.c_int, _dp, _dp, _dp]
     L.swm_track_stage_min.argtypes = [vp, ctypes.c_int]
+    L.swm_advance.argtypes = [vp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _ip, _dp]
     L.swm_set_execution.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.swm_execution_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
     L.swm_stage_min_h.argtypes = [vp, _dp]
@@ -173,6 +174,18 @@ class Solver:
         nonconv = self._check(lib().swm_run_steps(self._h, steps, _ptr(dts),
                                                   its.ctypes.data_as(_ip)), "swm_run_steps")
         return dts, its.reshape(steps, 13), nonconv
+
+    def advance(self, t, t_analyze, t_end, max_steps=1 << 30):
+        """The reference's time loop to the next output time (swm_advance):
+        returns (t, dts of the steps taken, non-converged solves)."""
+        tt = ctypes.c_double(t)
+        done = ctypes.c_int(0)
+        cap = min(max_steps, 1 << 20)
+        dts = np.zeros(cap)
+        nonconv = self._check(lib().swm_advance(self._h, ctypes.byref(tt), t_analyze, t_end,
+                                                cap, ctypes.byref(done), _ptr(dts)),
+                              "swm_advance")
+        return tt.value, dts[:done.value].copy(), nonconv
 
     def set_execution(self, use_graph=-1, margin=-1):
         """Step execution mode (results are identical in every mode): use_graph
@@ -400,6 +413,18 @@ class Engine:
     def set_jacobi_limits(self, max_iter, tol):
         self._check(_engine_lib().swm_engine_set_jacobi_limits(self._e, max_iter, tol),
                     "swm_engine_set_jacobi_limits")
+
+    def advance(self, t, t_analyze, t_end, max_steps=1 << 30):
+        """The reference's time loop to the next output time (swm_advance):
+        returns (t, dts of the steps taken, non-converged solves)."""
+        tt = ctypes.c_double(t)
+        done = ctypes.c_int(0)
+        cap = min(max_steps, 1 << 20)
+        dts = np.zeros(cap)
+        nonconv = self._check(lib().swm_advance(self._h, ctypes.byref(tt), t_analyze, t_end,
+                                                cap, ctypes.byref(done), _ptr(dts)),
+                              "swm_advance")
+        return tt.value, dts[:done.value].copy(), nonconv
 
     def set_execution(self, use_graph=-1, margin=-1):
         self._check(_engine_lib().swm_engine_set_execution(self._e, use_graph, margin),

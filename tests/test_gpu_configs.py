@@ -102,3 +102,55 @@ def test_dam_break_1024_thin_film_200_steps():
     assert nonconv == 0 and iters.max() < 1000
     rel = abs(mass(U) - m0) / m0
     assert rel <= 1e-11, rel
+
+
+def _min_r(a, b):
+    return a if a < b else b
+
+
+@pytest.mark.parametrize("make", [lambda: P.vortex(40, 11), lambda: P.lake_flag1(24, True)],
+                         ids=["vortex40", "lake_flag1_wb24"])
+def test_advance_reproduces_the_time_loop(make):
+    """swm_advance against the reference's loop (src/timediscretization.f90:
+    46-95) run on the oracle: two output intervals, dt clipped to the output
+    times and to tEnd, t accumulated in the same double arithmetic."""
+    p = make()
+    o = Oracle(p)
+    dt0, _, _ = o.timestep()
+    t_end = 9.5 * dt0
+    n_out = 2
+    dt_an = t_end / float(n_out)
+    # the oracle, as the reference's loop
+    t, t_an, ref_t, ref_dts = 0.0, dt_an, [], []
+    o = Oracle(p)
+    while True:
+        dt_min, _, _ = o.timestep()
+        dt = _min_r(_min_r(dt_min, t_an - t), _min_r(dt_min, t_end - t))
+        o.step(dt)
+        ref_dts.append(dt)
+        t = t + dt
+        if abs(t_an - t) < 1.0e-10:
+            ref_t.append(t)
+            t_an = t_an + dt_an
+            if t_an > t_end:
+                t_an = t_end
+        if abs(t - t_end) < 1.0e-10:
+            break
+    U_r = o.get_U()
+    # the device, one swm_advance per output interval
+    s = swm.Solver.from_problem(p)
+    t, t_an, got_t, got_dts = 0.0, dt_an, [], []
+    while True:
+        t, dts, _ = s.advance(t, t_an, t_end)
+        got_dts.extend(dts)
+        if abs(t_an - t) < 1.0e-10:
+            got_t.append(t)
+            t_an = t_an + dt_an
+            if t_an > t_end:
+                t_an = t_end
+        if abs(t - t_end) < 1.0e-10:
+            break
+    assert got_t == ref_t
+    assert same_bits(np.array(got_dts), np.array(ref_dts))
+    U = s.download()
+    assert same_bits(U, U_r), describe_diff(U, U_r)
