@@ -90,33 +90,56 @@ def run_batch(problems, nsteps, budget_bytes: int = 0, profile: bool = False):
     return res, t1 - t0, steps, prof, states
 
 
+def work(r: Request) -> int:
+    """Cell-steps of a request."""
+    return r.problem.n * r.problem.rows * r.nsteps
+
+
 def serve(requests: list[Request], budget_bytes: int, keep_results: bool = True,
-          log=None) -> dict:
-    """Continuous batching.  Requests are admitted in arrival order (FIFO; a
-    request that does not fit waits for memory to free up and blocks the ones
-    behind it), stepped together with all running solves, and retired as soon
-    as they finish."""
+          log=None, policy: str = "fifo", max_running_cells: int = 0) -> dict:
+    """Continuous batching.  Arrived requests are admitted while their device
+    footprint fits the budget, stepped together with all running solves, and
+    retired as soon as they finish.
+
+    policy "fifo": admission in arrival order (a request that does not fit
+    blocks the ones behind it).  policy "sjf": among the arrived requests the
+    one with the least work (cells x steps) first, and no admission beyond
+    max_running_cells running cells (0: no cap) -- the GPU is saturated well
+    below the memory limit, and short requests then finish without waiting
+    behind long ones."""
+    if policy not in ("fifo", "sjf"):
+        raise ValueError(policy)
     eng = swm.Engine(budget_bytes)
-    pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+    pending = sorted(requests, key=lambda r: r.arrival)
+    arrived: list[Request] = []
     running: dict[int, Request] = {}
     t0 = time.perf_counter()
     now = lambda: time.perf_counter() - t0  # noqa: E731
     steps = 0
     peak_bytes = 0
     peak_running = 0
-    while pending or running:
-        # admit everything that has arrived and fits
-        while pending and pending[0].arrival <= now():
-            r = pending[0]
+    cells = 0
+    while pending or arrived or running:
+        t = now()
+        while pending and pending[0].arrival <= t:
+            arrived.append(pending.pop(0))
+        if policy == "sjf":
+            arrived.sort(key=work)
+        while arrived:
+            r = arrived[0]
+            c = r.problem.n * r.problem.rows
+            if policy == "sjf" and max_running_cells and running and cells + c > max_running_cells:
+                break
             jid = eng.submit(r.problem, r.nsteps)
             if jid is None:
                 if not running:
                     raise RuntimeError(f"request {r.rid} alone exceeds the budget")
                 break
-            pending.popleft()
+            arrived.pop(0)
             r.admitted = now()
             r.bytes = eng.info(jid)["bytes"]
             running[jid] = r
+            cells += c
         if not running:
             # idle until the next arrival
             time.sleep(max(0.0, min(pending[0].arrival - now(), 0.05)))
@@ -130,13 +153,15 @@ def serve(requests: list[Request], budget_bytes: int, keep_results: bool = True,
             info = eng.info(jid)
             if info["state"] != swm.JOB_RUNNING:
                 r = running.pop(jid)
+                cells -= r.problem.n * r.problem.rows
                 r.done = t
                 r.state = info["state"]
                 if keep_results:
                     r.result = eng.fetch(jid)
                 eng.release(jid)
         if log and steps % 50 == 0:
-            log(f"serve: t={t:.2f}s step {steps} running {len(running)} pending {len(pending)}")
+            log(f"serve: t={t:.2f}s step {steps} running {len(running)} "
+                f"waiting {len(pending) + len(arrived)}")
     makespan = now()
     eng.close()
     lat = np.array([r.latency for r in requests])

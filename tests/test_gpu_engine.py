@@ -104,3 +104,22 @@ def test_failed_job_does_not_disturb_others():
     assert eng.info(dry)["state"] == swm.JOB_FAILED
     assert eng.info(g)["state"] == swm.JOB_DONE
     assert same_bits(eng.fetch(g), oracle_run(good, 3))
+
+
+def test_serve_policies_match_sequential():
+    """The continuous-batching loop (FIFO and shortest-work-first with a cap
+    on running cells) returns, request by request, the plain solver's
+    results bitwise."""
+    import serve
+
+    reqs = serve.request_mix(10, seed=3, rate=200.0, nmin=16, nmax=48, smin=2, smax=5)
+    serve.sequential(reqs, keep_results=True)
+    ref = [r.result.copy() for r in reqs]
+    for policy, cap in (("fifo", 0), ("sjf", 3000)):
+        for r in reqs:
+            r.result, r.done, r.admitted = None, float("nan"), float("nan")
+        out = serve.serve(reqs, budget_bytes=0, keep_results=True, policy=policy,
+                          max_running_cells=cap)
+        assert out["jobs"] == len(reqs) and out["failed"] == 0
+        for r, U in zip(reqs, ref):
+            assert same_bits(r.result, U), (policy, r.rid, r.problem.kind, r.problem.n)
