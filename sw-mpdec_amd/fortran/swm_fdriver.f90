@@ -1,24 +1,30 @@
 !===============================================================================!
 ! swm_fdriver: a Fortran caller of libswm through swm_bridge, shaped like the
 ! reference's TimeDiscretization loop (src/timediscretization.f90:57-95) with
-! timescheme = 5: dt = TimeStep(), one TimeDiscretizationByMPDeC5 step, repeat.
+! timescheme = 5.
+!   loop = 0: nsteps times dt = TimeStep(), one TimeDiscretizationByMPDeC5 step.
+!   loop = 1: the reference's output loop with tEnd = (nsteps - 1/2) * dt_min(0)
+!             and 2 output intervals, each interval one swm_advance call (dt
+!             clipped to tAnalyze and tEnd on the device).
 !
 ! Reads a raw input file
-!   int32  nx, ny, nsteps, wellbalanced, has_bathy, reserved
+!   int32  nx, ny, nsteps, wellbalanced, has_bathy, loop
 !   real64 dx, dy, gravity, cfl, wgp_bnd(4)
 !   real64 U(3,nx,ny) [, Uwb(3,nx,ny) if wellbalanced] [, dbdx, dbdy(nx,ny,4,4)]
 ! and writes the output file
-!   int32  nsteps, nonconverged_total
-!   real64 dt(nsteps), U(3,nx,ny)
+!   int32  nsteps_taken, nonconverged_total
+!   real64 dt(nsteps_taken), U(3,nx,ny)
 ! Usage: swm_fdriver <input> <output>
 !===============================================================================!
 PROGRAM swm_fdriver
 USE ISO_C_BINDING
 USE swm_bridge
 IMPLICIT NONE
-INTEGER(C_INT) :: hdr(6), nx, ny, nsteps, wb, hasb, rc, k, nonconv
+INTEGER(C_INT) :: hdr(6), nx, ny, nsteps, wb, hasb, rc, k, nonconv, loop, ndone, nout, nrec
 INTEGER(C_INT) :: iters(13)
 REAL(C_DOUBLE) :: par(8), dx, dy, g, cfl, wgp(4), t, dt, lmx, lmy
+REAL(C_DOUBLE) :: tend, tanalyze, dtanalyze
+REAL(C_DOUBLE), ALLOCATABLE, TARGET :: chunk(:)
 REAL(C_DOUBLE), ALLOCATABLE :: U(:,:,:), Uwb(:,:,:), bx(:,:,:,:), by(:,:,:,:), dts(:)
 TYPE(C_PTR) :: h
 CHARACTER(LEN=1024) :: fin, fout
@@ -28,7 +34,7 @@ CALL GET_COMMAND_ARGUMENT(1, fin)
 CALL GET_COMMAND_ARGUMENT(2, fout)
 OPEN(NEWUNIT=uin, FILE=TRIM(fin), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
 READ(uin) hdr
-nx = hdr(1); ny = hdr(2); nsteps = hdr(3); wb = hdr(4); hasb = hdr(5)
+nx = hdr(1); ny = hdr(2); nsteps = hdr(3); wb = hdr(4); hasb = hdr(5); loop = hdr(6)
 READ(uin) par
 dx = par(1); dy = par(2); g = par(3); cfl = par(4); wgp = par(5:8)
 ALLOCATE(U(3,nx,ny), dts(MAX(nsteps,1)))
@@ -59,24 +65,50 @@ CALL swm_check(h, swm_upload(h, U, 0_C_INT), "swm_upload")
 
 t = 0.0
 nonconv = 0
-DO k = 1, nsteps
+nrec = 0
+IF (loop .EQ. 0) THEN
+  DO k = 1, nsteps
+    CALL swm_check(h, swm_timestep(h, dt, lmx, lmy), "swm_timestep")
+    IF (.NOT. (dt .LT. HUGE(dt))) THEN
+      WRITE(*,*) "swm_fdriver: non-finite time step at step ", k
+      STOP 2
+    END IF
+    rc = swm_step_mpdec5(h, t, dt, iters)
+    CALL swm_check(h, rc, "swm_step_mpdec5")
+    nonconv = nonconv + rc
+    dts(k) = dt
+    t = t + dt
+  END DO
+  nrec = nsteps
+ELSE
+  ! src/timediscretization.f90:46-95 with nOutputFiles = 2
   CALL swm_check(h, swm_timestep(h, dt, lmx, lmy), "swm_timestep")
-  IF (.NOT. (dt .LT. HUGE(dt))) THEN
-    WRITE(*,*) "swm_fdriver: non-finite time step at step ", k
-    STOP 2
-  END IF
-  rc = swm_step_mpdec5(h, t, dt, iters)
-  CALL swm_check(h, rc, "swm_step_mpdec5")
-  nonconv = nonconv + rc
-  dts(k) = dt
-  t = t + dt
-END DO
+  tend = (REAL(nsteps) - 0.5) * dt
+  nout = 2
+  dtanalyze = tend / REAL(nout)
+  tanalyze = t + dtanalyze
+  DEALLOCATE(dts)
+  ALLOCATE(dts(4 * nsteps + 8), chunk(4 * nsteps + 8))
+  DO
+    rc = swm_advance(h, t, tanalyze, tend, 4 * nsteps + 8, ndone, C_LOC(chunk))
+    CALL swm_check(h, rc, "swm_advance")
+    nonconv = nonconv + rc
+    dts(nrec + 1:nrec + ndone) = chunk(1:ndone)
+    nrec = nrec + ndone
+    IF (ABS(tanalyze - t) .LT. 1.0E-10) THEN
+      ! WriteSolutionToDisk(t) would download here
+      tanalyze = tanalyze + dtanalyze
+      IF (tanalyze .GT. tend) tanalyze = tend
+    END IF
+    IF (ABS(t - tend) .LT. 1.0E-10) EXIT
+  END DO
+END IF
 CALL swm_check(h, swm_download(h, U), "swm_download")
 CALL swm_destroy(h)
 
 OPEN(NEWUNIT=uout, FILE=TRIM(fout), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
-WRITE(uout) nsteps, nonconv
-WRITE(uout) dts(1:nsteps)
+WRITE(uout) nrec, nonconv
+WRITE(uout) dts(1:nrec)
 WRITE(uout) U
 CLOSE(uout)
 END PROGRAM swm_fdriver

@@ -25,10 +25,10 @@ def with_steps(p, k):
     return p
 
 
-def write_input(path, p):
+def write_input(path, p, loop=0):
     m = p.mesh
     with open(path, "wb") as f:
-        np.array([p.n, p.rows, p.steps, int(p.wellbalanced), int(p.dbdx is not None), 0],
+        np.array([p.n, p.rows, p.steps, int(p.wellbalanced), int(p.dbdx is not None), loop],
                  dtype=np.int32).tofile(f)
         np.array([m.dx, p.mesh_y.dx, p.gravity, p.cfl, *m.wbnd], dtype=np.float64).tofile(f)
         np.ascontiguousarray(p.U0, dtype=np.float64).tofile(f)
@@ -69,6 +69,49 @@ def test_fortran_driver_matches_oracle(tmp_path, make):
         o.step(dt)
         dts_r.append(dt)
     assert same_bits(dts, np.array(dts_r))
+    U_r = o.get_U()
+    assert same_bits(U, U_r), describe_diff(U, U_r)
+    assert nonconv == 0
+
+
+def _min_r(a, b):
+    return a if a < b else b
+
+
+@pytest.mark.parametrize("make", [lambda: P.vortex(28, 6, steps=5),
+                                  lambda: with_steps(P.lake_flag1(20, True), 4)],
+                         ids=["vortex28", "lake_flag1_wb20"])
+def test_fortran_output_loop_with_advance(tmp_path, make):
+    """The driver's loop = 1: the reference's output loop (tEnd = (nsteps -
+    1/2) dt_min(0), two output intervals) with one swm_advance call per
+    interval, against the same loop on the oracle."""
+    if not FDRIVER.exists():
+        pytest.skip("Fortran driver not built (no Fortran compiler at build time)")
+    p = make()
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    write_input(fin, p, loop=1)
+    r = subprocess.run([str(FDRIVER), str(fin), str(fout)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    dts, U, nonconv = read_output(fout, p.n, p.rows)
+    o = Oracle(p)
+    dt0, _, _ = o.timestep()
+    t_end = (float(p.steps) - 0.5) * dt0
+    dt_an = t_end / 2.0
+    t, t_an, ref = 0.0, dt_an, []
+    while True:
+        dt_min, _, _ = o.timestep()
+        dt = _min_r(_min_r(dt_min, t_an - t), _min_r(dt_min, t_end - t))
+        o.step(dt)
+        ref.append(dt)
+        t = t + dt
+        if abs(t_an - t) < 1.0e-10:
+            t_an = t_an + dt_an
+            if t_an > t_end:
+                t_an = t_end
+        if abs(t - t_end) < 1.0e-10:
+            break
+    assert same_bits(dts, np.array(ref))
     U_r = o.get_U()
     assert same_bits(U, U_r), describe_diff(U, U_r)
     assert nonconv == 0
