@@ -75,6 +75,15 @@ def traffic_per_cell_node(kernel: str, size: int):
     return float(m.group(1)) if m else None
 
 
+def profiled(key: str, kernel: str, size: int):
+    """A per-kernel figure (key[kernel]: value) of profiles/traffic_<size>.md."""
+    f = ROOT / "profiles" / f"traffic_{size}.md"
+    if not f.exists():
+        return None
+    m = re.search(rf"{key}\[{kernel}\]:\s*([0-9.eE+]+)", f.read_text())
+    return float(m.group(1)) if m else None
+
+
 def cpu_baseline(budget_s: float = 20.0):
     """The reference Fortran itself (oracle/_ref, built from its own sources)
     on the host: one core, the reference's own lake-at-rest bathymetry
@@ -293,7 +302,12 @@ def main():
         ops = FP64_OPS_PER_FACE * units_per_step(dom, cells, 0) * args.steps / kern[dom]["launches"]
         tf = ops / avg_s / 1e12
         roof["fp64_valu"] = {"achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                             "frac": tf / FP64_PEAK_TFLOPS, "ops_per_launch": ops}
+                             "frac": tf / FP64_PEAK_TFLOPS, "ops_per_launch": ops,
+                             "note": "each +,-,*,/,sqrt counted once; the kernel is bound by "
+                                     "FP64 VALU issue, see issue_frac_profiled"}
+        roof["binding_roof"] = "fp64_valu_issue"
+        roof["issue_frac_profiled"] = profiled("valu_issue_fraction", dom, n)
+        roof["valu_instr_per_thread_node_profiled"] = profiled("valu_instr_per_thread_node", dom, n)
     out = {
         "metric": "mPDeC5 cell-steps/s FP64 at 2048\u00b2 vs CPU; batched jobs/s + p50/p99 latency",
         "value": value,

@@ -37,11 +37,6 @@ typedef __attribute__((address_space(1))) double gdouble;
 __device__ __forceinline__ const gdouble* G(const double* p) { return (const gdouble*)p; }
 __device__ __forceinline__ gdouble* G(double* p) { return (gdouble*)p; }
 
-// Minimum waves per SIMD the face kernels are compiled for (caps their VGPRs).
-#ifndef SWM_FACE_MIN_WAVES
-#define SWM_FACE_MIN_WAVES 1
-#endif
-
 // 0-based index of Fortran cell c (1..n or a periodic ghost) on an axis of n.
 // The tiles' halos keep c within [-2, n + 17], so one add or subtract of n
 // suffices for n >= 17; smaller axes take the modulo.
@@ -95,7 +90,7 @@ __device__ __forceinline__ void node_state(const JobDev& J, int m, const gdouble
 // face f = nx uses the unlimited left value of cell nx+1 while f = 0 uses the
 // limited one of cell 1: the two values of the periodic seam.
 // ------------------------------------------------------------------------
-__global__ __launch_bounds__(256, SWM_FACE_MIN_WAVES) void k_fvtd_x(EngineArgs E, const TileRef* tiles,
+__global__ __launch_bounds__(256) void k_fvtd_x(EngineArgs E, const TileRef* tiles,
                                                 NodeList L) {
   __shared__ double smem[3 * 400 + 2 * 3 * 320];
   double* sV = smem;          // [3][20 rows][20 cols]
@@ -171,7 +166,7 @@ __global__ __launch_bounds__(256, SWM_FACE_MIN_WAVES) void k_fvtd_x(EngineArgs E
 // y faces: the transpose of k_fvtd_x.  Tile (bx, by): columns
 // i = 16*bx + 1 + tx, Fortran rows r = 15*by + ty (0..ny+1); face g = r.
 // ------------------------------------------------------------------------
-__global__ __launch_bounds__(256, SWM_FACE_MIN_WAVES) void k_fvtd_y(EngineArgs E, const TileRef* tiles,
+__global__ __launch_bounds__(256) void k_fvtd_y(EngineArgs E, const TileRef* tiles,
                                                 NodeList L) {
   __shared__ double smem[3 * 400 + 2 * 3 * 320];
   double* sV = smem;          // [3][20 rows][20 cols]
