@@ -159,7 +159,7 @@ typedef struct {
   const double *Uwb;         /* (1:3, 1:nx, 1:ny) or NULL */
   const double *dbdx;        /* (1:nx, 1:ny, 1:4, 1:4) or NULL (flat bottom) */
   const double *dbdy;
-  int nsteps;                /* CFL steps to run; < 0: until released */
+  int nsteps;                /* CFL steps to run; 0: none (DONE at once); < 0: until released */
 } swm_job_desc;
 
 #define SWM_JOB_RUNNING 0

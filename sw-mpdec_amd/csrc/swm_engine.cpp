@@ -814,7 +814,10 @@ int engine_add(swm_engine* e, const swm_job_desc* d, int* id_out) {
     rc = init_wb(e, id, d->Uwb, 0);
     if (rc) return rc;
   }
-  return upload_state(e, J, d->U0, 0, J.dev.h, J.dev.hu, J.dev.hv);
+  rc = upload_state(e, J, d->U0, 0, J.dev.h, J.dev.hu, J.dev.hv);
+  if (rc) return rc;
+  if (d->nsteps == 0) J.state = JOB_DONE;  // nothing to run; holds U0 until released
+  return SWM_OK;
 }
 
 void engine_free_job(swm_engine* e, int id) {

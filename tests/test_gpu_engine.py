@@ -123,3 +123,16 @@ def test_serve_policies_match_sequential():
         assert out["jobs"] == len(reqs) and out["failed"] == 0
         for r, U in zip(reqs, ref):
             assert same_bits(r.result, U), (policy, r.rid, r.problem.kind, r.problem.n)
+
+
+def test_zero_step_job_is_done_and_keeps_its_state():
+    p = P.vortex(20, 2)
+    eng = swm.Engine()
+    a = eng.submit(p, 0)
+    b = eng.submit(P.vortex(24, 3), 2)
+    assert eng.info(a)["state"] == swm.JOB_DONE
+    while eng.step():
+        pass
+    assert eng.info(a)["steps_done"] == 0
+    assert same_bits(eng.fetch(a), p.U0)
+    assert eng.info(b)["state"] == swm.JOB_DONE and eng.info(b)["steps_done"] == 2

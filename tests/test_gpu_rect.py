@@ -29,6 +29,8 @@ CASES = {
     "vortex_40x17": lambda: P.vortex(40, 6, ny=17),
     "random_17x11": lambda: random_state(17, 11, 3),
     "random_3x29": lambda: random_state(3, 29, 4),
+    "random_29x3": lambda: random_state(29, 3, 7),
+    "random_3x3": lambda: random_state(3, 3, 8),
     "flag1wb_19x26": lambda: P.lake_flag1(19, True, ny=26),
     "flag1_33x16": lambda: P.lake_flag1(33, False, ny=16),
 }
