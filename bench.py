@@ -104,7 +104,7 @@ def cpu_baseline(budget_s: float = 20.0):
     p = P.lake_flag1(n, True)
 
     def run(steps):
-        hdr = np.array([5, n, steps, 1, 1, 0], dtype="<i4")
+        hdr = np.array([5, n, steps, 1, 1, 0, 5], dtype="<i4")
         fl = np.array([p.x0, p.x1, p.gravity, p.cfl], dtype="<f8")
         with tempfile.TemporaryDirectory() as td:
             fin, fout = Path(td) / "in.bin", Path(td) / "out.bin"
@@ -130,6 +130,34 @@ def cpu_baseline(budget_s: float = 20.0):
             "sample": (f"reference Fortran ({kind_note}), PATANKAR+WELLBALANCED, "
                        f"BathymetryFlag=1 lake at rest + 1e-3 bump, {n}x{n}, {k} CFL steps, "
                        f"{t:.1f} s, time loop only")}
+
+
+def schemes_section(n: int, seed: int, steps: int) -> dict:
+    """mPEuler (timescheme 6) and mPDeC2 (timescheme 7) with the by-node depth
+    view on the headline workload, `steps` CFL steps each after one warmup
+    step, through swm_run_steps.  (The reference's own depth view turns the
+    state NaN within a few steps, so it has no meaningful rate.)"""
+    import problems as P
+    import swm
+
+    prob = P.lake_bumps(n, seed)
+    res = {}
+    for ts, name in ((6, "mpeuler"), (7, "mpdec2")):
+        s = swm.Solver.from_problem(prob)
+        s.set_timescheme(ts)
+        s.set_hp_association(swm.HP_BY_NODE)
+        s.run_steps(1)
+        s.sync()
+        t0 = time.perf_counter()
+        _, iters, _ = s.run_steps(steps)
+        s.sync()
+        el = time.perf_counter() - t0
+        res[name] = {"timescheme": ts, "hp_association": "by_node", "steps": steps,
+                     "cell_steps_per_s": n * n * steps / el, "ms_per_step": 1e3 * el / steps,
+                     "jacobi_sweeps_per_step": float(np.sum(iters)) / steps}
+        log(f"bench: {name} {steps} steps {1e3 * el / steps:.2f} ms/step")
+        del s
+    return res
 
 
 def batched_section(count: int, seed: int) -> dict:
@@ -228,6 +256,8 @@ def main():
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--scheme-steps", type=int, default=20,
+                    help="steps of mPEuler and mPDeC2 at the headline size (0: skip)")
     ap.add_argument("--batch-jobs", type=int, default=256,
                     help="jobs of the batched run (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=48,
@@ -330,6 +360,8 @@ def main():
         "roofline": roof,
     }
     del s
+    if args.scheme_steps > 0:
+        out["schemes"] = schemes_section(n, args.seed, args.scheme_steps)
     if args.batch_jobs > 0:
         out["batched"] = batched_section(args.batch_jobs, args.seed)
     if args.serve_requests > 0:

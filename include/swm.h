@@ -83,10 +83,44 @@ int swm_timestep(swm_solver_t *h, double *dt_min, double *lambda_max_x, double *
  * the reference prints "Jacobi not converged" and continues -- or an error. */
 int swm_step_mpdec5(swm_solver_t *h, double t, double dt, int jacobi_iters[13]);
 
+/* The two other modified-Patankar schemes of the PATANKAR build, on the same
+ * kernels with MSteps = 2 and thetaDeC = reshape((/0, 1/2, 0, 1/2/), (/2,2/)):
+ * TimeDiscretizationByMPEuler (src/timediscretization.f90:671-764; KCorr = 1:
+ * one FVTimeDerivative, one solve) and TimeDiscretizationByMPDeC2 (:571-664;
+ * KCorr = 2: two of each).  Return values as swm_step_mpdec5; jacobi_iters
+ * receives 1 and 2 counts.  The depths they use follow swm_set_hp_association. */
+int swm_step_mpeuler(swm_solver_t *h, double t, double dt, int jacobi_iters[1]);
+int swm_step_mpdec2(swm_solver_t *h, double t, double dt, int jacobi_iters[2]);
+
+/* The scheme swm_run_steps and swm_advance step with, by the reference's
+ * timescheme number (src/parameters.f90:144; selected at
+ * src/timediscretization.f90:61-81): 5 mPDeC5 (the default), 6 mPEuler,
+ * 7 mPDeC2.  Other numbers: SWM_E_ARG. */
+int swm_set_timescheme(swm_solver_t *h, int timescheme);
+
+/* Which depths the two-node schemes put into the Patankar weights and the
+ * right-hand side (no effect on mPDeC5).  MODIFIED_PATANKAR_MATRIX_INVERSION
+ * declares hp(MSteps, nx, ny) (src/timediscretization.f90:783) but is passed
+ * Up(:,1,:,:), whose first extent is 4 (src/finitevolume.f90:161); with
+ * MSteps = 2 the dummy is sequence associated with that section, so
+ * hp(m, cell) reads Up at node ((m-1) + 2*cell) mod 4 of cell
+ * ((m-1) + 2*cell) / 4 (cells numbered from 0, x fastest), and nodes 3..4 are
+ * never written and hold 0.  The reference's mPEuler and mPDeC2 therefore
+ * drop the mass of every odd cell from the right-hand side (about half the
+ * mass in the first step) and reach NaN states within a few steps.
+ *   SWM_HP_AS_REFERENCE (default): exactly that, bitwise as the reference;
+ *   SWM_HP_BY_NODE: hp(m, cell) = Up(m, cell), the same rule as mPDeC5's.
+ * The by-node variant has no reference output to match; it conserves mass
+ * and keeps depths non-negative. */
+#define SWM_HP_AS_REFERENCE 0
+#define SWM_HP_BY_NODE 1
+int swm_set_hp_association(swm_solver_t *h, int mode);
+
 /* Several steps without returning to the caller: before each step dt is the
  * TimeStep value (a step-count stop instead of the tEnd stop of
  * src/timediscretization.f90:57-95).  dts (may be NULL) receives nsteps dt
- * values; stops early with SWM_E_STATE if dt is not finite. */
+ * values; jacobi_iters (may be NULL) nsteps rows of the scheme's solve count
+ * (13, 1 or 2).  Stops early with SWM_E_STATE if dt is not finite. */
 int swm_run_steps(swm_solver_t *h, int nsteps, double *dts, int *jacobi_iters);
 
 /* The reference's time loop on the device (src/timediscretization.f90:57-95)
@@ -142,9 +176,9 @@ int swm_sync(swm_solver_t *h);
 /* ------------------------------------------------------------------------
  * Batched and served solves (no counterpart in the reference, which runs one
  * grid per process): an engine holds any number of independent grids on one
- * GPU and advances every running grid by one mPDeC5 step per
+ * GPU and advances every running grid by one step of its scheme per
  * swm_engine_step, each with its own CFL time step, in shared kernel
- * launches.  Grids can be submitted and released between steps (continuous
+ * launches (grids of one scheme share launches).  Grids can be submitted and released between steps (continuous
  * batching); admission is checked against a device-memory budget.  Each
  * grid's results are bitwise those of the plain solver above.
  * ------------------------------------------------------------------------ */
@@ -160,6 +194,8 @@ typedef struct {
   const double *dbdx;        /* (1:nx, 1:ny, 1:4, 1:4) or NULL (flat bottom) */
   const double *dbdy;
   int nsteps;                /* CFL steps to run; 0: none (DONE at once); < 0: until released */
+  int timescheme;            /* 0 or 5: mPDeC5; 6: mPEuler; 7: mPDeC2 (swm_set_timescheme) */
+  int hp_association;        /* SWM_HP_AS_REFERENCE or SWM_HP_BY_NODE (swm_set_hp_association) */
 } swm_job_desc;
 
 #define SWM_JOB_RUNNING 0

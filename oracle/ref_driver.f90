@@ -17,9 +17,13 @@
 ! With WELLBALANCED builds the well-balanced reference state is installed the
 ! way InitializeWBVariables does it (src/finitevolume.f90:373-392).
 !
+! Builds without -DPATANKAR (the explicit schemes) have no mode 4 and write
+! no production/destruction arrays in mode 1.
+!
 ! Usage: swref <input.bin> <output.bin>
 ! Input (stream, little endian):
-!   int32 mode, nx, nsteps, bathy_flag, has_wb, ny (0: ny = nx)
+!   int32 mode, nx, nsteps, bathy_flag, has_wb, ny (0: ny = nx),
+!         timescheme (0: 5, mPDeC5; src/parameters.f90:144 numbering)
 !   real64 x0, x1, gravity, cfl    (the domain is [x0,x1]^2; dy = dx*nx/ny)
 !   real64 U(3,nx,ny)              [mode 1-3]   or  Mass(5nx*ny), Diag(nx*ny), rhs(nx*ny) [mode 4]
 !   real64 Uwb(3,nx,ny)            [if has_wb]
@@ -31,9 +35,11 @@ USE MOD_FiniteVolume2D,     ONLY: FVTimeDerivative
 USE MOD_Mesh,               ONLY: BuildMesh
 USE MOD_Equation,           ONLY: TimeStep
 USE MOD_TimeDiscretization, ONLY: TimeDiscretization
+#ifdef PATANKAR
 USE MOD_JacobiIteration,    ONLY: jacobi
+#endif
 IMPLICIT NONE
-INTEGER            :: mode, nx, ny, nsteps, bflag, haswb, k, nrow
+INTEGER            :: mode, nx, ny, nsteps, bflag, haswb, k, nrow, scheme
 INTEGER(KIND=8)    :: c0, c1, crate
 REAL               :: x0, x1, grav, cflin, dtmin
 REAL, ALLOCATABLE  :: Uic(:,:,:), Uwb(:,:,:), Mass(:), Diag(:), rhs(:), sol(:)
@@ -43,8 +49,9 @@ CALL get_command_argument(1, fin)
 CALL get_command_argument(2, fout)
 
 OPEN(10, FILE=TRIM(fin), ACCESS='stream', FORM='unformatted', STATUS='old')
-READ(10) mode, nx, nsteps, bflag, haswb, ny
+READ(10) mode, nx, nsteps, bflag, haswb, ny, scheme
 IF (ny .LE. 0) ny = nx
+IF (scheme .LE. 0) scheme = 5
 READ(10) x0, x1, grav, cflin
 
 #ifdef WELLBALANCED
@@ -70,8 +77,8 @@ BoundaryConditionsType = (/1,1,1,1/)
 BathymetryFlag    = bflag
 Reconstruction    = 4
 ReconstructionFix = Reconstruction
-timescheme        = 5
-WhichOutput       = 0   ! no snapshot files
+timescheme        = scheme
+WhichOutput      = 0   ! no snapshot files
 nOutputFiles      = 1
 InitialCondition  = 4   ! only consulted by WriteSolutionToDisk's error field
 
@@ -80,6 +87,7 @@ CALL BuildMesh()
 
 OPEN(11, FILE=TRIM(fout), ACCESS='stream', FORM='unformatted', STATUS='replace')
 
+#ifdef PATANKAR
 IF (mode .EQ. 4) THEN
   nrow = nx*ny
   ALLOCATE(Mass(5*nrow), Diag(nrow), rhs(nrow), sol(nrow))
@@ -91,6 +99,7 @@ IF (mode .EQ. 4) THEN
   CLOSE(10)
   STOP
 END IF
+#endif
 
 ALLOCATE(Uic(3,nx,ny))
 READ(10) Uic
@@ -116,7 +125,11 @@ U(1:3,1:nx,1:ny) = Uic
 SELECT CASE (mode)
   CASE (1)
     CALL FVTimeDerivative(0.)
+#ifdef PATANKAR
     WRITE(11) Ut, FX, FY, S, ProductionSparse, DestructionSparse
+#else
+    WRITE(11) Ut, FX, FY, S
+#endif
     WRITE(11) WeightsGPBnd, MESH_DX, MeshGP
   CASE (2)
     DO k = 1, nsteps

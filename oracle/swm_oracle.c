@@ -51,6 +51,7 @@ struct orc_state {
   double wbnd[NGP], wgp[NGP][NGP];
   int max_iter;
   double tol;
+  int hp_by_node;  /* 1: hp(m,i,j) = Up(m,1,i,j) instead of the sequence association */
   /* U, V: (3, -2:nx+3, -2:ny+3) */
   double *U, *V;
   /* WM, WP: (3, 4, 0:nx+1, 0:ny+1) */
@@ -673,23 +674,61 @@ int orc_jacobi(const orc_t *o, const double *Mass, const double *diag, const dou
   return k;
 }
 
-/* thetaDeC (column-major reshape) and betaDeC, src/timediscretization.f90:491-511 */
-static const double THETA[4][4] = {
-    /* THETA[m][r] = thetaDeC(m+1, r+1) */
-    {0.0000000000000000000, 0.0000000000000000000, 0.0000000000000000000, 0.0000000000000000000},
-    {0.1103005664791649049, 0.1896994335208351257, -0.0339073642291439076, 0.0103005664791649201},
-    {0.0730327668541684294, 0.4505740308958107176, 0.2269672331458314485, -0.0269672331458315692},
-    {0.0833333333333333287, 0.4166666666666666852, 0.4166666666666666852, 0.0833333333333333287}};
+/* ---- Patankar DeC schemes ------------------------------------------------- */
+
+/* A modified-Patankar DeC scheme: MSteps nodes, KCorr corrections and
+ * thetaDeC(MSteps, MSteps).  THETA[m][r] = thetaDeC(m+1, r+1) (the Fortran
+ * reshapes fill column-major).
+ *   timescheme 5  mPDeC5   src/timediscretization.f90:488-506
+ *   timescheme 6  mPEuler  src/timediscretization.f90:701-704
+ *   timescheme 7  mPDeC2   src/timediscretization.f90:601-604 */
+typedef struct {
+  int timescheme, msteps, kcorr;
+  double theta[4][4];
+} dec_scheme;
+
+static const dec_scheme SCHEMES[3] = {
+    {5, 4, 5,
+     {{0.0000000000000000000, 0.0000000000000000000, 0.0000000000000000000, 0.0000000000000000000},
+      {0.1103005664791649049, 0.1896994335208351257, -0.0339073642291439076, 0.0103005664791649201},
+      {0.0730327668541684294, 0.4505740308958107176, 0.2269672331458314485, -0.0269672331458315692},
+      {0.0833333333333333287, 0.4166666666666666852, 0.4166666666666666852, 0.0833333333333333287}}},
+    {6, 2, 1, {{0.0, 0.0}, {0.5, 0.5}}},
+    {7, 2, 2, {{0.0, 0.0}, {0.5, 0.5}}}};
+
+static const dec_scheme *find_scheme(int timescheme) {
+  for (int s = 0; s < 3; ++s)
+    if (SCHEMES[s].timescheme == timescheme) return &SCHEMES[s];
+  return NULL;
+}
 
 void orc_theta(double out[16]) {
   for (int m = 0; m < 4; ++m)
-    for (int r = 0; r < 4; ++r) out[4 * m + r] = THETA[m][r];
+    for (int r = 0; r < 4; ++r) out[4 * m + r] = SCHEMES[0].theta[m][r];
+}
+
+/* hp(m, ii, jj) of MODIFIED_PATANKAR_MATRIX_INVERSION (m 1-based).  The dummy
+ * is declared hp(MSteps, nElemsX, nElemsY) (src/timediscretization.f90:783)
+ * and the actual argument is the section Up(:,1,1:nElemsX,1:nElemsY) of an
+ * array whose first extent is 4 (src/finitevolume.f90:161), so the dummy is
+ * sequence associated with the section's elements in array element order:
+ * element L = (m-1) + MSteps*((ii-1) + nElemsX*(jj-1)) is Up(a,1,i,j) with
+ * L = (a-1) + 4*((i-1) + nElemsX*(j-1)).  For MSteps = 4 this is the
+ * identity; for MSteps = 2 it reads other cells, and nodes 3..4, which these
+ * schemes never write (they stay 0 from src/finitevolume.f90:207-208). */
+static double hp_at(const orc_t *o, int msteps, int m, int ii, int jj) {
+  if (o->hp_by_node) return o->Up[iN(o, m - 1, 0, ii, jj)];
+  size_t L = (size_t)(m - 1) + (size_t)msteps * ((size_t)(ii - 1) + (size_t)o->nx * (jj - 1));
+  int a = (int)(L % 4);
+  size_t c = L / 4;
+  int i = (int)(c % (size_t)o->nx) + 1, j = (int)(c / (size_t)o->nx) + 1;
+  return o->Up[iN(o, a, 0, i, j)];
 }
 
 /* MODIFIED_PATANKAR_MATRIX_INVERSION, src/timediscretization.f90:771-908.
- * m is the 0-based node; hp = Up(:,1,:,:) read from o->Up; ha -> Ua(m,1). */
-static int patankar_solve(orc_t *o, int m, double dt) {
-  const int nx = o->nx, ny = o->ny, n = nx * ny;
+ * m is the 0-based node (m_substep - 1); ha -> Ua(m,1). */
+static int patankar_solve(orc_t *o, const dec_scheme *sc, int m, double dt) {
+  const int nx = o->nx, ny = o->ny, n = nx * ny, ms = sc->msteps;
   for (int k = 0; k < 5 * n; ++k) o->Mass[k] = 0.;
   for (int k = 0; k < n; ++k) o->Diag[k] = 1.;
   /* neighbour offsets of slots S, W, E, N and their SparseIndexMat slot */
@@ -697,14 +736,14 @@ static int patankar_solve(orc_t *o, int m, double dt) {
   for (int jj = 1; jj <= ny; ++jj)
     for (int ii = 1; ii <= nx; ++ii) {
       int K = global_elem(o, ii, jj) - 1;
-      double hK = o->Up[iN(o, m, 0, ii, jj)];
+      double hK = hp_at(o, ms, m + 1, ii, jj);
       for (int s = 0; s < 4; ++s) {
         int iiL = ((ii + di[s] - 1) % nx + nx) % nx + 1;
         int jjL = ((jj + dj[s] - 1) % ny + ny) % ny + 1;
-        double hL = o->Up[iN(o, m, 0, iiL, jjL)];
+        double hL = hp_at(o, ms, m + 1, iiL, jjL);
         size_t KL = (size_t)5 * K + slot[s];
-        for (int rr = 0; rr < 4; ++rr) {
-          double th = THETA[m][rr];
+        for (int rr = 0; rr < ms; ++rr) {
+          double th = sc->theta[m][rr];
           double pr = o->ProdUp[4 * KL + rr], de = o->DestUp[4 * KL + rr];
           if (th > 0.) {
             o->Mass[KL] = o->Mass[KL] - dt * th * (orc_patankar_div(pr, hL));
@@ -715,7 +754,7 @@ static int patankar_solve(orc_t *o, int m, double dt) {
           }
         }
       }
-      o->rhs[K] = o->Up[iN(o, 0, 0, ii, jj)];
+      o->rhs[K] = hp_at(o, ms, 1, ii, jj);
     }
   int k = jacobi_impl(o, o->Mass, o->Diag, o->rhs, o->haVec, o->solp, o->prod);
   for (int jj = 1; jj <= ny; ++jj)
@@ -735,25 +774,35 @@ static void store_node_derivative(orc_t *o, int m) {
   }
 }
 
-static void momentum_update(orc_t *o, int m, double dt) {
+static void momentum_update(orc_t *o, const dec_scheme *sc, int m, double dt) {
   /* Ua(m,2:3) = Ua(1,2:3); DO r: Ua(m) = Ua(m) + dt*theta(m,r)*FUp(r)  (:531-543) */
   const int nx = o->nx, ny = o->ny;
   for (int jj = 1; jj <= ny; ++jj)
     for (int ii = 1; ii <= nx; ++ii)
       for (int v = 1; v < NVAR; ++v) o->Ua[iN(o, m, v, ii, jj)] = o->Ua[iN(o, 0, v, ii, jj)];
-  for (int r = 0; r < 4; ++r)
+  for (int r = 0; r < sc->msteps; ++r)
     for (int jj = 1; jj <= ny; ++jj)
       for (int ii = 1; ii <= nx; ++ii)
         for (int v = 1; v < NVAR; ++v)
           o->Ua[iN(o, m, v, ii, jj)] =
-              o->Ua[iN(o, m, v, ii, jj)] + dt * THETA[m][r] * o->FUp[iN(o, r, v, ii, jj)];
+              o->Ua[iN(o, m, v, ii, jj)] + dt * sc->theta[m][r] * o->FUp[iN(o, r, v, ii, jj)];
 }
 
-/* TimeDiscretizationByMPDeC5, src/timediscretization.f90:458-566 */
-int orc_step_mpdec5(orc_t *o, double dt, int *jacobi_iters) {
-  const int nx = o->nx, ny = o->ny;
+/* TimeDiscretizationByMPDeC5 / ByMPEuler / ByMPDeC2 (src/timediscretization.f90:
+ * 458-566, 671-764, 571-664): one loop body with the scheme's MSteps, KCorr
+ * and thetaDeC.  Nodes MSteps..3 of Ua/Up are those a run of this scheme
+ * alone leaves them: 0. */
+int orc_step_patankar(orc_t *o, int timescheme, double dt, int *jacobi_iters) {
+  const dec_scheme *sc = find_scheme(timescheme);
+  if (!sc) return -1;
+  const int nx = o->nx, ny = o->ny, ms = sc->msteps, kc = sc->kcorr;
+  const size_t node = (size_t)NVAR * nx * ny;
   int nsolve = 0, nfail = 0;
-  for (int m = 0; m < 4; ++m)
+  if (ms < 4) {
+    memset(o->Ua + ms * node, 0, (4 - ms) * node * sizeof(double));
+    memset(o->Up + ms * node, 0, (4 - ms) * node * sizeof(double));
+  }
+  for (int m = 0; m < ms; ++m)
     for (int jj = 1; jj <= ny; ++jj)
       for (int ii = 1; ii <= nx; ++ii)
         for (int v = 0; v < NVAR; ++v) {
@@ -761,18 +810,18 @@ int orc_step_mpdec5(orc_t *o, double dt, int *jacobi_iters) {
           o->Up[iN(o, m, v, ii, jj)] = o->U[iU(o, v, ii, jj)];
         }
   fv_time_derivative(o);
-  for (int m = 0; m < 4; ++m) store_node_derivative(o, m);
-  for (int kk = 1; kk <= 5; ++kk) {
-    int m0 = (kk == 5) ? 3 : 1;
-    for (int m = m0; m < 4; ++m) {
-      momentum_update(o, m, dt);
-      int it = patankar_solve(o, m, dt);
+  for (int m = 0; m < ms; ++m) store_node_derivative(o, m);
+  for (int kk = 1; kk <= kc; ++kk) {
+    int m0 = (kk == kc) ? ms - 1 : 1;
+    for (int m = m0; m < ms; ++m) {
+      momentum_update(o, sc, m, dt);
+      int it = patankar_solve(o, sc, m, dt);
       if (jacobi_iters) jacobi_iters[nsolve] = it;
       if (it >= o->max_iter) ++nfail;
       ++nsolve;
     }
-    if (kk != 5) {
-      for (int m = 1; m < 4; ++m) {
+    if (kk != kc) {
+      for (int m = 1; m < ms; ++m) {
         for (int jj = 1; jj <= ny; ++jj)
           for (int ii = 1; ii <= nx; ++ii)
             for (int v = 0; v < NVAR; ++v) o->U[iU(o, v, ii, jj)] = o->Ua[iN(o, m, v, ii, jj)];
@@ -780,10 +829,16 @@ int orc_step_mpdec5(orc_t *o, double dt, int *jacobi_iters) {
         store_node_derivative(o, m);
       }
     }
-    memcpy(o->Up, o->Ua, sizeof(double) * 4 * NVAR * (size_t)nx * ny);
+    memcpy(o->Up, o->Ua, sizeof(double) * 4 * node);
   }
   for (int jj = 1; jj <= ny; ++jj)
     for (int ii = 1; ii <= nx; ++ii)
-      for (int v = 0; v < NVAR; ++v) o->U[iU(o, v, ii, jj)] = o->Ua[iN(o, 3, v, ii, jj)];
+      for (int v = 0; v < NVAR; ++v) o->U[iU(o, v, ii, jj)] = o->Ua[iN(o, ms - 1, v, ii, jj)];
   return nfail;
 }
+
+int orc_step_mpdec5(orc_t *o, double dt, int *jacobi_iters) {
+  return orc_step_patankar(o, 5, dt, jacobi_iters);
+}
+
+void orc_set_hp_by_node(orc_t *o, int on) { o->hp_by_node = on ? 1 : 0; }

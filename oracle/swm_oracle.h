@@ -52,6 +52,18 @@ double orc_timestep(orc_t *o, double *lmx, double *lmy);
  * execution order.  Returns the number of solves that hit max_iter. */
 int orc_step_mpdec5(orc_t *o, double dt, int *jacobi_iters);
 
+/* One step of a modified-Patankar scheme, by the reference's timescheme
+ * number: 5 mPDeC5, 6 mPEuler (src/timediscretization.f90:671-764), 7 mPDeC2
+ * (:571-664).  jacobi_iters receives 13, 1 and 2 counts respectively.
+ * Returns the non-converged count, or -1 for another number. */
+int orc_step_patankar(orc_t *o, int timescheme, double dt, int *jacobi_iters);
+
+/* on = 1: the two-node schemes read hp(m,i,j) = Up(m,1,i,j) (the evident
+ * intent) instead of the reference's sequence association of hp with
+ * Up(:,1,:,:) (src/timediscretization.f90:783 vs src/finitevolume.f90:161).
+ * No reference output exists for on = 1; it is pinned by invariants only. */
+void orc_set_hp_by_node(orc_t *o, int on);
+
 /* Jacobi on the reference CSR pattern (src/jacobi_iteration.f90:52-87).
  * Mass (5n), diag (n), rhs (n) -> sol (n).  Returns the sweep count. */
 int orc_jacobi(const orc_t *o, const double *Mass, const double *diag,

@@ -145,10 +145,22 @@ struct Job {
   int last_iters[13] = {0};
   int pred[13];
   bool have_pred = false;  // pred holds the sweep counts of an accepted step
+  const SchemeDef* pred_sc = nullptr;  // ... of this scheme
   bool caps_set = false;   // the device status holds time-loop caps
   bool wb_ready = false;
+  const SchemeDef* scheme = &kSchemes[0];  // time scheme of engine / multi-step runs
+  int hp_assoc = SWM_HP_AS_REFERENCE;     // depth view of the two-node schemes
   Tables own;          // tables covering this job alone
 };
+
+// Jobs step together when they share a scheme and, for the two-node schemes,
+// the depth view: group 0 mPDeC5, 1-2 mPEuler, 3-4 mPDeC2 (reference view,
+// by-node view).
+constexpr int kGroups = 5;
+int group_of(const SchemeDef* d, int hp_assoc) {
+  const int si = (int)(d - kSchemes);
+  return si == 0 ? 0 : 1 + 2 * (si - 1) + (hp_assoc == SWM_HP_BY_NODE ? 1 : 0);
+}
 
 }  // namespace
 
@@ -170,7 +182,7 @@ struct swm_engine {
   };
   Caps* hcaps = nullptr;         // pinned: the time-loop clipping of one step
   bool dirty = true;             // device descriptors / tables out of date
-  Tables all;
+  Tables groups[kGroups];        // running jobs, by step group (group_of)
   int max_iter = 1000;
   double tol = 1.e-16;
   long long ambiguous = 0;
@@ -190,8 +202,12 @@ struct swm_engine {
     int device_dt = 0, track = 0;
     int sweeps[5] = {0, 0, 0, 0, 0};
     const void* tables = nullptr;
+    int group = -1;
+    long long used = 0;
     hipGraphExec_t exec = nullptr;
-  } cap;
+  };
+  Captured caps[kGroups];        // one captured step per table set in use
+  long long cap_clock = 0;
   Profiler prof;
   std::string err;
 };
@@ -281,16 +297,21 @@ int refresh(swm_engine* e) {
   std::vector<JobDev> host(nj);
   for (size_t i = 0; i < nj; ++i) host[i] = e->jobs[i]->dev;
   if (nj) HIPCK(e, hipMemcpy(e->djobs.p, host.data(), nj * sizeof(JobDev), hipMemcpyHostToDevice));
-  Tables& T = e->all;
-  T.xt.clear();
-  T.yt.clear();
-  T.ct.clear();
-  T.cb.clear();
-  T.jl.clear();
-  for (size_t i = 0; i < nj; ++i)
-    if (e->jobs[i]->state == JOB_RUNNING) add_tiles(T, (int)i, e->jobs[i]->dev);
-  int rc = upload_tables(e, T);
-  if (rc) return rc;
+  for (Tables& T : e->groups) {
+    T.xt.clear();
+    T.yt.clear();
+    T.ct.clear();
+    T.cb.clear();
+    T.jl.clear();
+  }
+  for (size_t i = 0; i < nj; ++i) {
+    const Job& J = *e->jobs[i];
+    if (J.state == JOB_RUNNING) add_tiles(e->groups[group_of(J.scheme, J.hp_assoc)], (int)i, J.dev);
+  }
+  for (Tables& T : e->groups) {
+    int rc = upload_tables(e, T);
+    if (rc) return rc;
+  }
   e->dirty = false;
   ++e->graph_epoch;  // job descriptors and tables may have moved
   return SWM_OK;
@@ -323,14 +344,16 @@ int read_status(swm_engine* e) {
 }
 
 // ---------------------------------------------------------------------------
-// One mPDeC5 step (src/timediscretization.f90:458-566) of every job in T.
+// One step of a modified-Patankar DeC scheme of every job in T: mPDeC5
+// (src/timediscretization.f90:458-566), mPEuler (:671-764) or mPDeC2
+// (:571-664), which differ only in MSteps, KCorr and thetaDeC.
 //
-// The launch sequence: [TimeStep] -> FVTD(U^n) -> 5 corrections, each
-// momentum -> assembly (+ first sweep) -> Jacobi sweeps -> record -> finalize
-// -> FVTD of the new nodes (corrections 1..4) -> (after the host check)
-// commit.  Each sweep is followed by the device-side stop decision, and a
-// sweep of a stopped system is a no-op, so launching more sweeps than a solve
-// needs changes nothing.
+// The launch sequence: [TimeStep] -> FVTD(U^n) -> KCorr corrections, each
+// [depth view] -> momentum -> assembly (+ first sweep) -> Jacobi sweeps ->
+// record -> finalize -> FVTD of the new nodes (all corrections but the last)
+// -> (after the host check) commit.  Each sweep is followed by the
+// device-side stop decision, and a sweep of a stopped system is a no-op, so
+// launching more sweeps than a solve needs changes nothing.
 //
 // Polling mode reads the status back during each correction until every
 // system has stopped.  Speculative mode launches a fixed number of sweeps per
@@ -341,10 +364,16 @@ int read_status(swm_engine* e) {
 // once as a HIP graph and replayed.
 // ---------------------------------------------------------------------------
 struct StepPlan {
+  const SchemeDef* sc;
+  int hp_view;    // the reference's depth view of the two-node schemes
   bool device_dt;
   bool speculative;
   int sweeps[5];  // speculative: sweeps per correction, counting the fused one
 };
+
+// Systems solved in correction kk (1-based): nodes 1..MSteps-1, or only the
+// last node in the last correction.
+int systems_in(const SchemeDef& D, int kk) { return kk == D.kcorr ? 1 : D.msteps - 1; }
 
 // The step's launches; returns after the last finalize / FVTD (no commit).
 int step_body(swm_engine* e, Tables& T, const StepPlan& P) {
@@ -354,7 +383,8 @@ int step_body(swm_engine* e, Tables& T, const StepPlan& P) {
     timed(e, 6, [&] { launch_timestep(a, T.dcb.p, T.ncb, T.djl.p, T.njobs, s); });
     LAUNCHCK(e);
   }
-  // FVTimeDerivative(U^n): node 0, shared by all four nodes at kk = 1
+  const SchemeDef& D = *P.sc;
+  // FVTimeDerivative(U^n): node 0, shared by all nodes at kk = 1
   NodeList L0{};
   L0.count = 1;
   L0.m[0] = 0;
@@ -365,13 +395,17 @@ int step_body(swm_engine* e, Tables& T, const StepPlan& P) {
   for (int r = 0; r < 4; ++r) S.src[r] = 0;
   S.hp_is_un = 1;
   S.track_min = e->track_min ? 1 : 0;
+  S.nr = D.msteps;
+  S.theta = D.theta;
+  S.hp_view = P.hp_view;
   int solve = 0;
-  for (int kk = 1; kk <= 5; ++kk) {
+  for (int kk = 1; kk <= D.kcorr; ++kk) {
     S.count = 0;
-    if (kk == 5)
-      S.m[S.count++] = 3;
+    if (kk == D.kcorr)
+      S.m[S.count++] = D.msteps - 1;
     else
-      for (int m = 1; m < 4; ++m) S.m[S.count++] = m;
+      for (int m = 1; m < D.msteps; ++m) S.m[S.count++] = m;
+    if (S.hp_view) timed(e, 4, [&] { launch_hp_view(a, T.dcb.p, T.ncb, kk == 1, s); });
     timed(e, 3, [&] { launch_momentum(a, T.dcb.p, T.ncb, S, s); });
     timed(e, 4, [&] { launch_assemble(a, T.dcb.p, T.ncb, T.djl.p, T.njobs, S, s); });
     LAUNCHCK(e);
@@ -413,7 +447,7 @@ int step_body(swm_engine* e, Tables& T, const StepPlan& P) {
     timed(e, 7, [&] { launch_finalize(a, T.dcb.p, T.ncb, T.djl.p, T.njobs, S, s); });
     LAUNCHCK(e);
     solve += S.count;
-    if (kk != 5) {
+    if (kk != D.kcorr) {
       NodeList L{};
       L.count = S.count;
       for (int k = 0; k < S.count; ++k) {
@@ -429,20 +463,36 @@ int step_body(swm_engine* e, Tables& T, const StepPlan& P) {
   return SWM_OK;
 }
 
-void drop_graph(swm_engine* e) {
-  if (e->cap.exec) (void)hipGraphExecDestroy(e->cap.exec);
-  e->cap = swm_engine::Captured{};
+void drop_graph(swm_engine::Captured& c) {
+  if (c.exec) (void)hipGraphExecDestroy(c.exec);
+  c = swm_engine::Captured{};
 }
 
-// Speculative body through the captured graph (capturing it when the job set
-// or the sweep counts changed).
+void drop_graphs(swm_engine* e) {
+  for (auto& c : e->caps) drop_graph(c);
+}
+
+// Speculative body through a captured graph (capturing it when the job set,
+// the scheme or the sweep counts changed).  Each table set keeps its own
+// graph, so groups of jobs stepped one after another do not recapture.
 int step_body_graph(swm_engine* e, Tables& T, const StepPlan& P) {
-  auto& c = e->cap;
+  const int group = group_of(P.sc, P.hp_view ? SWM_HP_AS_REFERENCE : SWM_HP_BY_NODE);
+  swm_engine::Captured* cp = nullptr;
+  for (auto& c : e->caps)
+    if (c.exec && c.tables == (const void*)&T) cp = &c;
+  if (!cp) {  // a free entry, else the least recently used
+    cp = &e->caps[0];
+    for (auto& c : e->caps)
+      if (!c.exec || (cp->exec && c.used < cp->used)) cp = &c;
+  }
+  auto& c = *cp;
   bool same = c.exec && c.epoch == e->graph_epoch && c.device_dt == (int)P.device_dt &&
-              c.track == (int)e->track_min && c.tables == (const void*)&T;
+              c.track == (int)e->track_min && c.tables == (const void*)&T && c.group == group;
   for (int i = 0; same && i < 5; ++i) same = c.sweeps[i] == P.sweeps[i];
+  c.used = ++e->cap_clock;
   if (!same) {
-    drop_graph(e);
+    drop_graph(c);
+    c.used = e->cap_clock;
     HIPCK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     int rc = step_body(e, T, P);
     hipGraph_t g = nullptr;
@@ -461,6 +511,7 @@ int step_body_graph(swm_engine* e, Tables& T, const StepPlan& P) {
     c.device_dt = P.device_dt;
     c.track = e->track_min;
     c.tables = &T;
+    c.group = group;
     for (int i = 0; i < 5; ++i) c.sweeps[i] = P.sweeps[i];
     ++e->graph_captures;
   }
@@ -501,8 +552,8 @@ void tune_record(swm_engine* e, double ms) {
 // Runs one step of the jobs in T and leaves their status in e->hstatus.
 // host_dt: the step uses this dt for every job (the plain solver's
 // swm_step_mpdec5); otherwise each job's TimeStep is computed on the device.
-int engine_step(swm_engine* e, Tables& T, const double* host_dt, int iters[13],
-                const double* caps = nullptr) {
+int engine_step(swm_engine* e, Tables& T, const SchemeDef* sc, int hp_assoc,
+                const double* host_dt, int iters[13], const double* caps = nullptr) {
   if (T.njobs == 0) return SWM_OK;
   hipStream_t s = e->stream;
   // the time-loop clipping (caps = {tAnalyze - t, tEnd - t}) or none
@@ -525,21 +576,27 @@ int engine_step(swm_engine* e, Tables& T, const double* host_dt, int iters[13],
     }
   }
   StepPlan P{};
+  P.sc = sc;
+  P.hp_view = (sc->msteps < 4 && hp_assoc == SWM_HP_AS_REFERENCE) ? 1 : 0;
   P.device_dt = host_dt == nullptr;
   bool spec = e->margin != -2;
-  for (int q = 0; q < T.njobs; ++q) spec = spec && e->jobs[T.jl[q]]->have_pred;
+  for (int q = 0; q < T.njobs; ++q) {
+    const Job& J = *e->jobs[T.jl[q]];
+    spec = spec && J.have_pred && J.pred_sc == sc;
+  }
   bool accepted = false;
   if (spec) {
     int graph, margin;
     tune_choice(e, graph, margin);
     P.speculative = true;
-    static const int first[5] = {0, 3, 6, 9, 12};
-    for (int kk = 0; kk < 5; ++kk) {
+    int first = 0;
+    for (int kk = 0; kk < sc->kcorr; ++kk) {
       int mx = 1;
-      const int ns = kk == 4 ? 1 : 3;
+      const int ns = systems_in(*sc, kk + 1);
       for (int q = 0; q < T.njobs; ++q)
-        for (int k = 0; k < ns; ++k) mx = std::max(mx, e->jobs[T.jl[q]]->pred[first[kk] + k]);
+        for (int k = 0; k < ns; ++k) mx = std::max(mx, e->jobs[T.jl[q]]->pred[first + k]);
       P.sweeps[kk] = std::min(mx + margin, e->max_iter);
+      first += ns;
     }
     const auto t0 = std::chrono::steady_clock::now();
     int rc = (graph && !e->prof.on) ? step_body_graph(e, T, P) : step_body(e, T, P);
@@ -564,22 +621,24 @@ int engine_step(swm_engine* e, Tables& T, const double* host_dt, int iters[13],
     if (rc) return rc;
   }
   EngineArgs a = eargs(e);
-  timed(e, 7, [&] { launch_commit(a, T.dcb.p, T.ncb, s); });
+  timed(e, 7, [&] { launch_commit(a, T.dcb.p, T.ncb, sc->msteps - 1, s); });
   LAUNCHCK(e);
   for (int q = 0; q < T.njobs; ++q) {
     Job& J = *e->jobs[T.jl[q]];
     const JobStatus& js = e->hstatus[T.jl[q]];
     for (int i = 0; i < 13; ++i) {
-      J.last_iters[i] = js.iters[i];
-      J.pred[i] = js.iters[i];
-      J.jacobi_sweeps += js.iters[i];
+      const int it = i < sc->nsolve ? js.iters[i] : 0;
+      J.last_iters[i] = it;
+      J.pred[i] = it;
+      J.jacobi_sweeps += it;
     }
     J.nonconv += js.nonconv;
     e->ambiguous += js.ambiguous;
     J.have_pred = true;
+    J.pred_sc = sc;
   }
   if (iters && T.njobs == 1)
-    for (int i = 0; i < 13; ++i) iters[i] = e->hstatus[T.jl[0]].iters[i];
+    for (int i = 0; i < sc->nsolve; ++i) iters[i] = e->hstatus[T.jl[0]].iters[i];
   return SWM_OK;
 }
 
@@ -755,6 +814,11 @@ int alloc_job(swm_engine* e, int nx, int ny, double dx, double dy, double g, dou
   J.nonconv = 0;
   J.wb_ready = false;
   for (int i = 0; i < 13; ++i) J.pred[i] = 4;
+  J.have_pred = false;
+  J.pred_sc = nullptr;
+  J.caps_set = false;
+  J.scheme = &kSchemes[0];
+  J.hp_assoc = SWM_HP_AS_REFERENCE;
   e->dirty = true;
   add_tiles(J.own, id, D);
   ++e->graph_epoch;
@@ -799,6 +863,10 @@ int engine_add(swm_engine* e, const swm_job_desc* d, int* id_out) {
   if (!d->U0) FAIL(e, SWM_E_ARG, "U0 is NULL");
   if (d->wellbalanced && !d->Uwb) FAIL(e, SWM_E_ARG, "wellbalanced job needs Uwb");
   if (!d->dbdx != !d->dbdy) FAIL(e, SWM_E_ARG, "dbdx and dbdy must both be set or both NULL");
+  const SchemeDef* sc = find_scheme(d->timescheme == 0 ? 5 : d->timescheme);
+  if (!sc) FAIL(e, SWM_E_ARG, "timescheme %d is not 0, 5, 6 or 7", d->timescheme);
+  if (d->hp_association != SWM_HP_AS_REFERENCE && d->hp_association != SWM_HP_BY_NODE)
+    FAIL(e, SWM_E_ARG, "hp_association must be SWM_HP_AS_REFERENCE or SWM_HP_BY_NODE");
   const long long extra = d->dbdx ? (long long)bathy_bytes((size_t)d->nx * d->ny) : 0;
   int id;
   int rc = alloc_job(e, d->nx, d->ny, d->dx, d->dy, d->gravity, d->cfl, d->wellbalanced ? 1 : 0,
@@ -806,6 +874,8 @@ int engine_add(swm_engine* e, const swm_job_desc* d, int* id_out) {
   if (rc) return rc;
   *id_out = id;
   Job& J = *e->jobs[id];
+  J.scheme = sc;
+  J.hp_assoc = d->hp_association;
   if (d->dbdx) {
     rc = job_set_bathy(e, id, d->dbdx, d->dbdy);
     if (rc) return rc;
@@ -888,7 +958,7 @@ int swm_engine_create(long long budget_bytes, swm_engine_t** out) {
 void swm_engine_destroy(swm_engine_t* e) {
   if (!e) return;
   if (e->stream) (void)hipStreamSynchronize(e->stream);
-  drop_graph(e);
+  drop_graphs(e);
   for (size_t i = 0; i < e->jobs.size(); ++i) engine_free_job(e, (int)i);
   e->jobs.clear();
   if (e->hstatus) (void)hipHostFree(e->hstatus);
@@ -934,26 +1004,30 @@ int swm_engine_step(swm_engine_t* e, int* n_running) {
   if (!e) return SWM_E_ARG;
   int rc = refresh(e);
   if (rc) return rc;
-  rc = engine_step(e, e->all, nullptr, nullptr);
-  if (rc) return rc;
-  HIPCK(e, hipStreamSynchronize(e->stream));
   int running = 0;
-  for (int q = 0; q < e->all.njobs; ++q) {
-    const int id = e->all.jl[q];
-    Job& J = *e->jobs[id];
-    const JobStatus& js = e->hstatus[id];
-    if (js.dt_bad) {  // the step was not committed: the job keeps its last state
-      J.state = JOB_FAILED;
-      e->dirty = true;
-      continue;
-    }
-    J.steps_done += 1;
-    J.t += js.dt;
-    if (J.steps_left > 0 && --J.steps_left == 0) {
-      J.state = JOB_DONE;
-      e->dirty = true;
-    } else {
-      ++running;
+  for (Tables& T : e->groups) {
+    if (T.njobs == 0) continue;
+    const Job& J0 = *e->jobs[T.jl[0]];
+    rc = engine_step(e, T, J0.scheme, J0.hp_assoc, nullptr, nullptr);
+    if (rc) return rc;
+    HIPCK(e, hipStreamSynchronize(e->stream));
+    for (int q = 0; q < T.njobs; ++q) {
+      const int id = T.jl[q];
+      Job& J = *e->jobs[id];
+      const JobStatus& js = e->hstatus[id];
+      if (js.dt_bad) {  // the step was not committed: the job keeps its last state
+        J.state = JOB_FAILED;
+        e->dirty = true;
+        continue;
+      }
+      J.steps_done += 1;
+      J.t += js.dt;
+      if (J.steps_left > 0 && --J.steps_left == 0) {
+        J.state = JOB_DONE;
+        e->dirty = true;
+      } else {
+        ++running;
+      }
     }
   }
   if (n_running) *n_running = running;
@@ -1166,23 +1240,59 @@ int swm_timestep(swm_solver_t* s, double* dt_min, double* lmx, double* lmy) {
   return SWM_OK;
 }
 
-int swm_step_mpdec5(swm_solver_t* s, double t, double dt, int jacobi_iters[13]) {
-  (void)t;
+// One step of the given scheme with a caller-provided dt (the TimeDiscretizationBy*
+// routines); jacobi_iters receives the scheme's solve count of values.
+static int plain_step(swm_solver_t* s, int timescheme, double dt, int* jacobi_iters) {
   if (!s) return SWM_E_ARG;
   swm_engine* e = s->e;
   Job& J = *plain_job(s);
   if (J.dev.wb && !J.wb_ready)
     FAIL(e, SWM_E_STATE, "well-balanced state not initialised (swm_init_wb)");
+  const SchemeDef* sc = find_scheme(timescheme);
   int rc = refresh(e);
   if (rc) return rc;
   const int before = J.nonconv;
   int iters[13] = {0};
-  rc = engine_step(e, J.own, &dt, iters);
+  rc = engine_step(e, J.own, sc, J.hp_assoc, &dt, iters);
   if (rc) return rc;
   if (jacobi_iters)
-    for (int i = 0; i < 13; ++i) jacobi_iters[i] = iters[i];
+    for (int i = 0; i < sc->nsolve; ++i) jacobi_iters[i] = iters[i];
   J.steps_done += 1;
   return J.nonconv - before;
+}
+
+int swm_step_mpdec5(swm_solver_t* s, double t, double dt, int jacobi_iters[13]) {
+  (void)t;
+  return plain_step(s, 5, dt, jacobi_iters);
+}
+
+int swm_step_mpeuler(swm_solver_t* s, double t, double dt, int jacobi_iters[1]) {
+  (void)t;
+  return plain_step(s, 6, dt, jacobi_iters);
+}
+
+int swm_step_mpdec2(swm_solver_t* s, double t, double dt, int jacobi_iters[2]) {
+  (void)t;
+  return plain_step(s, 7, dt, jacobi_iters);
+}
+
+int swm_set_timescheme(swm_solver_t* s, int timescheme) {
+  if (!s) return SWM_E_ARG;
+  const SchemeDef* sc = find_scheme(timescheme);
+  if (!sc)
+    FAIL(s->e, SWM_E_ARG, "timescheme %d is not a modified-Patankar scheme (5, 6 or 7)",
+         timescheme);
+  plain_job(s)->scheme = sc;
+  return SWM_OK;
+}
+
+int swm_set_hp_association(swm_solver_t* s, int mode) {
+  if (!s) return SWM_E_ARG;
+  if (mode != SWM_HP_AS_REFERENCE && mode != SWM_HP_BY_NODE)
+    FAIL(s->e, SWM_E_ARG, "hp association must be SWM_HP_AS_REFERENCE or SWM_HP_BY_NODE (got %d)",
+         mode);
+  plain_job(s)->hp_assoc = mode;
+  return SWM_OK;
 }
 
 int swm_run_steps(swm_solver_t* s, int nsteps, double* dts, int* jacobi_iters) {
@@ -1199,7 +1309,7 @@ int swm_run_steps(swm_solver_t* s, int nsteps, double* dts, int* jacobi_iters) {
     // dt from TimeStep on the device, within the step; a step whose dt is
     // not finite is not committed, so the state stays that of step k
     int iters[13] = {0};
-    rc = engine_step(e, J.own, nullptr, iters);
+    rc = engine_step(e, J.own, J.scheme, J.hp_assoc, nullptr, iters);
     if (rc) return rc;
     const JobStatus& js = e->hstatus[s->job];
     if (dts) dts[k] = js.dt;
@@ -1207,8 +1317,9 @@ int swm_run_steps(swm_solver_t* s, int nsteps, double* dts, int* jacobi_iters) {
       HIPCK(e, hipStreamSynchronize(e->stream));
       FAIL(e, SWM_E_STATE, "time step %d: dt is not finite (%g)", k, js.dt);
     }
+    const int ns = J.scheme->nsolve;
     if (jacobi_iters)
-      for (int i = 0; i < 13; ++i) jacobi_iters[13 * k + i] = iters[i];
+      for (int i = 0; i < ns; ++i) jacobi_iters[ns * k + i] = iters[i];
     J.steps_done += 1;
   }
   HIPCK(e, hipStreamSynchronize(e->stream));
@@ -1231,7 +1342,7 @@ int swm_advance(swm_solver_t* s, double* t, double t_analyze, double t_end, int 
   while (k < max_steps) {
     // src/timediscretization.f90:58-59 and :83-94
     const double caps[2] = {t_analyze - *t, t_end - *t};
-    rc = engine_step(e, J.own, nullptr, nullptr, caps);
+    rc = engine_step(e, J.own, J.scheme, J.hp_assoc, nullptr, nullptr, caps);
     if (rc) return rc;
     const JobStatus& js = e->hstatus[s->job];
     if (js.dt_bad) {

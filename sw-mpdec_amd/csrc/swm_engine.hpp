@@ -2,7 +2,8 @@
 // launch interface shared by swm_kernels.hip and swm_engine.cpp.
 //
 // The engine advances any number of independent periodic grids ("jobs") in
-// lockstep, one mPDeC5 step per engine step.  Every kernel launch covers all
+// lockstep, one step of its modified-Patankar scheme (mPDeC5, mPEuler or
+// mPDeC2) per engine step, jobs of one scheme together.  Every kernel launch covers all
 // active jobs: a block looks up (job, tile) in a tile table built on the host
 // whenever the job set changes.  A single plain solve is a one-job engine.
 //
@@ -117,7 +118,25 @@ struct SysList {
   int src[4];           // buffer index r' holding node r's FU/q (0 at kk = 1, else r)
   int hp_is_un;         // 1: the denominators use h^n (first correction)
   int track_min;        // 1: fold the new stage depths into JobStatus::hmin
+  int nr;               // thetaDeC columns summed (MSteps): 4 or 2
+  int theta;            // THETA block: 0 mPDeC5, 1 the two-node schemes
+  int hp_view;          // 1: denominators from hres[3], rhs from hres[2] (k_hp_view)
 };
+
+// The modified-Patankar schemes, by the reference's timescheme number.
+struct SchemeDef {
+  int timescheme;  // src/parameters.f90:144 numbering
+  int msteps;      // DeC nodes
+  int kcorr;       // corrections
+  int theta;       // SysList::theta
+  int nsolve;      // Jacobi solves per step
+};
+constexpr SchemeDef kSchemes[3] = {{5, 4, 5, 0, 13}, {6, 2, 1, 1, 1}, {7, 2, 2, 1, 2}};
+inline const SchemeDef* find_scheme(int timescheme) {
+  for (const SchemeDef& d : kSchemes)
+    if (d.timescheme == timescheme) return &d;
+  return nullptr;
+}
 
 // ---- launches (swm_kernels.hip) -------------------------------------------
 void launch_fvtd(const EngineArgs& E, const TileRef* xt, int nxt, const TileRef* yt, int nyt,
@@ -132,7 +151,9 @@ void launch_finalize(const EngineArgs& E, const TileRef* cb, int ncb, const int*
                      const SysList& S, hipStream_t s);
 void launch_timestep(const EngineArgs& E, const TileRef* cb, int ncb, const int* jl, int njobs,
                      hipStream_t s);
-void launch_commit(const EngineArgs& E, const TileRef* cb, int ncb, hipStream_t s);
+void launch_commit(const EngineArgs& E, const TileRef* cb, int ncb, int final_node,
+                   hipStream_t s);
+void launch_hp_view(const EngineArgs& E, const TileRef* cb, int ncb, int from_un, hipStream_t s);
 void launch_solve_done(const EngineArgs& E, const int* jl, int njobs, const SysList& S,
                        int first_solve, hipStream_t s);
 void launch_copy_wb(const EngineArgs& E, const TileRef* cb, int ncb, int slot, hipStream_t s);

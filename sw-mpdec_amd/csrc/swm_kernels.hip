@@ -22,12 +22,16 @@
 
 namespace swm {
 
-// thetaDeC (src/timediscretization.f90:491-506): THETA[m][r] = thetaDeC(m+1, r+1)
-__constant__ double THETA[4][4] = {
-    {0.0000000000000000000, 0.0000000000000000000, 0.0000000000000000000, 0.0000000000000000000},
-    {0.1103005664791649049, 0.1896994335208351257, -0.0339073642291439076, 0.0103005664791649201},
-    {0.0730327668541684294, 0.4505740308958107176, 0.2269672331458314485, -0.0269672331458315692},
-    {0.0833333333333333287, 0.4166666666666666852, 0.4166666666666666852, 0.0833333333333333287}};
+// thetaDeC: THETA[b][m][r] = thetaDeC(m+1, r+1) of block b
+//   b = 0  mPDeC5  (src/timediscretization.f90:491-506)
+//   b = 1  mPEuler and mPDeC2, reshape((/0, 1/2, 0, 1/2/), (/2,2/))
+//          (src/timediscretization.f90:604,704)
+__constant__ double THETA[2][4][4] = {
+    {{0.0000000000000000000, 0.0000000000000000000, 0.0000000000000000000, 0.0000000000000000000},
+     {0.1103005664791649049, 0.1896994335208351257, -0.0339073642291439076, 0.0103005664791649201},
+     {0.0730327668541684294, 0.4505740308958107176, 0.2269672331458314485, -0.0269672331458315692},
+     {0.0833333333333333287, 0.4166666666666666852, 0.4166666666666666852, 0.0833333333333333287}},
+    {{0.0, 0.0, 0.0, 0.0}, {0.5, 0.5, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}}};
 
 // Global-address-space view of a device array.  The arrays' pointers are read
 // from the job table in memory, so the compiler cannot infer their address
@@ -352,9 +356,11 @@ __global__ __launch_bounds__(256) void k_fvtd_update(EngineArgs E, const TileRef
 }
 
 // ------------------------------------------------------------------------
-// DeC momentum update, src/timediscretization.f90:531-543:
-//   Ua(m,2:3) = Ua(1,2:3); DO r=1,4: Ua(m) = Ua(m) + (dt*theta(m,r))*FUp(r)
+// DeC momentum update, src/timediscretization.f90:531-543 (and :617-621,
+// :717-721): Ua(m,2:3) = Ua(1,2:3); DO r=1,R: Ua(m) = Ua(m) + (dt*theta(m,r))*FUp(r)
+// with R = MSteps.
 // ------------------------------------------------------------------------
+template <int R>
 __global__ __launch_bounds__(256) void k_momentum(EngineArgs E, const TileRef* blocks,
                                                   SysList S) {
   const TileRef T = blocks[blockIdx.x];
@@ -366,8 +372,8 @@ __global__ __launch_bounds__(256) void k_momentum(EngineArgs E, const TileRef* b
     const int m = S.m[s];
     double u = G(J.hu)[k], v = G(J.hv)[k];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const double c = dt * THETA[m][r];
+    for (int r = 0; r < R; ++r) {
+      const double c = dt * THETA[S.theta][m][r];
       u = u + c * G(J.FU2[S.src[r]])[k];
       v = v + c * G(J.FU3[S.src[r]])[k];
     }
@@ -450,9 +456,11 @@ __device__ __forceinline__ double patankar_div_zero(double z, double h) {
   return (h < kMinDepth) ? 0. : z * (h * 0. + 1.);
 }
 
-__device__ __forceinline__ void patankar_slot(int m, double dt, const double q[4], bool east,
-                                              double hK, double hL, double& M, double& D) {
-  // production / destruction of the slot at the 4 DeC nodes, from the face
+template <int R>
+__device__ __forceinline__ void patankar_slot(const double (&th_m)[4], double dt, const double q[R],
+                                              bool east, double hK, double hL, double& M,
+                                              double& D) {
+  // production / destruction of the slot at the R DeC nodes, from the face
   // mass flux q = F/dx: west/south slots take max(F,0)/dx, -min(F,0)/dx; the
   // east/north slots the same of -F (src/finitevolume.f90:496-523).
   // theta > 0: M = M - c*PD(P, hL), D = D + c*PD(De, hK);
@@ -461,7 +469,7 @@ __device__ __forceinline__ void patankar_slot(int m, double dt, const double q[4
   // zero one goes through patankar_div_zero, which gives the same value.
   M = 0.;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
+  for (int r = 0; r < R; ++r) {
     const double qr = q[r];
     double P, De;
     if (!east) {
@@ -471,7 +479,7 @@ __device__ __forceinline__ void patankar_slot(int m, double dt, const double q[4
       P = (qr < 0.) ? -qr : 0.;
       De = (qr > 0.) ? qr : -0.;
     }
-    const double th = THETA[m][r];
+    const double th = th_m[r];
     const double c = dt * th;
     const bool pos = th > 0.;
     const double numL = pos ? P : De, numK = pos ? De : P;
@@ -484,10 +492,21 @@ __device__ __forceinline__ void patankar_slot(int m, double dt, const double q[4
   }
 }
 
+// The denominators and right-hand side of the systems: hp(m,.) is h^n in the
+// first correction and node m's depth after it; the rhs is h^n.  With
+// S.hp_view both come from the buffers k_hp_view fills.
+__device__ __forceinline__ const gdouble* system_hp(const JobDev& J, const SysList& S, int m) {
+  return G(S.hp_view ? J.hres[3] : (S.hp_is_un ? J.h : J.hres[m]));
+}
+__device__ __forceinline__ const gdouble* system_rhs(const JobDev& J, const SysList& S) {
+  return G(S.hp_view ? J.hres[2] : J.h);
+}
+
+template <int R>
 __global__ __launch_bounds__(256) void k_assemble(EngineArgs E, const TileRef* blocks,
                                                   SysList S) {
   __shared__ double red[4];
-  __shared__ double sq[16 * 256];  // [slot*4 + node][thread]: face mass fluxes
+  __shared__ double sq[4 * R * 256];  // [slot*R + node][thread]: face mass fluxes
   const TileRef T = blocks[blockIdx.x];
   const JobDev& J = E.jobs[T.job];
   const int nx = J.nx, ny = J.ny, n = J.n;
@@ -508,17 +527,18 @@ __global__ __launch_bounds__(256) void k_assemble(EngineArgs E, const TileRef* b
     const size_t fW = (size_t)j * (nx + 1) + i, fE = fW + 1;
     const size_t gS = (size_t)j * nx + i, gN = gS + nx;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      sq[(0 * 4 + r) * 256 + tid] = G(J.qy[S.src[r]])[gS];
-      sq[(1 * 4 + r) * 256 + tid] = G(J.qx[S.src[r]])[fW];
-      sq[(2 * 4 + r) * 256 + tid] = G(J.qx[S.src[r]])[fE];
-      sq[(3 * 4 + r) * 256 + tid] = G(J.qy[S.src[r]])[gN];
+    for (int r = 0; r < R; ++r) {
+      sq[(0 * R + r) * 256 + tid] = G(J.qy[S.src[r]])[gS];
+      sq[(1 * R + r) * 256 + tid] = G(J.qx[S.src[r]])[fW];
+      sq[(2 * R + r) * 256 + tid] = G(J.qx[S.src[r]])[fE];
+      sq[(3 * R + r) * 256 + tid] = G(J.qy[S.src[r]])[gN];
     }
-    b = G(J.h)[k];
-    bS = G(J.h)[kS];
-    bW = G(J.h)[kW];
-    bE = G(J.h)[kE];
-    bN = G(J.h)[kN];
+    const gdouble* rhs = system_rhs(J, S);
+    b = rhs[k];
+    bS = rhs[kS];
+    bW = rhs[kW];
+    bE = rhs[kE];
+    bN = rhs[kN];
   }
   // One system at a time, and within it one stencil slot at a time (S, W, E,
   // N: the CSR order), so that only one slot's divisions are live.  Each
@@ -528,7 +548,7 @@ __global__ __launch_bounds__(256) void k_assemble(EngineArgs E, const TileRef* b
     double d = 0.;
     if (in) {
       const int m = S.m[s];
-      const gdouble* hp = G(S.hp_is_un ? J.h : J.hres[m]);
+      const gdouble* hp = system_hp(J, S, m);
       const double hK = hp[k], hS = hp[kS], hW = hp[kW], hE = hp[kE], hN = hp[kN];
       double D = 1.;
       // sparse_matmul in CSR order S, W, self (value 0), E, N; then
@@ -539,11 +559,11 @@ __global__ __launch_bounds__(256) void k_assemble(EngineArgs E, const TileRef* b
         // the slot's neighbour values, selected from the ones loaded above
         const double hL = sl == 0 ? hS : sl == 1 ? hW : sl == 2 ? hE : hN;
         const double bL = sl == 0 ? bS : sl == 1 ? bW : sl == 2 ? bE : bN;
-        double q[4];
+        double q[R];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) q[r] = sq[(sl * 4 + r) * 256 + tid];
+        for (int r = 0; r < R; ++r) q[r] = sq[(sl * R + r) * 256 + tid];
         double M;
-        patankar_slot(m, dt, q, sl >= 2, hK, hL, M, D);
+        patankar_slot<R>(THETA[S.theta][m], dt, q, sl >= 2, hK, hL, M, D);
         G(J.M[s][sl])[k] = M;
         p = p + M * bL;
         if (sl == 1) p = p + 0. * b;
@@ -584,7 +604,7 @@ __global__ __launch_bounds__(256) void k_sweep(EngineArgs E, const TileRef* bloc
     const int kN = (j == ny - 1 ? 0 : j + 1) * nx + i;
     const int kW = j * nx + (i == 0 ? nx - 1 : i - 1);
     const int kE = j * nx + (i == nx - 1 ? 0 : i + 1);
-    const double b = G(J.h)[k];
+    const double b = system_rhs(J, S)[k];
     for (int s = 0; s < S.count; ++s) {
       if (!act[s]) continue;
       const int m = S.m[s];
@@ -625,12 +645,13 @@ __global__ __launch_bounds__(256) void k_decide(EngineArgs E, const int* jobs, S
     if (act[s]) tot[s] = block_sum_array(J.part + (size_t)s * nb, nb, red);
   if (FIRST) tot[kMaxSys] = block_sum_array(J.part + (size_t)kMaxSys * nb, nb, red);
   if (threadIdx.x != 0) return;
+  const double* rhs = S.hp_view ? J.hres[2] : J.h;
   if (FIRST) {
     // initial residual SUM(ABS(0 - rhs))/n, common to all systems
     int c0 = certified_continue(tot[kMaxSys], n, E.tol);
     int amb0 = 0;
     if (c0 < 0) {
-      double r = sequential_residual(n, J.h, nullptr, 0);
+      double r = sequential_residual(n, rhs, nullptr, 0);
       c0 = (E.tol < r) ? 1 : 0;
       amb0 = 1;
     }
@@ -650,7 +671,7 @@ __global__ __launch_bounds__(256) void k_decide(EngineArgs E, const int* jobs, S
       q.zero = 0;
       int c1 = certified_continue(tot[s], n, E.tol);
       if (c1 < 0) {
-        double r = sequential_residual(n, J.xj[S.m[s]][1], J.h, 1);
+        double r = sequential_residual(n, J.xj[S.m[s]][1], rhs, 1);
         c1 = (E.tol < r) ? 1 : 0;
         q.ambiguous += 1;
       }
@@ -754,16 +775,41 @@ __global__ void k_solve_done(EngineArgs E, const int* jobs, SysList S, int first
   }
 }
 
-// U^{n+1} = Ua(4) (src/timediscretization.f90:564); a job whose dt was not
-// finite keeps U^n.
-__global__ __launch_bounds__(256) void k_commit(EngineArgs E, const TileRef* blocks) {
+// U^{n+1} = Ua(MSteps) (src/timediscretization.f90:564,662,762), node
+// mf = MSteps - 1 here; a job whose dt was not finite keeps U^n.
+__global__ __launch_bounds__(256) void k_commit(EngineArgs E, const TileRef* blocks, int mf) {
   const TileRef T = blocks[blockIdx.x];
   const JobDev& J = E.jobs[T.job];
   const int k = T.tile * kCellBlock + threadIdx.x;
   if (k >= J.n || J.status->dt_bad) return;
-  G(J.h)[k] = G(J.hres[3])[k];
-  G(J.hu)[k] = G(J.hun[3])[k];
-  G(J.hv)[k] = G(J.hvn[3])[k];
+  G(J.h)[k] = G(J.hres[mf])[k];
+  G(J.hu)[k] = G(J.hun[mf])[k];
+  G(J.hv)[k] = G(J.hvn[mf])[k];
+}
+
+// The reference's view of the depths in the two-node schemes.
+// MODIFIED_PATANKAR_MATRIX_INVERSION declares hp(MSteps, nx, ny)
+// (src/timediscretization.f90:783) and receives Up(:,1,1:nx,1:ny), whose
+// first extent is 4 (src/finitevolume.f90:161), so hp is sequence associated:
+// hp(m, cell) is element L = (m-1) + 2*cell of Up(1:4, 1, cells) in array
+// element order, i.e. node L % 4 of cell L / 4 (cell = (i-1) + nx*(j-1), our
+// k).  Nodes 3 and 4 are never written by these schemes and hold 0
+// (src/finitevolume.f90:207-208).  So, with m = 2 the node solved:
+//   hp(2, k) = k even ? Up(2)[k/2] : 0       -> hres[3]
+//   rhs(k)  = hp(1, k) = k even ? h^n[k/2] : 0 -> hres[2]
+// where Up(2) is h^n in the first correction (from_un) and node 1's depth
+// after it.  hres[2] and hres[3] are free: only node 1 is solved.
+__global__ __launch_bounds__(256) void k_hp_view(EngineArgs E, const TileRef* blocks,
+                                                 int from_un) {
+  const TileRef T = blocks[blockIdx.x];
+  const JobDev& J = E.jobs[T.job];
+  const int k = T.tile * kCellBlock + threadIdx.x;
+  if (k >= J.n) return;
+  const gdouble* up2 = G(from_un ? J.h : J.hres[1]);
+  const bool odd = k & 1;
+  const int c = k >> 1;
+  G(J.hres[3])[k] = odd ? 0. : up2[c];
+  G(J.hres[2])[k] = odd ? 0. : G(J.h)[c];
 }
 
 // FXWB/FYWB from the slot scratch after the well-balanced FVTD.  The job's
@@ -959,14 +1005,25 @@ void launch_fvtd(const EngineArgs& E, const TileRef* xt, int nxt, const TileRef*
 
 void launch_momentum(const EngineArgs& E, const TileRef* cb, int ncb, const SysList& S,
                      hipStream_t s) {
-  if (ncb) hipLaunchKernelGGL(k_momentum, dim3(ncb), dim3(256), 0, s, E, cb, S);
+  if (!ncb) return;
+  if (S.nr == 2)
+    hipLaunchKernelGGL(k_momentum<2>, dim3(ncb), dim3(256), 0, s, E, cb, S);
+  else
+    hipLaunchKernelGGL(k_momentum<4>, dim3(ncb), dim3(256), 0, s, E, cb, S);
 }
 
 void launch_assemble(const EngineArgs& E, const TileRef* cb, int ncb, const int* jl, int njobs,
                      const SysList& S, hipStream_t s) {
   if (!ncb) return;
-  hipLaunchKernelGGL(k_assemble, dim3(ncb), dim3(256), 0, s, E, cb, S);
+  if (S.nr == 2)
+    hipLaunchKernelGGL(k_assemble<2>, dim3(ncb), dim3(256), 0, s, E, cb, S);
+  else
+    hipLaunchKernelGGL(k_assemble<4>, dim3(ncb), dim3(256), 0, s, E, cb, S);
   hipLaunchKernelGGL(k_decide<true>, dim3(njobs), dim3(256), 0, s, E, jl, S);
+}
+
+void launch_hp_view(const EngineArgs& E, const TileRef* cb, int ncb, int from_un, hipStream_t s) {
+  if (ncb) hipLaunchKernelGGL(k_hp_view, dim3(ncb), dim3(256), 0, s, E, cb, from_un);
 }
 
 void launch_sweep(const EngineArgs& E, const TileRef* cb, int ncb, const int* jl, int njobs,
@@ -990,8 +1047,9 @@ void launch_timestep(const EngineArgs& E, const TileRef* cb, int ncb, const int*
   hipLaunchKernelGGL(k_timestep_final, dim3(njobs), dim3(256), 0, s, E, jl);
 }
 
-void launch_commit(const EngineArgs& E, const TileRef* cb, int ncb, hipStream_t s) {
-  if (ncb) hipLaunchKernelGGL(k_commit, dim3(ncb), dim3(256), 0, s, E, cb);
+void launch_commit(const EngineArgs& E, const TileRef* cb, int ncb, int final_node,
+                   hipStream_t s) {
+  if (ncb) hipLaunchKernelGGL(k_commit, dim3(ncb), dim3(256), 0, s, E, cb, final_node);
 }
 
 void launch_solve_done(const EngineArgs& E, const int* jl, int njobs, const SysList& S,

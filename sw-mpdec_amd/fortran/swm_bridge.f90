@@ -16,6 +16,7 @@ INTEGER(C_INT), PARAMETER, PUBLIC :: SWM_OK = 0
 
 PUBLIC :: swm_create, swm_destroy, swm_set_bathy_grad, swm_init_wb
 PUBLIC :: swm_upload, swm_download, swm_timestep, swm_step_mpdec5, swm_advance
+PUBLIC :: swm_step_mpeuler, swm_step_mpdec2, swm_set_timescheme, swm_set_hp_association
 PUBLIC :: swm_error_message, swm_check
 
 INTERFACE
@@ -75,6 +76,36 @@ INTERFACE
     TYPE(C_PTR), VALUE :: h
     REAL(C_DOUBLE), VALUE :: t, dt
     INTEGER(C_INT), INTENT(OUT) :: iters(13)
+  END FUNCTION
+
+  ! TimeDiscretizationByMPEuler (src/timediscretization.f90:671-764)
+  INTEGER(C_INT) FUNCTION swm_step_mpeuler(h, t, dt, iters) BIND(C, NAME='swm_step_mpeuler')
+    IMPORT :: C_INT, C_DOUBLE, C_PTR
+    TYPE(C_PTR), VALUE :: h
+    REAL(C_DOUBLE), VALUE :: t, dt
+    INTEGER(C_INT), INTENT(OUT) :: iters(1)
+  END FUNCTION
+
+  ! TimeDiscretizationByMPDeC2 (src/timediscretization.f90:571-664)
+  INTEGER(C_INT) FUNCTION swm_step_mpdec2(h, t, dt, iters) BIND(C, NAME='swm_step_mpdec2')
+    IMPORT :: C_INT, C_DOUBLE, C_PTR
+    TYPE(C_PTR), VALUE :: h
+    REAL(C_DOUBLE), VALUE :: t, dt
+    INTEGER(C_INT), INTENT(OUT) :: iters(2)
+  END FUNCTION
+
+  ! timescheme (src/parameters.f90:144) of swm_advance: 5, 6 or 7
+  INTEGER(C_INT) FUNCTION swm_set_timescheme(h, timescheme) BIND(C, NAME='swm_set_timescheme')
+    IMPORT :: C_INT, C_PTR
+    TYPE(C_PTR), VALUE :: h
+    INTEGER(C_INT), VALUE :: timescheme
+  END FUNCTION
+
+  ! 0: the reference's depths for timescheme 6 and 7; 1: by node
+  INTEGER(C_INT) FUNCTION swm_set_hp_association(h, mode) BIND(C, NAME='swm_set_hp_association')
+    IMPORT :: C_INT, C_PTR
+    TYPE(C_PTR), VALUE :: h
+    INTEGER(C_INT), VALUE :: mode
   END FUNCTION
 
   ! the time loop of TimeDiscretization to the next output time

@@ -1,14 +1,16 @@
 !===============================================================================!
 ! swm_fdriver: a Fortran caller of libswm through swm_bridge, shaped like the
 ! reference's TimeDiscretization loop (src/timediscretization.f90:57-95) with
-! timescheme = 5.
-!   loop = 0: nsteps times dt = TimeStep(), one TimeDiscretizationByMPDeC5 step.
+! timescheme = 5, 6 or 7.
+!   loop = 0: nsteps times dt = TimeStep(), one step of the scheme selected as
+!             the reference's SELECT CASE(timescheme) does (:61-81).
 !   loop = 1: the reference's output loop with tEnd = (nsteps - 1/2) * dt_min(0)
 !             and 2 output intervals, each interval one swm_advance call (dt
 !             clipped to tAnalyze and tEnd on the device).
 !
 ! Reads a raw input file
-!   int32  nx, ny, nsteps, wellbalanced, has_bathy, loop
+!   int32  nx, ny, nsteps, wellbalanced, has_bathy, loop, timescheme (0: 5),
+!          hp_association (swm_set_hp_association)
 !   real64 dx, dy, gravity, cfl, wgp_bnd(4)
 !   real64 U(3,nx,ny) [, Uwb(3,nx,ny) if wellbalanced] [, dbdx, dbdy(nx,ny,4,4)]
 ! and writes the output file
@@ -20,7 +22,8 @@ PROGRAM swm_fdriver
 USE ISO_C_BINDING
 USE swm_bridge
 IMPLICIT NONE
-INTEGER(C_INT) :: hdr(6), nx, ny, nsteps, wb, hasb, rc, k, nonconv, loop, ndone, nout, nrec
+INTEGER(C_INT) :: hdr(8), nx, ny, nsteps, wb, hasb, rc, k, nonconv, loop, ndone, nout, nrec
+INTEGER(C_INT) :: timescheme, hpa
 INTEGER(C_INT) :: iters(13)
 REAL(C_DOUBLE) :: par(8), dx, dy, g, cfl, wgp(4), t, dt, lmx, lmy
 REAL(C_DOUBLE) :: tend, tanalyze, dtanalyze
@@ -35,6 +38,8 @@ CALL GET_COMMAND_ARGUMENT(2, fout)
 OPEN(NEWUNIT=uin, FILE=TRIM(fin), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
 READ(uin) hdr
 nx = hdr(1); ny = hdr(2); nsteps = hdr(3); wb = hdr(4); hasb = hdr(5); loop = hdr(6)
+timescheme = hdr(7); hpa = hdr(8)
+IF (timescheme .EQ. 0) timescheme = 5
 READ(uin) par
 dx = par(1); dy = par(2); g = par(3); cfl = par(4); wgp = par(5:8)
 ALLOCATE(U(3,nx,ny), dts(MAX(nsteps,1)))
@@ -62,6 +67,8 @@ IF (wb .NE. 0) THEN
   CALL swm_check(h, swm_init_wb(h, Uwb, 0_C_INT), "swm_init_wb")
 END IF
 CALL swm_check(h, swm_upload(h, U, 0_C_INT), "swm_upload")
+CALL swm_check(h, swm_set_timescheme(h, timescheme), "swm_set_timescheme")
+CALL swm_check(h, swm_set_hp_association(h, hpa), "swm_set_hp_association")
 
 t = 0.0
 nonconv = 0
@@ -73,8 +80,18 @@ IF (loop .EQ. 0) THEN
       WRITE(*,*) "swm_fdriver: non-finite time step at step ", k
       STOP 2
     END IF
-    rc = swm_step_mpdec5(h, t, dt, iters)
-    CALL swm_check(h, rc, "swm_step_mpdec5")
+    SELECT CASE(timescheme)
+      CASE(5)
+        rc = swm_step_mpdec5(h, t, dt, iters)
+      CASE(6)
+        rc = swm_step_mpeuler(h, t, dt, iters(1:1))
+      CASE(7)
+        rc = swm_step_mpdec2(h, t, dt, iters(1:2))
+      CASE DEFAULT
+        WRITE(*,*) "Time discretization not implemented"
+        STOP 1
+    END SELECT
+    CALL swm_check(h, rc, "swm_step")
     nonconv = nonconv + rc
     dts(k) = dt
     t = t + dt

@@ -66,6 +66,10 @@ def lib():
     L.swm_download.argtypes = [vp, _dp]
     L.swm_timestep.argtypes = [vp, _dp, _dp, _dp]
     L.swm_step_mpdec5.argtypes = [vp, ctypes.c_double, ctypes.c_double, _ip]
+    L.swm_step_mpeuler.argtypes = [vp, ctypes.c_double, ctypes.c_double, _ip]
+    L.swm_step_mpdec2.argtypes = [vp, ctypes.c_double, ctypes.c_double, _ip]
+    L.swm_set_timescheme.argtypes = [vp, ctypes.c_int]
+    L.swm_set_hp_association.argtypes = [vp, ctypes.c_int]
     L.swm_run_steps.argtypes = [vp, ctypes.c_int, _dp, _ip]
     L.swm_fvtd.argtypes = [vp] + [_dp] * 6
     L.swm_set_jacobi_limits.argtypes = [vp, ctypes.c_int, ctypes.c_double]
@@ -92,6 +96,12 @@ def footprint_bytes(nx, ny, wellbalanced, bathymetry) -> int:
     return int(lib().swm_footprint_bytes(nx, ny, int(bool(wellbalanced)), int(bool(bathymetry))))
 
 
+# modified-Patankar schemes by the reference's timescheme number: entry point
+# of one step and Jacobi solves per step
+SCHEMES = {5: ("swm_step_mpdec5", 13), 6: ("swm_step_mpeuler", 1), 7: ("swm_step_mpdec2", 2)}
+HP_AS_REFERENCE, HP_BY_NODE = 0, 1
+
+
 class Solver:
     """One grid on the device; mirrors the reference module state of one run."""
 
@@ -104,6 +114,7 @@ class Solver:
         if rc != SWM_OK:
             raise SwmError(f"swm_create: {L.swm_last_error(None).decode()} (rc={rc})")
         self.nx, self.ny = nx, ny
+        self.timescheme = 5
 
     @classmethod
     def from_problem(cls, p):
@@ -154,9 +165,19 @@ class Solver:
                                        ctypes.byref(ly)), "swm_timestep")
         return dt.value, lx.value, ly.value
 
-    def step(self, dt, t=0.0):
-        it = (ctypes.c_int * 13)()
-        nonconv = self._check(lib().swm_step_mpdec5(self._h, t, dt, it), "swm_step_mpdec5")
+    def set_timescheme(self, timescheme):
+        """The scheme of step/run_steps/advance: 5 mPDeC5, 6 mPEuler, 7 mPDeC2."""
+        self._check(lib().swm_set_timescheme(self._h, int(timescheme)), "swm_set_timescheme")
+        self.timescheme = int(timescheme)
+
+    def set_hp_association(self, mode):
+        self._check(lib().swm_set_hp_association(self._h, int(mode)), "swm_set_hp_association")
+
+    def step(self, dt, t=0.0, timescheme=None):
+        ts = self.timescheme if timescheme is None else int(timescheme)
+        name, ns = SCHEMES[ts]
+        it = (ctypes.c_int * ns)()
+        nonconv = self._check(getattr(lib(), name)(self._h, t, dt, it), name)
         return list(it), nonconv
 
     def run(self, steps):
@@ -169,11 +190,12 @@ class Solver:
         return hist, self.download()
 
     def run_steps(self, steps):
+        ns = SCHEMES[self.timescheme][1]
         dts = np.zeros(steps)
-        its = np.zeros(13 * steps, dtype=np.int32)
+        its = np.zeros(ns * steps, dtype=np.int32)
         nonconv = self._check(lib().swm_run_steps(self._h, steps, _ptr(dts),
                                                   its.ctypes.data_as(_ip)), "swm_run_steps")
-        return dts, its.reshape(steps, 13), nonconv
+        return dts, its.reshape(steps, ns), nonconv
 
     def advance(self, t, t_analyze, t_end, max_steps=1 << 30):
         """The reference's time loop to the next output time (swm_advance):
@@ -280,7 +302,8 @@ class JobDesc(ctypes.Structure):
                 ("gravity", ctypes.c_double), ("cfl", ctypes.c_double),
                 ("wellbalanced", ctypes.c_int), ("wgp_bnd", ctypes.c_double * 4),
                 ("U0", _dp), ("Uwb", _dp), ("dbdx", _dp), ("dbdy", _dp),
-                ("nsteps", ctypes.c_int)]
+                ("nsteps", ctypes.c_int), ("timescheme", ctypes.c_int),
+                ("hp_association", ctypes.c_int)]
 
 
 class JobInfo(ctypes.Structure):
@@ -355,7 +378,7 @@ class Engine:
         except Exception:
             pass
 
-    def submit(self, p, nsteps):
+    def submit(self, p, nsteps, timescheme=5, hp_association=HP_AS_REFERENCE):
         """Copies problem p to the device; returns its job id, or None when it
         does not fit the memory budget right now."""
         m = p.mesh
@@ -378,6 +401,8 @@ class Engine:
             arrs.append(np.ascontiguousarray(p.dbdy, dtype=np.float64))
             d.dbdy = _ptr(arrs[-1])
         d.nsteps = int(nsteps)
+        d.timescheme = int(timescheme)
+        d.hp_association = int(hp_association)
         jid = ctypes.c_int(-1)
         rc = _engine_lib().swm_engine_submit(self._e, ctypes.byref(d), ctypes.byref(jid))
         if rc == SWM_E_BUDGET:

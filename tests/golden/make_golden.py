@@ -47,20 +47,27 @@ CASES = {
     # rectangular grids (nx != ny, hence dx != dy on the unit square)
     "vortex_rect13x20": (lambda: P.vortex(13, 103, ny=20), 3),
     "lakeflag1_wb_rect11x8": (lambda: P.lake_flag1(11, True, ny=8), 2),
+    # the two-node Patankar schemes (timescheme 6 mPEuler, 7 mPDeC2), run by
+    # the reference until its state turns NaN
+    "vortex12_mpeuler": (lambda: P.vortex(12, 104), 3, 6),
+    "lakeflag1_wb10_mpdec2": (lambda: P.lake_flag1(10, True), 2, 7),
+    "vortex_rect13x9_mpdec2": (lambda: P.vortex(13, 105, ny=9), 2, 7),
 }
 
 
 def main(only=None):
-    for name, (mk, steps) in CASES.items():
+    for name, (mk, steps, *rest) in CASES.items():
         if only and name not in only:
             continue
+        scheme = rest[0] if rest else 5
         p = mk()
         out = problem_arrays(p)
+        out["scheme"] = scheme
         f = run_swref(p, 1)
         for k in ("Ut", "FX", "FY", "S", "Prod", "Dest", "wbnd", "MeshGP"):
             out["fvtd_" + k] = f[k]
         if steps:
-            hist, U = run_swref(p, 2, steps=steps)
+            hist, U = run_swref(p, 2, steps=steps, scheme=scheme)
             out["steps_hist"] = np.array(hist)
             out["steps_U"] = U if U is not None else np.zeros(0)
         out["ts"] = np.array(run_swref(p, 3))

@@ -22,6 +22,9 @@ import numpy as np
 ROOT = Path(__file__).resolve().parents[1]
 ORACLE_LIB = ROOT / "oracle" / "lib" / "liboracle.so"
 SWREF = {False: ROOT / "oracle" / "_ref" / "swref_p", True: ROOT / "oracle" / "_ref" / "swref_pwb"}
+# builds without -DPATANKAR, for the explicit schemes (timescheme 1-4)
+SWREF_EXPLICIT = {False: ROOT / "oracle" / "_ref" / "swref_e",
+                  True: ROOT / "oracle" / "_ref" / "swref_ewb"}
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -56,6 +59,9 @@ def oracle_lib():
         L.orc_timestep.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_step_mpdec5.restype = ctypes.c_int
         L.orc_step_mpdec5.argtypes = [ctypes.c_void_p, ctypes.c_double, _ip]
+        L.orc_step_patankar.restype = ctypes.c_int
+        L.orc_step_patankar.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _ip]
+        L.orc_set_hp_by_node.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_jacobi.restype = ctypes.c_int
         L.orc_jacobi.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
         L.orc_set_jacobi_limits.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
@@ -68,12 +74,19 @@ def oracle_lib():
     return _lib
 
 
-class Oracle:
-    """One problem instance on the C oracle."""
+# Jacobi solves per step of each modified-Patankar scheme (timescheme number)
+NSOLVES = {5: 13, 6: 1, 7: 2}
 
-    def __init__(self, problem):
+
+class Oracle:
+    """One problem instance on the C oracle.  scheme: the reference's
+    timescheme (5 mPDeC5, 6 mPEuler, 7 mPDeC2); hp_by_node: the two-node
+    schemes' by-node depth view instead of the reference's."""
+
+    def __init__(self, problem, scheme: int = 5, hp_by_node: bool = False):
         L = oracle_lib()
         self.p = problem
+        self.scheme = scheme
         m = problem.mesh
         self.n = problem.n
         self.rows = problem.rows
@@ -83,6 +96,7 @@ class Oracle:
                               1 if problem.wellbalanced else 0, _ptr(w))
         if not self.h:
             raise ValueError("orc_create failed")
+        L.orc_set_hp_by_node(self.h, 1 if hp_by_node else 0)
         if problem.dbdx is not None:
             L.orc_set_bathy(self.h, _ptr(np.ascontiguousarray(problem.dbdx)),
                             _ptr(np.ascontiguousarray(problem.dbdy)))
@@ -117,8 +131,10 @@ class Oracle:
 
     def step(self, dt):
         it = (ctypes.c_int * 13)()
-        nfail = oracle_lib().orc_step_mpdec5(self.h, dt, it)
-        return list(it), nfail
+        nfail = oracle_lib().orc_step_patankar(self.h, self.scheme, dt, it)
+        if nfail < 0:
+            raise ValueError(f"timescheme {self.scheme} is not a Patankar scheme")
+        return list(it)[:NSOLVES[self.scheme]], nfail
 
     def run(self, steps):
         """steps CFL steps; returns per-step (dt, lmx, lmy, iters) and final U."""
@@ -151,17 +167,20 @@ def have_swref():
     return SWREF[False].exists() and SWREF[True].exists()
 
 
-def run_swref(problem, mode: int, steps: int = 0, jac=None, timeout=600):
+def run_swref(problem, mode: int, steps: int = 0, jac=None, timeout=600, scheme: int = 5):
     """Run the reference build on problem.  mode 1: one FVTimeDerivative,
     2: `steps` CFL steps through TimeDiscretization, 3: TimeStep only,
-    4: jacobi on jac=(Mass, diag, rhs)."""
-    exe = SWREF[bool(problem.wellbalanced)]
+    4: jacobi on jac=(Mass, diag, rhs).  scheme: the reference's timescheme
+    (5 mPDeC5, 6 mPEuler, 7 mPDeC2 with the PATANKAR build; 1-4 with the
+    build without it)."""
+    patankar = scheme >= 5
+    exe = (SWREF if patankar else SWREF_EXPLICIT)[bool(problem.wellbalanced)]
     if not exe.exists():
         raise RuntimeError(f"{exe} not built")
     n, r = problem.n, problem.rows
     bflag = 1 if problem.dbdx is not None else 0
-    hdr = np.array([mode, n, steps, bflag, 1 if problem.wellbalanced else 0, r if r != n else 0],
-                   dtype="<i4")
+    hdr = np.array([mode, n, steps, bflag, 1 if problem.wellbalanced else 0, r if r != n else 0,
+                    scheme], dtype="<i4")
     fl = np.array([problem.x0, problem.x1, problem.gravity, problem.cfl], dtype="<f8")
     with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as td:
         fin, fout = Path(td) / "in.bin", Path(td) / "out.bin"
@@ -183,9 +202,10 @@ def run_swref(problem, mode: int, steps: int = 0, jac=None, timeout=600):
     if mode == 3:
         return tuple(raw[:3])
     if mode == 1:
-        sizes = [("Ut", (r, n, 3)), ("FX", (r, n + 1, 3)), ("FY", (r + 1, n, 3)), ("S", (r, n, 3)),
-                 ("Prod", (5 * n * r,)), ("Dest", (5 * n * r,)), ("wbnd", (4,)), ("dx", (2,)),
-                 ("MeshGP", (4, 4, r, n, 2))]
+        sizes = [("Ut", (r, n, 3)), ("FX", (r, n + 1, 3)), ("FY", (r + 1, n, 3)), ("S", (r, n, 3))]
+        if patankar:
+            sizes += [("Prod", (5 * n * r,)), ("Dest", (5 * n * r,))]
+        sizes += [("wbnd", (4,)), ("dx", (2,)), ("MeshGP", (4, 4, r, n, 2))]
         out, off = {}, 0
         for k, shp in sizes:
             cnt = int(np.prod(shp))

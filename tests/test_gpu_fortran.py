@@ -25,11 +25,11 @@ def with_steps(p, k):
     return p
 
 
-def write_input(path, p, loop=0):
+def write_input(path, p, loop=0, scheme=5, hp_by_node=False):
     m = p.mesh
     with open(path, "wb") as f:
-        np.array([p.n, p.rows, p.steps, int(p.wellbalanced), int(p.dbdx is not None), loop],
-                 dtype=np.int32).tofile(f)
+        np.array([p.n, p.rows, p.steps, int(p.wellbalanced), int(p.dbdx is not None), loop,
+                  scheme, int(hp_by_node)], dtype=np.int32).tofile(f)
         np.array([m.dx, p.mesh_y.dx, p.gravity, p.cfl, *m.wbnd], dtype=np.float64).tofile(f)
         np.ascontiguousarray(p.U0, dtype=np.float64).tofile(f)
         if p.wellbalanced:
@@ -115,3 +115,33 @@ def test_fortran_output_loop_with_advance(tmp_path, make):
     U_r = o.get_U()
     assert same_bits(U, U_r), describe_diff(U, U_r)
     assert nonconv == 0
+
+
+@pytest.mark.parametrize("scheme,by_node,make",
+                         [(6, False, lambda: P.vortex(24, 3, steps=2)),
+                          (7, False, lambda: P.vortex(20, 5, steps=1)),
+                          (6, True, lambda: P.vortex(24, 3, steps=3)),
+                          (7, True, lambda: with_steps(P.lake_flag1(16, True), 3))],
+                         ids=["mpeuler_ref", "mpdec2_ref", "mpeuler_by_node", "mpdec2_by_node"])
+def test_fortran_driver_two_node_schemes(tmp_path, scheme, by_node, make):
+    """The driver's SELECT CASE(timescheme) with 6 (swm_step_mpeuler) and 7
+    (swm_step_mpdec2), both depth views, against the oracle (the reference's
+    view only for the steps before its states turn NaN)."""
+    if not FDRIVER.exists():
+        pytest.skip("Fortran driver not built (no Fortran compiler at build time)")
+    p = make()
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    write_input(fin, p, scheme=scheme, hp_by_node=by_node)
+    r = subprocess.run([str(FDRIVER), str(fin), str(fout)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    dts, U, _ = read_output(fout, p.n, p.rows)
+    o = Oracle(p, scheme, hp_by_node=by_node)
+    dts_r = []
+    for _ in range(p.steps):
+        dt, _, _ = o.timestep()
+        o.step(dt)
+        dts_r.append(dt)
+    assert same_bits(dts, np.array(dts_r))
+    U_r = o.get_U()
+    assert same_bits(U, U_r), describe_diff(U, U_r)

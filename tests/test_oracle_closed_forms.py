@@ -228,3 +228,59 @@ def test_shallow_dam_break_positivity():
         U = o.get_U()
         assert (U[..., 0] >= 0).all()
     assert abs(U[..., 0].sum() - m0) <= 1e-12 * m0
+
+
+# The two-node schemes with the by-node depth view (hp(m,i,j) = Up(m,1,i,j)).
+# The reference has no output for this variant; these invariants pin it.
+@pytest.mark.parametrize("scheme,order", [(6, 1), (7, 2)])
+def test_by_node_two_node_schemes_have_their_temporal_order(scheme, order):
+    """mPEuler is first and mPDeC2 second order in time: on a fixed grid, the
+    error against a fine mPDeC5 solution falls by 2^order per halving of dt.
+    A wrong thetaDeC entry, node or correction count breaks the ratio."""
+    p = P.vortex(16, 5)
+    dt0, _, _ = Oracle(p).timestep()
+    T = 4 * dt0
+
+    def run(sch, k):
+        o = Oracle(p, sch, hp_by_node=True)
+        for _ in range(k):
+            o.step(T / k)
+        return o.get_U()
+
+    ref = run(5, 64)
+    for v in (0, 1):
+        errs = [np.abs(run(scheme, k)[..., v] - ref[..., v]).sum() for k in (8, 16, 32)]
+        for a, b in zip(errs, errs[1:]):
+            assert abs(math.log2(a / b) - order) < 0.15, (v, errs)
+
+
+@pytest.mark.parametrize("scheme", [6, 7])
+def test_by_node_two_node_schemes_conserve_mass_and_positivity(scheme):
+    for p in (P.vortex(24, 5), P.dam_break(20, dry=0.1)):
+        o = Oracle(p, scheme, hp_by_node=True)
+        m0 = p.U0[..., 0].sum()
+        for _ in range(4):
+            dt, _, _ = o.timestep()
+            o.step(dt)
+            U = o.get_U()
+            assert (U[..., 0] >= 0).all()
+        assert abs(U[..., 0].sum() - m0) <= 1e-12 * m0
+
+
+@pytest.mark.parametrize("scheme", [6, 7])
+def test_by_node_two_node_schemes_keep_lake_at_rest(scheme):
+    p = P.lake_bumps(16, 9, amp=0.0)
+    o = Oracle(p, scheme, hp_by_node=True)
+    for _ in range(2):
+        dt, _, _ = o.timestep()
+        o.step(dt)
+    assert same_bits(o.get_U(), p.U0)
+
+
+def test_hp_view_is_irrelevant_for_mpdec5():
+    p = P.vortex(12, 3)
+    a, b = Oracle(p, 5), Oracle(p, 5, hp_by_node=True)
+    for o in (a, b):
+        dt, _, _ = o.timestep()
+        o.step(dt)
+    assert same_bits(a.get_U(), b.get_U())

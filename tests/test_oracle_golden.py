@@ -60,7 +60,7 @@ def test_timestep_matches_reference(f):
                          ids=lambda f: f.stem)
 def test_steps_match_reference(f):
     z = np.load(f)
-    o = Oracle(load_problem(z))
+    o = Oracle(load_problem(z), int(z["scheme"]) if "scheme" in z else 5)
     hist = z["steps_hist"]
     U = None
     for k, row in enumerate(hist):

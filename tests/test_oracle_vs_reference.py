@@ -81,3 +81,54 @@ def test_jacobi_bitwise_random_matrices():
         sol, k = Oracle(p).jacobi(mass, diag, rhs)
         assert k > 5
         assert same_bits(sol, ref)
+
+
+# The two-node modified-Patankar schemes (timescheme 6 mPEuler, 7 mPDeC2).  In
+# the reference their depths come through the sequence association of
+# hp(MSteps,nx,ny) with Up(:,1,:,:) (first extent 4), so states turn NaN
+# within a few steps; the history is compared up to the first non-finite dt.
+@pytest.mark.parametrize("scheme", [6, 7])
+@pytest.mark.parametrize("case,steps", [("vortex21", 4), ("random13", 3), ("lakeflag1_wb14", 4),
+                                        ("dam15_mindepth", 3), ("vortex_rect21x34", 3),
+                                        ("random_rect17x11", 3)])
+def test_two_node_schemes_bitwise(case, steps, scheme):
+    p = CASES[case]()
+    hist_r, U_r = run_swref(p, 2, steps=steps, scheme=scheme)
+    o = Oracle(p, scheme)
+    U = None
+    for k, row in enumerate(hist_r):
+        dt, lx, ly = o.timestep()
+        assert same_bits([dt, lx, ly], row), f"step {k}"
+        if not np.isfinite(dt):
+            break
+        o.step(dt)
+        U = o.get_U()
+    assert same_bits(U, U_r), describe_diff(U, U_r)
+
+
+@pytest.mark.parametrize("scheme", [6, 7])
+def test_reference_two_node_schemes_lose_mass(scheme):
+    """What the sequence association does in the reference: the first step's
+    right-hand side is h^n of cell k/2 at even cells k and 0 at odd ones; the
+    matrix conserves mass, so the new total depth is the sum of that
+    right-hand side -- the first half of the cells, about half the mass."""
+    p = CASES["vortex21"]()
+    _, U_r = run_swref(p, 2, steps=1, scheme=scheme)
+    h0 = p.U0[..., 0].ravel()
+    n = h0.size
+    expect = sum(h0[k // 2] for k in range(0, n, 2))
+    assert abs(U_r[..., 0].sum() - expect) < 1e-11 * n
+    assert abs(U_r[..., 0].sum() - h0.sum()) > 0.4 * h0.sum()
+
+
+def test_by_node_mpeuler_momentum_matches_reference():
+    """The momentum update of mPEuler's single correction does not read hp,
+    so the by-node variant's first-step momenta are the reference's."""
+    p = CASES["vortex21"]()
+    _, U_r = run_swref(p, 2, steps=1, scheme=6)
+    o = Oracle(p, 6, hp_by_node=True)
+    dt, _, _ = o.timestep()
+    o.step(dt)
+    U = o.get_U()
+    assert same_bits(U[..., 1:], U_r[..., 1:])
+    assert not same_bits(U[..., 0], U_r[..., 0])
