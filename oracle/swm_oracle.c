@@ -66,6 +66,8 @@ struct orc_state {
   double *Ua, *Up, *FUp, *ProdUp, *DestUp;
   /* Patankar / Jacobi scratch */
   double *Mass, *Diag, *rhs, *haVec, *solp, *prod;
+  /* explicit schemes: K0..K5 and UN0 (3,nx,ny) (src/finitevolume.f90:152-158) */
+  double *K[6], *UN0;
 };
 
 /* ---- index helpers mirroring the Fortran bounds ------------------------- */
@@ -454,7 +456,8 @@ static void source_terms(orc_t *o) {
   free(vt);
 }
 
-/* UpdateTimeDerivative, PATANKAR branch, src/finitevolume.f90:446-540 */
+/* UpdateTimeDerivative, src/finitevolume.f90:446-540: the PATANKAR branch's
+ * production/destruction entries and Ut(2:3), and Ut(1) of the other branch */
 static void update_time_derivative(orc_t *o) {
   const int nx = o->nx, ny = o->ny;
   if (o->wb) {
@@ -477,7 +480,10 @@ static void update_time_derivative(orc_t *o) {
       o->Prod[sv] = fmaxr(f, 0.) / dx; o->Dest[sv] = -(fminr(f, 0.) / dx); ++sv;
       f = -o->FY[iFY(o, 0, ii, jj)];
       o->Prod[sv] = fmaxr(f, 0.) / dy; o->Dest[sv] = -(fminr(f, 0.) / dy); ++sv;
-      for (int v = 1; v < NVAR; ++v)
+      /* v = 0 is the build without PATANKAR (:529-533); the PATANKAR build
+       * leaves Ut(1) untouched and never reads it, so computing it here
+       * changes nothing there */
+      for (int v = 0; v < NVAR; ++v)
         o->Ut[iC(o, v, ii, jj)] = o->S[iC(o, v, ii, jj)]
                                   - (o->FX[iFX(o, v, ii, jj)] - o->FX[iFX(o, v, ii - 1, jj)]) / dx
                                   - (o->FY[iFY(o, v, ii, jj)] - o->FY[iFY(o, v, ii, jj - 1)]) / dy;
@@ -539,6 +545,8 @@ orc_t *orc_create(int nx, int ny, double dx, double dy, double gravity, double c
   o->haVec = calloc(nc, sizeof(double));
   o->solp = calloc(nc, sizeof(double));
   o->prod = calloc(nc, sizeof(double));
+  for (int k = 0; k < 6; ++k) o->K[k] = calloc(NVAR * nc, sizeof(double));
+  o->UN0 = calloc(NVAR * nc, sizeof(double));
   return o;
 }
 
@@ -547,7 +555,7 @@ void orc_destroy(orc_t *o) {
   double *p[] = {o->U, o->V, o->WM, o->WP, o->FX, o->FY, o->FluxX, o->FluxY, o->S, o->Ut,
                  o->FXWB, o->FYWB, o->SWB, o->Prod, o->Dest, o->dbdx, o->dbdy, o->Ua, o->Up,
                  o->FUp, o->ProdUp, o->DestUp, o->Mass, o->Diag, o->rhs, o->haVec, o->solp,
-                 o->prod};
+                 o->prod, o->K[0], o->K[1], o->K[2], o->K[3], o->K[4], o->K[5], o->UN0};
   for (size_t k = 0; k < sizeof(p) / sizeof(p[0]); ++k) free(p[k]);
   free(o);
 }
@@ -842,3 +850,155 @@ int orc_step_mpdec5(orc_t *o, double dt, int *jacobi_iters) {
 }
 
 void orc_set_hp_by_node(orc_t *o, int on) { o->hp_by_node = on ? 1 : 0; }
+
+/* ---- explicit schemes (the build without PATANKAR) ------------------------ */
+
+/* The interior of U as a (3,nx,ny) array, and back. */
+static void get_state(const orc_t *o, double *dst) {
+  for (int jj = 1; jj <= o->ny; ++jj)
+    for (int ii = 1; ii <= o->nx; ++ii)
+      for (int v = 0; v < NVAR; ++v) dst[iC(o, v, ii, jj)] = o->U[iU(o, v, ii, jj)];
+}
+static void set_state(orc_t *o, const double *src) {
+  for (int jj = 1; jj <= o->ny; ++jj)
+    for (int ii = 1; ii <= o->nx; ++ii)
+      for (int v = 0; v < NVAR; ++v) o->U[iU(o, v, ii, jj)] = src[iC(o, v, ii, jj)];
+}
+
+/* TimeDiscretizationByForwardEuler, src/timediscretization.f90:198-232 */
+static void step_forward_euler(orc_t *o, double dt) {
+  const size_t n3 = (size_t)NVAR * o->nx * o->ny;
+  double *K0 = o->K[0], *Ut = o->Ut;
+  get_state(o, K0);
+  fv_time_derivative(o);
+  for (size_t i = 0; i < n3; ++i) K0[i] = K0[i] + Ut[i] * dt; /* U = K0 + Ut*dt */
+  set_state(o, K0);
+}
+
+/* TimeDiscretizationBySSPRK4, src/timediscretization.f90:104-192 (the stage
+ * weights are the decimal literals of the source, read as doubles) */
+static void step_ssprk4(orc_t *o, double dt) {
+  const size_t n3 = (size_t)NVAR * o->nx * o->ny;
+  double *K0 = o->K[0], *K1 = o->K[1], *K2 = o->K[2], *K3 = o->K[3], *K4 = o->K[4],
+         *K5 = o->K[5], *Ut = o->Ut, *W = o->UN0;
+  get_state(o, K0);
+  fv_time_derivative(o);
+  for (size_t i = 0; i < n3; ++i)
+    K1[i] = 1.00000000000000 * K0[i] + 0.39175222700392 * Ut[i] * dt;
+  set_state(o, K1);
+  fv_time_derivative(o);
+  for (size_t i = 0; i < n3; ++i)
+    K2[i] = 0.44437049406734 * K0[i] + 0.55562950593266 * K1[i] + 0.36841059262959 * Ut[i] * dt;
+  set_state(o, K2);
+  fv_time_derivative(o);
+  for (size_t i = 0; i < n3; ++i)
+    K3[i] = 0.62010185138540 * K0[i] + 0.37989814861460 * K2[i] + 0.25189177424738 * Ut[i] * dt;
+  set_state(o, K3);
+  fv_time_derivative(o);
+  for (size_t i = 0; i < n3; ++i)
+    K4[i] = 0.17807995410773 * K0[i] + 0.82192004589227 * K3[i] + 0.54497475021237 * Ut[i] * dt;
+  set_state(o, K4);
+  memcpy(K5, Ut, n3 * sizeof(double));
+  fv_time_derivative(o);
+  for (size_t i = 0; i < n3; ++i)
+    W[i] = 0.00683325884039 * K0[i] + 0.51723167208978 * K2[i] + 0.12759831133288 * K3[i] +
+           0.34833675773694 * K4[i] + 0.08460416338212 * K5[i] * dt +
+           0.22600748319395 * Ut[i] * dt;
+  set_state(o, W);
+}
+
+/* TimeDiscretizationByRK65, src/timediscretization.f90:238-356.  ARK(i,j) of
+ * the column-major reshape (:264-268), bRK (:269). */
+static const double ARK65[6][5] = {
+    {0., 0., 0., 0., 0.},
+    {0.25, 0., 0., 0., 0.},
+    {0.125, 0.125, 0., 0., 0.},
+    {0., 0., 0.5, 0., 0.},
+    {0.1875, -0.375, 0.375, 0.5625, 0.},
+    {-0.42857142857142855, 1.1428571428571428, 0.8571428571428571, -1.7142857142857142,
+     1.1428571428571428}};
+static const double BRK65[6] = {0.077777777777777778, 0., 0.35555555555555556,
+                                0.13333333333333333, 0.35555555555555556, 0.077777777777777778};
+
+static void step_rk65(orc_t *o, double dt) {
+  const size_t n3 = (size_t)NVAR * o->nx * o->ny;
+  /* W holds each stage's state; it is K5's array, which is written only
+   * after the last stage state has been consumed */
+  double *UN0 = o->UN0, *Ut = o->Ut, *W = o->K[5];
+  get_state(o, UN0);
+  fv_time_derivative(o);
+  memcpy(o->K[0], Ut, n3 * sizeof(double));
+  /* stages 1..5: U = UN0 + (sum_j ARK(s+1,j) K_{j-1})*dt, then K_s = F(U) */
+  for (int s = 1; s <= 5; ++s) {
+    for (size_t i = 0; i < n3; ++i) {
+      double acc = ARK65[s][0] * o->K[0][i];
+      for (int j = 1; j < s; ++j) acc = acc + ARK65[s][j] * o->K[j][i];
+      W[i] = UN0[i] + acc * dt;
+    }
+    set_state(o, W);
+    fv_time_derivative(o);
+    memcpy(o->K[s], Ut, n3 * sizeof(double));
+  }
+  for (size_t i = 0; i < n3; ++i) {
+    double acc = BRK65[0] * o->K[0][i];
+    for (int j = 1; j < 6; ++j) acc = acc + BRK65[j] * o->K[j][i];
+    o->Ut[i] = UN0[i] + acc * dt;
+  }
+  set_state(o, o->Ut);
+}
+
+/* TimeDiscretizationByDeC5, src/timediscretization.f90:362-451: the mPDeC5
+ * loop with every variable updated explicitly. */
+static void step_dec5(orc_t *o, double dt) {
+  const dec_scheme *sc = &SCHEMES[0];
+  const int nx = o->nx, ny = o->ny;
+  const size_t node = (size_t)NVAR * nx * ny;
+  for (int m = 0; m < 4; ++m)
+    for (int jj = 1; jj <= ny; ++jj)
+      for (int ii = 1; ii <= nx; ++ii)
+        for (int v = 0; v < NVAR; ++v) {
+          o->Ua[iN(o, m, v, ii, jj)] = o->U[iU(o, v, ii, jj)];
+          o->Up[iN(o, m, v, ii, jj)] = o->U[iU(o, v, ii, jj)];
+        }
+  fv_time_derivative(o);
+  for (int m = 0; m < 4; ++m)
+    for (int jj = 1; jj <= ny; ++jj)
+      for (int ii = 1; ii <= nx; ++ii)
+        for (int v = 0; v < NVAR; ++v) o->FUp[iN(o, m, v, ii, jj)] = o->Ut[iC(o, v, ii, jj)];
+  for (int kk = 1; kk <= 5; ++kk) {
+    int m0 = (kk == 5) ? 3 : 1;
+    for (int m = m0; m < 4; ++m)
+      for (int jj = 1; jj <= ny; ++jj)
+        for (int ii = 1; ii <= nx; ++ii)
+          for (int v = 0; v < NVAR; ++v) {
+            double a = o->Ua[iN(o, 0, v, ii, jj)];
+            for (int r = 0; r < 4; ++r) a = a + dt * sc->theta[m][r] * o->FUp[iN(o, r, v, ii, jj)];
+            o->Ua[iN(o, m, v, ii, jj)] = a;
+          }
+    if (kk != 5) {
+      for (int m = 1; m < 4; ++m) {
+        for (int jj = 1; jj <= ny; ++jj)
+          for (int ii = 1; ii <= nx; ++ii)
+            for (int v = 0; v < NVAR; ++v) o->U[iU(o, v, ii, jj)] = o->Ua[iN(o, m, v, ii, jj)];
+        fv_time_derivative(o);
+        for (int jj = 1; jj <= ny; ++jj)
+          for (int ii = 1; ii <= nx; ++ii)
+            for (int v = 0; v < NVAR; ++v) o->FUp[iN(o, m, v, ii, jj)] = o->Ut[iC(o, v, ii, jj)];
+      }
+    }
+    memcpy(o->Up, o->Ua, sizeof(double) * 4 * node);
+  }
+  for (int jj = 1; jj <= ny; ++jj)
+    for (int ii = 1; ii <= nx; ++ii)
+      for (int v = 0; v < NVAR; ++v) o->U[iU(o, v, ii, jj)] = o->Ua[iN(o, 3, v, ii, jj)];
+}
+
+int orc_step_explicit(orc_t *o, int timescheme, double dt) {
+  switch (timescheme) {
+    case 1: step_forward_euler(o, dt); return 0;
+    case 2: step_ssprk4(o, dt); return 0;
+    case 3: step_rk65(o, dt); return 0;
+    case 4: step_dec5(o, dt); return 0;
+    default: return -1;
+  }
+}

@@ -58,6 +58,13 @@ int orc_step_mpdec5(orc_t *o, double dt, int *jacobi_iters);
  * Returns the non-converged count, or -1 for another number. */
 int orc_step_patankar(orc_t *o, int timescheme, double dt, int *jacobi_iters);
 
+/* One step of an explicit scheme of the build without PATANKAR
+ * (src/timediscretization.f90): 1 ForwardEuler (:198-232), 2 SSPRK4
+ * (:104-192), 3 RK65 (:238-356), 4 DeC5 (:362-451).  Every FVTD of the oracle
+ * also forms Ut(1) as that build does (src/finitevolume.f90:529-533).
+ * Returns 0, or -1 for another number. */
+int orc_step_explicit(orc_t *o, int timescheme, double dt);
+
 /* on = 1: the two-node schemes read hp(m,i,j) = Up(m,1,i,j) (the evident
  * intent) instead of the reference's sequence association of hp with
  * Up(:,1,:,:) (src/timediscretization.f90:783 vs src/finitevolume.f90:161).

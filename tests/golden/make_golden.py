@@ -52,6 +52,11 @@ CASES = {
     "vortex12_mpeuler": (lambda: P.vortex(12, 104), 3, 6),
     "lakeflag1_wb10_mpdec2": (lambda: P.lake_flag1(10, True), 2, 7),
     "vortex_rect13x9_mpdec2": (lambda: P.vortex(13, 105, ny=9), 2, 7),
+    # the explicit schemes of the build without PATANKAR (timescheme 1-4)
+    "vortex12_euler": (lambda: P.vortex(12, 106), 3, 1),
+    "lakeflag1_wb10_ssprk4": (lambda: P.lake_flag1(10, True), 2, 2),
+    "vortex_rect11x14_rk65": (lambda: P.vortex(11, 107, ny=14), 2, 3),
+    "shallowdam12_dec5": (lambda: P.dam_break(12, dry=0.1), 2, 4),
 }
 
 
@@ -66,6 +71,8 @@ def main(only=None):
         f = run_swref(p, 1)
         for k in ("Ut", "FX", "FY", "S", "Prod", "Dest", "wbnd", "MeshGP"):
             out["fvtd_" + k] = f[k]
+        if scheme < 5:  # that build's Ut(1:3)
+            out["fvtd_explicit_Ut"] = run_swref(p, 1, scheme=scheme)["Ut"]
         if steps:
             hist, U = run_swref(p, 2, steps=steps, scheme=scheme)
             out["steps_hist"] = np.array(hist)

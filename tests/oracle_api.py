@@ -62,6 +62,8 @@ def oracle_lib():
         L.orc_step_patankar.restype = ctypes.c_int
         L.orc_step_patankar.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _ip]
         L.orc_set_hp_by_node.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_step_explicit.restype = ctypes.c_int
+        L.orc_step_explicit.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
         L.orc_jacobi.restype = ctypes.c_int
         L.orc_jacobi.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
         L.orc_set_jacobi_limits.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
@@ -74,13 +76,15 @@ def oracle_lib():
     return _lib
 
 
-# Jacobi solves per step of each modified-Patankar scheme (timescheme number)
-NSOLVES = {5: 13, 6: 1, 7: 2}
+# Jacobi solves per step of each scheme (timescheme number): the explicit
+# schemes 1-4 of the build without PATANKAR solve nothing
+NSOLVES = {1: 0, 2: 0, 3: 0, 4: 0, 5: 13, 6: 1, 7: 2}
 
 
 class Oracle:
     """One problem instance on the C oracle.  scheme: the reference's
-    timescheme (5 mPDeC5, 6 mPEuler, 7 mPDeC2); hp_by_node: the two-node
+    timescheme (1 ForwardEuler, 2 SSPRK4, 3 RK65, 4 DeC5 of the build without
+    PATANKAR; 5 mPDeC5, 6 mPEuler, 7 mPDeC2); hp_by_node: the two-node
     schemes' by-node depth view instead of the reference's."""
 
     def __init__(self, problem, scheme: int = 5, hp_by_node: bool = False):
@@ -130,6 +134,10 @@ class Oracle:
         return dt, lx.value, ly.value
 
     def step(self, dt):
+        if self.scheme < 5:
+            if oracle_lib().orc_step_explicit(self.h, self.scheme, dt) < 0:
+                raise ValueError(f"timescheme {self.scheme} is not an explicit scheme")
+            return [], 0
         it = (ctypes.c_int * 13)()
         nfail = oracle_lib().orc_step_patankar(self.h, self.scheme, dt, it)
         if nfail < 0:
@@ -165,6 +173,10 @@ def big_stack():
 
 def have_swref():
     return SWREF[False].exists() and SWREF[True].exists()
+
+
+def have_swref_explicit():
+    return SWREF_EXPLICIT[False].exists() and SWREF_EXPLICIT[True].exists()
 
 
 def run_swref(problem, mode: int, steps: int = 0, jac=None, timeout=600, scheme: int = 5):

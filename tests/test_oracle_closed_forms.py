@@ -284,3 +284,39 @@ def test_hp_view_is_irrelevant_for_mpdec5():
         dt, _, _ = o.timestep()
         o.step(dt)
     assert same_bits(a.get_U(), b.get_U())
+
+
+# The explicit schemes of the build without PATANKAR (timescheme 1-4).
+@pytest.mark.parametrize("scheme", [1, 2, 3, 4])
+def test_explicit_schemes_conserve_mass_and_keep_lake_at_rest(scheme):
+    """Ut(1) is a difference of face fluxes, so every stage conserves the
+    total depth to rounding.  The well-balanced lake at rest (Ut = 0) stays
+    bitwise where the stages add dt-multiples of Ut to a state; SSPRK4's
+    convex combinations a*K0 + b*K1 of equal states round, so it holds to
+    rounding only."""
+    p = P.vortex(20, 7)
+    o = Oracle(p, scheme)
+    m0 = p.U0[..., 0].sum()
+    for _ in range(3):
+        dt, _, _ = o.timestep()
+        o.step(dt)
+    assert abs(o.get_U()[..., 0].sum() - m0) <= 1e-12 * m0
+    lake = P.lake_bumps(16, 9, amp=0.0)
+    o = Oracle(lake, scheme)
+    for _ in range(2):
+        dt, _, _ = o.timestep()
+        o.step(dt)
+    if scheme == 2:
+        assert np.abs(o.get_U() - lake.U0).max() < 1e-11
+    else:
+        assert same_bits(o.get_U(), lake.U0)
+
+
+def test_explicit_fvtd_mass_derivative_is_the_flux_divergence():
+    """Ut(1) = 0 - (FX(1,i)-FX(1,i-1))/dx - (FY(1,j)-FY(1,j-1))/dy."""
+    p = P.vortex(14, 2, ny=9)
+    f = Oracle(p).fvtd()
+    FX, FY = f["FX"][..., 0], f["FY"][..., 0]
+    dx, dy = p.mesh.dx, p.mesh_y.dx
+    expect = 0.0 - (FX[:, 1:] - FX[:, :-1]) / dx - (FY[1:, :] - FY[:-1, :]) / dy
+    assert same_bits(f["Ut"][..., 0], expect)

@@ -38,6 +38,15 @@ def test_fvtd_matches_reference_fixture(f):
         assert same_bits(a, b), f"{k}: {describe_diff(a, b)}"
 
 
+@pytest.mark.parametrize("f", [f for f in FILES if "fvtd_explicit_Ut" in np.load(f)],
+                         ids=lambda f: f.stem)
+def test_explicit_build_ut_matches_reference_fixture(f):
+    """Ut(1:3) of the build without PATANKAR, Ut(1) included."""
+    z = np.load(f)
+    got = Oracle(load_problem(z)).fvtd()["Ut"]
+    assert same_bits(got, z["fvtd_explicit_Ut"]), describe_diff(got, z["fvtd_explicit_Ut"])
+
+
 @pytest.mark.parametrize("f", FILES, ids=lambda f: f.stem)
 def test_mesh_quadrature_matches_reference(f):
     z = np.load(f)

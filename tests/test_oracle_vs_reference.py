@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import problems as P
-from oracle_api import Oracle, have_swref, run_swref
+from oracle_api import Oracle, have_swref, have_swref_explicit, run_swref
 from parity_util import FVTD_KEYS, describe_diff, same_bits
 
 pytestmark = pytest.mark.skipif(not have_swref(), reason="oracle/_ref not built")
@@ -132,3 +132,41 @@ def test_by_node_mpeuler_momentum_matches_reference():
     U = o.get_U()
     assert same_bits(U[..., 1:], U_r[..., 1:])
     assert not same_bits(U[..., 0], U_r[..., 0])
+
+
+# The explicit schemes of the build without PATANKAR (timescheme 1 ForwardEuler,
+# 2 SSPRK4, 3 RK65, 4 DeC5), against that build of the reference.
+explicit = pytest.mark.skipif(not have_swref_explicit(), reason="oracle/_ref/swref_e not built")
+
+
+@explicit
+@pytest.mark.parametrize("case", ["vortex21", "random16_rough", "lakeflag1_wb14", "dam15_mindepth",
+                                  "vortex_rect21x34", "lakeflag1_wb_rect19x26"])
+def test_fvtd_explicit_build_bitwise(case):
+    """That build's FVTimeDerivative: Ut(1:3), with Ut(1) from the mass
+    fluxes (src/finitevolume.f90:529-533), and the same FX, FY, S."""
+    p = CASES[case]()
+    ref = run_swref(p, 1, scheme=1)
+    got = Oracle(p).fvtd()
+    for k in ("Ut", "FX", "FY", "S"):
+        assert same_bits(got[k], ref[k]), f"{case} {k}: {describe_diff(got[k], ref[k])}"
+
+
+@explicit
+@pytest.mark.parametrize("scheme", [1, 2, 3, 4])
+@pytest.mark.parametrize("case,steps", [("vortex21", 3), ("random13", 3), ("lakeflag1_wb14", 3),
+                                        ("dam15_mindepth", 2), ("vortex_rect21x34", 2),
+                                        ("random_rect17x11", 3)])
+def test_explicit_schemes_bitwise(case, steps, scheme):
+    p = CASES[case]()
+    hist_r, U_r = run_swref(p, 2, steps=steps, scheme=scheme)
+    o = Oracle(p, scheme)
+    U = None
+    for k, row in enumerate(hist_r):
+        dt, lx, ly = o.timestep()
+        assert same_bits([dt, lx, ly], row), f"step {k}"
+        if not np.isfinite(dt):
+            break
+        o.step(dt)
+        U = o.get_U()
+    assert same_bits(U, U_r), describe_diff(U, U_r)
