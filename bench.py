@@ -84,27 +84,33 @@ def profiled(key: str, kernel: str, size: int):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(budget_s: float = 20.0):
+def cpu_baseline(budget_s: float = 20.0, scheme: int = 5):
     """The reference Fortran itself (oracle/_ref, built from its own sources)
     on the host: one core, the reference's own lake-at-rest bathymetry
     (BathymetryFlag=1) with the well-balanced build, 128x128, the number of
-    CFL steps that fits the time budget.  Release-style build if present."""
+    CFL steps that fits the time budget.  Release-style build if present.
+    scheme: the timescheme (1-4 run the well-balanced build without
+    PATANKAR, which has no release-style variant here)."""
     sys.path.insert(0, str(ROOT / "tests"))
     import problems as P
     from oracle_api import big_stack
 
-    exe = ROOT / "oracle" / "_ref" / "swref_pwb_fast"
-    kind_note = "amdflang -O3 -march=x86-64-v3"
-    if not exe.exists():
-        exe = ROOT / "oracle" / "_ref" / "swref_pwb"
-        kind_note = "amdflang -O2 -ffp-contract=off"
+    if scheme >= 5:
+        exe = ROOT / "oracle" / "_ref" / "swref_pwb_fast"
+        kind_note = "amdflang -O3 -march=x86-64-v3"
+        if not exe.exists():
+            exe = ROOT / "oracle" / "_ref" / "swref_pwb"
+            kind_note = "amdflang -O2 -ffp-contract=off"
+    else:
+        exe = ROOT / "oracle" / "_ref" / "swref_ewb"
+        kind_note = "amdflang -O2 -ffp-contract=off, built without PATANKAR"
     if not exe.exists():
         return None
     n = 128
     p = P.lake_flag1(n, True)
 
     def run(steps):
-        hdr = np.array([5, n, steps, 1, 1, 0, 5], dtype="<i4")
+        hdr = np.array([5, n, steps, 1, 1, 0, scheme], dtype="<i4")
         fl = np.array([p.x0, p.x1, p.gravity, p.cfl], dtype="<f8")
         with tempfile.TemporaryDirectory() as td:
             fin, fout = Path(td) / "in.bin", Path(td) / "out.bin"
@@ -127,22 +133,24 @@ def cpu_baseline(budget_s: float = 20.0):
     t, k = run(steps)
     rate = n * n * k / t
     return {"value": rate, "unit": "cell-steps/s", "cores": 1, "kind": "reference",
-            "sample": (f"reference Fortran ({kind_note}), PATANKAR+WELLBALANCED, "
+            "sample": (f"reference Fortran ({kind_note}), timescheme {scheme}, WELLBALANCED, "
                        f"BathymetryFlag=1 lake at rest + 1e-3 bump, {n}x{n}, {k} CFL steps, "
                        f"{t:.1f} s, time loop only")}
 
 
-def schemes_section(n: int, seed: int, steps: int) -> dict:
-    """mPEuler (timescheme 6) and mPDeC2 (timescheme 7) with the by-node depth
-    view on the headline workload, `steps` CFL steps each after one warmup
-    step, through swm_run_steps.  (The reference's own depth view turns the
-    state NaN within a few steps, so it has no meaningful rate.)"""
+def schemes_section(n: int, seed: int, steps: int, cpu: bool) -> dict:
+    """The other time schemes on the headline workload, `steps` CFL steps each
+    after one warmup step, through swm_run_steps: the explicit ones of the
+    build without PATANKAR (timescheme 1-4) and mPEuler / mPDeC2 (6, 7) with
+    the by-node depth view (the reference's own view turns the state NaN
+    within a few steps, so it has no meaningful rate)."""
     import problems as P
     import swm
 
     prob = P.lake_bumps(n, seed)
     res = {}
-    for ts, name in ((6, "mpeuler"), (7, "mpdec2")):
+    for ts, name in ((1, "forward_euler"), (2, "ssprk4"), (3, "rk65"), (4, "dec5"),
+                     (6, "mpeuler"), (7, "mpdec2")):
         s = swm.Solver.from_problem(prob)
         s.set_timescheme(ts)
         s.set_hp_association(swm.HP_BY_NODE)
@@ -152,11 +160,20 @@ def schemes_section(n: int, seed: int, steps: int) -> dict:
         _, iters, _ = s.run_steps(steps)
         s.sync()
         el = time.perf_counter() - t0
-        res[name] = {"timescheme": ts, "hp_association": "by_node", "steps": steps,
+        res[name] = {"timescheme": ts, "steps": steps,
                      "cell_steps_per_s": n * n * steps / el, "ms_per_step": 1e3 * el / steps,
                      "jacobi_sweeps_per_step": float(np.sum(iters)) / steps}
+        if ts >= 6:
+            res[name]["hp_association"] = "by_node"
         log(f"bench: {name} {steps} steps {1e3 * el / steps:.2f} ms/step")
         del s
+        if cpu and ts <= 4:
+            # the reference's own rate for the scheme (its mPEuler / mPDeC2
+            # states turn NaN, so they have none to compare)
+            cb = cpu_baseline(budget_s=4.0, scheme=ts)
+            if cb:
+                res[name]["cpu_reference"] = cb
+                res[name]["gpu_vs_cpu"] = res[name]["cell_steps_per_s"] / cb["value"]
     return res
 
 
@@ -257,7 +274,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--scheme-steps", type=int, default=20,
-                    help="steps of mPEuler and mPDeC2 at the headline size (0: skip)")
+                    help="steps of each other time scheme at the headline size (0: skip)")
     ap.add_argument("--batch-jobs", type=int, default=256,
                     help="jobs of the batched run (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=48,
@@ -361,7 +378,7 @@ def main():
     }
     del s
     if args.scheme_steps > 0:
-        out["schemes"] = schemes_section(n, args.seed, args.scheme_steps)
+        out["schemes"] = schemes_section(n, args.seed, args.scheme_steps, not args.no_cpu)
     if args.batch_jobs > 0:
         out["batched"] = batched_section(args.batch_jobs, args.seed)
     if args.serve_requests > 0:

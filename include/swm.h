@@ -92,10 +92,26 @@ int swm_step_mpdec5(swm_solver_t *h, double t, double dt, int jacobi_iters[13]);
 int swm_step_mpeuler(swm_solver_t *h, double t, double dt, int jacobi_iters[1]);
 int swm_step_mpdec2(swm_solver_t *h, double t, double dt, int jacobi_iters[2]);
 
+/* The explicit schemes, as the reference built without -DPATANKAR runs them:
+ * there UpdateTimeDerivative forms Ut(1:3) from the face fluxes
+ * (src/finitevolume.f90:529-533) instead of production/destruction terms, and
+ * the step combines FVTimeDerivative stages with the source's weights, in its
+ * evaluation order:
+ *   TimeDiscretizationByForwardEuler (src/timediscretization.f90:198-232),
+ *   TimeDiscretizationBySSPRK4 (:104-192, 5 stages),
+ *   TimeDiscretizationByRK65 (:238-356, 6 stages),
+ *   TimeDiscretizationByDeC5 (:362-451, 13 FVTimeDerivative evaluations).
+ * No Jacobi solves; returns SWM_OK or an error. */
+int swm_step_forward_euler(swm_solver_t *h, double t, double dt);
+int swm_step_ssprk4(swm_solver_t *h, double t, double dt);
+int swm_step_rk65(swm_solver_t *h, double t, double dt);
+int swm_step_dec5(swm_solver_t *h, double t, double dt);
+
 /* The scheme swm_run_steps and swm_advance step with, by the reference's
  * timescheme number (src/parameters.f90:144; selected at
- * src/timediscretization.f90:61-81): 5 mPDeC5 (the default), 6 mPEuler,
- * 7 mPDeC2.  Other numbers: SWM_E_ARG. */
+ * src/timediscretization.f90:61-81): 1 ForwardEuler, 2 SSPRK4, 3 RK65,
+ * 4 DeC5 (the explicit schemes, as above), 5 mPDeC5 (the default),
+ * 6 mPEuler, 7 mPDeC2.  Other numbers: SWM_E_ARG. */
 int swm_set_timescheme(swm_solver_t *h, int timescheme);
 
 /* Which depths the two-node schemes put into the Patankar weights and the
@@ -120,7 +136,8 @@ int swm_set_hp_association(swm_solver_t *h, int mode);
  * TimeStep value (a step-count stop instead of the tEnd stop of
  * src/timediscretization.f90:57-95).  dts (may be NULL) receives nsteps dt
  * values; jacobi_iters (may be NULL) nsteps rows of the scheme's solve count
- * (13, 1 or 2).  Stops early with SWM_E_STATE if dt is not finite. */
+ * (13, 1 or 2; none for the explicit schemes).  Stops early with SWM_E_STATE
+ * if dt is not finite. */
 int swm_run_steps(swm_solver_t *h, int nsteps, double *dts, int *jacobi_iters);
 
 /* The reference's time loop on the device (src/timediscretization.f90:57-95)
@@ -194,7 +211,7 @@ typedef struct {
   const double *dbdx;        /* (1:nx, 1:ny, 1:4, 1:4) or NULL (flat bottom) */
   const double *dbdy;
   int nsteps;                /* CFL steps to run; 0: none (DONE at once); < 0: until released */
-  int timescheme;            /* 0 or 5: mPDeC5; 6: mPEuler; 7: mPDeC2 (swm_set_timescheme) */
+  int timescheme;            /* 0 or 5: mPDeC5; 1..4, 6, 7 as swm_set_timescheme */
   int hp_association;        /* SWM_HP_AS_REFERENCE or SWM_HP_BY_NODE (swm_set_hp_association) */
 } swm_job_desc;
 

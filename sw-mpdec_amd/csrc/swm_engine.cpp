@@ -155,11 +155,13 @@ struct Job {
 
 // Jobs step together when they share a scheme and, for the two-node schemes,
 // the depth view: group 0 mPDeC5, 1-2 mPEuler, 3-4 mPDeC2 (reference view,
-// by-node view).
-constexpr int kGroups = 5;
+// by-node view), 5-8 the explicit schemes 1-4.
+constexpr int kGroups = 9;
 int group_of(const SchemeDef* d, int hp_assoc) {
   const int si = (int)(d - kSchemes);
-  return si == 0 ? 0 : 1 + 2 * (si - 1) + (hp_assoc == SWM_HP_BY_NODE ? 1 : 0);
+  if (si == 0) return 0;
+  if (si <= 2) return 1 + 2 * (si - 1) + (hp_assoc == SWM_HP_BY_NODE ? 1 : 0);
+  return 5 + (si - 3);
 }
 
 }  // namespace
@@ -375,6 +377,149 @@ struct StepPlan {
 // last node in the last correction.
 int systems_in(const SchemeDef& D, int kk) { return kk == D.kcorr ? 1 : D.msteps - 1; }
 
+// ---------------------------------------------------------------------------
+// The explicit schemes of the build without PATANKAR
+// (src/timediscretization.f90:104-451): FVTD launches that write Ut(1:3) into
+// derivative slots, and stage combinations (k_combine) with the reference's
+// weights and evaluation order.  States live in slots 0..5 (0 = U^n); the
+// result is left in the scheme's final_state slot for the commit.
+// ---------------------------------------------------------------------------
+struct Term {
+  int deriv;  // 1: derivative slot, 0: state slot
+  int src;
+  double c;
+};
+
+CombOp comb_op(int mode, int out, int base, std::initializer_list<Term> terms) {
+  CombOp op{};
+  op.mode = mode;
+  op.out = out;
+  op.base = base;
+  for (const Term& t : terms) {
+    op.deriv[op.nterm] = t.deriv;
+    op.src[op.nterm] = t.src;
+    op.c[op.nterm] = t.c;
+    ++op.nterm;
+  }
+  return op;
+}
+
+// thetaDeC of DeC5 (src/timediscretization.f90:389-404; the same literals as
+// mPDeC5's), [m][r] = thetaDeC(m+1, r+1)
+constexpr double kThetaDeC5[4][4] = {
+    {0.0000000000000000000, 0.0000000000000000000, 0.0000000000000000000, 0.0000000000000000000},
+    {0.1103005664791649049, 0.1896994335208351257, -0.0339073642291439076, 0.0103005664791649201},
+    {0.0730327668541684294, 0.4505740308958107176, 0.2269672331458314485, -0.0269672331458315692},
+    {0.0833333333333333287, 0.4166666666666666852, 0.4166666666666666852, 0.0833333333333333287}};
+
+// RK65 (src/timediscretization.f90:264-269): ARK(i,j) of the column-major
+// reshape as [i-1][j-1], and bRK
+constexpr double kARK65[6][5] = {
+    {0., 0., 0., 0., 0.},
+    {0.25, 0., 0., 0., 0.},
+    {0.125, 0.125, 0., 0., 0.},
+    {0., 0., 0.5, 0., 0.},
+    {0.1875, -0.375, 0.375, 0.5625, 0.},
+    {-0.42857142857142855, 1.1428571428571428, 0.8571428571428571, -1.7142857142857142,
+     1.1428571428571428}};
+constexpr double kBRK65[6] = {0.077777777777777778, 0., 0.35555555555555556,
+                              0.13333333333333333, 0.35555555555555556, 0.077777777777777778};
+
+int explicit_body(swm_engine* e, Tables& T, const SchemeDef& D) {
+  EngineArgs a = eargs(e);
+  hipStream_t s = e->stream;
+  // FVTimeDerivative of state slots into derivative slots, Ut(1:3)
+  auto fvtd = [&](std::initializer_list<std::pair<int, int>> in_out) {
+    NodeList L{};
+    L.ut1 = 1;
+    for (const auto& io : in_out) {
+      L.m[L.count] = io.first;
+      L.out[L.count] = io.second;
+      L.slot[L.count] = L.count;
+      ++L.count;
+    }
+    return fvtd_launch(e, T, L);
+  };
+  auto comb = [&](const CombOp& op) {
+    timed(e, 3, [&] { launch_combine(a, T.dcb.p, T.ncb, op, s); });
+  };
+  constexpr int X = 0, F = 1;
+  int rc = SWM_OK;
+  switch (D.timescheme) {
+    case 1:  // ForwardEuler (:198-232): U = K0 + Ut*dt
+      if ((rc = fvtd({{0, 0}}))) return rc;
+      comb(comb_op(0, 1, 0, {{X, 0, 1.0}, {F, 0, 1.0}}));
+      break;
+    case 2:  // SSPRK4 (:104-192)
+      if ((rc = fvtd({{0, 0}}))) return rc;
+      comb(comb_op(0, 1, 0, {{X, 0, 1.00000000000000}, {F, 0, 0.39175222700392}}));
+      if ((rc = fvtd({{1, 0}}))) return rc;
+      comb(comb_op(0, 2, 0,
+                   {{X, 0, 0.44437049406734}, {X, 1, 0.55562950593266}, {F, 0, 0.36841059262959}}));
+      if ((rc = fvtd({{2, 0}}))) return rc;
+      comb(comb_op(0, 3, 0,
+                   {{X, 0, 0.62010185138540}, {X, 2, 0.37989814861460}, {F, 0, 0.25189177424738}}));
+      if ((rc = fvtd({{3, 0}}))) return rc;
+      comb(comb_op(0, 4, 0,
+                   {{X, 0, 0.17807995410773}, {X, 3, 0.82192004589227}, {F, 0, 0.54497475021237}}));
+      // K5 = Ut of stage 4 stays in derivative slot 0; stage 5's goes to 1
+      if ((rc = fvtd({{4, 1}}))) return rc;
+      comb(comb_op(0, 1, 0,
+                   {{X, 0, 0.00683325884039}, {X, 2, 0.51723167208978}, {X, 3, 0.12759831133288},
+                    {X, 4, 0.34833675773694}, {F, 0, 0.08460416338212},
+                    {F, 1, 0.22600748319395}}));
+      break;
+    case 3: {  // RK65 (:238-356): K_j in derivative slot j, stage states in slot 1
+      if ((rc = fvtd({{0, 0}}))) return rc;
+      for (int st = 1; st <= 5; ++st) {
+        CombOp op = comb_op(1, 1, 0, {});
+        for (int j = 0; j < st; ++j) {
+          op.deriv[op.nterm] = F;
+          op.src[op.nterm] = j;
+          op.c[op.nterm] = kARK65[st][j];
+          ++op.nterm;
+        }
+        comb(op);
+        if ((rc = fvtd({{1, st}}))) return rc;
+      }
+      CombOp op = comb_op(1, 1, 0, {});
+      for (int j = 0; j < 6; ++j) {
+        op.deriv[op.nterm] = F;
+        op.src[op.nterm] = j;
+        op.c[op.nterm] = kBRK65[j];
+        ++op.nterm;
+      }
+      comb(op);
+      break;
+    }
+    case 4: {  // DeC5 (:362-451): the mPDeC5 loop on all three variables
+      if ((rc = fvtd({{0, 0}}))) return rc;
+      int src[4] = {0, 0, 0, 0};  // FUp(r): FVTD(U^n) for every r at kk = 1
+      for (int kk = 1; kk <= 5; ++kk) {
+        for (int m = (kk == 5 ? 3 : 1); m < 4; ++m) {
+          CombOp op = comb_op(2, m, 0, {});
+          for (int r = 0; r < 4; ++r) {
+            op.deriv[op.nterm] = F;
+            op.src[op.nterm] = src[r];
+            op.c[op.nterm] = kThetaDeC5[m][r];
+            ++op.nterm;
+          }
+          comb(op);
+        }
+        if (kk != 5) {
+          if ((rc = fvtd({{1, 1}, {2, 2}, {3, 3}}))) return rc;
+          for (int r = 0; r < 4; ++r) src[r] = r;
+        }
+      }
+      break;
+    }
+    default:
+      FAIL(e, SWM_E_ARG, "internal: timescheme %d is not explicit", D.timescheme);
+  }
+  LAUNCHCK(e);
+  return SWM_OK;
+}
+
 // The step's launches; returns after the last finalize / FVTD (no commit).
 int step_body(swm_engine* e, Tables& T, const StepPlan& P) {
   EngineArgs a = eargs(e);
@@ -384,10 +529,12 @@ int step_body(swm_engine* e, Tables& T, const StepPlan& P) {
     LAUNCHCK(e);
   }
   const SchemeDef& D = *P.sc;
+  if (is_explicit(D)) return explicit_body(e, T, D);
   // FVTimeDerivative(U^n): node 0, shared by all nodes at kk = 1
   NodeList L0{};
   L0.count = 1;
   L0.m[0] = 0;
+  L0.out[0] = 0;
   L0.slot[0] = 0;
   int rc = fvtd_launch(e, T, L0);
   if (rc) return rc;
@@ -452,6 +599,7 @@ int step_body(swm_engine* e, Tables& T, const StepPlan& P) {
       L.count = S.count;
       for (int k = 0; k < S.count; ++k) {
         L.m[k] = S.m[k];
+        L.out[k] = S.m[k];
         L.slot[k] = k;
       }
       rc = fvtd_launch(e, T, L);
@@ -579,7 +727,7 @@ int engine_step(swm_engine* e, Tables& T, const SchemeDef* sc, int hp_assoc,
   P.sc = sc;
   P.hp_view = (sc->msteps < 4 && hp_assoc == SWM_HP_AS_REFERENCE) ? 1 : 0;
   P.device_dt = host_dt == nullptr;
-  bool spec = e->margin != -2;
+  bool spec = e->margin != -2 && !is_explicit(*sc);  // nothing to predict without solves
   for (int q = 0; q < T.njobs; ++q) {
     const Job& J = *e->jobs[T.jl[q]];
     spec = spec && J.have_pred && J.pred_sc == sc;
@@ -621,7 +769,7 @@ int engine_step(swm_engine* e, Tables& T, const SchemeDef* sc, int hp_assoc,
     if (rc) return rc;
   }
   EngineArgs a = eargs(e);
-  timed(e, 7, [&] { launch_commit(a, T.dcb.p, T.ncb, sc->msteps - 1, s); });
+  timed(e, 7, [&] { launch_commit(a, T.dcb.p, T.ncb, sc->final_state, s); });
   LAUNCHCK(e);
   for (int q = 0; q < T.njobs; ++q) {
     Job& J = *e->jobs[T.jl[q]];
@@ -632,8 +780,10 @@ int engine_step(swm_engine* e, Tables& T, const SchemeDef* sc, int hp_assoc,
       J.pred[i] = it;
       J.jacobi_sweeps += it;
     }
-    J.nonconv += js.nonconv;
-    e->ambiguous += js.ambiguous;
+    if (sc->nsolve > 0) {  // the solve counters are this step's only with solves
+      J.nonconv += js.nonconv;
+      e->ambiguous += js.ambiguous;
+    }
     J.have_pred = true;
     J.pred_sc = sc;
   }
@@ -677,6 +827,7 @@ int init_wb(swm_engine* e, int id, const double* Uwb, int ghosts) {
   NodeList L{};
   L.count = 1;
   L.m[0] = 1;
+  L.out[0] = 1;
   L.slot[0] = 0;
   L.wb_init = 1;
   rc = fvtd_launch(e, J.own, L);
@@ -794,6 +945,32 @@ int alloc_job(swm_engine* e, int nx, int ny, double dx, double dy, double g, dou
     }
   for (int q = 0; q < kMaxSys; ++q)
     for (int k = 0; k < 5; ++k) D.M[q][k] = take(lay.n);
+  // state and derivative slots of the explicit schemes (JobDev)
+  D.sh[0] = D.h;
+  D.shu[0] = D.hu;
+  D.shv[0] = D.hv;
+  for (int m = 1; m < 4; ++m) {
+    D.sh[m] = D.hres[m];
+    D.shu[m] = D.hun[m];
+    D.shv[m] = D.hvn[m];
+  }
+  D.sh[4] = D.M[0][0];
+  D.shu[4] = D.M[0][1];
+  D.shv[4] = D.M[0][2];
+  D.sh[5] = D.M[0][3];
+  D.shu[5] = D.M[0][4];
+  D.shv[5] = D.M[1][0];
+  for (int r = 0; r < 4; ++r) {
+    D.d1[r] = D.qx[r];
+    D.d2[r] = D.FU2[r];
+    D.d3[r] = D.FU3[r];
+  }
+  D.d1[4] = D.M[1][1];
+  D.d2[4] = D.M[1][2];
+  D.d3[4] = D.M[1][3];
+  D.d1[5] = D.M[1][4];
+  D.d2[5] = D.M[2][0];
+  D.d3[5] = D.M[2][1];
   if (wb) {
     for (int v = 0; v < 3; ++v) {
       D.FXWB[v] = take(lay.nfx);
@@ -864,7 +1041,7 @@ int engine_add(swm_engine* e, const swm_job_desc* d, int* id_out) {
   if (d->wellbalanced && !d->Uwb) FAIL(e, SWM_E_ARG, "wellbalanced job needs Uwb");
   if (!d->dbdx != !d->dbdy) FAIL(e, SWM_E_ARG, "dbdx and dbdy must both be set or both NULL");
   const SchemeDef* sc = find_scheme(d->timescheme == 0 ? 5 : d->timescheme);
-  if (!sc) FAIL(e, SWM_E_ARG, "timescheme %d is not 0, 5, 6 or 7", d->timescheme);
+  if (!sc) FAIL(e, SWM_E_ARG, "timescheme %d is not one of 0..7", d->timescheme);
   if (d->hp_association != SWM_HP_AS_REFERENCE && d->hp_association != SWM_HP_BY_NODE)
     FAIL(e, SWM_E_ARG, "hp_association must be SWM_HP_AS_REFERENCE or SWM_HP_BY_NODE");
   const long long extra = d->dbdx ? (long long)bathy_bytes((size_t)d->nx * d->ny) : 0;
@@ -1276,12 +1453,30 @@ int swm_step_mpdec2(swm_solver_t* s, double t, double dt, int jacobi_iters[2]) {
   return plain_step(s, 7, dt, jacobi_iters);
 }
 
+int swm_step_forward_euler(swm_solver_t* s, double t, double dt) {
+  (void)t;
+  return plain_step(s, 1, dt, nullptr);
+}
+
+int swm_step_ssprk4(swm_solver_t* s, double t, double dt) {
+  (void)t;
+  return plain_step(s, 2, dt, nullptr);
+}
+
+int swm_step_rk65(swm_solver_t* s, double t, double dt) {
+  (void)t;
+  return plain_step(s, 3, dt, nullptr);
+}
+
+int swm_step_dec5(swm_solver_t* s, double t, double dt) {
+  (void)t;
+  return plain_step(s, 4, dt, nullptr);
+}
+
 int swm_set_timescheme(swm_solver_t* s, int timescheme) {
   if (!s) return SWM_E_ARG;
   const SchemeDef* sc = find_scheme(timescheme);
-  if (!sc)
-    FAIL(s->e, SWM_E_ARG, "timescheme %d is not a modified-Patankar scheme (5, 6 or 7)",
-         timescheme);
+  if (!sc) FAIL(s->e, SWM_E_ARG, "timescheme %d is not one of 1..7", timescheme);
   plain_job(s)->scheme = sc;
   return SWM_OK;
 }
@@ -1371,6 +1566,7 @@ int swm_fvtd(swm_solver_t* s, double* Ut, double* FX, double* FY, double* S, dou
   NodeList L{};
   L.count = 1;
   L.m[0] = 0;
+  L.out[0] = 0;
   L.slot[0] = 0;
   L.debug_S = 1;
   rc = fvtd_launch(e, J.own, L);

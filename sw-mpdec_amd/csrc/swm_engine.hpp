@@ -25,6 +25,7 @@ constexpr int kFaceTile = kTile - 1;  // faces a tile produces along its sweep
 constexpr int kCellBlock = 256;       // cells per block of the pointwise kernels
 constexpr int kMaxSys = 3;            // Patankar systems solved together (nodes 2..4)
 constexpr int kSlots = 3;             // concurrent FVTD evaluations per job
+constexpr int kStates = 6;            // state / derivative slots of the explicit schemes
 
 // Jacobi state of one system, written by the decide kernel.
 struct JacState {
@@ -84,6 +85,14 @@ struct JobDev {
   double *FXWB[3], *FYWB[3];
   double *SWB2, *SWB3;
   double *bx, *by;
+  // State slots s = 0..5 (h, hu, hv), read by the FVTD: 0 is U^n, 1..3 the
+  // DeC nodes (hres, hun, hvn); 4..5 alias Patankar matrices M, which the
+  // explicit schemes that use them never assemble.
+  double *sh[kStates], *shu[kStates], *shv[kStates];
+  // Time-derivative slots r = 0..5 of the explicit schemes (Ut(1:3)): d2, d3
+  // are FU2, FU3 for r < 4; d1 aliases the face-flux arrays qx (r < 4);
+  // r = 4..5 alias M as well.
+  double *d1[kStates], *d2[kStates], *d3[kStates];
   // reductions
   double* part;     // [kMaxSys + 1][nblk] Jacobi partial sums
   double* tspart;   // [nblk][3] TimeStep partial maxima
@@ -105,10 +114,25 @@ struct EngineArgs {
 // Which DeC nodes an FVTD launch evaluates (blockIdx.y picks one).
 struct NodeList {
   int count;
-  int m[kSlots];      // node evaluated (0 = U^n)
+  int m[kSlots];      // state slot evaluated (0 = U^n, 1..3 DeC nodes, 4..5 explicit)
+  int out[kSlots];    // output slot: FU2/FU3/qx/qy[out], or d1..d3[out] with ut1
   int slot[kSlots];   // scratch slot used
   int wb_init;        // 1: building the well-balanced state (outputs to FXWB...)
   int debug_S;        // 1: also store S(2:3) to the slot's FX[.][0..1] scratch tail
+  int ut1;            // 1: the build without PATANKAR: Ut(1:3) to d1..d3, no face fluxes
+};
+
+// One stage combination of an explicit scheme, per cell and variable, in the
+// reference's evaluation order (every sum left to right):
+//   mode 0: out = t_0 + t_1 + ...,   t_i = c_i*X_i (state) or (c_i*D_i)*dt (derivative)
+//   mode 1: out = X_base + (c_0*D_0 + c_1*D_1 + ...)*dt
+//   mode 2: out = X_base; out = out + (dt*c_i)*D_i for each term
+constexpr int kMaxTerms = 8;
+struct CombOp {
+  int mode, out, base, nterm;
+  int deriv[kMaxTerms];
+  int src[kMaxTerms];
+  double c[kMaxTerms];
 };
 
 // Patankar/Jacobi launch parameters (the same for every job of a step).
@@ -123,15 +147,22 @@ struct SysList {
   int hp_view;          // 1: denominators from hres[3], rhs from hres[2] (k_hp_view)
 };
 
-// The modified-Patankar schemes, by the reference's timescheme number.
+// The time schemes, by the reference's timescheme number: the modified-
+// Patankar ones of the PATANKAR build and the explicit ones of the build
+// without it (which compute Ut(1) instead of production/destruction terms).
 struct SchemeDef {
   int timescheme;  // src/parameters.f90:144 numbering
-  int msteps;      // DeC nodes
-  int kcorr;       // corrections
-  int theta;       // SysList::theta
-  int nsolve;      // Jacobi solves per step
+  int msteps;      // DeC nodes (Patankar)
+  int kcorr;       // corrections (Patankar)
+  int theta;       // SysList::theta (Patankar)
+  int nsolve;      // Jacobi solves per step (0: explicit)
+  int final_state; // state slot holding U^{n+1} before the commit
 };
-constexpr SchemeDef kSchemes[3] = {{5, 4, 5, 0, 13}, {6, 2, 1, 1, 1}, {7, 2, 2, 1, 2}};
+constexpr int kNumSchemes = 7;
+constexpr SchemeDef kSchemes[kNumSchemes] = {
+    {5, 4, 5, 0, 13, 3}, {6, 2, 1, 1, 1, 1}, {7, 2, 2, 1, 2, 1},
+    {1, 0, 0, 0, 0, 1},  {2, 0, 0, 0, 0, 1}, {3, 0, 0, 0, 0, 1}, {4, 4, 5, 0, 0, 3}};
+inline bool is_explicit(const SchemeDef& d) { return d.timescheme < 5; }
 inline const SchemeDef* find_scheme(int timescheme) {
   for (const SchemeDef& d : kSchemes)
     if (d.timescheme == timescheme) return &d;
@@ -154,6 +185,8 @@ void launch_timestep(const EngineArgs& E, const TileRef* cb, int ncb, const int*
 void launch_commit(const EngineArgs& E, const TileRef* cb, int ncb, int final_node,
                    hipStream_t s);
 void launch_hp_view(const EngineArgs& E, const TileRef* cb, int ncb, int from_un, hipStream_t s);
+void launch_combine(const EngineArgs& E, const TileRef* cb, int ncb, const CombOp& op,
+                    hipStream_t s);
 void launch_solve_done(const EngineArgs& E, const int* jl, int njobs, const SysList& S,
                        int first_solve, hipStream_t s);
 void launch_copy_wb(const EngineArgs& E, const TileRef* cb, int ncb, int slot, hipStream_t s);

@@ -71,18 +71,12 @@ __device__ __forceinline__ void cell_ij(int i0, int j0, int t, int nx, int& i, i
 
 __device__ __forceinline__ int cdiv_d(int a, int b) { return (a + b - 1) / b; }
 
-// State of DeC node m of a job: m = 0 is U^n.
+// State slot m of a job: m = 0 is U^n, 1..3 the DeC nodes.
 __device__ __forceinline__ void node_state(const JobDev& J, int m, const gdouble*& h,
                                            const gdouble*& hu, const gdouble*& hv) {
-  if (m == 0) {
-    h = G(J.h);
-    hu = G(J.hu);
-    hv = G(J.hv);
-  } else {
-    h = G(J.hres[m]);
-    hu = G(J.hun[m]);
-    hv = G(J.hvn[m]);
-  }
+  h = G(J.sh[m]);
+  hu = G(J.shu[m]);
+  hv = G(J.shv[m]);
 }
 
 // ------------------------------------------------------------------------
@@ -344,14 +338,25 @@ __global__ __launch_bounds__(256) void k_fvtd_update(EngineArgs E, const TileRef
       }
     }
     // Ut(2:3) = S - (FX(i)-FX(i-1))/dx - (FY(j)-FY(j-1))/dy  (:525-527)
-    G(J.FU2[m])[k] = S2 - (fxe[1] - fxw[1]) / J.dx - (fyn[1] - fys[1]) / J.dy;
-    G(J.FU3[m])[k] = S3 - (fxe[2] - fxw[2]) / J.dx - (fyn[2] - fys[2]) / J.dy;
+    const int o = L.out[node];
+    const double ut2 = S2 - (fxe[1] - fxw[1]) / J.dx - (fyn[1] - fys[1]) / J.dy;
+    const double ut3 = S3 - (fxe[2] - fxw[2]) / J.dx - (fyn[2] - fys[2]) / J.dy;
+    if (L.ut1) {
+      // the build without PATANKAR: Ut(1:3) (:529-533), with S(1) = 0 exactly
+      // (SourceFunc's first component is 0., src/equation.f90:408)
+      G(J.d1[o])[k] = 0.0 - (fxe[0] - fxw[0]) / J.dx - (fyn[0] - fys[0]) / J.dy;
+      G(J.d2[o])[k] = ut2;
+      G(J.d3[o])[k] = ut3;
+      return;
+    }
+    G(J.FU2[o])[k] = ut2;
+    G(J.FU3[o])[k] = ut3;
     // mass flux per face / dx: the production/destruction entries of
     // src/finitevolume.f90:496-523 are max(+-q, 0) and -min(+-q, 0) of these.
-    G(J.qx[m])[fe] = fxe[0] / J.dx;
-    G(J.qy[m])[gn] = fyn[0] / J.dy;
-    if (i == 1) G(J.qx[m])[fw] = fxw[0] / J.dx;
-    if (j == 1) G(J.qy[m])[gs] = fys[0] / J.dy;
+    G(J.qx[o])[fe] = fxe[0] / J.dx;
+    G(J.qy[o])[gn] = fyn[0] / J.dy;
+    if (i == 1) G(J.qx[o])[fw] = fxw[0] / J.dx;
+    if (j == 1) G(J.qy[o])[gs] = fys[0] / J.dy;
   }
 }
 
@@ -812,6 +817,46 @@ __global__ __launch_bounds__(256) void k_hp_view(EngineArgs E, const TileRef* bl
   G(J.hres[2])[k] = odd ? 0. : G(J.h)[c];
 }
 
+// A stage combination of an explicit scheme (CombOp, mode 0/1/2), for the
+// three variables of every cell.  Reads only slots other than op.out.
+__device__ __forceinline__ const gdouble* state_var(const JobDev& J, int v, int s) {
+  return G(v == 0 ? J.sh[s] : v == 1 ? J.shu[s] : J.shv[s]);
+}
+__device__ __forceinline__ const gdouble* deriv_var(const JobDev& J, int v, int r) {
+  return G(v == 0 ? J.d1[r] : v == 1 ? J.d2[r] : J.d3[r]);
+}
+
+__global__ __launch_bounds__(256) void k_combine(EngineArgs E, const TileRef* blocks, CombOp op) {
+  const TileRef T = blocks[blockIdx.x];
+  const JobDev& J = E.jobs[T.job];
+  const int k = T.tile * kCellBlock + threadIdx.x;
+  if (k >= J.n) return;
+  const double dt = J.status->dt;
+#pragma unroll 1
+  for (int v = 0; v < 3; ++v) {
+    double acc = 0.;
+    if (op.mode == 0) {
+      for (int t = 0; t < op.nterm; ++t) {
+        const double term = op.deriv[t] ? (op.c[t] * deriv_var(J, v, op.src[t])[k]) * dt
+                                        : op.c[t] * state_var(J, v, op.src[t])[k];
+        acc = t == 0 ? term : acc + term;
+      }
+    } else if (op.mode == 1) {
+      double sum = 0.;
+      for (int t = 0; t < op.nterm; ++t) {
+        const double term = op.c[t] * deriv_var(J, v, op.src[t])[k];
+        sum = t == 0 ? term : sum + term;
+      }
+      acc = state_var(J, v, op.base)[k] + sum * dt;
+    } else {
+      acc = state_var(J, v, op.base)[k];
+      for (int t = 0; t < op.nterm; ++t)
+        acc = acc + (dt * op.c[t]) * deriv_var(J, v, op.src[t])[k];
+    }
+    G(v == 0 ? J.sh[op.out] : v == 1 ? J.shu[op.out] : J.shv[op.out])[k] = acc;
+  }
+}
+
 // FXWB/FYWB from the slot scratch after the well-balanced FVTD.  The job's
 // cell blocks cover n indices; the loop also covers the nx (ny) extra faces.
 __global__ __launch_bounds__(256) void k_copy_wb(EngineArgs E, const TileRef* blocks, int slot) {
@@ -1024,6 +1069,11 @@ void launch_assemble(const EngineArgs& E, const TileRef* cb, int ncb, const int*
 
 void launch_hp_view(const EngineArgs& E, const TileRef* cb, int ncb, int from_un, hipStream_t s) {
   if (ncb) hipLaunchKernelGGL(k_hp_view, dim3(ncb), dim3(256), 0, s, E, cb, from_un);
+}
+
+void launch_combine(const EngineArgs& E, const TileRef* cb, int ncb, const CombOp& op,
+                    hipStream_t s) {
+  if (ncb) hipLaunchKernelGGL(k_combine, dim3(ncb), dim3(256), 0, s, E, cb, op);
 }
 
 void launch_sweep(const EngineArgs& E, const TileRef* cb, int ncb, const int* jl, int njobs,

@@ -17,6 +17,7 @@ INTEGER(C_INT), PARAMETER, PUBLIC :: SWM_OK = 0
 PUBLIC :: swm_create, swm_destroy, swm_set_bathy_grad, swm_init_wb
 PUBLIC :: swm_upload, swm_download, swm_timestep, swm_step_mpdec5, swm_advance
 PUBLIC :: swm_step_mpeuler, swm_step_mpdec2, swm_set_timescheme, swm_set_hp_association
+PUBLIC :: swm_step_forward_euler, swm_step_ssprk4, swm_step_rk65, swm_step_dec5
 PUBLIC :: swm_error_message, swm_check
 
 INTERFACE
@@ -94,7 +95,31 @@ INTERFACE
     INTEGER(C_INT), INTENT(OUT) :: iters(2)
   END FUNCTION
 
-  ! timescheme (src/parameters.f90:144) of swm_advance: 5, 6 or 7
+  ! the explicit schemes as the build without PATANKAR runs them:
+  ! TimeDiscretizationByForwardEuler/BySSPRK4/ByRK65/ByDeC5
+  ! (src/timediscretization.f90:198-232, 104-192, 238-356, 362-451)
+  INTEGER(C_INT) FUNCTION swm_step_forward_euler(h, t, dt) BIND(C, NAME='swm_step_forward_euler')
+    IMPORT :: C_INT, C_DOUBLE, C_PTR
+    TYPE(C_PTR), VALUE :: h
+    REAL(C_DOUBLE), VALUE :: t, dt
+  END FUNCTION
+  INTEGER(C_INT) FUNCTION swm_step_ssprk4(h, t, dt) BIND(C, NAME='swm_step_ssprk4')
+    IMPORT :: C_INT, C_DOUBLE, C_PTR
+    TYPE(C_PTR), VALUE :: h
+    REAL(C_DOUBLE), VALUE :: t, dt
+  END FUNCTION
+  INTEGER(C_INT) FUNCTION swm_step_rk65(h, t, dt) BIND(C, NAME='swm_step_rk65')
+    IMPORT :: C_INT, C_DOUBLE, C_PTR
+    TYPE(C_PTR), VALUE :: h
+    REAL(C_DOUBLE), VALUE :: t, dt
+  END FUNCTION
+  INTEGER(C_INT) FUNCTION swm_step_dec5(h, t, dt) BIND(C, NAME='swm_step_dec5')
+    IMPORT :: C_INT, C_DOUBLE, C_PTR
+    TYPE(C_PTR), VALUE :: h
+    REAL(C_DOUBLE), VALUE :: t, dt
+  END FUNCTION
+
+  ! timescheme (src/parameters.f90:144) of swm_advance: 1..7
   INTEGER(C_INT) FUNCTION swm_set_timescheme(h, timescheme) BIND(C, NAME='swm_set_timescheme')
     IMPORT :: C_INT, C_PTR
     TYPE(C_PTR), VALUE :: h

@@ -1,7 +1,7 @@
 !===============================================================================!
 ! swm_fdriver: a Fortran caller of libswm through swm_bridge, shaped like the
 ! reference's TimeDiscretization loop (src/timediscretization.f90:57-95) with
-! timescheme = 5, 6 or 7.
+! timescheme = 1..7.
 !   loop = 0: nsteps times dt = TimeStep(), one step of the scheme selected as
 !             the reference's SELECT CASE(timescheme) does (:61-81).
 !   loop = 1: the reference's output loop with tEnd = (nsteps - 1/2) * dt_min(0)
@@ -81,6 +81,14 @@ IF (loop .EQ. 0) THEN
       STOP 2
     END IF
     SELECT CASE(timescheme)
+      CASE(1)
+        rc = swm_step_forward_euler(h, t, dt)
+      CASE(2)
+        rc = swm_step_ssprk4(h, t, dt)
+      CASE(3)
+        rc = swm_step_rk65(h, t, dt)
+      CASE(4)
+        rc = swm_step_dec5(h, t, dt)
       CASE(5)
         rc = swm_step_mpdec5(h, t, dt, iters)
       CASE(6)

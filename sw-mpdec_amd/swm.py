@@ -69,6 +69,8 @@ def lib():
     L.swm_step_mpeuler.argtypes = [vp, ctypes.c_double, ctypes.c_double, _ip]
     L.swm_step_mpdec2.argtypes = [vp, ctypes.c_double, ctypes.c_double, _ip]
     L.swm_set_timescheme.argtypes = [vp, ctypes.c_int]
+    for name in ("swm_step_forward_euler", "swm_step_ssprk4", "swm_step_rk65", "swm_step_dec5"):
+        getattr(L, name).argtypes = [vp, ctypes.c_double, ctypes.c_double]
     L.swm_set_hp_association.argtypes = [vp, ctypes.c_int]
     L.swm_run_steps.argtypes = [vp, ctypes.c_int, _dp, _ip]
     L.swm_fvtd.argtypes = [vp] + [_dp] * 6
@@ -96,9 +98,12 @@ def footprint_bytes(nx, ny, wellbalanced, bathymetry) -> int:
     return int(lib().swm_footprint_bytes(nx, ny, int(bool(wellbalanced)), int(bool(bathymetry))))
 
 
-# modified-Patankar schemes by the reference's timescheme number: entry point
-# of one step and Jacobi solves per step
-SCHEMES = {5: ("swm_step_mpdec5", 13), 6: ("swm_step_mpeuler", 1), 7: ("swm_step_mpdec2", 2)}
+# time schemes by the reference's timescheme number: entry point of one step
+# and Jacobi solves per step (1-4: the explicit schemes of the build without
+# PATANKAR; 5-7: the modified-Patankar ones)
+SCHEMES = {1: ("swm_step_forward_euler", 0), 2: ("swm_step_ssprk4", 0), 3: ("swm_step_rk65", 0),
+           4: ("swm_step_dec5", 0), 5: ("swm_step_mpdec5", 13), 6: ("swm_step_mpeuler", 1),
+           7: ("swm_step_mpdec2", 2)}
 HP_AS_REFERENCE, HP_BY_NODE = 0, 1
 
 
@@ -166,7 +171,8 @@ class Solver:
         return dt.value, lx.value, ly.value
 
     def set_timescheme(self, timescheme):
-        """The scheme of step/run_steps/advance: 5 mPDeC5, 6 mPEuler, 7 mPDeC2."""
+        """The scheme of step/run_steps/advance: 1 ForwardEuler, 2 SSPRK4,
+        3 RK65, 4 DeC5, 5 mPDeC5, 6 mPEuler, 7 mPDeC2."""
         self._check(lib().swm_set_timescheme(self._h, int(timescheme)), "swm_set_timescheme")
         self.timescheme = int(timescheme)
 
@@ -176,6 +182,9 @@ class Solver:
     def step(self, dt, t=0.0, timescheme=None):
         ts = self.timescheme if timescheme is None else int(timescheme)
         name, ns = SCHEMES[ts]
+        if ns == 0:
+            self._check(getattr(lib(), name)(self._h, t, dt), name)
+            return [], 0
         it = (ctypes.c_int * ns)()
         nonconv = self._check(getattr(lib(), name)(self._h, t, dt, it), name)
         return list(it), nonconv

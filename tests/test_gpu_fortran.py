@@ -118,15 +118,21 @@ def test_fortran_output_loop_with_advance(tmp_path, make):
 
 
 @pytest.mark.parametrize("scheme,by_node,make",
-                         [(6, False, lambda: P.vortex(24, 3, steps=2)),
+                         [(1, False, lambda: P.vortex(20, 3, steps=3)),
+                          (2, False, lambda: P.lake_bumps(18, 4, steps=2)),
+                          (3, False, lambda: with_steps(P.lake_flag1(16, True), 2)),
+                          (4, False, lambda: P.vortex(17, 5, steps=2, ny=23)),
+                          (6, False, lambda: P.vortex(24, 3, steps=2)),
                           (7, False, lambda: P.vortex(20, 5, steps=1)),
                           (6, True, lambda: P.vortex(24, 3, steps=3)),
                           (7, True, lambda: with_steps(P.lake_flag1(16, True), 3))],
-                         ids=["mpeuler_ref", "mpdec2_ref", "mpeuler_by_node", "mpdec2_by_node"])
-def test_fortran_driver_two_node_schemes(tmp_path, scheme, by_node, make):
-    """The driver's SELECT CASE(timescheme) with 6 (swm_step_mpeuler) and 7
-    (swm_step_mpdec2), both depth views, against the oracle (the reference's
-    view only for the steps before its states turn NaN)."""
+                         ids=["euler", "ssprk4", "rk65", "dec5", "mpeuler_ref", "mpdec2_ref",
+                              "mpeuler_by_node", "mpdec2_by_node"])
+def test_fortran_driver_other_schemes(tmp_path, scheme, by_node, make):
+    """The driver's SELECT CASE(timescheme) with the explicit schemes 1-4
+    (swm_step_forward_euler ... swm_step_dec5), 6 (swm_step_mpeuler) and 7
+    (swm_step_mpdec2) in both depth views, against the oracle (the
+    reference's view only for the steps before its states turn NaN)."""
     if not FDRIVER.exists():
         pytest.skip("Fortran driver not built (no Fortran compiler at build time)")
     p = make()

@@ -170,13 +170,14 @@ def test_same_job_alone_and_in_mix():
 def test_bad_scheme_is_rejected():
     p = P.vortex(12, 1)
     s = swm.Solver.from_problem(p)
-    with pytest.raises(swm.SwmError):
-        s.set_timescheme(4)
+    for bad in (0, 8, -1):
+        with pytest.raises(swm.SwmError):
+            s.set_timescheme(bad)
     with pytest.raises(swm.SwmError):
         s.set_hp_association(2)
     eng = swm.Engine()
     with pytest.raises(swm.SwmError):
-        eng.submit(p, 1, timescheme=3)
+        eng.submit(p, 1, timescheme=9)
 
 
 @pytest.mark.parametrize("scheme", [6, 7])
