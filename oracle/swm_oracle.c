@@ -1,8 +1,9 @@
 /*
  * swm_oracle.c -- TEST ORACLE.  A plain-C restatement of the reference
- * sw-mpdec mPDeC5 / WENO5 PATANKAR time step (accdavlo/sw-mpdec,
- * fv-solver-sw/src/).  Only tests/, __graft_entry__.smoke() and bench.py's
- * cpu_baseline leg may load it; the product library never does.
+ * sw-mpdec WENO5 time step (accdavlo/sw-mpdec, fv-solver-sw/src/): mPDeC5 and
+ * the other modified-Patankar schemes of the PATANKAR build, and the explicit
+ * schemes of the build without it.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it; the product library never does.
  *
  * It follows the reference's loops, array bounds and expression order
  * literally (Fortran evaluates left to right, honouring parentheses; every
@@ -21,6 +22,14 @@
  *    PATANKAR_DIV, Gauss/Lobatto constants, constant states give Ut = 0,
  *    lake at rest gives Ut = 0 with the well-balanced variant, Jacobi matches
  *    a dense solve, mass is conserved.
+ *  - the time schemes: mPDeC5, mPEuler and mPDeC2 (orc_step_patankar, with the
+ *    reference's sequence-associated depths) against the PATANKAR builds; the
+ *    explicit schemes (orc_step_explicit) and Ut(1) against the builds without
+ *    PATANKAR.  The by-node depth view (orc_set_hp_by_node(o, 1)) has no
+ *    reference output: it is pinned only by mathematics -- temporal order 1
+ *    and 2 of mPEuler / mPDeC2, mass conservation, positivity, lake at rest,
+ *    and the first-step momenta, which do not read the depths and equal the
+ *    reference's.
  *
  * Not restated: the Jameson shock indicator (src/shocksindicator.f90).  With
  * ReconstructionFix == Reconstruction (src/parameters.f90:289-290) both

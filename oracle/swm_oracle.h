@@ -1,7 +1,8 @@
 /*
- * swm_oracle.h -- CPU restatement of the sw-mpdec mPDeC5 / WENO5 PATANKAR time
- * step, used ONLY as a test oracle (by tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py).  The product path never links it.
+ * swm_oracle.h -- CPU restatement of the sw-mpdec WENO5 time steps (mPDeC5 and
+ * the other schemes of the reference's two builds), used ONLY as a test oracle
+ * (by tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py).
+ * The product path never links it.
  *
  * Arrays use the reference's Fortran layouts (column-major, first index
  * fastest) so that dumps from the reference build (oracle/_ref) compare

@@ -1,5 +1,7 @@
-// swm_kernels.hip -- FP64 HIP kernels (gfx950) of the mPDeC5 / WENO5
-// PATANKAR time step of sw-mpdec, over all active jobs of an engine.
+// swm_kernels.hip -- FP64 HIP kernels (gfx950) of the WENO5 time steps of
+// sw-mpdec -- mPDeC5 and the other modified-Patankar schemes of the PATANKAR
+// build, and the explicit schemes of the build without it -- over all active
+// jobs of an engine.
 //
 // One FVTimeDerivative (src/finitevolume.f90:402-440) is three launches, each
 // covering every job and up to three DeC nodes (blockIdx.y in the face
