@@ -393,12 +393,13 @@ struct Term {
 CombOp comb_op(int mode, int out, int base, std::initializer_list<Term> terms) {
   CombOp op{};
   op.mode = mode;
-  op.out = out;
+  op.nout = 1;
+  op.out[0] = out;
   op.base = base;
   for (const Term& t : terms) {
     op.deriv[op.nterm] = t.deriv;
     op.src[op.nterm] = t.src;
-    op.c[op.nterm] = t.c;
+    op.c[0][op.nterm] = t.c;
     ++op.nterm;
   }
   return op;
@@ -476,7 +477,7 @@ int explicit_body(swm_engine* e, Tables& T, const SchemeDef& D) {
         for (int j = 0; j < st; ++j) {
           op.deriv[op.nterm] = F;
           op.src[op.nterm] = j;
-          op.c[op.nterm] = kARK65[st][j];
+          op.c[0][op.nterm] = kARK65[st][j];
           ++op.nterm;
         }
         comb(op);
@@ -486,7 +487,7 @@ int explicit_body(swm_engine* e, Tables& T, const SchemeDef& D) {
       for (int j = 0; j < 6; ++j) {
         op.deriv[op.nterm] = F;
         op.src[op.nterm] = j;
-        op.c[op.nterm] = kBRK65[j];
+        op.c[0][op.nterm] = kBRK65[j];
         ++op.nterm;
       }
       comb(op);
@@ -496,16 +497,20 @@ int explicit_body(swm_engine* e, Tables& T, const SchemeDef& D) {
       if ((rc = fvtd({{0, 0}}))) return rc;
       int src[4] = {0, 0, 0, 0};  // FUp(r): FVTD(U^n) for every r at kk = 1
       for (int kk = 1; kk <= 5; ++kk) {
+        // the correction's nodes in one launch: they share Ua(1) and FUp
+        CombOp op = comb_op(2, 0, 0, {});
+        op.nout = 0;
         for (int m = (kk == 5 ? 3 : 1); m < 4; ++m) {
-          CombOp op = comb_op(2, m, 0, {});
-          for (int r = 0; r < 4; ++r) {
-            op.deriv[op.nterm] = F;
-            op.src[op.nterm] = src[r];
-            op.c[op.nterm] = kThetaDeC5[m][r];
-            ++op.nterm;
-          }
-          comb(op);
+          op.out[op.nout] = m;
+          for (int r = 0; r < 4; ++r) op.c[op.nout][r] = kThetaDeC5[m][r];
+          ++op.nout;
         }
+        for (int r = 0; r < 4; ++r) {
+          op.deriv[r] = F;
+          op.src[r] = src[r];
+        }
+        op.nterm = 4;
+        comb(op);
         if (kk != 5) {
           if ((rc = fvtd({{1, 1}, {2, 2}, {3, 3}}))) return rc;
           for (int r = 0; r < 4; ++r) src[r] = r;

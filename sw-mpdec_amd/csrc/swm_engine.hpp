@@ -127,12 +127,16 @@ struct NodeList {
 //   mode 0: out = t_0 + t_1 + ...,   t_i = c_i*X_i (state) or (c_i*D_i)*dt (derivative)
 //   mode 1: out = X_base + (c_0*D_0 + c_1*D_1 + ...)*dt
 //   mode 2: out = X_base; out = out + (dt*c_i)*D_i for each term
+// Up to kMaxOut outputs share the sources and differ in their weights c[o]
+// (the DeC nodes of one correction).
 constexpr int kMaxTerms = 8;
+constexpr int kMaxOut = 3;
 struct CombOp {
-  int mode, out, base, nterm;
+  int mode, base, nterm, nout;
+  int out[kMaxOut];
   int deriv[kMaxTerms];
   int src[kMaxTerms];
-  double c[kMaxTerms];
+  double c[kMaxOut][kMaxTerms];
 };
 
 // Patankar/Jacobi launch parameters (the same for every job of a step).

@@ -828,34 +828,53 @@ __device__ __forceinline__ const gdouble* deriv_var(const JobDev& J, int v, int 
   return G(v == 0 ? J.d1[r] : v == 1 ? J.d2[r] : J.d3[r]);
 }
 
+// Every source value is loaded once per variable (fully unrolled over
+// kMaxTerms, guarded by nterm, so the values stay in registers) and serves
+// each of the op.nout outputs, which differ only in their weights.
 __global__ __launch_bounds__(256) void k_combine(EngineArgs E, const TileRef* blocks, CombOp op) {
   const TileRef T = blocks[blockIdx.x];
   const JobDev& J = E.jobs[T.job];
   const int k = T.tile * kCellBlock + threadIdx.x;
   if (k >= J.n) return;
   const double dt = J.status->dt;
-#pragma unroll 1
+#pragma unroll
   for (int v = 0; v < 3; ++v) {
-    double acc = 0.;
-    if (op.mode == 0) {
-      for (int t = 0; t < op.nterm; ++t) {
-        const double term = op.deriv[t] ? (op.c[t] * deriv_var(J, v, op.src[t])[k]) * dt
-                                        : op.c[t] * state_var(J, v, op.src[t])[k];
-        acc = t == 0 ? term : acc + term;
+    double x[kMaxTerms];
+#pragma unroll
+    for (int t = 0; t < kMaxTerms; ++t)
+      if (t < op.nterm)
+        x[t] = op.deriv[t] ? deriv_var(J, v, op.src[t])[k] : state_var(J, v, op.src[t])[k];
+    const double base = op.mode == 0 ? 0. : state_var(J, v, op.base)[k];
+#pragma unroll
+    for (int o = 0; o < kMaxOut; ++o) {
+      if (o >= op.nout) break;
+      const double* c = op.c[o];
+      double acc = 0.;
+      if (op.mode == 0) {
+#pragma unroll
+        for (int t = 0; t < kMaxTerms; ++t)
+          if (t < op.nterm) {
+            const double term = op.deriv[t] ? (c[t] * x[t]) * dt : c[t] * x[t];
+            acc = t == 0 ? term : acc + term;
+          }
+      } else if (op.mode == 1) {
+        double sum = 0.;
+#pragma unroll
+        for (int t = 0; t < kMaxTerms; ++t)
+          if (t < op.nterm) {
+            const double term = c[t] * x[t];
+            sum = t == 0 ? term : sum + term;
+          }
+        acc = base + sum * dt;
+      } else {
+        acc = base;
+#pragma unroll
+        for (int t = 0; t < kMaxTerms; ++t)
+          if (t < op.nterm) acc = acc + (dt * c[t]) * x[t];
       }
-    } else if (op.mode == 1) {
-      double sum = 0.;
-      for (int t = 0; t < op.nterm; ++t) {
-        const double term = op.c[t] * deriv_var(J, v, op.src[t])[k];
-        sum = t == 0 ? term : sum + term;
-      }
-      acc = state_var(J, v, op.base)[k] + sum * dt;
-    } else {
-      acc = state_var(J, v, op.base)[k];
-      for (int t = 0; t < op.nterm; ++t)
-        acc = acc + (dt * op.c[t]) * deriv_var(J, v, op.src[t])[k];
+      const int out = op.out[o];
+      G(v == 0 ? J.sh[out] : v == 1 ? J.shu[out] : J.shv[out])[k] = acc;
     }
-    G(v == 0 ? J.sh[op.out] : v == 1 ? J.shu[op.out] : J.shv[op.out])[k] = acc;
   }
 }
 

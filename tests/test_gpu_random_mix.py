@@ -1,0 +1,69 @@
+"""A seeded random mix of jobs in one engine -- every time scheme (1-7) and
+depth view, square and rectangular grids from 3 to 70 cells a side, with and
+without the well-balanced state and bathymetry, 1 to 4 steps -- each job
+bitwise against the oracle.  Broad coverage of the shape-dependent index
+arithmetic (tiles, seams, wraps on tiny axes) across the scheme paths."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import problems as P
+import swm
+from oracle_api import Oracle
+from parity_util import describe_diff, same_bits
+
+pytestmark = pytest.mark.gpu
+
+
+def random_state(n, ny, rng, amp=0.3):
+    U = np.empty((ny, n, 3))
+    U[..., 0] = 1.0 + amp * rng.uniform(-1, 1, (ny, n))
+    U[..., 1] = U[..., 0] * rng.normal(0, 0.3, (ny, n))
+    U[..., 2] = U[..., 0] * rng.normal(0, 0.3, (ny, n))
+    return P.Problem(n, U, kind="random", ny=ny)
+
+
+def make_job(rng):
+    n = int(rng.integers(3, 71))
+    ny = int(rng.integers(3, 71)) if rng.random() < 0.5 else n
+    kind = rng.choice(["random", "vortex", "lake_flag1", "lake_bumps"])
+    seed = int(rng.integers(1 << 20))
+    if kind == "random":
+        p = random_state(n, ny, np.random.default_rng(seed))
+    elif kind == "vortex":
+        p = P.vortex(n, seed, ny=0 if ny == n else ny)
+    elif kind == "lake_flag1":
+        p = P.lake_flag1(n, bool(rng.random() < 0.7), ny=0 if ny == n else ny)
+    else:
+        p = P.lake_bumps(max(n, 8), seed)
+    scheme = int(rng.integers(1, 8))
+    by_node = bool(rng.random() < 0.7)
+    steps = int(rng.integers(1, 5))
+    return p, scheme, by_node, steps
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_random_mix_matches_oracle(seed):
+    rng = np.random.default_rng(seed)
+    jobs = [make_job(rng) for _ in range(24)]
+    eng = swm.Engine()
+    ids = [eng.submit(p, k, timescheme=sc,
+                      hp_association=swm.HP_BY_NODE if bn else swm.HP_AS_REFERENCE)
+           for p, sc, bn, k in jobs]
+    while eng.step():
+        pass
+    for (p, sc, bn, k), jid in zip(jobs, ids):
+        o = Oracle(p, sc, hp_by_node=bn)
+        done = 0
+        for _ in range(k):
+            dt, _, _ = o.timestep()
+            if not np.isfinite(dt):
+                break
+            o.step(dt)
+            done += 1
+        info = eng.info(jid)
+        what = f"{p.kind} {p.n}x{p.rows} scheme {sc} by_node {bn} steps {k}"
+        assert info["steps_done"] == done, what
+        U, U_r = eng.fetch(jid), o.get_U()
+        assert same_bits(U, U_r), f"{what}: {describe_diff(U, U_r)}"
