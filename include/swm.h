@@ -162,8 +162,9 @@ int swm_fvtd(swm_solver_t *h, double *Ut, double *FX, double *FY, double *S, dou
 int swm_set_jacobi_limits(swm_solver_t *h, int max_iter, double tol);
 
 /* Stage positivity check (no counterpart in the reference): while on, the
- * device folds the minimum depth of every DeC stage solution (the 13 Jacobi
- * results of each step) into a per-handle record; NaN if any depth was NaN.
+ * device folds the minimum depth of every DeC stage solution (the result of
+ * every Patankar solve: 13 per mPDeC5 step; the explicit schemes solve
+ * nothing and record nothing) into a per-handle record; NaN if any depth was NaN.
  * Enabling resets the record.  swm_stage_min_h returns SWM_E_STATE when no
  * stage has run since. */
 int swm_track_stage_min(swm_solver_t *h, int on);

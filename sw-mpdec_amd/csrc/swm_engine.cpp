@@ -730,7 +730,7 @@ int engine_step(swm_engine* e, Tables& T, const SchemeDef* sc, int hp_assoc,
   }
   StepPlan P{};
   P.sc = sc;
-  P.hp_view = (sc->msteps < 4 && hp_assoc == SWM_HP_AS_REFERENCE) ? 1 : 0;
+  P.hp_view = (!is_explicit(*sc) && sc->msteps < 4 && hp_assoc == SWM_HP_AS_REFERENCE) ? 1 : 0;
   P.device_dt = host_dt == nullptr;
   bool spec = e->margin != -2 && !is_explicit(*sc);  // nothing to predict without solves
   for (int q = 0; q < T.njobs; ++q) {
