@@ -101,7 +101,9 @@ int swm_step_mpdec2(swm_solver_t *h, double t, double dt, int jacobi_iters[2]);
  *   TimeDiscretizationBySSPRK4 (:104-192, 5 stages),
  *   TimeDiscretizationByRK65 (:238-356, 6 stages),
  *   TimeDiscretizationByDeC5 (:362-451, 13 FVTimeDerivative evaluations).
- * No Jacobi solves; returns SWM_OK or an error. */
+ * No Jacobi solves; returns SWM_OK or an error.  (The PATANKAR build also
+ * accepts these timescheme numbers, but it never writes Ut(1), so there the
+ * depth stays frozen; that degenerate variant is not offered.) */
 int swm_step_forward_euler(swm_solver_t *h, double t, double dt);
 int swm_step_ssprk4(swm_solver_t *h, double t, double dt);
 int swm_step_rk65(swm_solver_t *h, double t, double dt);

@@ -179,13 +179,15 @@ def have_swref_explicit():
     return SWREF_EXPLICIT[False].exists() and SWREF_EXPLICIT[True].exists()
 
 
-def run_swref(problem, mode: int, steps: int = 0, jac=None, timeout=600, scheme: int = 5):
+def run_swref(problem, mode: int, steps: int = 0, jac=None, timeout=600, scheme: int = 5,
+              patankar=None):
     """Run the reference build on problem.  mode 1: one FVTimeDerivative,
     2: `steps` CFL steps through TimeDiscretization, 3: TimeStep only,
     4: jacobi on jac=(Mass, diag, rhs).  scheme: the reference's timescheme
     (5 mPDeC5, 6 mPEuler, 7 mPDeC2 with the PATANKAR build; 1-4 with the
-    build without it)."""
-    patankar = scheme >= 5
+    build without it, unless patankar=True forces the PATANKAR build)."""
+    if patankar is None:
+        patankar = scheme >= 5
     exe = (SWREF if patankar else SWREF_EXPLICIT)[bool(problem.wellbalanced)]
     if not exe.exists():
         raise RuntimeError(f"{exe} not built")

@@ -170,3 +170,14 @@ def test_explicit_schemes_bitwise(case, steps, scheme):
         o.step(dt)
         U = o.get_U()
     assert same_bits(U, U_r), describe_diff(U, U_r)
+
+
+@pytest.mark.parametrize("scheme", [1, 3, 4])
+def test_patankar_build_explicit_schemes_freeze_the_depth(scheme):
+    """Why the explicit schemes follow the build without PATANKAR: in the
+    PATANKAR build nothing writes Ut(1) (src/finitevolume.f90:525-527 set
+    Ut(2:3) only), so these schemes leave h bitwise unchanged there."""
+    p = CASES["vortex21"]()
+    _, U = run_swref(p, 2, steps=1, scheme=scheme, patankar=True)
+    assert same_bits(U[..., 0], p.U0[..., 0])
+    assert not same_bits(U[..., 1], p.U0[..., 1])
