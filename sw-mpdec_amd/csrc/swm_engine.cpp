@@ -1,7 +1,8 @@
 // swm_engine.cpp -- host side of the solve engine and of the C ABI
-// (include/swm.h): per-job device memory, tile tables, the lockstep mPDeC5
-// step over all running jobs, the device-decided Jacobi loop, admission
-// against a device-memory budget, and HIP-event kernel timing.
+// (include/swm.h): per-job device memory, tile tables, the lockstep step of
+// each time scheme over all running jobs of that scheme, the device-decided
+// Jacobi loop, admission against a device-memory budget, and HIP-event
+// kernel timing.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
