@@ -281,10 +281,15 @@ def main():
                     help="requests of the served run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0,
                     help="Poisson arrival rate of the served run, per second")
+    ap.add_argument("--serve-rates", type=str, default="",
+                    help="comma-separated further arrival rates, each a served run of its own "
+                         "(a rate sweep, reported under served_sweep)")
     ap.add_argument("--graph", type=int, default=-1,
                     help="step execution: 1 replay a captured HIP graph, 0 no graph, -1 autotuned")
     ap.add_argument("--margin", type=int, default=-1,
                     help="extra speculative Jacobi sweeps per correction (-1 autotuned, -2 none)")
+    ap.add_argument("--tile-shape", type=int, default=-1,
+                    help="face-kernel tile shape: 0 16x16, 1 32x16/16x32, -1 autotuned")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -301,6 +306,7 @@ def main():
     log(f"bench: problem {n}x{n} lake_bumps seed {args.seed} built in {time.time() - t0:.1f} s")
     s = swm.Solver.from_problem(prob)
     s.set_execution(args.graph, args.margin)
+    s.set_tile_shape(args.tile_shape)
     del prob
     for _ in range(args.warmup):
         dt, _, _ = s.timestep()
@@ -372,7 +378,7 @@ def main():
                                f"8 Gaussian bumps, CFL 0.5",
                    "grid": [n, n], "cfl": 0.5, "wellbalanced": True, "seed": args.seed},
         "jacobi_sweeps_per_step": jsum / args.steps,
-        "execution": s.execution_stats(),
+        "execution": dict(s.execution_stats(), tile_shape=s.tile_shape()[1]),
         "kernels": kern,
         "roofline": roof,
     }
@@ -383,6 +389,10 @@ def main():
         out["batched"] = batched_section(args.batch_jobs, args.seed)
     if args.serve_requests > 0:
         out["served"] = served_section(args.serve_requests, args.seed, args.serve_rate)
+        rates = [float(r) for r in args.serve_rates.split(",") if r.strip()]
+        if rates:
+            out["served_sweep"] = [served_section(args.serve_requests, args.seed, r)
+                                   for r in rates]
     if not args.no_cpu:
         cb = cpu_baseline()
         if cb:

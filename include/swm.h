@@ -1,7 +1,9 @@
 /*
  * swm.h -- C ABI of the MI355X (gfx950) implementation of sw-mpdec's
  * PATANKAR-build time step: timescheme=5 (mPDeC5) with Reconstruction=4
- * (WENO5 at 4-point Gauss nodes), periodic boundaries, FP64.
+ * (WENO5 at 4-point Gauss nodes), periodic boundaries, FP64 -- and, on the
+ * same kernels, the reference's other time schemes (timescheme 1-4 of the
+ * build without PATANKAR, 6 and 7 of the PATANKAR build).
  *
  * The reference (accdavlo/sw-mpdec, Fortran) has no FFI; its hot path sits
  * behind module procedures that share module-global state.  Each entry point
@@ -172,9 +174,12 @@ int swm_set_jacobi_limits(swm_solver_t *h, int max_iter, double tol);
 int swm_track_stage_min(swm_solver_t *h, int on);
 int swm_stage_min_h(swm_solver_t *h, double *hmin);
 
-/* swm_engine_set_execution / swm_engine_execution_stats for a plain handle. */
+/* swm_engine_set_execution / swm_engine_execution_stats /
+ * swm_engine_set_tile_shape / swm_engine_tile_shape for a plain handle. */
 int swm_set_execution(swm_solver_t *h, int use_graph, int margin);
 int swm_execution_stats(const swm_solver_t *h, long long stats[6]);
+int swm_set_tile_shape(swm_solver_t *h, int shape);
+int swm_tile_shape(const swm_solver_t *h, int dims[4]);
 
 /* Number of Jacobi stop decisions so far that fell inside the rounding window
  * of the parallel residual sum and were settled by the exact sequential sum. */
@@ -255,12 +260,22 @@ int swm_engine_set_jacobi_limits(swm_engine_t *e, int max_iter, double tol);
  * reading back, replayed from a captured HIP graph when use_graph is 1, and
  * checks once before committing; a step whose solves needed more sweeps is
  * redone with read-backs.  -1 lets the built-in autotuner time the
- * candidates (graph 0/1, margin 1/2) on the first steps and keep the fastest;
- * margin = -2 turns speculation off (every step reads back).
+ * candidates (face-kernel tile shape, then graph 0/1 x margin 1/2) on the
+ * first speculative steps and keep the fastest; margin = -2 turns
+ * speculation off (every step reads back).
  * stats: speculative steps, redone steps, graph launches, graph captures,
  * graph and margin in use. */
 int swm_engine_set_execution(swm_engine_t *e, int use_graph, int margin);
 int swm_engine_execution_stats(const swm_engine_t *e, long long stats[6]);
+/* Tile shape of the face kernels (no counterpart in the reference; results
+ * are bitwise the same with every shape): 0 = 16x16 threads for both face
+ * directions, 1 = 32x16 for x faces and 16x32 for y faces (fewer redundant
+ * halo sweeps, more idle threads on small grids), -1 = chosen by the
+ * autotuner (the default).  swm_engine_tile_shape returns the shape in use
+ * (the tuned one once tuning has finished, 0 before) and, if dims is not
+ * NULL, its x-kernel and y-kernel thread extents. */
+int swm_engine_set_tile_shape(swm_engine_t *e, int shape);
+int swm_engine_tile_shape(const swm_engine_t *e, int dims[4]);
 
 /* The stage positivity check of swm_track_stage_min, per job of an engine. */
 int swm_engine_track_stage_min(swm_engine_t *e, int on);

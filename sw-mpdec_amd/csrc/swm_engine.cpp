@@ -123,11 +123,11 @@ struct DevVec {
 };
 
 struct Tables {
-  std::vector<TileRef> xt, yt, ct, cb;
+  std::vector<TileRef> xt[kNumShapes], yt[kNumShapes], ct, cb;  // face tiles per shape
   std::vector<int> jl;
-  DevVec<TileRef> dxt, dyt, dct, dcb;
+  DevVec<TileRef> dxt[kNumShapes], dyt[kNumShapes], dct, dcb;
   DevVec<int> djl;
-  int nxt = 0, nyt = 0, nct = 0, ncb = 0, njobs = 0;
+  int nxt[kNumShapes] = {}, nyt[kNumShapes] = {}, nct = 0, ncb = 0, njobs = 0;
 };
 
 enum JobState { JOB_RUNNING = 0, JOB_DONE = 1, JOB_FAILED = 2, JOB_FREE = 3 };
@@ -158,6 +158,9 @@ struct Job {
 // the depth view: group 0 mPDeC5, 1-2 mPEuler, 3-4 mPDeC2 (reference view,
 // by-node view), 5-8 the explicit schemes 1-4.
 constexpr int kGroups = 9;
+// autotuner candidates: each tile shape with (graph 0, margin 1), then
+// (1,1), (1,2), (0,2) with the faster shape
+constexpr int kTuneCands = kNumShapes + 3;
 int group_of(const SchemeDef* d, int hp_assoc) {
   const int si = (int)(d - kSchemes);
   if (si == 0) return 0;
@@ -195,9 +198,11 @@ struct swm_engine {
   // reads and check once at the end; they may be replayed from a captured
   // HIP graph.  -1 = chosen by the autotuner.
   int use_graph = -1, margin = -1;
-  int tuned_graph = 1, tuned_margin = 1;
+  int tile_shape = -1;           // face-kernel tile shape (kShapes), -1 = autotuned
+  int tuned_graph = 1, tuned_margin = 1, tuned_shape = 0;
+  int cur_shape = 0;             // the shape the launches use now
   int tune_phase = 0;            // autotuner: candidates timed so far
-  double tune_ms[4] = {0, 0, 0, 0};
+  double tune_ms[kTuneCands] = {};
   long long graph_epoch = 0;     // bumped whenever captured arguments change
   long long spec_steps = 0, redone_steps = 0, graph_launches = 0, graph_captures = 0;
   struct Captured {
@@ -205,7 +210,7 @@ struct swm_engine {
     int device_dt = 0, track = 0;
     int sweeps[5] = {0, 0, 0, 0, 0};
     const void* tables = nullptr;
-    int group = -1;
+    int group = -1, shape = -1;
     long long used = 0;
     hipGraphExec_t exec = nullptr;
   };
@@ -262,29 +267,38 @@ void timed(swm_engine* e, int cat, F&& launch) {
 }
 
 void add_tiles(Tables& T, int id, const JobDev& d) {
-  const int nxt = cdiv(d.nx + 1, kFaceTile) * cdiv(d.ny, kTile);
-  const int nyt = cdiv(d.nx, kTile) * cdiv(d.ny + 1, kFaceTile);
+  for (int sh = 0; sh < kNumShapes; ++sh) {
+    const FaceShape& F = kShapes[sh];
+    const int nxt = cdiv(d.nx + 1, F.xw - 1) * cdiv(d.ny, F.xh);
+    const int nyt = cdiv(d.nx, F.yw) * cdiv(d.ny + 1, F.yh - 1);
+    for (int t = 0; t < nxt; ++t) T.xt[sh].push_back({id, t});
+    for (int t = 0; t < nyt; ++t) T.yt[sh].push_back({id, t});
+  }
   const int nct = cdiv(d.nx, kTile) * cdiv(d.ny, kTile);
-  for (int t = 0; t < nxt; ++t) T.xt.push_back({id, t});
-  for (int t = 0; t < nyt; ++t) T.yt.push_back({id, t});
   for (int t = 0; t < nct; ++t) T.ct.push_back({id, t});
   for (int t = 0; t < d.nblk; ++t) T.cb.push_back({id, t});
   T.jl.push_back(id);
 }
 
 int upload_tables(swm_engine* e, Tables& T) {
-  T.nxt = (int)T.xt.size();
-  T.nyt = (int)T.yt.size();
+  for (int sh = 0; sh < kNumShapes; ++sh) {
+    T.nxt[sh] = (int)T.xt[sh].size();
+    T.nyt[sh] = (int)T.yt[sh].size();
+    HIPCK(e, T.dxt[sh].reserve(T.xt[sh].size()));
+    HIPCK(e, T.dyt[sh].reserve(T.yt[sh].size()));
+    if (T.nxt[sh])
+      HIPCK(e, hipMemcpy(T.dxt[sh].p, T.xt[sh].data(), T.nxt[sh] * sizeof(TileRef),
+                         hipMemcpyHostToDevice));
+    if (T.nyt[sh])
+      HIPCK(e, hipMemcpy(T.dyt[sh].p, T.yt[sh].data(), T.nyt[sh] * sizeof(TileRef),
+                         hipMemcpyHostToDevice));
+  }
   T.nct = (int)T.ct.size();
   T.ncb = (int)T.cb.size();
   T.njobs = (int)T.jl.size();
-  HIPCK(e, T.dxt.reserve(T.xt.size()));
-  HIPCK(e, T.dyt.reserve(T.yt.size()));
   HIPCK(e, T.dct.reserve(T.ct.size()));
   HIPCK(e, T.dcb.reserve(T.cb.size()));
   HIPCK(e, T.djl.reserve(T.jl.size()));
-  if (T.nxt) HIPCK(e, hipMemcpy(T.dxt.p, T.xt.data(), T.nxt * sizeof(TileRef), hipMemcpyHostToDevice));
-  if (T.nyt) HIPCK(e, hipMemcpy(T.dyt.p, T.yt.data(), T.nyt * sizeof(TileRef), hipMemcpyHostToDevice));
   if (T.nct) HIPCK(e, hipMemcpy(T.dct.p, T.ct.data(), T.nct * sizeof(TileRef), hipMemcpyHostToDevice));
   if (T.ncb) HIPCK(e, hipMemcpy(T.dcb.p, T.cb.data(), T.ncb * sizeof(TileRef), hipMemcpyHostToDevice));
   if (T.njobs) HIPCK(e, hipMemcpy(T.djl.p, T.jl.data(), T.njobs * sizeof(int), hipMemcpyHostToDevice));
@@ -301,8 +315,10 @@ int refresh(swm_engine* e) {
   for (size_t i = 0; i < nj; ++i) host[i] = e->jobs[i]->dev;
   if (nj) HIPCK(e, hipMemcpy(e->djobs.p, host.data(), nj * sizeof(JobDev), hipMemcpyHostToDevice));
   for (Tables& T : e->groups) {
-    T.xt.clear();
-    T.yt.clear();
+    for (int sh = 0; sh < kNumShapes; ++sh) {
+      T.xt[sh].clear();
+      T.yt[sh].clear();
+    }
     T.ct.clear();
     T.cb.clear();
     T.jl.clear();
@@ -331,9 +347,10 @@ EngineArgs eargs(swm_engine* e) {
 int fvtd_launch(swm_engine* e, Tables& T, const NodeList& L) {
   EngineArgs a = eargs(e);
   hipStream_t s = e->stream;
-  timed(e, 0, [&] { launch_fvtd(a, T.dxt.p, T.nxt, nullptr, 0, nullptr, 0, L, s); });
-  timed(e, 1, [&] { launch_fvtd(a, nullptr, 0, T.dyt.p, T.nyt, nullptr, 0, L, s); });
-  timed(e, 2, [&] { launch_fvtd(a, nullptr, 0, nullptr, 0, T.dct.p, T.nct, L, s); });
+  const int sh = e->cur_shape;
+  timed(e, 0, [&] { launch_fvtd(a, sh, T.dxt[sh].p, T.nxt[sh], nullptr, 0, nullptr, 0, L, s); });
+  timed(e, 1, [&] { launch_fvtd(a, sh, nullptr, 0, T.dyt[sh].p, T.nyt[sh], nullptr, 0, L, s); });
+  timed(e, 2, [&] { launch_fvtd(a, sh, nullptr, 0, nullptr, 0, T.dct.p, T.nct, L, s); });
   LAUNCHCK(e);
   return SWM_OK;
 }
@@ -641,7 +658,8 @@ int step_body_graph(swm_engine* e, Tables& T, const StepPlan& P) {
   }
   auto& c = *cp;
   bool same = c.exec && c.epoch == e->graph_epoch && c.device_dt == (int)P.device_dt &&
-              c.track == (int)e->track_min && c.tables == (const void*)&T && c.group == group;
+              c.track == (int)e->track_min && c.tables == (const void*)&T && c.group == group &&
+              c.shape == e->cur_shape;
   for (int i = 0; same && i < 5; ++i) same = c.sweeps[i] == P.sweeps[i];
   c.used = ++e->cap_clock;
   if (!same) {
@@ -666,6 +684,7 @@ int step_body_graph(swm_engine* e, Tables& T, const StepPlan& P) {
     c.track = e->track_min;
     c.tables = &T;
     c.group = group;
+    c.shape = e->cur_shape;
     for (int i = 0; i < 5; ++i) c.sweeps[i] = P.sweeps[i];
     ++e->graph_captures;
   }
@@ -674,32 +693,50 @@ int step_body_graph(swm_engine* e, Tables& T, const StepPlan& P) {
   return SWM_OK;
 }
 
-// Autotuner over (graph, margin): the first speculative steps of an engine
-// run the candidates (1,1), (0,1), (1,2), (0,2) one step each and keep the
-// fastest; explicit settings (swm_engine_set_execution) bypass it.
-void tune_choice(swm_engine* e, int& graph, int& margin) {
+// Autotuner over (tile shape, graph, margin).  The first speculative steps of
+// an engine time one candidate each: the face-kernel tile shapes first,
+// without a graph (so no capture time enters that comparison), then graph
+// on/off x margin 1/2 with the faster shape; the fastest candidate is kept.
+// Explicit settings (swm_engine_set_execution, swm_engine_set_tile_shape)
+// override their dimension.
+struct TuneCand {
+  int graph, margin, shape;
+};
+
+TuneCand tune_cand(const swm_engine* e, int phase) {
+  if (phase < kNumShapes) return {0, 1, phase};
+  int best = 0;  // the faster shape of the first phases
+  for (int i = 1; i < kNumShapes; ++i)
+    if (e->tune_ms[i] < e->tune_ms[best]) best = i;
   static const int cg[4] = {1, 0, 1, 0}, cm[4] = {1, 1, 2, 2};
-  if (e->tune_phase < 4 && (e->use_graph < 0 || e->margin < 0)) {
-    graph = cg[e->tune_phase];
-    margin = cm[e->tune_phase];
-  } else {
-    graph = e->tuned_graph;
-    margin = e->tuned_margin;
-  }
-  if (e->use_graph >= 0) graph = e->use_graph;
-  if (e->margin >= 0) margin = e->margin;
+  const int q = phase - kNumShapes;  // (0,1) was timed with each shape already
+  const int idx = q == 0 ? 0 : q + 1;
+  return {cg[idx], cm[idx], best};
+}
+
+bool tuning(const swm_engine* e) {
+  return e->tune_phase < kTuneCands && (e->use_graph < 0 || e->margin < 0 || e->tile_shape < 0);
+}
+
+void tune_choice(swm_engine* e, int& graph, int& margin) {
+  TuneCand c = tuning(e) ? tune_cand(e, e->tune_phase)
+                         : TuneCand{e->tuned_graph, e->tuned_margin, e->tuned_shape};
+  graph = e->use_graph >= 0 ? e->use_graph : c.graph;
+  margin = e->margin >= 0 ? e->margin : c.margin;
+  e->cur_shape = e->tile_shape >= 0 ? e->tile_shape : c.shape;
 }
 
 void tune_record(swm_engine* e, double ms) {
-  if (e->tune_phase >= 4 || (e->use_graph >= 0 && e->margin >= 0)) return;
-  static const int cg[4] = {1, 0, 1, 0}, cm[4] = {1, 1, 2, 2};
+  if (!tuning(e)) return;
   e->tune_ms[e->tune_phase++] = ms;
-  if (e->tune_phase == 4) {
+  if (e->tune_phase == kTuneCands) {
     int best = 0;
-    for (int i = 1; i < 4; ++i)
+    for (int i = 1; i < kTuneCands; ++i)
       if (e->tune_ms[i] < e->tune_ms[best]) best = i;
-    e->tuned_graph = cg[best];
-    e->tuned_margin = cm[best];
+    const TuneCand c = tune_cand(e, best);
+    e->tuned_graph = c.graph;
+    e->tuned_margin = c.margin;
+    e->tuned_shape = c.shape;
   }
 }
 
@@ -710,6 +747,7 @@ int engine_step(swm_engine* e, Tables& T, const SchemeDef* sc, int hp_assoc,
                 const double* host_dt, int iters[13], const double* caps = nullptr) {
   if (T.njobs == 0) return SWM_OK;
   hipStream_t s = e->stream;
+  e->cur_shape = e->tile_shape >= 0 ? e->tile_shape : e->tuned_shape;  // tune_choice may vary it
   // the time-loop clipping (caps = {tAnalyze - t, tEnd - t}) or none
   *e->hcaps = caps ? swm_engine::Caps{1, 0, caps[0], caps[1]} : swm_engine::Caps{0, 0, 0.0, 0.0};
   for (int q = 0; q < T.njobs; ++q) {
@@ -1299,6 +1337,27 @@ int swm_engine_set_execution(swm_engine_t* e, int use_graph, int margin) {
   return SWM_OK;
 }
 
+int swm_engine_set_tile_shape(swm_engine_t* e, int shape) {
+  if (!e) return SWM_E_ARG;
+  if (shape < -1 || shape >= kNumShapes)
+    FAIL(e, SWM_E_ARG, "tile shape must be -1 (autotuned) or 0..%d (got %d)", kNumShapes - 1,
+         shape);
+  e->tile_shape = shape;
+  return SWM_OK;
+}
+
+int swm_engine_tile_shape(const swm_engine_t* e, int dims[4]) {
+  if (!e) return SWM_E_ARG;
+  const int sh = e->tile_shape >= 0 ? e->tile_shape : e->tuned_shape;
+  if (dims) {
+    dims[0] = kShapes[sh].xw;
+    dims[1] = kShapes[sh].xh;
+    dims[2] = kShapes[sh].yw;
+    dims[3] = kShapes[sh].yh;
+  }
+  return sh;
+}
+
 int swm_engine_execution_stats(const swm_engine_t* e, long long out[6]) {
   if (!e || !out) return SWM_E_ARG;
   out[0] = e->spec_steps;
@@ -1660,6 +1719,16 @@ int swm_stage_min_h(swm_solver_t* s, double* hmin) {
 int swm_set_execution(swm_solver_t* s, int use_graph, int margin) {
   if (!s) return SWM_E_ARG;
   return swm_engine_set_execution(s->e, use_graph, margin);
+}
+
+int swm_set_tile_shape(swm_solver_t* s, int shape) {
+  if (!s) return SWM_E_ARG;
+  return swm_engine_set_tile_shape(s->e, shape);
+}
+
+int swm_tile_shape(const swm_solver_t* s, int dims[4]) {
+  if (!s) return SWM_E_ARG;
+  return swm_engine_tile_shape(s->e, dims);
 }
 
 int swm_execution_stats(const swm_solver_t* s, long long stats[6]) {

@@ -20,8 +20,17 @@
 
 namespace swm {
 
-constexpr int kTile = 16;             // FVTD tiles: 16x16 cells, 256 threads
-constexpr int kFaceTile = kTile - 1;  // faces a tile produces along its sweep
+constexpr int kTile = 16;             // k_fvtd_update tiles: 16x16 cells, 256 threads
+
+// Tile shapes of the face kernels: k_fvtd_x runs xw x xh threads per tile and
+// produces xw-1 faces per row; k_fvtd_y runs yw x yh threads and yh-1 faces
+// per column.  Wider tiles recompute fewer halo sweeps; narrower ones waste
+// fewer threads on small grids.  The engine's autotuner picks one.
+struct FaceShape {
+  int xw, xh, yw, yh;
+};
+constexpr int kNumShapes = 2;
+constexpr FaceShape kShapes[kNumShapes] = {{16, 16, 16, 16}, {32, 16, 16, 32}};
 constexpr int kCellBlock = 256;       // cells per block of the pointwise kernels
 constexpr int kMaxSys = 3;            // Patankar systems solved together (nodes 2..4)
 constexpr int kSlots = 3;             // concurrent FVTD evaluations per job
@@ -174,8 +183,9 @@ inline const SchemeDef* find_scheme(int timescheme) {
 }
 
 // ---- launches (swm_kernels.hip) -------------------------------------------
-void launch_fvtd(const EngineArgs& E, const TileRef* xt, int nxt, const TileRef* yt, int nyt,
-                 const TileRef* ct, int nct, const NodeList& L, hipStream_t s);
+// xt / yt: face tiles of shape kShapes[shape]; ct: k_fvtd_update tiles.
+void launch_fvtd(const EngineArgs& E, int shape, const TileRef* xt, int nxt, const TileRef* yt,
+                 int nyt, const TileRef* ct, int nct, const NodeList& L, hipStream_t s);
 void launch_momentum(const EngineArgs& E, const TileRef* cb, int ncb, const SysList& S,
                      hipStream_t s);
 void launch_assemble(const EngineArgs& E, const TileRef* cb, int ncb, const int* jl, int njobs,

@@ -82,74 +82,93 @@ __device__ __forceinline__ void node_state(const JobDev& J, int m, const gdouble
 }
 
 // ------------------------------------------------------------------------
-// x faces.  Tile (bx, by) of a job: Fortran cells c = 15*bx + tx (tx = 0..15,
-// c in 0..nx+1; c = 0 is the ghost of nx, c = nx+1 the ghost of 1), rows
-// j = 16*by + 1 + ty.  Thread (tx, ty) reconstructs cell (c, j) and, for
-// tx < 15, computes face f = c between cells c and c+1 (f = 0..nx).  The
-// positivity limiter runs for c = 0..nx only (src/finitevolume.f90:670), so
-// face f = nx uses the unlimited left value of cell nx+1 while f = 0 uses the
-// limited one of cell 1: the two values of the periodic seam.
+// Face kernels, templated on the tile shape TW x TH (threads in x, y).
+//
+// x faces.  Tile (bx, by) of a job: Fortran cells c = (TW-1)*bx + tx
+// (tx = 0..TW-1, c in 0..nx+1; c = 0 is the ghost of nx, c = nx+1 the ghost
+// of 1), rows j = TH*by + 1 + ty.  Thread (tx, ty) reconstructs cell (c, j)
+// and, for tx < TW-1, computes face f = c between cells c and c+1
+// (f = 0..nx).  The positivity limiter runs for c = 0..nx only
+// (src/finitevolume.f90:670), so face f = nx uses the unlimited left value
+// of cell nx+1 while f = 0 uses the limited one of cell 1: the two values of
+// the periodic seam.  Every face is computed by exactly one thread with the
+// same arithmetic whatever the shape, so the results do not depend on it.
 // ------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_fvtd_x(EngineArgs E, const TileRef* tiles,
-                                                NodeList L) {
-  __shared__ double smem[3 * 400 + 2 * 3 * 320];
-  double* sV = smem;          // [3][20 rows][20 cols]
-  double* sQM = smem + 1200;  // [3][20 rows][16 cols]
-  double* sQP = sQM + 960;
+template <int TW, int TH>
+struct FaceTile {
+  static constexpr int NT = TW * TH;               // threads
+  static constexpr int VW = TW + 4, VH = TH + 4;   // halo tile of primitives
+  static constexpr int NV = VW * VH;
+  // first-sweep results per variable: VH x TW (x faces) or TH x VW (y faces)
+  static constexpr int NQ = (TW * VH > TH * VW) ? TW * VH : TH * VW;
+  // doubles of LDS: primitives + both first sweeps, reused for the WM exchange
+  static constexpr int LDS = (3 * NV + 6 * NQ > 12 * NT) ? 3 * NV + 6 * NQ : 12 * NT;
+};
+
+template <int TW, int TH>
+__global__ __launch_bounds__(TW * TH) void k_fvtd_x(EngineArgs E, const TileRef* tiles,
+                                                    NodeList L) {
+  using FT = FaceTile<TW, TH>;
+  constexpr int NT = FT::NT, VW = FT::VW, VH = FT::VH, NV = FT::NV;
+  constexpr int NQ = VH * TW;  // first sweeps per variable: VH rows x TW cells
+  __shared__ double smem[FT::LDS];
+  double* sV = smem;            // [3][VH rows][VW cols]
+  double* sQM = smem + 3 * NV;  // [3][VH rows][TW cols]
+  double* sQP = sQM + 3 * NQ;
   const TileRef T = tiles[blockIdx.x];
   const JobDev& J = E.jobs[T.job];
   const int m = L.m[blockIdx.y], slot = L.slot[blockIdx.y];
   const int nx = J.nx, ny = J.ny;
-  const int ntx = cdiv_d(nx + 1, kFaceTile);
-  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-  const int c0 = (T.tile % ntx) * kFaceTile;
-  const int j0 = (T.tile / ntx) * kTile + 1;
+  const int ntx = cdiv_d(nx + 1, TW - 1);
+  const int tid = threadIdx.x, tx = tid % TW, ty = tid / TW;
+  const int c0 = (T.tile % ntx) * (TW - 1);
+  const int j0 = (T.tile / ntx) * TH + 1;
   const gdouble *h, *hu, *hv;
   node_state(J, m, h, hu, hv);
 
-  for (int t = tid; t < 400; t += 256) {
-    int lx = t % 20, ly = t / 20;
+  for (int t = tid; t < NV; t += NT) {
+    int lx = t % VW, ly = t / VW;
     size_t k = (size_t)wrap1(j0 - 2 + ly, ny) * nx + wrap1(c0 - 2 + lx, nx);
     double hh = h[k], u, v;
     cons_to_prim(hh, hu[k], hv[k], u, v);
     sV[t] = hh;
-    sV[400 + t] = u;
-    sV[800 + t] = v;
+    sV[NV + t] = u;
+    sV[2 * NV + t] = v;
   }
   __syncthreads();
-  for (int t = tid; t < 960; t += 256) {
-    int var = t / 320, r = t % 320, ly = r >> 4, lx = r & 15;
-    const double* q = sV + var * 400 + ly * 20 + lx;
+  for (int t = tid; t < 3 * NQ; t += NT) {
+    int var = t / NQ, r = t % NQ, ly = r / TW, lx = r % TW;
+    const double* q = sV + var * NV + ly * VW + lx;
     double wm, wp;
     weno5_first(q[0], q[1], q[2], q[3], q[4], wm, wp);
-    sQM[var * 320 + r] = wm;
-    sQP[var * 320 + r] = wp;
+    sQM[var * NQ + r] = wm;
+    sQP[var * NQ + r] = wp;
   }
   __syncthreads();
   double WM[3][4], WP[3][4];
 #pragma unroll
   for (int var = 0; var < 3; ++var) {
-    const double* qm = sQM + var * 320 + ty * 16 + tx;
-    weno5_second(qm[0], qm[16], qm[32], qm[48], qm[64], WM[var]);
-    const double* qp = sQP + var * 320 + ty * 16 + tx;
-    weno5_second(qp[0], qp[16], qp[32], qp[48], qp[64], WP[var]);
+    const double* qm = sQM + var * NQ + ty * TW + tx;
+    weno5_second(qm[0], qm[TW], qm[2 * TW], qm[3 * TW], qm[4 * TW], WM[var]);
+    const double* qp = sQP + var * NQ + ty * TW + tx;
+    weno5_second(qp[0], qp[TW], qp[2 * TW], qp[3 * TW], qp[4 * TW], WP[var]);
   }
-  const double vh = sV[(ty + 2) * 20 + tx + 2];
+  const double vh = sV[(ty + 2) * VW + tx + 2];
   const int c = c0 + tx;
   if (c <= nx) positivity_limit(vh, J.wbnd, WM[0], WP[0]);
   __syncthreads();
-  double* sWM = smem;  // [12][256]
+  double* sWM = smem;  // [12][NT]
 #pragma unroll
-  for (int a = 0; a < 12; ++a) sWM[a * 256 + tid] = WM[a >> 2][a & 3];
+  for (int a = 0; a < 12; ++a) sWM[a * NT + tid] = WM[a >> 2][a & 3];
   __syncthreads();
   const int j = j0 + ty;
-  if (tx < kFaceTile && c <= nx && j <= ny) {
+  if (tx < TW - 1 && c <= nx && j <= ny) {
     double acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
       double F1, F2, F3;
-      rusanov<1, 0, 0, 1>(WP[0][g], WP[1][g], WP[2][g], sWM[(0 + g) * 256 + tid + 1],
-                          sWM[(4 + g) * 256 + tid + 1], sWM[(8 + g) * 256 + tid + 1], J.g, F1,
+      rusanov<1, 0, 0, 1>(WP[0][g], WP[1][g], WP[2][g], sWM[(0 + g) * NT + tid + 1],
+                          sWM[(4 + g) * NT + tid + 1], sWM[(8 + g) * NT + tid + 1], J.g, F1,
                           F2, F3);
       acc1 = acc1 + J.wbnd[g] * F1;
       acc2 = acc2 + J.wbnd[g] * F2;
@@ -164,68 +183,72 @@ __global__ __launch_bounds__(256) void k_fvtd_x(EngineArgs E, const TileRef* til
 
 // ------------------------------------------------------------------------
 // y faces: the transpose of k_fvtd_x.  Tile (bx, by): columns
-// i = 16*bx + 1 + tx, Fortran rows r = 15*by + ty (0..ny+1); face g = r.
+// i = TW*bx + 1 + tx, Fortran rows r = (TH-1)*by + ty (0..ny+1); face g = r.
 // ------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_fvtd_y(EngineArgs E, const TileRef* tiles,
-                                                NodeList L) {
-  __shared__ double smem[3 * 400 + 2 * 3 * 320];
-  double* sV = smem;          // [3][20 rows][20 cols]
-  double* sQM = smem + 1200;  // [3][16 rows][20 cols]
-  double* sQP = sQM + 960;
+template <int TW, int TH>
+__global__ __launch_bounds__(TW * TH) void k_fvtd_y(EngineArgs E, const TileRef* tiles,
+                                                    NodeList L) {
+  using FT = FaceTile<TW, TH>;
+  constexpr int NT = FT::NT, VW = FT::VW, NV = FT::NV;
+  constexpr int NQ = TH * VW;  // first sweeps per variable: TH rows x VW columns
+  __shared__ double smem[FT::LDS];
+  double* sV = smem;            // [3][VH rows][VW cols]
+  double* sQM = smem + 3 * NV;  // [3][TH rows][VW cols]
+  double* sQP = sQM + 3 * NQ;
   const TileRef T = tiles[blockIdx.x];
   const JobDev& J = E.jobs[T.job];
   const int m = L.m[blockIdx.y], slot = L.slot[blockIdx.y];
   const int nx = J.nx, ny = J.ny;
-  const int ntx = cdiv_d(nx, kTile);
-  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-  const int i0 = (T.tile % ntx) * kTile + 1;
-  const int r0 = (T.tile / ntx) * kFaceTile;
+  const int ntx = cdiv_d(nx, TW);
+  const int tid = threadIdx.x, tx = tid % TW, ty = tid / TW;
+  const int i0 = (T.tile % ntx) * TW + 1;
+  const int r0 = (T.tile / ntx) * (TH - 1);
   const gdouble *h, *hu, *hv;
   node_state(J, m, h, hu, hv);
 
-  for (int t = tid; t < 400; t += 256) {
-    int lx = t % 20, ly = t / 20;
+  for (int t = tid; t < NV; t += NT) {
+    int lx = t % VW, ly = t / VW;
     size_t k = (size_t)wrap1(r0 - 2 + ly, ny) * nx + wrap1(i0 - 2 + lx, nx);
     double hh = h[k], u, v;
     cons_to_prim(hh, hu[k], hv[k], u, v);
     sV[t] = hh;
-    sV[400 + t] = u;
-    sV[800 + t] = v;
+    sV[NV + t] = u;
+    sV[2 * NV + t] = v;
   }
   __syncthreads();
-  for (int t = tid; t < 960; t += 256) {
-    int var = t / 320, r = t % 320, ly = r / 20, lx = r % 20;
-    const double* q = sV + var * 400 + ly * 20 + lx;
+  for (int t = tid; t < 3 * NQ; t += NT) {
+    int var = t / NQ, r = t % NQ, ly = r / VW, lx = r % VW;
+    const double* q = sV + var * NV + ly * VW + lx;
     double wm, wp;
-    weno5_first(q[0], q[20], q[40], q[60], q[80], wm, wp);
-    sQM[var * 320 + r] = wm;
-    sQP[var * 320 + r] = wp;
+    weno5_first(q[0], q[VW], q[2 * VW], q[3 * VW], q[4 * VW], wm, wp);
+    sQM[var * NQ + r] = wm;
+    sQP[var * NQ + r] = wp;
   }
   __syncthreads();
   double WM[3][4], WP[3][4];
 #pragma unroll
   for (int var = 0; var < 3; ++var) {
-    const double* qm = sQM + var * 320 + ty * 20 + tx;
+    const double* qm = sQM + var * NQ + ty * VW + tx;
     weno5_second(qm[0], qm[1], qm[2], qm[3], qm[4], WM[var]);
-    const double* qp = sQP + var * 320 + ty * 20 + tx;
+    const double* qp = sQP + var * NQ + ty * VW + tx;
     weno5_second(qp[0], qp[1], qp[2], qp[3], qp[4], WP[var]);
   }
-  const double vh = sV[(ty + 2) * 20 + tx + 2];
+  const double vh = sV[(ty + 2) * VW + tx + 2];
   const int r = r0 + ty;
   if (r <= ny) positivity_limit(vh, J.wbnd, WM[0], WP[0]);
   __syncthreads();
-  double* sWM = smem;
+  double* sWM = smem;  // [12][NT]
 #pragma unroll
-  for (int a = 0; a < 12; ++a) sWM[a * 256 + tid] = WM[a >> 2][a & 3];
+  for (int a = 0; a < 12; ++a) sWM[a * NT + tid] = WM[a >> 2][a & 3];
   __syncthreads();
   const int i = i0 + tx;
-  if (ty < kFaceTile && r <= ny && i <= nx) {
+  if (ty < TH - 1 && r <= ny && i <= nx) {
     double acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
       double F1, F2, F3;
-      rusanov<0, 1, -1, 0>(WP[0][g], WP[1][g], WP[2][g], sWM[(0 + g) * 256 + tid + 16],
-                           sWM[(4 + g) * 256 + tid + 16], sWM[(8 + g) * 256 + tid + 16], J.g,
+      rusanov<0, 1, -1, 0>(WP[0][g], WP[1][g], WP[2][g], sWM[(0 + g) * NT + tid + TW],
+                           sWM[(4 + g) * NT + tid + TW], sWM[(8 + g) * NT + tid + TW], J.g,
                            F1, F2, F3);
       acc1 = acc1 + J.wbnd[g] * F1;
       acc2 = acc2 + J.wbnd[g] * F2;
@@ -1060,10 +1083,25 @@ __global__ void k_weno5(int n, const double* q, double* first, double* second) {
 // ------------------------------------------------------------------------
 static inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
-void launch_fvtd(const EngineArgs& E, const TileRef* xt, int nxt, const TileRef* yt, int nyt,
-                 const TileRef* ct, int nct, const NodeList& L, hipStream_t s) {
-  if (nxt) hipLaunchKernelGGL(k_fvtd_x, dim3(nxt, L.count), dim3(256), 0, s, E, xt, L);
-  if (nyt) hipLaunchKernelGGL(k_fvtd_y, dim3(nyt, L.count), dim3(256), 0, s, E, yt, L);
+template <int XW, int XH, int YW, int YH>
+static void launch_faces(const EngineArgs& E, const TileRef* xt, int nxt, const TileRef* yt,
+                         int nyt, const NodeList& L, hipStream_t s) {
+  if (nxt)
+    hipLaunchKernelGGL((k_fvtd_x<XW, XH>), dim3(nxt, L.count), dim3(XW * XH), 0, s, E, xt, L);
+  if (nyt)
+    hipLaunchKernelGGL((k_fvtd_y<YW, YH>), dim3(nyt, L.count), dim3(YW * YH), 0, s, E, yt, L);
+}
+
+void launch_fvtd(const EngineArgs& E, int shape, const TileRef* xt, int nxt, const TileRef* yt,
+                 int nyt, const TileRef* ct, int nct, const NodeList& L, hipStream_t s) {
+  static_assert(kShapes[0].xw == 16 && kShapes[0].xh == 16 && kShapes[0].yw == 16 &&
+                    kShapes[0].yh == 16 && kShapes[1].xw == 32 && kShapes[1].xh == 16 &&
+                    kShapes[1].yw == 16 && kShapes[1].yh == 32,
+                "the instantiations below follow kShapes");
+  if (shape == 1)
+    launch_faces<32, 16, 16, 32>(E, xt, nxt, yt, nyt, L, s);
+  else
+    launch_faces<16, 16, 16, 16>(E, xt, nxt, yt, nyt, L, s);
   if (nct)
     hipLaunchKernelGGL(k_fvtd_update, dim3(cdiv(nct, 8) * 8 * L.count), dim3(256), 0, s, E, ct,
                        nct, L);

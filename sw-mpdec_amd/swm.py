@@ -83,6 +83,8 @@ def lib():
     L.swm_track_stage_min.argtypes = [vp, ctypes.c_int]
     L.swm_advance.argtypes = [vp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _ip, _dp]
     L.swm_set_execution.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.swm_set_tile_shape.argtypes = [vp, ctypes.c_int]
+    L.swm_tile_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     L.swm_execution_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
     L.swm_stage_min_h.argtypes = [vp, _dp]
     L.swm_debug_weno5.argtypes = [ctypes.c_int, _dp, _dp, _dp]
@@ -229,6 +231,16 @@ class Solver:
         self._check(lib().swm_execution_stats(self._h, out), "swm_execution_stats")
         return dict(zip(EXEC_STATS, list(out)))
 
+    def set_tile_shape(self, shape=-1):
+        """Face-kernel tile shape: 0 (16x16), 1 (32x16 / 16x32), -1 autotuned."""
+        self._check(lib().swm_set_tile_shape(self._h, int(shape)), "swm_set_tile_shape")
+
+    def tile_shape(self):
+        """(shape in use, (x-kernel w, h, y-kernel w, h))."""
+        d = (ctypes.c_int * 4)()
+        sh = self._check(lib().swm_tile_shape(self._h, d), "swm_tile_shape")
+        return sh, tuple(d)
+
     def track_stage_min(self, on=True):
         """Record the minimum depth over every DeC stage solution from now on."""
         self._check(lib().swm_track_stage_min(self._h, int(bool(on))), "swm_track_stage_min")
@@ -346,6 +358,8 @@ def _engine_lib():
     L.swm_engine_profile_read.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
     L.swm_engine_track_stage_min.argtypes = [vp, ctypes.c_int]
     L.swm_engine_set_execution.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.swm_engine_set_tile_shape.argtypes = [vp, ctypes.c_int]
+    L.swm_engine_tile_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     L.swm_engine_execution_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
     L.swm_engine_stage_min_h.argtypes = [vp, ctypes.c_int, _dp]
     L._engine_bound = True
@@ -469,6 +483,15 @@ class Engine:
         self._check(_engine_lib().swm_engine_execution_stats(self._e, out),
                     "swm_engine_execution_stats")
         return dict(zip(EXEC_STATS, list(out)))
+
+    def set_tile_shape(self, shape=-1):
+        self._check(_engine_lib().swm_engine_set_tile_shape(self._e, int(shape)),
+                    "swm_engine_set_tile_shape")
+
+    def tile_shape(self):
+        d = (ctypes.c_int * 4)()
+        sh = self._check(_engine_lib().swm_engine_tile_shape(self._e, d), "swm_engine_tile_shape")
+        return sh, tuple(d)
 
     def track_stage_min(self, on=True):
         self._check(_engine_lib().swm_engine_track_stage_min(self._e, int(bool(on))),

@@ -1,0 +1,80 @@
+"""The face kernels' tile shapes (swm_set_tile_shape): every shape computes
+each face with the same arithmetic, so results are bitwise the same whatever
+the shape -- checked against the oracle for each shape on square and
+rectangular grids down to 3 cells, under several schemes -- and the
+autotuner settles on one of them."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import problems as P
+import swm
+from oracle_api import Oracle
+from parity_util import describe_diff, same_bits
+
+pytestmark = pytest.mark.gpu
+
+
+def random_state(n, ny, seed, amp=0.3):
+    rng = np.random.default_rng(seed)
+    U = np.empty((ny, n, 3))
+    U[..., 0] = 1.0 + amp * rng.uniform(-1, 1, (ny, n))
+    U[..., 1] = U[..., 0] * rng.normal(0, 0.3, (ny, n))
+    U[..., 2] = U[..., 0] * rng.normal(0, 0.3, (ny, n))
+    return P.Problem(n, U, kind="random", ny=ny)
+
+
+CASES = {
+    "vortex40x33": lambda: P.vortex(40, 2, ny=33),
+    "lake_bumps70": lambda: P.lake_bumps(70, 3),
+    "random_3x41": lambda: random_state(3, 41, 4),
+    "random_37x5": lambda: random_state(37, 5, 5),
+    "flag1wb31x62": lambda: P.lake_flag1(31, True, ny=62),
+}
+
+
+@pytest.mark.parametrize("shape", [0, 1])
+@pytest.mark.parametrize("scheme", [5, 2, 7])
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_shapes_match_oracle(case, scheme, shape):
+    p = CASES[case]()
+    s = swm.Solver.from_problem(p)
+    s.set_tile_shape(shape)
+    s.set_timescheme(scheme)
+    s.set_hp_association(swm.HP_BY_NODE)
+    assert s.tile_shape()[0] == shape
+    dts, _, _ = s.run_steps(3)
+    o = Oracle(p, scheme, hp_by_node=True)
+    for k in range(3):
+        dt, _, _ = o.timestep()
+        o.step(dt)
+        assert dt == dts[k]
+    U, U_r = s.download(), o.get_U()
+    assert same_bits(U, U_r), f"{case} scheme {scheme} shape {shape}: {describe_diff(U, U_r)}"
+
+
+def test_fvtd_outputs_are_shape_independent():
+    p = P.lake_bumps(45, 8)
+    outs = []
+    for shape in (0, 1):
+        s = swm.Solver.from_problem(p)
+        s.set_tile_shape(shape)
+        outs.append(s.fvtd())
+    for k in outs[0]:
+        assert same_bits(outs[0][k], outs[1][k]), k
+
+
+def test_autotuner_settles_on_a_shape():
+    p = P.vortex(96, 7)
+    s = swm.Solver.from_problem(p)
+    s.run_steps(8)  # one polling step, then the tuning candidates
+    shape, dims = s.tile_shape()
+    assert shape in (0, 1) and len(dims) == 4
+    U = s.download()
+    o = Oracle(p)
+    o.run(8)
+    assert same_bits(U, o.get_U())
+    eng = swm.Engine()
+    with pytest.raises(swm.SwmError):
+        eng.set_tile_shape(2)
