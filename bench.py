@@ -292,8 +292,9 @@ def main():
                     help="face-kernel tile shape: 0 16x16, 1 32x16/16x32, -1 autotuned")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
-              "batched and served solves shard across GPUs without communication, see DESIGN.md",
+        print(f"bench.py: this benchmark measures one GPU only (asked for --gpus {args.gpus}); "
+              "batched and served solves would shard across GPUs by job, one engine per GPU, "
+              "with no communication",
               file=sys.stderr)
         sys.exit(2)
 
