@@ -78,3 +78,24 @@ def test_autotuner_settles_on_a_shape():
     eng = swm.Engine()
     with pytest.raises(swm.SwmError):
         eng.set_tile_shape(2)
+
+
+def test_engine_with_wide_tiles_matches_oracle():
+    """An engine forced to the wide shape, graph replay on, with tiny,
+    rectangular and larger grids under several schemes."""
+    jobs = [(random_state(3, 29, 9), 3, 5), (random_state(29, 3, 10), 2, 1),
+            (P.vortex(47, 3), 3, 5), (P.lake_bumps(33, 2), 2, 7), (P.vortex(20, 1), 4, 3),
+            (P.lake_flag1(18, True), 3, 4)]
+    eng = swm.Engine()
+    eng.set_tile_shape(1)
+    eng.set_execution(1, 1)
+    ids = [eng.submit(p, k, timescheme=sc, hp_association=swm.HP_BY_NODE) for p, k, sc in jobs]
+    while eng.step():
+        pass
+    assert eng.tile_shape()[0] == 1
+    for (p, k, sc), jid in zip(jobs, ids):
+        o = Oracle(p, sc, hp_by_node=True)
+        for _ in range(k):
+            dt, _, _ = o.timestep()
+            o.step(dt)
+        assert same_bits(eng.fetch(jid), o.get_U()), (p.kind, p.n, p.rows, sc)
