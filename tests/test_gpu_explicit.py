@@ -142,3 +142,26 @@ def test_large_grid_conserves_mass_and_keeps_lake(scheme):
         assert np.abs(U - lake.U0).max() < 1e-7
     else:
         assert same_bits(U, lake.U0)
+
+
+@pytest.mark.parametrize("scheme", [1, 2, 3, 4, 6, 7])
+def test_dry_dam_break_nan_pattern(scheme):
+    """A dam break over a dry bed (h = 0 outside the column): ConsToPrim
+    divides by zero depths, the states turn NaN after one step and the next
+    dt is NaN, Inf or absurd.  The dt sequence and the NaN pattern of every
+    step agree with the oracle (NaNs compared by position)."""
+    p = P.dam_break(20)
+    s = swm.Solver.from_problem(p)
+    s.set_timescheme(scheme)
+    s.set_hp_association(swm.HP_BY_NODE)
+    o = Oracle(p, scheme, hp_by_node=True)
+    for k in range(3):
+        dt, lx, ly = s.timestep()
+        ref = o.timestep()
+        assert same_bits([dt, lx, ly], list(ref)), (scheme, k)
+        if not np.isfinite(dt):
+            break
+        s.step(dt)
+        o.step(dt)
+        U, U_r = s.download(), o.get_U()
+        assert same_bits(U, U_r), f"scheme {scheme} step {k}: {describe_diff(U, U_r)}"

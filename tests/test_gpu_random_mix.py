@@ -43,7 +43,7 @@ def make_job(rng):
     return p, scheme, by_node, steps
 
 
-@pytest.mark.parametrize("seed", [11, 12])
+@pytest.mark.parametrize("seed", [11, 12, 13, 14])
 def test_random_mix_matches_oracle(seed):
     rng = np.random.default_rng(seed)
     jobs = [make_job(rng) for _ in range(24)]
