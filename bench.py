@@ -110,7 +110,7 @@ def cpu_baseline(budget_s: float = 20.0, scheme: int = 5):
     p = P.lake_flag1(n, True)
 
     def run(steps):
-        hdr = np.array([5, n, steps, 1, 1, 0, scheme], dtype="<i4")
+        hdr = np.array([5, n, steps, 1, 1, 0, scheme, 0], dtype="<i4")
         fl = np.array([p.x0, p.x1, p.gravity, p.cfl], dtype="<f8")
         with tempfile.TemporaryDirectory() as td:
             fin, fout = Path(td) / "in.bin", Path(td) / "out.bin"

@@ -136,6 +136,24 @@ int swm_set_timescheme(swm_solver_t *h, int timescheme);
 #define SWM_HP_BY_NODE 1
 int swm_set_hp_association(swm_solver_t *h, int mode);
 
+/* ReconstructionFix (src/finitevolume_vars.f90:94), which the reference sets
+ * equal to Reconstruction = 4 (src/parameters.f90:142).  When it differs,
+ * every FVTD runs the Jameson shock indicator on the depths
+ * (src/shocksindicator.f90:29-129) and the flagged cells take
+ * ReconstructionFix's reconstruction instead of WENO5
+ * (src/reconstruction.f90:190-342; FVTimeDerivative, src/finitevolume.f90:425-433):
+ *   4 (default): WENO5 everywhere; the indicator cannot change a result;
+ *   1: first order, the cell's primitives at every Gauss point;
+ *   2: MUSCL with the minmod limiter.
+ * The indicator writes each cell's verdict over Ind(:, ii-1:ii+1, jj) in loop
+ * order, so the flag of cell ii is the verdict of cell min(ii+1, n): the
+ * library reproduces that.  3 (WENO3) is rejected with SWM_E_ARG:
+ * WENO3_SecondSweep defines W(1:2) only, so with 4 Gauss points the
+ * reference reads undefined values.  The well-balanced reference state is an
+ * FVTD under this setting: changing it on a handle discards that state, and
+ * swm_init_wb must be called again before stepping. */
+int swm_set_reconstruction_fix(swm_solver_t *h, int recfix);
+
 /* Several steps without returning to the caller: before each step dt is the
  * TimeStep value (a step-count stop instead of the tEnd stop of
  * src/timediscretization.f90:57-95).  dts (may be NULL) receives nsteps dt
@@ -222,6 +240,7 @@ typedef struct {
   int nsteps;                /* CFL steps to run; 0: none (DONE at once); < 0: until released */
   int timescheme;            /* 0 or 5: mPDeC5; 1..4, 6, 7 as swm_set_timescheme */
   int hp_association;        /* SWM_HP_AS_REFERENCE or SWM_HP_BY_NODE (swm_set_hp_association) */
+  int reconstruction_fix;    /* 0 or 4: WENO5 everywhere; 1, 2 as swm_set_reconstruction_fix */
 } swm_job_desc;
 
 #define SWM_JOB_RUNNING 0

@@ -23,7 +23,8 @@
 ! Usage: swref <input.bin> <output.bin>
 ! Input (stream, little endian):
 !   int32 mode, nx, nsteps, bathy_flag, has_wb, ny (0: ny = nx),
-!         timescheme (0: 5, mPDeC5; src/parameters.f90:144 numbering)
+!         timescheme (0: 5, mPDeC5; src/parameters.f90:144 numbering),
+!         reconstruction_fix (0: 4 = Reconstruction, as src/parameters.f90:142)
 !   real64 x0, x1, gravity, cfl    (the domain is [x0,x1]^2; dy = dx*nx/ny)
 !   real64 U(3,nx,ny)              [mode 1-3]   or  Mass(5nx*ny), Diag(nx*ny), rhs(nx*ny) [mode 4]
 !   real64 Uwb(3,nx,ny)            [if has_wb]
@@ -39,7 +40,7 @@ USE MOD_TimeDiscretization, ONLY: TimeDiscretization
 USE MOD_JacobiIteration,    ONLY: jacobi
 #endif
 IMPLICIT NONE
-INTEGER            :: mode, nx, ny, nsteps, bflag, haswb, k, nrow, scheme
+INTEGER            :: mode, nx, ny, nsteps, bflag, haswb, k, nrow, scheme, recfix
 INTEGER(KIND=8)    :: c0, c1, crate
 REAL               :: x0, x1, grav, cflin, dtmin
 REAL, ALLOCATABLE  :: Uic(:,:,:), Uwb(:,:,:), Mass(:), Diag(:), rhs(:), sol(:)
@@ -49,9 +50,10 @@ CALL get_command_argument(1, fin)
 CALL get_command_argument(2, fout)
 
 OPEN(10, FILE=TRIM(fin), ACCESS='stream', FORM='unformatted', STATUS='old')
-READ(10) mode, nx, nsteps, bflag, haswb, ny, scheme
+READ(10) mode, nx, nsteps, bflag, haswb, ny, scheme, recfix
 IF (ny .LE. 0) ny = nx
 IF (scheme .LE. 0) scheme = 5
+IF (recfix .LE. 0) recfix = 4
 READ(10) x0, x1, grav, cflin
 
 #ifdef WELLBALANCED
@@ -76,7 +78,7 @@ CFL     = cflin
 BoundaryConditionsType = (/1,1,1,1/)
 BathymetryFlag    = bflag
 Reconstruction    = 4
-ReconstructionFix = Reconstruction
+ReconstructionFix = recfix   ! src/parameters.f90:142 sets it to Reconstruction
 timescheme        = scheme
 WhichOutput      = 0   ! no snapshot files
 nOutputFiles      = 1

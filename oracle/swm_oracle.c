@@ -31,11 +31,14 @@
  *    and the first-step momenta, which do not read the depths and equal the
  *    reference's.
  *
- * Not restated: the Jameson shock indicator (src/shocksindicator.f90).  With
- * ReconstructionFix == Reconstruction (src/parameters.f90:289-290) both
+ * The Jameson shock indicator (src/shocksindicator.f90) and
+ * ReconstructionFixX/Y (src/reconstruction.f90:190-342) run only when
+ * ReconstructionFix differs from Reconstruction = 4 (orc_set_reconstruction_fix;
+ * the reference sets them equal, src/parameters.f90:142).  With them equal,
  * ReconstructionX and ReconstructionFixX call WENO_XDIR(...,4) on disjoint
  * cell sets that cover every cell (src/reconstruction.f90:96-108,243-255), so
- * the indicator cannot change any result.
+ * the indicator cannot change any result and is skipped.  ReconstructionFix 1
+ * and 2 are pinned against the reference built with them.
  */
 #include "swm_oracle.h"
 
@@ -61,6 +64,8 @@ struct orc_state {
   int max_iter;
   double tol;
   int hp_by_node;  /* 1: hp(m,i,j) = Up(m,1,i,j) instead of the sequence association */
+  int recfix;      /* ReconstructionFix: 4 (= Reconstruction), 1 or 2 */
+  unsigned char *ind; /* Ind(1:2, 0:nx+1, 0:ny+1) of the shock indicator */
   /* U, V: (3, -2:nx+3, -2:ny+3) */
   double *U, *V;
   /* WM, WP: (3, 4, 0:nx+1, 0:ny+1) */
@@ -321,17 +326,106 @@ static void weno_ydir(const orc_t *o, int ii, int jj, double *wm, double *wp) {
   }
 }
 
-/* ReconstructionX (+ReconstructionFixX), src/reconstruction.f90:96-108 */
-static void reconstruction_x(orc_t *o) {
-  for (int jj = 1; jj <= o->ny; ++jj)
-    for (int ii = 0; ii <= o->nx + 1; ++ii)
-      weno_xdir(o, ii, jj, &o->WM[iW(o, 0, 0, ii, jj)], &o->WP[iW(o, 0, 0, ii, jj)]);
+/* ---- shocksindicator.f90 ------------------------------------------------- */
+static inline size_t iInd(const orc_t *o, int d, int i, int j) {
+  /* Ind(1:2, 0:nx+1, 0:ny+1), d 0-based */
+  return (size_t)d + 2 * ((size_t)i + (size_t)(o->nx + 2) * (size_t)j);
 }
-/* ReconstructionY (+ReconstructionFixY), src/reconstruction.f90:164-176 */
-static void reconstruction_y(orc_t *o) {
+
+/* JamesonIndicator on the depths of three cells, src/shocksindicator.f90:93-129 */
+static int jameson_indicator(double hm, double h0, double hp) {
+  double ind = fabs(hm + hp - 2.0 * h0);
+  ind = ind / (fabs(hm) + fabs(hp) + fabs(2.0 * h0));
+  return ind > 0.5E-03;
+}
+
+/* ShocksIndicatorX, src/shocksindicator.f90:29-57.  Each cell's verdict
+ * overwrites Ind(1, ii-1:ii+1, jj) in loop order, so a later cell's verdict
+ * replaces an earlier one. */
+static void shocks_indicator_x(orc_t *o) {
   for (int jj = 0; jj <= o->ny + 1; ++jj)
-    for (int ii = 1; ii <= o->nx; ++ii)
-      weno_ydir(o, ii, jj, &o->WM[iW(o, 0, 0, ii, jj)], &o->WP[iW(o, 0, 0, ii, jj)]);
+    for (int ii = 0; ii <= o->nx + 1; ++ii) o->ind[iInd(o, 0, ii, jj)] = 0;
+  for (int jj = 1; jj <= o->ny; ++jj)
+    for (int ii = 1; ii <= o->nx; ++ii) {
+      int shock = jameson_indicator(o->V[iU(o, 0, ii - 1, jj)], o->V[iU(o, 0, ii, jj)],
+                                    o->V[iU(o, 0, ii + 1, jj)]);
+      for (int c = ii - 1; c <= ii + 1; ++c) o->ind[iInd(o, 0, c, jj)] = (unsigned char)shock;
+    }
+}
+/* ShocksIndicatorY, src/shocksindicator.f90:63-91 */
+static void shocks_indicator_y(orc_t *o) {
+  for (int jj = 0; jj <= o->ny + 1; ++jj)
+    for (int ii = 0; ii <= o->nx + 1; ++ii) o->ind[iInd(o, 1, ii, jj)] = 0;
+  for (int jj = 1; jj <= o->ny; ++jj)
+    for (int ii = 1; ii <= o->nx; ++ii) {
+      int shock = jameson_indicator(o->V[iU(o, 0, ii, jj - 1)], o->V[iU(o, 0, ii, jj)],
+                                    o->V[iU(o, 0, ii, jj + 1)]);
+      for (int r = jj - 1; r <= jj + 1; ++r) o->ind[iInd(o, 1, ii, r)] = (unsigned char)shock;
+    }
+}
+
+/* MINMOD, src/reconstruction.f90:387-402; SIGN(1.0, x) is copysign(1.0, x) */
+static inline double minmod(double x, double y) {
+  return 0.5 * (copysign(1.0, x) + copysign(1.0, y)) * fminr(fabs(x), fabs(y));
+}
+
+/* The ReconstructionFix values of one flagged cell from its stencil
+ * q[v][s] = V(v, -2+s) along the sweep direction (ReconstructionFixX/Y,
+ * src/reconstruction.f90:219-255): 1 the cell's primitives at every Gauss
+ * point; 2 MUSCL (src/reconstruction.f90:348-381). */
+static void reconstruction_fix_cell(const orc_t *o, const double q[NVAR][5], double d,
+                                    double *wm, double *wp) {
+  for (int g = 0; g < NGP; ++g)
+    for (int v = 0; v < NVAR; ++v) {
+      if (o->recfix == 1) {
+        wm[iW(o, v, g, 0, 0)] = q[v][2];
+        wp[iW(o, v, g, 0, 0)] = q[v][2];
+      } else {
+        double sp = (q[v][3] - q[v][2]) / d;
+        double sm = (q[v][2] - q[v][1]) / d;
+        double slope = minmod(sm, sp);
+        wm[iW(o, v, g, 0, 0)] = q[v][2] - 0.5 * slope * d;
+        wp[iW(o, v, g, 0, 0)] = q[v][2] + 0.5 * slope * d;
+      }
+    }
+}
+
+/* ShocksIndicatorX, ReconstructionX and ReconstructionFixX,
+ * src/reconstruction.f90:96-108,219-255: WENO5 for the cells the indicator
+ * leaves, ReconstructionFix for the flagged ones. */
+static void reconstruction_x(orc_t *o) {
+  const int fix = o->recfix != 4;
+  if (fix) shocks_indicator_x(o);
+  for (int jj = 1; jj <= o->ny; ++jj)
+    for (int ii = 0; ii <= o->nx + 1; ++ii) {
+      double *wm = &o->WM[iW(o, 0, 0, ii, jj)], *wp = &o->WP[iW(o, 0, 0, ii, jj)];
+      if (fix && o->ind[iInd(o, 0, ii, jj)]) {
+        double q[NVAR][5];
+        for (int v = 0; v < NVAR; ++v)
+          for (int s = -2; s <= 2; ++s) q[v][s + 2] = o->V[iU(o, v, ii + s, jj)];
+        reconstruction_fix_cell(o, q, o->dx, wm, wp);
+      } else {
+        weno_xdir(o, ii, jj, wm, wp);
+      }
+    }
+}
+/* ShocksIndicatorY, ReconstructionY and ReconstructionFixY,
+ * src/reconstruction.f90:164-176,298-334 */
+static void reconstruction_y(orc_t *o) {
+  const int fix = o->recfix != 4;
+  if (fix) shocks_indicator_y(o);
+  for (int jj = 0; jj <= o->ny + 1; ++jj)
+    for (int ii = 1; ii <= o->nx; ++ii) {
+      double *wm = &o->WM[iW(o, 0, 0, ii, jj)], *wp = &o->WP[iW(o, 0, 0, ii, jj)];
+      if (fix && o->ind[iInd(o, 1, ii, jj)]) {
+        double q[NVAR][5];
+        for (int v = 0; v < NVAR; ++v)
+          for (int s = -2; s <= 2; ++s) q[v][s + 2] = o->V[iU(o, v, ii, jj + s)];
+        reconstruction_fix_cell(o, q, o->dy, wm, wp);
+      } else {
+        weno_ydir(o, ii, jj, wm, wp);
+      }
+    }
 }
 
 /* ---- finitevolume.f90 ----------------------------------------------------- */
@@ -521,6 +615,7 @@ orc_t *orc_create(int nx, int ny, double dx, double dy, double gravity, double c
   o->nx = nx; o->ny = ny; o->dx = dx; o->dy = dy; o->g = gravity; o->cfl = cfl;
   o->wb = wellbalanced ? 1 : 0;
   o->max_iter = 1000; o->tol = 1.e-16;
+  o->recfix = 4;
   for (int i = 0; i < 4; ++i) o->wbnd[i] = wbnd[i];
   for (int i = 0; i < 4; ++i)
     for (int j = 0; j < 4; ++j) o->wgp[i][j] = wbnd[i] * wbnd[j]; /* src/mesh.f90:136 */
@@ -528,6 +623,7 @@ orc_t *orc_create(int nx, int ny, double dx, double dy, double gravity, double c
   size_t nc = (size_t)nx * ny;
   o->U = calloc(pad, sizeof(double));
   o->V = calloc(pad, sizeof(double));
+  o->ind = calloc((size_t)2 * (nx + 2) * (ny + 2), 1);
   o->WM = calloc((size_t)NVAR * NGP * (nx + 2) * (ny + 2), sizeof(double));
   o->WP = calloc((size_t)NVAR * NGP * (nx + 2) * (ny + 2), sizeof(double));
   o->FX = calloc((size_t)NVAR * (nx + 1) * ny, sizeof(double));
@@ -566,6 +662,7 @@ void orc_destroy(orc_t *o) {
                  o->FUp, o->ProdUp, o->DestUp, o->Mass, o->Diag, o->rhs, o->haVec, o->solp,
                  o->prod, o->K[0], o->K[1], o->K[2], o->K[3], o->K[4], o->K[5], o->UN0};
   for (size_t k = 0; k < sizeof(p) / sizeof(p[0]); ++k) free(p[k]);
+  free(o->ind);
   free(o);
 }
 
@@ -859,6 +956,12 @@ int orc_step_mpdec5(orc_t *o, double dt, int *jacobi_iters) {
 }
 
 void orc_set_hp_by_node(orc_t *o, int on) { o->hp_by_node = on ? 1 : 0; }
+
+int orc_set_reconstruction_fix(orc_t *o, int recfix) {
+  if (recfix != 1 && recfix != 2 && recfix != 4) return -1;
+  o->recfix = recfix;
+  return 0;
+}
 
 /* ---- explicit schemes (the build without PATANKAR) ------------------------ */
 

@@ -72,6 +72,12 @@ int orc_step_explicit(orc_t *o, int timescheme, double dt);
  * No reference output exists for on = 1; it is pinned by invariants only. */
 void orc_set_hp_by_node(orc_t *o, int on);
 
+/* ReconstructionFix (src/finitevolume_vars.f90:94): 4 (the reference's
+ * setting, = Reconstruction, src/parameters.f90:142), or 1 (first order) or
+ * 2 (MUSCL) for the cells the Jameson indicator flags.  Returns 0, or -1 for
+ * another value. */
+int orc_set_reconstruction_fix(orc_t *o, int recfix);
+
 /* Jacobi on the reference CSR pattern (src/jacobi_iteration.f90:52-87).
  * Mass (5n), diag (n), rhs (n) -> sol (n).  Returns the sweep count. */
 int orc_jacobi(const orc_t *o, const double *Mass, const double *diag,

@@ -1051,6 +1051,14 @@ int alloc_job(swm_engine* e, int nx, int ny, double dx, double dy, double g, dou
   return SWM_OK;
 }
 
+// JobDev::recfix for a ReconstructionFix value: 4 (= Reconstruction, the
+// reference's setting) -> 0, 1 and 2 -> themselves, anything else -> -1.
+int recfix_code(int recfix) {
+  if (recfix == 4) return 0;
+  if (recfix == 1 || recfix == 2) return recfix;
+  return -1;
+}
+
 int job_set_bathy(swm_engine* e, int id, const double* bx, const double* by) {
   Job& J = *e->jobs[id];
   if (!bx != !by) FAIL(e, SWM_E_ARG, "dbdx and dbdy must both be set or both NULL");
@@ -1088,6 +1096,9 @@ int engine_add(swm_engine* e, const swm_job_desc* d, int* id_out) {
   if (!sc) FAIL(e, SWM_E_ARG, "timescheme %d is not one of 0..7", d->timescheme);
   if (d->hp_association != SWM_HP_AS_REFERENCE && d->hp_association != SWM_HP_BY_NODE)
     FAIL(e, SWM_E_ARG, "hp_association must be SWM_HP_AS_REFERENCE or SWM_HP_BY_NODE");
+  const int fix = recfix_code(d->reconstruction_fix == 0 ? 4 : d->reconstruction_fix);
+  if (fix < 0)
+    FAIL(e, SWM_E_ARG, "reconstruction_fix %d is not 0, 4, 1 or 2", d->reconstruction_fix);
   const long long extra = d->dbdx ? (long long)bathy_bytes((size_t)d->nx * d->ny) : 0;
   int id;
   int rc = alloc_job(e, d->nx, d->ny, d->dx, d->dy, d->gravity, d->cfl, d->wellbalanced ? 1 : 0,
@@ -1097,6 +1108,7 @@ int engine_add(swm_engine* e, const swm_job_desc* d, int* id_out) {
   Job& J = *e->jobs[id];
   J.scheme = sc;
   J.hp_assoc = d->hp_association;
+  J.dev.recfix = fix;
   if (d->dbdx) {
     rc = job_set_bathy(e, id, d->dbdx, d->dbdy);
     if (rc) return rc;
@@ -1543,6 +1555,23 @@ int swm_set_timescheme(swm_solver_t* s, int timescheme) {
   const SchemeDef* sc = find_scheme(timescheme);
   if (!sc) FAIL(s->e, SWM_E_ARG, "timescheme %d is not one of 1..7", timescheme);
   plain_job(s)->scheme = sc;
+  return SWM_OK;
+}
+
+int swm_set_reconstruction_fix(swm_solver_t* s, int recfix) {
+  if (!s) return SWM_E_ARG;
+  swm_engine* e = s->e;
+  const int code = recfix_code(recfix);
+  if (code < 0)
+    FAIL(e, SWM_E_ARG,
+         "reconstruction fix %d is not 4, 1 or 2 (3, WENO3, defines 2 of the 4 Gauss values)",
+         recfix);
+  Job& J = *plain_job(s);
+  if (J.dev.recfix != code) {
+    J.dev.recfix = code;
+    J.wb_ready = false;  // the well-balanced reference is an FVTD with this setting
+    e->dirty = true;
+  }
   return SWM_OK;
 }
 

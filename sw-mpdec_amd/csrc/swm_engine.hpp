@@ -72,6 +72,7 @@ struct JobDev {
   // geometry and physics
   int nx, ny, n;
   int wb, bathy;
+  int recfix;     // ReconstructionFix on flagged cells: 0 none (= WENO5), 1 first order, 2 MUSCL
   int nblk;       // cell blocks (kCellBlock cells) of this job
   double dx, dy, g, cfl;
   double wbnd[4];

@@ -153,8 +153,24 @@ __global__ __launch_bounds__(TW * TH) void k_fvtd_x(EngineArgs E, const TileRef*
     const double* qp = sQP + var * NQ + ty * TW + tx;
     weno5_second(qp[0], qp[TW], qp[2 * TW], qp[3 * TW], qp[4 * TW], WP[var]);
   }
-  const double vh = sV[(ty + 2) * VW + tx + 2];
   const int c = c0 + tx;
+  if (J.recfix) {
+    // ReconstructionFix 1 or 2 on the cells the shock indicator flags.
+    // ShocksIndicatorX writes each cell's verdict to Ind(1, ii-1:ii+1, jj) in
+    // loop order (src/shocksindicator.f90:50-55), so the last writer wins:
+    // Ind(1, c, j) = Shock(min(c + 1, nx), j), from the depths at columns
+    // min(c + 1, nx) - 1 .. + 1, all inside the halo tile.
+    const int lc = min(c + 1, nx) - c0 + 2;
+    const double* hrow = sV + (ty + 2) * VW;
+    if (jameson_shock(hrow[lc - 1], hrow[lc], hrow[lc + 1])) {
+#pragma unroll
+      for (int var = 0; var < 3; ++var) {
+        const double* q = sV + var * NV + (ty + 2) * VW + tx + 2;
+        reconstruction_fix(J.recfix, q[-1], q[0], q[1], J.dx, WM[var], WP[var]);
+      }
+    }
+  }
+  const double vh = sV[(ty + 2) * VW + tx + 2];
   if (c <= nx) positivity_limit(vh, J.wbnd, WM[0], WP[0]);
   __syncthreads();
   double* sWM = smem;  // [12][NT]
@@ -233,8 +249,21 @@ __global__ __launch_bounds__(TW * TH) void k_fvtd_y(EngineArgs E, const TileRef*
     const double* qp = sQP + var * NQ + ty * VW + tx;
     weno5_second(qp[0], qp[1], qp[2], qp[3], qp[4], WP[var]);
   }
-  const double vh = sV[(ty + 2) * VW + tx + 2];
   const int r = r0 + ty;
+  if (J.recfix) {
+    // Ind(2, i, r) = Shock(i, min(r + 1, ny)) (src/shocksindicator.f90:84-89),
+    // as in k_fvtd_x
+    const int lr = min(r + 1, ny) - r0 + 2;
+    const double* hcol = sV + tx + 2;
+    if (jameson_shock(hcol[(lr - 1) * VW], hcol[lr * VW], hcol[(lr + 1) * VW])) {
+#pragma unroll
+      for (int var = 0; var < 3; ++var) {
+        const double* q = sV + var * NV + (ty + 2) * VW + tx + 2;
+        reconstruction_fix(J.recfix, q[-VW], q[0], q[VW], J.dy, WM[var], WP[var]);
+      }
+    }
+  }
+  const double vh = sV[(ty + 2) * VW + tx + 2];
   if (r <= ny) positivity_limit(vh, J.wbnd, WM[0], WP[0]);
   __syncthreads();
   double* sWM = smem;  // [12][NT]

@@ -314,6 +314,40 @@ __device__ __forceinline__ void rusanov(double hl, double ul, double vl, double 
   F3 = n2 * f2 + t2 * f3;
 }
 
+// JamesonIndicator, src/shocksindicator.f90:93-129, on the depths of three
+// neighbouring cells.
+__device__ __forceinline__ bool jameson_shock(double hm, double h0, double hp) {
+  double ind = fabs(hm + hp - 2.0 * h0);
+  ind = ind / (fabs(hm) + fabs(hp) + fabs(2.0 * h0));
+  return ind > 0.5E-03;
+}
+
+// MINMOD, src/reconstruction.f90:387-402; SIGN(1.0, x) is copysign(1.0, x).
+__device__ __forceinline__ double minmod(double x, double y) {
+  return 0.5 * (copysign(1.0, x) + copysign(1.0, y)) * min_r(fabs(x), fabs(y));
+}
+
+// ReconstructionFixX/Y (src/reconstruction.f90:219-242, :298-321) for one
+// variable of a flagged cell, from the cell's value q0 and its neighbours
+// along the sweep direction: recfix 1 puts q0 at every Gauss point, 2 the
+// MUSCL values (src/reconstruction.f90:348-381) with d = MESH_DX(dir).
+__device__ __forceinline__ void reconstruction_fix(int recfix, double qm1, double q0, double qp1,
+                                                   double d, double wm[4], double wp[4]) {
+  double m = q0, p = q0;
+  if (recfix == 2) {
+    double sp = (qp1 - q0) / d;
+    double sm = (q0 - qm1) / d;
+    double slope = minmod(sm, sp);
+    m = q0 - 0.5 * slope * d;
+    p = q0 + 0.5 * slope * d;
+  }
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    wm[g] = m;
+    wp[g] = p;
+  }
+}
+
 // PATANKAR_DIV, src/timediscretization.f90:913-924
 __device__ __forceinline__ double patankar_div(double prod, double h) {
   if (h < kMinDepth) return 0.;

@@ -17,6 +17,7 @@ INTEGER(C_INT), PARAMETER, PUBLIC :: SWM_OK = 0
 PUBLIC :: swm_create, swm_destroy, swm_set_bathy_grad, swm_init_wb
 PUBLIC :: swm_upload, swm_download, swm_timestep, swm_step_mpdec5, swm_advance
 PUBLIC :: swm_step_mpeuler, swm_step_mpdec2, swm_set_timescheme, swm_set_hp_association
+PUBLIC :: swm_set_reconstruction_fix
 PUBLIC :: swm_step_forward_euler, swm_step_ssprk4, swm_step_rk65, swm_step_dec5
 PUBLIC :: swm_error_message, swm_check
 
@@ -131,6 +132,15 @@ INTERFACE
     IMPORT :: C_INT, C_PTR
     TYPE(C_PTR), VALUE :: h
     INTEGER(C_INT), VALUE :: mode
+  END FUNCTION
+
+  ! ReconstructionFix (src/parameters.f90:142): 4 as the reference, 1 first
+  ! order or 2 MUSCL on the cells the shock indicator flags; before swm_init_wb
+  INTEGER(C_INT) FUNCTION swm_set_reconstruction_fix(h, recfix) &
+      BIND(C, NAME='swm_set_reconstruction_fix')
+    IMPORT :: C_INT, C_PTR
+    TYPE(C_PTR), VALUE :: h
+    INTEGER(C_INT), VALUE :: recfix
   END FUNCTION
 
   ! the time loop of TimeDiscretization to the next output time

@@ -10,7 +10,8 @@
 !
 ! Reads a raw input file
 !   int32  nx, ny, nsteps, wellbalanced, has_bathy, loop, timescheme (0: 5),
-!          hp_association (swm_set_hp_association)
+!          hp_association (swm_set_hp_association),
+!          reconstruction_fix (0: 4, the reference's; swm_set_reconstruction_fix)
 !   real64 dx, dy, gravity, cfl, wgp_bnd(4)
 !   real64 U(3,nx,ny) [, Uwb(3,nx,ny) if wellbalanced] [, dbdx, dbdy(nx,ny,4,4)]
 ! and writes the output file
@@ -22,8 +23,8 @@ PROGRAM swm_fdriver
 USE ISO_C_BINDING
 USE swm_bridge
 IMPLICIT NONE
-INTEGER(C_INT) :: hdr(8), nx, ny, nsteps, wb, hasb, rc, k, nonconv, loop, ndone, nout, nrec
-INTEGER(C_INT) :: timescheme, hpa
+INTEGER(C_INT) :: hdr(9), nx, ny, nsteps, wb, hasb, rc, k, nonconv, loop, ndone, nout, nrec
+INTEGER(C_INT) :: timescheme, hpa, recfix
 INTEGER(C_INT) :: iters(13)
 REAL(C_DOUBLE) :: par(8), dx, dy, g, cfl, wgp(4), t, dt, lmx, lmy
 REAL(C_DOUBLE) :: tend, tanalyze, dtanalyze
@@ -38,8 +39,9 @@ CALL GET_COMMAND_ARGUMENT(2, fout)
 OPEN(NEWUNIT=uin, FILE=TRIM(fin), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD')
 READ(uin) hdr
 nx = hdr(1); ny = hdr(2); nsteps = hdr(3); wb = hdr(4); hasb = hdr(5); loop = hdr(6)
-timescheme = hdr(7); hpa = hdr(8)
+timescheme = hdr(7); hpa = hdr(8); recfix = hdr(9)
 IF (timescheme .EQ. 0) timescheme = 5
+IF (recfix .EQ. 0) recfix = 4
 READ(uin) par
 dx = par(1); dy = par(2); g = par(3); cfl = par(4); wgp = par(5:8)
 ALLOCATE(U(3,nx,ny), dts(MAX(nsteps,1)))
@@ -60,6 +62,7 @@ IF (rc .NE. SWM_OK) THEN
   WRITE(*,*) "swm_create failed: ", rc
   STOP 1
 END IF
+CALL swm_check(h, swm_set_reconstruction_fix(h, recfix), "swm_set_reconstruction_fix")
 IF (hasb .NE. 0) THEN
   CALL swm_check(h, swm_set_bathy_grad(h, bx, by), "swm_set_bathy_grad")
 END IF

@@ -72,6 +72,7 @@ def lib():
     for name in ("swm_step_forward_euler", "swm_step_ssprk4", "swm_step_rk65", "swm_step_dec5"):
         getattr(L, name).argtypes = [vp, ctypes.c_double, ctypes.c_double]
     L.swm_set_hp_association.argtypes = [vp, ctypes.c_int]
+    L.swm_set_reconstruction_fix.argtypes = [vp, ctypes.c_int]
     L.swm_run_steps.argtypes = [vp, ctypes.c_int, _dp, _ip]
     L.swm_fvtd.argtypes = [vp] + [_dp] * 6
     L.swm_set_jacobi_limits.argtypes = [vp, ctypes.c_int, ctypes.c_double]
@@ -124,9 +125,13 @@ class Solver:
         self.timescheme = 5
 
     @classmethod
-    def from_problem(cls, p):
+    def from_problem(cls, p, recfix=4):
+        """A handle holding problem p; recfix: ReconstructionFix (set before
+        the well-balanced state is built, which depends on it)."""
         m = p.mesh
         s = cls(p.n, p.rows, m.dx, p.mesh_y.dx, p.gravity, p.cfl, p.wellbalanced, m.wbnd)
+        if recfix != 4:
+            s.set_reconstruction_fix(recfix)
         if p.dbdx is not None:
             s.set_bathy_grad(p.dbdx, p.dbdy)
         if p.wellbalanced:
@@ -180,6 +185,13 @@ class Solver:
 
     def set_hp_association(self, mode):
         self._check(lib().swm_set_hp_association(self._h, int(mode)), "swm_set_hp_association")
+
+    def set_reconstruction_fix(self, recfix):
+        """ReconstructionFix for the cells the shock indicator flags: 4 (WENO5,
+        the reference's setting), 1 first order, 2 MUSCL.  Discards the
+        well-balanced state (init_wb again)."""
+        self._check(lib().swm_set_reconstruction_fix(self._h, int(recfix)),
+                    "swm_set_reconstruction_fix")
 
     def step(self, dt, t=0.0, timescheme=None):
         ts = self.timescheme if timescheme is None else int(timescheme)
@@ -324,7 +336,7 @@ class JobDesc(ctypes.Structure):
                 ("wellbalanced", ctypes.c_int), ("wgp_bnd", ctypes.c_double * 4),
                 ("U0", _dp), ("Uwb", _dp), ("dbdx", _dp), ("dbdy", _dp),
                 ("nsteps", ctypes.c_int), ("timescheme", ctypes.c_int),
-                ("hp_association", ctypes.c_int)]
+                ("hp_association", ctypes.c_int), ("reconstruction_fix", ctypes.c_int)]
 
 
 class JobInfo(ctypes.Structure):
@@ -401,7 +413,7 @@ class Engine:
         except Exception:
             pass
 
-    def submit(self, p, nsteps, timescheme=5, hp_association=HP_AS_REFERENCE):
+    def submit(self, p, nsteps, timescheme=5, hp_association=HP_AS_REFERENCE, recfix=4):
         """Copies problem p to the device; returns its job id, or None when it
         does not fit the memory budget right now."""
         m = p.mesh
@@ -426,6 +438,7 @@ class Engine:
         d.nsteps = int(nsteps)
         d.timescheme = int(timescheme)
         d.hp_association = int(hp_association)
+        d.reconstruction_fix = int(recfix)
         jid = ctypes.c_int(-1)
         rc = _engine_lib().swm_engine_submit(self._e, ctypes.byref(d), ctypes.byref(jid))
         if rc == SWM_E_BUDGET:

@@ -57,6 +57,11 @@ CASES = {
     "lakeflag1_wb10_ssprk4": (lambda: P.lake_flag1(10, True), 2, 2),
     "vortex_rect11x14_rk65": (lambda: P.vortex(11, 107, ny=14), 2, 3),
     "shallowdam12_dec5": (lambda: P.dam_break(12, dry=0.1), 2, 4),
+    # ReconstructionFix 1 (first order) and 2 (MUSCL) on the cells the Jameson
+    # indicator flags (the reference's own setting is ReconstructionFix = 4)
+    "lakeflag1_wb10_fix1": (lambda: P.lake_flag1(10, True, pert=0.05), 2, 5, 1),
+    "vortex_rect13x11_fix2": (lambda: P.vortex(13, 108, ny=11), 3, 5, 2),
+    "shallowdam12_ssprk4_fix2": (lambda: P.dam_break(12, dry=0.1), 2, 2, 2),
 }
 
 
@@ -65,19 +70,21 @@ def main(only=None):
         if only and name not in only:
             continue
         scheme = rest[0] if rest else 5
+        recfix = rest[1] if len(rest) > 1 else 4
         p = mk()
         out = problem_arrays(p)
         out["scheme"] = scheme
-        f = run_swref(p, 1)
+        out["recfix"] = recfix
+        f = run_swref(p, 1, recfix=recfix)
         for k in ("Ut", "FX", "FY", "S", "Prod", "Dest", "wbnd", "MeshGP"):
             out["fvtd_" + k] = f[k]
         if scheme < 5:  # that build's Ut(1:3)
-            out["fvtd_explicit_Ut"] = run_swref(p, 1, scheme=scheme)["Ut"]
+            out["fvtd_explicit_Ut"] = run_swref(p, 1, scheme=scheme, recfix=recfix)["Ut"]
         if steps:
-            hist, U = run_swref(p, 2, steps=steps, scheme=scheme)
+            hist, U = run_swref(p, 2, steps=steps, scheme=scheme, recfix=recfix)
             out["steps_hist"] = np.array(hist)
             out["steps_U"] = U if U is not None else np.zeros(0)
-        out["ts"] = np.array(run_swref(p, 3))
+        out["ts"] = np.array(run_swref(p, 3, recfix=recfix))
         np.savez_compressed(HERE / f"{name}.npz", **out)
         print(f"wrote {name}.npz")
     if only:

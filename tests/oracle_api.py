@@ -62,6 +62,8 @@ def oracle_lib():
         L.orc_step_patankar.restype = ctypes.c_int
         L.orc_step_patankar.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _ip]
         L.orc_set_hp_by_node.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_set_reconstruction_fix.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_set_reconstruction_fix.restype = ctypes.c_int
         L.orc_step_explicit.restype = ctypes.c_int
         L.orc_step_explicit.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
         L.orc_jacobi.restype = ctypes.c_int
@@ -85,9 +87,11 @@ class Oracle:
     """One problem instance on the C oracle.  scheme: the reference's
     timescheme (1 ForwardEuler, 2 SSPRK4, 3 RK65, 4 DeC5 of the build without
     PATANKAR; 5 mPDeC5, 6 mPEuler, 7 mPDeC2); hp_by_node: the two-node
-    schemes' by-node depth view instead of the reference's."""
+    schemes' by-node depth view instead of the reference's; recfix: the
+    reference's ReconstructionFix (4, its own setting; 1 first order, 2 MUSCL
+    for the cells the shock indicator flags)."""
 
-    def __init__(self, problem, scheme: int = 5, hp_by_node: bool = False):
+    def __init__(self, problem, scheme: int = 5, hp_by_node: bool = False, recfix: int = 4):
         L = oracle_lib()
         self.p = problem
         self.scheme = scheme
@@ -101,6 +105,8 @@ class Oracle:
         if not self.h:
             raise ValueError("orc_create failed")
         L.orc_set_hp_by_node(self.h, 1 if hp_by_node else 0)
+        if L.orc_set_reconstruction_fix(self.h, recfix) != 0:
+            raise ValueError(f"reconstruction fix {recfix} is not 1, 2 or 4")
         if problem.dbdx is not None:
             L.orc_set_bathy(self.h, _ptr(np.ascontiguousarray(problem.dbdx)),
                             _ptr(np.ascontiguousarray(problem.dbdy)))
@@ -180,12 +186,13 @@ def have_swref_explicit():
 
 
 def run_swref(problem, mode: int, steps: int = 0, jac=None, timeout=600, scheme: int = 5,
-              patankar=None):
+              patankar=None, recfix: int = 4):
     """Run the reference build on problem.  mode 1: one FVTimeDerivative,
     2: `steps` CFL steps through TimeDiscretization, 3: TimeStep only,
     4: jacobi on jac=(Mass, diag, rhs).  scheme: the reference's timescheme
     (5 mPDeC5, 6 mPEuler, 7 mPDeC2 with the PATANKAR build; 1-4 with the
-    build without it, unless patankar=True forces the PATANKAR build)."""
+    build without it, unless patankar=True forces the PATANKAR build).
+    recfix: the ReconstructionFix the build runs with (4: its own setting)."""
     if patankar is None:
         patankar = scheme >= 5
     exe = (SWREF if patankar else SWREF_EXPLICIT)[bool(problem.wellbalanced)]
@@ -194,7 +201,7 @@ def run_swref(problem, mode: int, steps: int = 0, jac=None, timeout=600, scheme:
     n, r = problem.n, problem.rows
     bflag = 1 if problem.dbdx is not None else 0
     hdr = np.array([mode, n, steps, bflag, 1 if problem.wellbalanced else 0, r if r != n else 0,
-                    scheme], dtype="<i4")
+                    scheme, recfix], dtype="<i4")
     fl = np.array([problem.x0, problem.x1, problem.gravity, problem.cfl], dtype="<f8")
     with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as td:
         fin, fout = Path(td) / "in.bin", Path(td) / "out.bin"

@@ -25,11 +25,11 @@ def with_steps(p, k):
     return p
 
 
-def write_input(path, p, loop=0, scheme=5, hp_by_node=False):
+def write_input(path, p, loop=0, scheme=5, hp_by_node=False, recfix=0):
     m = p.mesh
     with open(path, "wb") as f:
         np.array([p.n, p.rows, p.steps, int(p.wellbalanced), int(p.dbdx is not None), loop,
-                  scheme, int(hp_by_node)], dtype=np.int32).tofile(f)
+                  scheme, int(hp_by_node), recfix], dtype=np.int32).tofile(f)
         np.array([m.dx, p.mesh_y.dx, p.gravity, p.cfl, *m.wbnd], dtype=np.float64).tofile(f)
         np.ascontiguousarray(p.U0, dtype=np.float64).tofile(f)
         if p.wellbalanced:
@@ -143,6 +143,35 @@ def test_fortran_driver_other_schemes(tmp_path, scheme, by_node, make):
     assert r.returncode == 0, r.stdout + r.stderr
     dts, U, _ = read_output(fout, p.n, p.rows)
     o = Oracle(p, scheme, hp_by_node=by_node)
+    dts_r = []
+    for _ in range(p.steps):
+        dt, _, _ = o.timestep()
+        o.step(dt)
+        dts_r.append(dt)
+    assert same_bits(dts, np.array(dts_r))
+    U_r = o.get_U()
+    assert same_bits(U, U_r), describe_diff(U, U_r)
+
+
+@pytest.mark.parametrize("recfix,scheme,make",
+                         [(1, 5, lambda: with_steps(P.lake_flag1(16, True, pert=0.05), 2)),
+                          (2, 5, lambda: P.vortex(20, 3, steps=2)),
+                          (2, 2, lambda: P.dam_break(18, steps=2, dry=0.1))],
+                         ids=["first_order_wb", "muscl", "muscl_ssprk4"])
+def test_fortran_driver_reconstruction_fix(tmp_path, recfix, scheme, make):
+    """The driver's header reconstruction_fix: swm_set_reconstruction_fix
+    before the well-balanced state is built, then the time loop, against the
+    oracle with the same ReconstructionFix."""
+    if not FDRIVER.exists():
+        pytest.skip("Fortran driver not built (no Fortran compiler at build time)")
+    p = make()
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    write_input(fin, p, scheme=scheme, recfix=recfix)
+    r = subprocess.run([str(FDRIVER), str(fin), str(fout)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    dts, U, _ = read_output(fout, p.n, p.rows)
+    o = Oracle(p, scheme, recfix=recfix)
     dts_r = []
     for _ in range(p.steps):
         dt, _, _ = o.timestep()

@@ -16,6 +16,11 @@ GOLDEN = Path(__file__).resolve().parent / "golden"
 FILES = sorted(f for f in GOLDEN.glob("*.npz") if not f.name.startswith("jacobi"))
 
 
+def recfix(z):
+    """The ReconstructionFix the fixture was written with (4: the reference's)."""
+    return int(z["recfix"]) if "recfix" in z else 4
+
+
 def load_problem(z):
     return P.Problem(int(z["n"]), z["U0"], ny=int(z["ny"]) if "ny" in z else 0,
                      wellbalanced=bool(int(z["wellbalanced"])),
@@ -30,7 +35,7 @@ def load_problem(z):
 def test_fvtd_matches_reference_fixture(f):
     z = np.load(f)
     p = load_problem(z)
-    got = Oracle(p).fvtd()
+    got = Oracle(p, recfix=recfix(z)).fvtd()
     for k in FVTD_KEYS:
         a, b = got[k], z["fvtd_" + k]
         if k in ("Ut", "S"):
@@ -43,7 +48,7 @@ def test_fvtd_matches_reference_fixture(f):
 def test_explicit_build_ut_matches_reference_fixture(f):
     """Ut(1:3) of the build without PATANKAR, Ut(1) included."""
     z = np.load(f)
-    got = Oracle(load_problem(z)).fvtd()["Ut"]
+    got = Oracle(load_problem(z), recfix=recfix(z)).fvtd()["Ut"]
     assert same_bits(got, z["fvtd_explicit_Ut"]), describe_diff(got, z["fvtd_explicit_Ut"])
 
 
@@ -69,7 +74,7 @@ def test_timestep_matches_reference(f):
                          ids=lambda f: f.stem)
 def test_steps_match_reference(f):
     z = np.load(f)
-    o = Oracle(load_problem(z), int(z["scheme"]) if "scheme" in z else 5)
+    o = Oracle(load_problem(z), int(z["scheme"]) if "scheme" in z else 5, recfix=recfix(z))
     hist = z["steps_hist"]
     U = None
     for k, row in enumerate(hist):

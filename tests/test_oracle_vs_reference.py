@@ -181,3 +181,99 @@ def test_patankar_build_explicit_schemes_freeze_the_depth(scheme):
     _, U = run_swref(p, 2, steps=1, scheme=scheme, patankar=True)
     assert same_bits(U[..., 0], p.U0[..., 0])
     assert not same_bits(U[..., 1], p.U0[..., 1])
+
+
+# ---- ReconstructionFix != Reconstruction: the shock-indicator path ----------
+# The reference fixes ReconstructionFix = Reconstruction (src/parameters.f90:142);
+# the driver of oracle/_ref sets it from its input header instead.
+
+FIX_CASES = ["vortex21", "random13", "random16_rough", "lakeflag1_wb14", "dam15_mindepth",
+             "vortex_rect21x34", "random_rect17x11", "lakeflag1_wb_rect19x26"]
+
+
+def jameson_np(hm, h0, hp):
+    ind = np.abs(hm + hp - 2.0 * h0)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        ind = ind / (np.abs(hm) + np.abs(hp) + np.abs(2.0 * h0))
+    return ind > 0.5e-3
+
+
+def indicator_readings(h):
+    """Flags of cells 0..n+1 along the last axis of the periodic depths h:
+    the reference's (each cell's verdict overwrites Ind(ii-1:ii+1) in loop
+    order, src/shocksindicator.f90:50-55) and an OR over the three writers."""
+    n = h.shape[-1]
+    shock = jameson_np(np.roll(h, 1, -1), h, np.roll(h, -1, -1))  # cells 1..n
+    idx = np.arange(n + 2)
+    last = shock[..., np.minimum(idx + 1, n) - 1]
+    ors = np.zeros(h.shape[:-1] + (n + 2,), bool)
+    for w in range(1, n + 1):  # writer w sets cells w-1..w+1
+        ors[..., w - 1:w + 2] |= shock[..., w - 1:w]
+    return last, ors
+
+
+@pytest.mark.parametrize("recfix", [1, 2])
+@pytest.mark.parametrize("case", FIX_CASES)
+def test_reconstruction_fix_fvtd_bitwise(case, recfix):
+    """One FVTimeDerivative with ReconstructionFix 1 (first order) or 2
+    (MUSCL) on the flagged cells: bitwise, and different from the WENO5-only
+    result (the indicator flags cells in every case)."""
+    p = CASES[case]()
+    ref = run_swref(p, 1, recfix=recfix)
+    assert not same_bits(ref["FX"], run_swref(p, 1)["FX"]), "no cell flagged"
+    got = Oracle(p, recfix=recfix).fvtd()
+    for k in FVTD_KEYS:
+        a, b = got[k], ref[k]
+        if k in ("Ut", "S"):
+            a, b = a[..., 1:], b[..., 1:]
+        assert same_bits(a, b), f"{case} fix {recfix} {k}: {describe_diff(a, b)}"
+
+
+def test_fix_cases_separate_the_indicator_readings():
+    """On the smooth cases the reference's overwrite reading of the indicator
+    and an OR of the three writers flag different cells along x and along y,
+    so the bitwise comparisons above and the device tests would catch an
+    implementation that ORs.  (The random states are rough enough that every
+    cell is flagged: they exercise the fix arithmetic everywhere.)"""
+    for case in FIX_CASES:
+        h = CASES[case]().U0[..., 0]
+        for hh in (h, h.T):
+            last, ors = indicator_readings(hh)
+            if not case.startswith("random"):
+                assert (last != ors).any(), case
+                assert 0.05 < last.mean() < 0.95, case
+
+
+@pytest.mark.parametrize("recfix", [1, 2])
+@pytest.mark.parametrize("case,steps", [("vortex21", 3), ("random13", 2), ("lakeflag1_wb14", 3),
+                                        ("dam15_mindepth", 2), ("vortex_rect21x34", 2),
+                                        ("random_rect17x11", 2)])
+def test_reconstruction_fix_steps_bitwise(case, steps, recfix):
+    """mPDeC5 steps with the fix (the well-balanced state built under it too)."""
+    p = CASES[case]()
+    hist_r, U_r = run_swref(p, 2, steps=steps, recfix=recfix)
+    o = Oracle(p, recfix=recfix)
+    for k, row in enumerate(hist_r):
+        dt, lx, ly = o.timestep()
+        assert same_bits([dt, lx, ly], row), f"step {k}"
+        if not np.isfinite(dt):
+            break
+        o.step(dt)
+    U = o.get_U()
+    assert same_bits(U, U_r), f"{case} fix {recfix}: {describe_diff(U, U_r)}"
+
+
+@explicit
+@pytest.mark.parametrize("scheme", [2, 4])
+@pytest.mark.parametrize("recfix", [1, 2])
+@pytest.mark.parametrize("case", ["vortex21", "random16_rough", "lakeflag1_wb14"])
+def test_reconstruction_fix_explicit_schemes_bitwise(case, recfix, scheme):
+    p = CASES[case]()
+    hist_r, U_r = run_swref(p, 2, steps=2, scheme=scheme, recfix=recfix)
+    o = Oracle(p, scheme, recfix=recfix)
+    for k, row in enumerate(hist_r):
+        dt, lx, ly = o.timestep()
+        assert same_bits([dt, lx, ly], row), f"step {k}"
+        o.step(dt)
+    U = o.get_U()
+    assert same_bits(U, U_r), f"{case} fix {recfix} scheme {scheme}: {describe_diff(U, U_r)}"
