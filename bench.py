@@ -84,13 +84,14 @@ def profiled(key: str, kernel: str, size: int):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(budget_s: float = 20.0, scheme: int = 5):
+def cpu_baseline(budget_s: float = 20.0, scheme: int = 5, recfix: int = 0):
     """The reference Fortran itself (oracle/_ref, built from its own sources)
     on the host: one core, the reference's own lake-at-rest bathymetry
     (BathymetryFlag=1) with the well-balanced build, 128x128, the number of
     CFL steps that fits the time budget.  Release-style build if present.
     scheme: the timescheme (1-4 run the well-balanced build without
-    PATANKAR, which has no release-style variant here)."""
+    PATANKAR, which has no release-style variant here).  recfix: the
+    ReconstructionFix the reference runs with (0: its own, 4)."""
     sys.path.insert(0, str(ROOT / "tests"))
     import problems as P
     from oracle_api import big_stack
@@ -110,7 +111,7 @@ def cpu_baseline(budget_s: float = 20.0, scheme: int = 5):
     p = P.lake_flag1(n, True)
 
     def run(steps):
-        hdr = np.array([5, n, steps, 1, 1, 0, scheme, 0], dtype="<i4")
+        hdr = np.array([5, n, steps, 1, 1, 0, scheme, recfix], dtype="<i4")
         fl = np.array([p.x0, p.x1, p.gravity, p.cfl], dtype="<f8")
         with tempfile.TemporaryDirectory() as td:
             fin, fout = Path(td) / "in.bin", Path(td) / "out.bin"
@@ -133,7 +134,8 @@ def cpu_baseline(budget_s: float = 20.0, scheme: int = 5):
     t, k = run(steps)
     rate = n * n * k / t
     return {"value": rate, "unit": "cell-steps/s", "cores": 1, "kind": "reference",
-            "sample": (f"reference Fortran ({kind_note}), timescheme {scheme}, WELLBALANCED, "
+            "sample": (f"reference Fortran ({kind_note}), timescheme {scheme}, "
+                       f"{f'ReconstructionFix {recfix}, ' if recfix else ''}WELLBALANCED, "
                        f"BathymetryFlag=1 lake at rest + 1e-3 bump, {n}x{n}, {k} CFL steps, "
                        f"{t:.1f} s, time loop only")}
 
@@ -174,6 +176,50 @@ def schemes_section(n: int, seed: int, steps: int, cpu: bool) -> dict:
             if cb:
                 res[name]["cpu_reference"] = cb
                 res[name]["gpu_vs_cpu"] = res[name]["cell_steps_per_s"] / cb["value"]
+    return res
+
+
+def flagged_fraction(h: np.ndarray) -> float:
+    """Fraction of the cells of the periodic depths h (rows, cols) that the
+    Jameson indicator flags along x (src/shocksindicator.f90:93-129)."""
+    hm, hp = np.roll(h, 1, 1), np.roll(h, -1, 1)
+    ind = np.abs(hm + hp - 2.0 * h)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        ind = ind / (np.abs(hm) + np.abs(hp) + np.abs(2.0 * h))
+    return float(np.mean(ind > 0.5e-3))
+
+
+def recfix_section(n: int, steps: int, cpu: bool) -> dict:
+    """ReconstructionFix 1 (first order) and 2 (MUSCL) on the shock indicator's
+    flagged cells, against 4 (WENO5 everywhere, the reference's setting), on a
+    shallow dam break of n x n cells (h = 1 inside radius 0.25 of the centre,
+    0.1 outside: the indicator flags the front), mPDeC5, `steps` CFL steps
+    each after one warmup step.  The face kernels compute WENO5 for every cell
+    and replace the flagged cells' values, so the fix adds work only where
+    cells are flagged."""
+    import problems as P
+    import swm
+
+    prob = P.dam_break(n, dry=0.1)
+    res = {"workload": f"shallow dam break {n}x{n}, mPDeC5",
+           "flagged_fraction_x_initial": flagged_fraction(prob.U0[..., 0])}
+    for rf in (4, 1, 2):
+        s = swm.Solver.from_problem(prob, recfix=rf)
+        s.run_steps(1)
+        s.sync()
+        t0 = time.perf_counter()
+        s.run_steps(steps)
+        s.sync()
+        el = time.perf_counter() - t0
+        res[f"fix{rf}"] = {"steps": steps, "ms_per_step": 1e3 * el / steps,
+                           "cell_steps_per_s": n * n * steps / el}
+        log(f"bench: reconstruction fix {rf}: {1e3 * el / steps:.2f} ms/step")
+        del s
+    if cpu:
+        cb = cpu_baseline(budget_s=4.0, scheme=5, recfix=2)
+        if cb:
+            res["fix2"]["cpu_reference"] = cb
+            res["fix2"]["gpu_vs_cpu"] = res["fix2"]["cell_steps_per_s"] / cb["value"]
     return res
 
 
@@ -275,6 +321,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--scheme-steps", type=int, default=20,
                     help="steps of each other time scheme at the headline size (0: skip)")
+    ap.add_argument("--fix-steps", type=int, default=10,
+                    help="steps of each ReconstructionFix run on a dam break at the headline "
+                         "size (0: skip)")
     ap.add_argument("--batch-jobs", type=int, default=256,
                     help="jobs of the batched run (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=48,
@@ -386,6 +435,8 @@ def main():
     del s
     if args.scheme_steps > 0:
         out["schemes"] = schemes_section(n, args.seed, args.scheme_steps, not args.no_cpu)
+    if args.fix_steps > 0:
+        out["reconstruction_fix"] = recfix_section(n, args.fix_steps, not args.no_cpu)
     if args.batch_jobs > 0:
         out["batched"] = batched_section(args.batch_jobs, args.seed)
     if args.serve_requests > 0:
