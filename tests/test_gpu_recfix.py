@@ -161,17 +161,17 @@ def test_dry_bed_first_order_nan_pattern():
 
 
 @pytest.mark.parametrize("recfix", [1, 2])
-def test_large_grid_mass_and_lake(recfix):
-    """1024^2: mass conserved to rounding over two mPDeC5 steps; the
-    well-balanced lake at rest stays bitwise at rest (the flags of U and of
-    the well-balanced state agree, so FVTD(U) - FVTD(Uwb) is exactly 0)."""
-    p = P.lake_bumps(1024, 11, amp=0.05)
+def test_large_grid_positivity_and_lake(recfix):
+    """1024^2: depths stay non-negative over two mPDeC5 steps of a dam break
+    (mass is not a property here: under the fix the reference leaks mass
+    through the periodic seam, whose two face copies see different flags);
+    the well-balanced lake at rest stays bitwise at rest (the flags of U and
+    of the well-balanced state agree, so FVTD(U) - FVTD(Uwb) is exactly 0)."""
+    p = P.dam_break(1024, dry=0.1)
     s = swm.Solver.from_problem(p, recfix=recfix)
-    m0 = p.U0[..., 0].sum()
     s.run_steps(2)
     U = s.download()
-    assert abs(U[..., 0].sum() - m0) <= 1e-11 * m0
-    assert U[..., 0].min() >= 0.0
+    assert np.isfinite(U).all() and U[..., 0].min() >= 0.0
     lake = P.lake_bumps(1024, 11, amp=0.0)
     s = swm.Solver.from_problem(lake, recfix=recfix)
     s.run_steps(2)

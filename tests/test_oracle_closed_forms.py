@@ -320,3 +320,110 @@ def test_explicit_fvtd_mass_derivative_is_the_flux_divergence():
     dx, dy = p.mesh.dx, p.mesh_y.dx
     expect = 0.0 - (FX[:, 1:] - FX[:, :-1]) / dx - (FY[1:, :] - FY[:-1, :]) / dy
     assert same_bits(f["Ut"][..., 0], expect)
+
+
+# ReconstructionFix 1 / 2 on the flagged cells (the reference itself pins the
+# oracle bitwise; these fix what the two reconstructions must be).
+def _rusanov_x(L, R, g):
+    """Rusanov flux along x between primitive states (..., 3), with the
+    reference's wave-speed bound max(|u_L + c_L|, |u_R + c_R|)
+    (src/equation.f90:1096-1101, WaveSpeeds1D's fastest = u + c)."""
+    def flux(W):
+        h, u, v = W[..., 0], W[..., 1], W[..., 2]
+        return np.stack([h * u, h * u * u + 0.5 * g * h * h, h * u * v], -1)
+
+    def cons(W):
+        return np.stack([W[..., 0], W[..., 0] * W[..., 1], W[..., 0] * W[..., 2]], -1)
+
+    lam = np.maximum(np.abs(L[..., 1] + np.sqrt(g * L[..., 0])),
+                     np.abs(R[..., 1] + np.sqrt(g * R[..., 0])))
+    return 0.5 * (flux(L) + flux(R)) - 0.5 * lam[..., None] * (cons(R) - cons(L))
+
+
+def _minmod(x, y):
+    return 0.5 * (np.sign(x) + np.sign(y)) * np.minimum(np.abs(x), np.abs(y))
+
+
+@pytest.mark.parametrize("recfix", [1, 2])
+def test_fully_flagged_fix_gives_first_order_and_muscl_fluxes(recfix):
+    """A rough state flags every cell, so every x face carries the Rusanov flux
+    of the fix values: the two cells' primitives (1) or the minmod-limited
+    linear values at the face (2), constant over the Gauss points (the
+    positivity limiter leaves them).  FX = (sum of the Gauss weights) x that
+    flux, to rounding."""
+    rng = np.random.default_rng(41)
+    n = 16
+    U = np.empty((n, n, 3))
+    U[..., 0] = 1.0 + 0.9 * rng.uniform(-1, 1, (n, n))
+    U[..., 1] = U[..., 0] * rng.normal(0, 0.3, (n, n))
+    U[..., 2] = U[..., 0] * rng.normal(0, 0.3, (n, n))
+    p = P.Problem(n, U, kind="random")
+    h = U[..., 0]
+    ht = h + 1e-6 / h
+    V = np.stack([h, U[..., 1] / ht, U[..., 2] / ht], -1)  # ConsToPrim
+    hm, hp = np.roll(h, 1, 1), np.roll(h, -1, 1)
+    assert (np.abs(hm + hp - 2 * h) / (np.abs(hm) + np.abs(hp) + np.abs(2 * h)) > 0.5e-3).all()
+    dx = p.mesh.dx
+    if recfix == 1:
+        WM = WP = V
+    else:
+        sp = (np.roll(V, -1, 1) - V) / dx
+        sm = (V - np.roll(V, 1, 1)) / dx
+        slope = _minmod(sm, sp)
+        WM, WP = V - 0.5 * slope * dx, V + 0.5 * slope * dx
+    F = _rusanov_x(WP, np.roll(WM, -1, 1), p.gravity)  # face between cell i and i+1
+    FX = Oracle(p, recfix=recfix).fvtd()["FX"]  # (rows, faces 0..n, 3)
+    want = np.sum(p.mesh.wbnd) * np.concatenate([F[:, -1:], F], axis=1)
+    assert np.allclose(FX, want, rtol=1e-12, atol=1e-12)
+    # and not the WENO5 fluxes
+    assert not np.allclose(Oracle(p).fvtd()["FX"], want, rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("recfix", [1, 2])
+def test_fix_keeps_positivity_and_lake_at_rest(recfix):
+    p = P.dam_break(20, dry=0.1)
+    o = Oracle(p, recfix=recfix)
+    for _ in range(4):
+        dt, _, _ = o.timestep()
+        o.step(dt)
+        assert (o.get_U()[..., 0] >= 0).all()
+    lake = P.lake_flag1(14, True, pert=0.0)
+    o = Oracle(lake, recfix=recfix)
+    assert (o.fvtd()["Ut"][..., 1:] == 0).all()
+
+
+def _flags(h):
+    """Ind along the last axis of the periodic depths h, cells 0..n+1, as
+    the reference's loop leaves it: the verdict of cell min(ii+1, n)."""
+    n = h.shape[-1]
+    hm, hp = np.roll(h, 1, -1), np.roll(h, -1, -1)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        shock = np.abs(hm + hp - 2 * h) / (np.abs(hm) + np.abs(hp) + np.abs(2 * h)) > 0.5e-3
+    return shock[..., np.minimum(np.arange(n + 2) + 1, n) - 1]
+
+
+@pytest.mark.parametrize("recfix", [1, 2])
+def test_fix_leaks_mass_only_through_the_seam(recfix):
+    """Under the fix the ghost cells take a neighbour's verdict (Ind(0) =
+    Shock(1), Ind(n+1) = Shock(n)), not their image cell's, so the two
+    copies of the periodic seam face, F(0) and F(n), differ exactly in the
+    rows where ghost 0 and cell n, or cell 1 and ghost n+1, are flagged
+    differently.  Every interior face is shared by its two cells, so the
+    total mass derivative is the seam mismatch and nothing else."""
+    p = P.dam_break(20, dry=0.1)
+    o = Oracle(p, recfix=recfix)
+    dt, _, _ = o.timestep()
+    o.step(dt)  # the front's wake reaches the seam
+    n = p.n
+    U = o.get_U()
+    f = o.fvtd()
+    FX, FY = f["FX"], f["FY"]
+    for F, h in ((FX, U[..., 0]), (np.swapaxes(FY, 0, 1), U[..., 0].T)):
+        fl = _flags(h)
+        want = (fl[:, 0] != fl[:, n]) | (fl[:, 1] != fl[:, n + 1])
+        got = (F[:, 0] != F[:, n]).any(-1)
+        assert want.any() and (got == want).all()
+    seam = ((FX[:, n, 0] - FX[:, 0, 0]).sum() / p.mesh.dx
+            + (FY[n, :, 0] - FY[0, :, 0]).sum() / p.mesh_y.dx)
+    total = f["Ut"][..., 0].sum()
+    assert seam != 0.0 and abs(total + seam) <= 1e-12 * np.abs(f["Ut"][..., 0]).sum()

@@ -277,3 +277,17 @@ def test_reconstruction_fix_explicit_schemes_bitwise(case, recfix, scheme):
         o.step(dt)
     U = o.get_U()
     assert same_bits(U, U_r), f"{case} fix {recfix} scheme {scheme}: {describe_diff(U, U_r)}"
+
+
+def test_reference_fix_does_not_conserve_mass():
+    """The reference itself, built with ReconstructionFix 1 or 2, loses mass
+    on a dam break (through the periodic seam, whose two face copies see
+    different flags; test_oracle_closed_forms pins the mechanism), and the
+    oracle reproduces the loss bitwise (test_reconstruction_fix_steps_bitwise)."""
+    p = P.dam_break(20, dry=0.1)
+    m0 = p.U0[..., 0].sum()
+    _, U4 = run_swref(p, 2, steps=3)
+    assert abs(U4[..., 0].sum() - m0) <= 1e-12 * m0
+    for recfix in (1, 2):
+        _, U = run_swref(p, 2, steps=3, recfix=recfix)
+        assert abs(U[..., 0].sum() - m0) > 1e-10 * m0
