@@ -208,11 +208,12 @@ def recfix_section(n: int, steps: int, cpu: bool) -> dict:
         s.run_steps(1)
         s.sync()
         t0 = time.perf_counter()
-        s.run_steps(steps)
+        _, iters, _ = s.run_steps(steps)
         s.sync()
         el = time.perf_counter() - t0
         res[f"fix{rf}"] = {"steps": steps, "ms_per_step": 1e3 * el / steps,
-                           "cell_steps_per_s": n * n * steps / el}
+                           "cell_steps_per_s": n * n * steps / el,
+                           "jacobi_sweeps_per_step": float(np.sum(iters)) / steps}
         log(f"bench: reconstruction fix {rf}: {1e3 * el / steps:.2f} ms/step")
         del s
     if cpu:
