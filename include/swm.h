@@ -67,6 +67,25 @@ int swm_set_bathy_grad(swm_solver_t *h, const double *dbdx, const double *dbdy);
  * the well-balanced state Uwb on the device, kept as FXWB/FYWB/SWB. */
 int swm_init_wb(swm_solver_t *h, const double *Uwb, int ghosts);
 
+/* The reference's initial data made on the device instead of by the host
+ * (src/main.f90:22-26): the state of FillInitialConditions
+ * (src/finitevolume.f90:281-333), i.e. the 4x4-Gauss cell averages of
+ * ExactFunction(initial_condition, t = 0) (src/equation.f90:70-230,
+ * initial_condition 1..6); the Bathymetry_X/_Y tables of bathymetry_flag
+ * (src/equation.f90:419-509; 0: flat bottom); and, for a wellbalanced
+ * handle, the averages of ExactFunctionWB (initial conditions 2 and 3 only,
+ * else SWM_E_ARG) with their FVTD, as swm_init_wb.  One call replaces
+ * swm_set_bathy_grad + swm_init_wb + swm_upload with host-evaluated data.
+ * The domain is MESH_X0 = x0, MESH_X1 = x1 (src/parameters.f90), which must
+ * give the handle's dx and dy exactly (|x1-x0|/n, src/mesh.f90:39-41), else
+ * SWM_E_ARG.  Gravity is the handle's.  Parity: sqrt, + and * are IEEE,
+ * the powers are expanded as the reference's compiler does, and the dam
+ * breaks (4, 5) are bitwise the reference's; exp, sin, cos and atan are the
+ * device library's, so the other initial data agree with the reference's to
+ * a few ulps. */
+int swm_fill_initial_data(swm_solver_t *h, int initial_condition, int bathymetry_flag,
+                          const double x0[2], const double x1[2]);
+
 /* Copy the state U into / out of the device (the module array U). */
 int swm_upload(swm_solver_t *h, const double *U, int ghosts);
 int swm_download(swm_solver_t *h, double *U);
@@ -241,6 +260,11 @@ typedef struct {
   int timescheme;            /* 0 or 5: mPDeC5; 1..4, 6, 7 as swm_set_timescheme */
   int hp_association;        /* SWM_HP_AS_REFERENCE or SWM_HP_BY_NODE (swm_set_hp_association) */
   int reconstruction_fix;    /* 0 or 4: WENO5 everywhere; 1, 2 as swm_set_reconstruction_fix */
+  /* With U0 == NULL and initial_condition > 0 the job's initial data are made
+   * on the device as swm_fill_initial_data does (Uwb, dbdx, dbdy unused). */
+  int initial_condition;     /* 1..6 */
+  int bathymetry_flag;       /* 0..2 */
+  double mesh_x0[2], mesh_x1[2];
 } swm_job_desc;
 
 #define SWM_JOB_RUNNING 0

@@ -14,6 +14,9 @@
 !   jacobi            (src/jacobi_iteration.f90:52) -> mode 4
 !   mode 5: like mode 2, but only the wall time of the step loop (SYSTEM_CLOCK)
 !           and the final state are written (CPU baseline timing)
+!   mode 6: the initial data (InitializeWBVariables' well-balanced averages in
+!           WELLBALANCED builds, then FillInitialConditions) for
+!           InitialCondition = nsteps; no state is read
 ! With WELLBALANCED builds the well-balanced reference state is installed the
 ! way InitializeWBVariables does it (src/finitevolume.f90:373-392).
 !
@@ -26,6 +29,7 @@
 !         timescheme (0: 5, mPDeC5; src/parameters.f90:144 numbering),
 !         reconstruction_fix (0: 4 = Reconstruction, as src/parameters.f90:142)
 !   real64 x0, x1, gravity, cfl    (the domain is [x0,x1]^2; dy = dx*nx/ny)
+!   real64 y0, y1                  [mode 6 only: the domain is [x0,x1] x [y0,y1]]
 !   real64 U(3,nx,ny)              [mode 1-3]   or  Mass(5nx*ny), Diag(nx*ny), rhs(nx*ny) [mode 4]
 !   real64 Uwb(3,nx,ny)            [if has_wb]
 !===============================================================================!
@@ -33,6 +37,10 @@ PROGRAM swref
 USE MOD_FiniteVolume2D_vars
 USE MOD_FiniteVolume2D,     ONLY: InitializeFiniteVolume
 USE MOD_FiniteVolume2D,     ONLY: FVTimeDerivative
+USE MOD_FiniteVolume2D,     ONLY: FillInitialConditions
+#ifdef WELLBALANCED
+USE MOD_FiniteVolume2D,     ONLY: InitializeWBVariables
+#endif
 USE MOD_Mesh,               ONLY: BuildMesh
 USE MOD_Equation,           ONLY: TimeStep
 USE MOD_TimeDiscretization, ONLY: TimeDiscretization
@@ -42,7 +50,7 @@ USE MOD_JacobiIteration,    ONLY: jacobi
 IMPLICIT NONE
 INTEGER            :: mode, nx, ny, nsteps, bflag, haswb, k, nrow, scheme, recfix
 INTEGER(KIND=8)    :: c0, c1, crate
-REAL               :: x0, x1, grav, cflin, dtmin
+REAL               :: x0, x1, y0, y1, grav, cflin, dtmin
 REAL, ALLOCATABLE  :: Uic(:,:,:), Uwb(:,:,:), Mass(:), Diag(:), rhs(:), sol(:)
 CHARACTER(LEN=1024):: fin, fout
 
@@ -55,6 +63,9 @@ IF (ny .LE. 0) ny = nx
 IF (scheme .LE. 0) scheme = 5
 IF (recfix .LE. 0) recfix = 4
 READ(10) x0, x1, grav, cflin
+y0 = x0
+y1 = x1
+IF (mode .EQ. 6) READ(10) y0, y1
 
 #ifdef WELLBALANCED
 IF (haswb .NE. 1) THEN
@@ -71,8 +82,8 @@ END IF
 ! Module globals as InitializeParameters (src/parameters.f90:24-171) sets them.
 nElemsX = nx
 nElemsY = ny
-MESH_X0 = (/x0, x0/)
-MESH_X1 = (/x1, x1/)
+MESH_X0 = (/x0, y0/)
+MESH_X1 = (/x1, y1/)
 Gravity = grav
 CFL     = cflin
 BoundaryConditionsType = (/1,1,1,1/)
@@ -102,6 +113,25 @@ IF (mode .EQ. 4) THEN
   STOP
 END IF
 #endif
+
+IF (mode .EQ. 6) THEN
+  ! The reference's own initial data (src/main.f90:22-26) for
+  ! InitialCondition = nsteps on [x0,x1]^2 with BathymetryFlag = bathy_flag:
+  ! the well-balanced cell averages of InitializeWBVariables (WELLBALANCED
+  ! builds; U is 0 before it, as after InitializeFiniteVolume), then the cell
+  ! averages of FillInitialConditions.
+  CLOSE(10)
+  InitialCondition = nsteps
+#ifdef WELLBALANCED
+  U = 0.
+  CALL InitializeWBVariables()
+  WRITE(11) U(1:3,1:nx,1:ny)
+#endif
+  CALL FillInitialConditions()
+  WRITE(11) U(1:3,1:nx,1:ny)
+  CLOSE(11)
+  STOP
+END IF
 
 ALLOCATE(Uic(3,nx,ny))
 READ(10) Uic

@@ -606,6 +606,184 @@ static void fv_time_derivative(orc_t *o) {
   update_time_derivative(o);
 }
 
+/* ---- initial data: BuildMesh's quadrature, ExactFunction, ExactFunctionWB,
+ *      Bathymetry and its derivatives (src/mesh.f90:116-146,
+ *      src/equation.f90:70-509, src/finitevolume.f90:281-396) ------------------ */
+typedef struct {
+  int ic, bflag;
+  double x0[2], sx[2], g, pi;
+} exact_ctx;
+
+/* Bathymetry, src/equation.f90:419-446 */
+static double bathymetry(const exact_ctx *c, double x, double y) {
+  if (c->bflag == 1) return 0.1 * sin(2. * c->pi * x) * cos(2. * c->pi * y);
+  if (c->bflag == 2) {
+    double r2 = x * x + y * y; /* X**2. is folded to X*X by the compiler */
+    return (r2 < 1) ? exp(1.0 - 1.0 / (1.0 - r2)) : 0.0;
+  }
+  return 0.;
+}
+/* Bathymetry_X, src/equation.f90:451-478 */
+static double bathymetry_x(const exact_ctx *c, double x, double y) {
+  if (c->bflag == 1) return 0.1 * 2. * c->pi * cos(2. * c->pi * x) * cos(2. * c->pi * y);
+  if (c->bflag == 2) {
+    double r2 = x * x + y * y;
+    return (r2 < 1) ? -2.0 * x / ((1.0 - r2) * (1.0 - r2)) * exp(1.0 - 1.0 / (1.0 - r2)) : 0.0;
+  }
+  return 0.;
+}
+/* Bathymetry_Y, src/equation.f90:484-509 */
+static double bathymetry_y(const exact_ctx *c, double x, double y) {
+  if (c->bflag == 1) return -(0.1 * 2. * c->pi * sin(2. * c->pi * x) * sin(2. * c->pi * y));
+  if (c->bflag == 2) {
+    double r2 = x * x + y * y;
+    return (r2 < 1) ? -2.0 * y / ((1.0 - r2) * (1.0 - r2)) * exp(1.0 - 1.0 / (1.0 - r2)) : 0.0;
+  }
+  return 0.;
+}
+
+/* hSmoothAuxiliary and hDerivSmoothAuxiliary, src/equation.f90:216-228.
+ * The reference compiler evaluates a**3. (an integral real exponent) and a**4
+ * as left-to-right products, a*a*a and a*a*a*a: the reference's own initial
+ * data pin that (a library pow differs in the last bit at some points). */
+static double h_smooth(double x) {
+  double b = atan(1. - x);
+  return 1. - 0.1 * exp(-1. / (b * b * b));
+}
+static double h_deriv_smooth(double x) {
+  double a = atan(x - 1.);
+  return 3. * 0.1 * exp(1. / (a * a * a)) / (a * a * a * a * ((x - 1.) * (x - 1.) + 1.));
+}
+
+/* MODULO(a, p) for reals: a - FLOOR(a/p)*p; exact through fmod */
+static double fmodulo(double a, double p) {
+  double m = fmod(a, p);
+  if (m != 0. && ((m < 0.) != (p < 0.))) m += p;
+  return m;
+}
+
+/* ExactFunction at t = 0, src/equation.f90:70-211 (conservative variables) */
+static int exact_function(const exact_ctx *c, double x, double y, double *cons) {
+  double h = 0., u = 0., v = 0.;
+  switch (c->ic) {
+    case 1: { /* unsteady smooth vortex */
+      const double u_inf = 2., v_inf = 3., h_inf = 1.;
+      double xm = c->x0[0] + 0.5 * c->sx[0], ym = c->x0[1] + 0.5 * c->sx[1];
+      double xc = fmodulo(x - u_inf * 0.0 - c->x0[0], c->sx[0]) + c->x0[0] - xm;
+      double yc = fmodulo(y - v_inf * 0.0 - c->x0[1], c->sx[1]) + c->x0[1] - ym;
+      double r = xc * xc + yc * yc;
+      double omega = sqrt(2. * c->g * h_deriv_smooth(r));
+      h = h_inf;
+      u = u_inf;
+      v = v_inf;
+      if (r < 1) {
+        h = h_smooth(r);
+        u = u + omega * (+yc);
+        v = v + omega * (-xc);
+      }
+      break;
+    }
+    case 2: /* lake at rest */
+      h = 1. - 0.1 * sin(2. * c->pi * x) * cos(2. * c->pi * y);
+      break;
+    case 3: { /* perturbation of a wet-dry lake at rest */
+      h = fmaxr(0.7 - bathymetry(c, x, y), MIN_DEPTH);
+      double r2 = ((x + 2) * (x + 2) + (y - 0.5) * (y - 0.5)) * 9.;
+      if (r2 < 1) h = h + 0.05 * exp(1. - 1. / ((1 - r2) * (1 - r2)));
+      break;
+    }
+    case 4:
+    case 5: { /* circular dam breaks 1 and 2 */
+      double xc = x - (c->x0[0] + 0.5 * c->sx[0]), yc = y - (c->x0[1] + 0.5 * c->sx[1]);
+      double r = sqrt(xc * xc + yc * yc);
+      if (c->ic == 4)
+        h = (r <= 7.) ? 2.5 : MIN_DEPTH;
+      else
+        h = (r <= 15.) ? 10. : 0.5;
+      break;
+    }
+    case 6: { /* wave over a dry island */
+      double r2 = (x + 2) * (x + 2);
+      h = fmaxr(0.7 - bathymetry(c, x, y), MIN_DEPTH);
+      if (r2 < 1) h = h + 0.5 * exp(1. - 1. / ((1 - r2) * (1 - r2)));
+      u = (h > MIN_DEPTH * 10) ? 1.0 : 0.0;
+      break;
+    }
+    default:
+      return -1;
+  }
+  cons[0] = h;
+  cons[1] = h * u;
+  cons[2] = h * v;
+  return 0;
+}
+
+/* ExactFunctionWB, src/equation.f90:236-278 */
+static int exact_function_wb(const exact_ctx *c, double x, double y, double *cons) {
+  double h;
+  if (c->ic == 2)
+    h = 1. - 0.1 * sin(2. * c->pi * x) * cos(2. * c->pi * y);
+  else if (c->ic == 3)
+    h = fmaxr(0.7 - bathymetry(c, x, y), MIN_DEPTH);
+  else
+    return -1;
+  cons[0] = h;
+  cons[1] = h * 0.;
+  cons[2] = h * 0.;
+  return 0;
+}
+
+int orc_initial_data(int ic, int bathy_flag, int nx, int ny, const double x0[2],
+                     const double x1[2], double gravity, double *U, double *Uwb, double *dbdx,
+                     double *dbdy) {
+  if (nx < 1 || ny < 1 || ic < 1 || ic > 6) return -1;
+  if (Uwb && ic != 2 && ic != 3) return -1;
+  exact_ctx c = {ic, bathy_flag, {x0[0], x0[1]}, {0., 0.}, gravity, acos(-1.0)};
+  double dxy[2];
+  const int nel[2] = {nx, ny};
+  for (int d = 0; d < 2; ++d) { /* src/mesh.f90:39-41 */
+    c.sx[d] = fabs(x1[d] - x0[d]);
+    dxy[d] = fabs(c.sx[d]) / (double)nel[d];
+  }
+  /* quadNodes1D / quadWeights1D for nGPs = 4, src/mesh.f90:127-132 */
+  const double q1 = 0.5 * sqrt(3. / 7. + 2. / 7. * sqrt(6. / 5.));
+  const double q2 = 0.5 * sqrt(3. / 7. - 2. / 7. * sqrt(6. / 5.));
+  const double node[4] = {-q1, -q2, q2, q1};
+  const double wlo = (18. - sqrt(30.)) / 72., whi = (18. + sqrt(30.)) / 72.;
+  const double wq[4] = {wlo, whi, whi, wlo};
+  const size_t n = (size_t)nx * ny;
+  for (int jj = 1; jj <= ny; ++jj)
+    for (int ii = 1; ii <= nx; ++ii) {
+      /* MeshBary = MeshNodes(ii-1, jj-1) + 0.5*Mesh_DX (src/mesh.f90:43-54) */
+      double bx = (x0[0] + (double)(ii - 1) * dxy[0]) + 0.5 * dxy[0];
+      double by = (x0[1] + (double)(jj - 1) * dxy[1]) + 0.5 * dxy[1];
+      size_t k = (size_t)(jj - 1) * nx + (ii - 1);
+      double acc[3] = {0., 0., 0.}, accwb[3] = {0., 0., 0.};
+      for (int ig = 0; ig < 4; ++ig)
+        for (int jg = 0; jg < 4; ++jg) {
+          double x = bx + node[ig] * dxy[0], y = by + node[jg] * dxy[1]; /* MeshGP */
+          double w = wq[ig] * wq[jg], t[3];
+          if (exact_function(&c, x, y, t)) return -1;
+          for (int v = 0; v < 3; ++v) acc[v] = acc[v] + w * t[v];
+          if (Uwb) {
+            exact_function_wb(&c, x, y, t);
+            for (int v = 0; v < 3; ++v) accwb[v] = accwb[v] + w * t[v];
+          }
+        }
+      for (int v = 0; v < 3; ++v) {
+        if (U) U[3 * k + v] = acc[v];
+        if (Uwb) Uwb[3 * k + v] = accwb[v];
+      }
+      for (int yg = 0; yg < 4; ++yg)
+        for (int xg = 0; xg < 4; ++xg) {
+          double x = bx + node[xg] * dxy[0], y = by + node[yg] * dxy[1];
+          if (dbdx) dbdx[(size_t)(yg * 4 + xg) * n + k] = bathymetry_x(&c, x, y);
+          if (dbdy) dbdy[(size_t)(yg * 4 + xg) * n + k] = bathymetry_y(&c, x, y);
+        }
+    }
+  return 0;
+}
+
 /* ---- public API ----------------------------------------------------------- */
 
 orc_t *orc_create(int nx, int ny, double dx, double dy, double gravity, double cfl,

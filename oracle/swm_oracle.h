@@ -78,6 +78,18 @@ void orc_set_hp_by_node(orc_t *o, int on);
  * another value. */
 int orc_set_reconstruction_fix(orc_t *o, int recfix);
 
+/* The reference's initial data for InitialCondition ic (1..6) with
+ * BathymetryFlag bathy_flag (0..2) on [x0[0],x1[0]] x [x0[1],x1[1]], nx x ny
+ * cells (src/main.f90:22-26): U, the cell averages of ExactFunction at the
+ * 4x4 Gauss points (FillInitialConditions, src/finitevolume.f90:281-333);
+ * Uwb, those of ExactFunctionWB (ic 2 and 3; InitializeWBVariables); and
+ * Bathymetry_X/_Y at every Gauss point in the bathymetry-table layout.
+ * Any output may be NULL.  Returns 0, or -1 for an unknown ic (or Uwb
+ * requested for an ic without a well-balanced state). */
+int orc_initial_data(int ic, int bathy_flag, int nx, int ny, const double x0[2],
+                     const double x1[2], double gravity, double *U, double *Uwb, double *dbdx,
+                     double *dbdy);
+
 /* Jacobi on the reference CSR pattern (src/jacobi_iteration.f90:52-87).
  * Mass (5n), diag (n), rhs (n) -> sol (n).  Returns the sweep count. */
 int orc_jacobi(const orc_t *o, const double *Mass, const double *diag,

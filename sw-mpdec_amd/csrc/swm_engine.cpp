@@ -860,13 +860,11 @@ int upload_state(swm_engine* e, Job& J, const double* U, int ghosts, double* h, 
   return SWM_OK;
 }
 
-// FVTimeDerivative of the well-balanced state into FXWB/FYWB/SWB
-// (InitializeWBVariables, src/finitevolume.f90:373-392).
-int init_wb(swm_engine* e, int id, const double* Uwb, int ghosts) {
+// FVTimeDerivative of the well-balanced state held in node 1's arrays into
+// FXWB/FYWB/SWB (InitializeWBVariables, src/finitevolume.f90:373-392).
+int build_wb(swm_engine* e, int id) {
   Job& J = *e->jobs[id];
-  int rc = upload_state(e, J, Uwb, ghosts, J.dev.hres[1], J.dev.hun[1], J.dev.hvn[1]);
-  if (rc) return rc;
-  rc = refresh(e);
+  int rc = refresh(e);
   if (rc) return rc;
   NodeList L{};
   L.count = 1;
@@ -881,6 +879,13 @@ int init_wb(swm_engine* e, int id, const double* Uwb, int ghosts) {
   HIPCK(e, hipStreamSynchronize(e->stream));
   J.wb_ready = true;
   return SWM_OK;
+}
+
+int init_wb(swm_engine* e, int id, const double* Uwb, int ghosts) {
+  Job& J = *e->jobs[id];
+  int rc = upload_state(e, J, Uwb, ghosts, J.dev.hres[1], J.dev.hun[1], J.dev.hvn[1]);
+  if (rc) return rc;
+  return build_wb(e, id);
 }
 
 int status_capacity(swm_engine* e, size_t nj) {
@@ -1059,15 +1064,8 @@ int recfix_code(int recfix) {
   return -1;
 }
 
-int job_set_bathy(swm_engine* e, int id, const double* bx, const double* by) {
-  Job& J = *e->jobs[id];
-  if (!bx != !by) FAIL(e, SWM_E_ARG, "dbdx and dbdy must both be set or both NULL");
-  J.wb_ready = false;  // the well-balanced reference depends on the source term
-  if (!bx) {
-    J.dev.bathy = 0;
-    e->dirty = true;
-    return SWM_OK;
-  }
+// The job's bathymetry tables, allocated on first use (within the budget).
+int ensure_bathy(swm_engine* e, Job& J) {
   const size_t n = J.lay.n;
   if (!J.bathy_buf) {
     const long long b = (long long)bathy_bytes(n);
@@ -1078,6 +1076,21 @@ int job_set_bathy(swm_engine* e, int id, const double* bx, const double* by) {
   }
   J.dev.bx = J.bathy_buf;
   J.dev.by = J.bathy_buf + 16 * n;
+  return SWM_OK;
+}
+
+int job_set_bathy(swm_engine* e, int id, const double* bx, const double* by) {
+  Job& J = *e->jobs[id];
+  if (!bx != !by) FAIL(e, SWM_E_ARG, "dbdx and dbdy must both be set or both NULL");
+  J.wb_ready = false;  // the well-balanced reference depends on the source term
+  if (!bx) {
+    J.dev.bathy = 0;
+    e->dirty = true;
+    return SWM_OK;
+  }
+  const size_t n = J.lay.n;
+  int rc = ensure_bathy(e, J);
+  if (rc) return rc;
   J.dev.bathy = 1;
   HIPCK(e, hipMemcpyAsync(J.dev.bx, bx, 16 * n * sizeof(double), hipMemcpyHostToDevice, e->stream));
   HIPCK(e, hipMemcpyAsync(J.dev.by, by, 16 * n * sizeof(double), hipMemcpyHostToDevice, e->stream));
@@ -1086,11 +1099,65 @@ int job_set_bathy(swm_engine* e, int id, const double* bx, const double* by) {
   return SWM_OK;
 }
 
+// The reference's initial data on the device (src/main.f90:22-26): the state
+// of InitialCondition ic, the bathymetry tables of BathymetryFlag bflag (0:
+// flat bottom) and, for a well-balanced job, the well-balanced state and its
+// FVTD.  The domain [x0[0], x1[0]] x [x0[1], x1[1]] must give the job's
+// Mesh_DX (src/mesh.f90:39-41) bitwise.
+int fill_initial_data(swm_engine* e, int id, int ic, int bflag, const double* x0,
+                      const double* x1) {
+  Job& J = *e->jobs[id];
+  if (ic < 1 || ic > 6) FAIL(e, SWM_E_ARG, "initial condition %d is not one of 1..6", ic);
+  if (bflag < 0 || bflag > 2) FAIL(e, SWM_E_ARG, "bathymetry flag %d is not one of 0..2", bflag);
+  if (!x0 || !x1) FAIL(e, SWM_E_ARG, "domain corners are NULL");
+  if (J.dev.wb && ic != 2 && ic != 3)
+    FAIL(e, SWM_E_ARG,
+         "initial condition %d has no well-balanced state (ExactFunctionWB covers 2 and 3)", ic);
+  InitialSetup c{};
+  c.ic = ic;
+  c.bflag = bflag;
+  const int nel[2] = {J.dev.nx, J.dev.ny};
+  const double hdx[2] = {J.dev.dx, J.dev.dy};
+  for (int d = 0; d < 2; ++d) {
+    c.x0[d] = x0[d];
+    c.sx[d] = std::fabs(x1[d] - x0[d]);
+    c.dx[d] = std::fabs(c.sx[d]) / (double)nel[d];
+    if (!(c.dx[d] == hdx[d]))
+      FAIL(e, SWM_E_ARG, "domain gives Mesh_DX(%d) = %.17g, the handle has %.17g", d + 1,
+           c.dx[d], hdx[d]);
+  }
+  c.g = J.dev.g;
+  c.pi = std::acos(-1.0);
+  const double q1 = 0.5 * std::sqrt(3. / 7. + 2. / 7. * std::sqrt(6. / 5.));
+  const double q2 = 0.5 * std::sqrt(3. / 7. - 2. / 7. * std::sqrt(6. / 5.));
+  const double nodes[4] = {-q1, -q2, q2, q1};
+  for (int g = 0; g < 4; ++g) {
+    c.node[g] = nodes[g];
+    c.w[g] = J.dev.wbnd[g];  // WeightsGP(i,j) = quadWeights1D(i)*quadWeights1D(j)
+  }
+  if (bflag) {
+    int rc = ensure_bathy(e, J);
+    if (rc) return rc;
+  }
+  const bool wb = J.dev.wb != 0;
+  launch_initial_data(c, J.dev.nx, J.dev.ny, J.dev.h, J.dev.hu, J.dev.hv,
+                      wb ? J.dev.hres[1] : nullptr, wb ? J.dev.hun[1] : nullptr,
+                      wb ? J.dev.hvn[1] : nullptr, bflag ? J.dev.bx : nullptr,
+                      bflag ? J.dev.by : nullptr, e->stream);
+  LAUNCHCK(e);
+  HIPCK(e, hipStreamSynchronize(e->stream));
+  J.dev.bathy = bflag ? 1 : 0;
+  J.wb_ready = false;
+  e->dirty = true;
+  return wb ? build_wb(e, id) : SWM_OK;
+}
+
 int engine_add(swm_engine* e, const swm_job_desc* d, int* id_out) {
   *id_out = -1;
   if (!d) FAIL(e, SWM_E_ARG, "job description is NULL");
-  if (!d->U0) FAIL(e, SWM_E_ARG, "U0 is NULL");
-  if (d->wellbalanced && !d->Uwb) FAIL(e, SWM_E_ARG, "wellbalanced job needs Uwb");
+  const bool exact = !d->U0 && d->initial_condition > 0;  // initial data made on the device
+  if (!d->U0 && !exact) FAIL(e, SWM_E_ARG, "U0 is NULL and no initial_condition is set");
+  if (d->wellbalanced && !d->Uwb && !exact) FAIL(e, SWM_E_ARG, "wellbalanced job needs Uwb");
   if (!d->dbdx != !d->dbdy) FAIL(e, SWM_E_ARG, "dbdx and dbdy must both be set or both NULL");
   const SchemeDef* sc = find_scheme(d->timescheme == 0 ? 5 : d->timescheme);
   if (!sc) FAIL(e, SWM_E_ARG, "timescheme %d is not one of 0..7", d->timescheme);
@@ -1099,7 +1166,8 @@ int engine_add(swm_engine* e, const swm_job_desc* d, int* id_out) {
   const int fix = recfix_code(d->reconstruction_fix == 0 ? 4 : d->reconstruction_fix);
   if (fix < 0)
     FAIL(e, SWM_E_ARG, "reconstruction_fix %d is not 0, 4, 1 or 2", d->reconstruction_fix);
-  const long long extra = d->dbdx ? (long long)bathy_bytes((size_t)d->nx * d->ny) : 0;
+  const bool tables = exact ? d->bathymetry_flag > 0 : d->dbdx != nullptr;
+  const long long extra = tables ? (long long)bathy_bytes((size_t)d->nx * d->ny) : 0;
   int id;
   int rc = alloc_job(e, d->nx, d->ny, d->dx, d->dy, d->gravity, d->cfl, d->wellbalanced ? 1 : 0,
                      d->wgp_bnd, d->nsteps, extra, &id);
@@ -1109,16 +1177,22 @@ int engine_add(swm_engine* e, const swm_job_desc* d, int* id_out) {
   J.scheme = sc;
   J.hp_assoc = d->hp_association;
   J.dev.recfix = fix;
-  if (d->dbdx) {
-    rc = job_set_bathy(e, id, d->dbdx, d->dbdy);
+  if (exact) {
+    rc = fill_initial_data(e, id, d->initial_condition, d->bathymetry_flag, d->mesh_x0,
+                           d->mesh_x1);
+    if (rc) return rc;
+  } else {
+    if (d->dbdx) {
+      rc = job_set_bathy(e, id, d->dbdx, d->dbdy);
+      if (rc) return rc;
+    }
+    if (d->wellbalanced) {
+      rc = init_wb(e, id, d->Uwb, 0);
+      if (rc) return rc;
+    }
+    rc = upload_state(e, J, d->U0, 0, J.dev.h, J.dev.hu, J.dev.hv);
     if (rc) return rc;
   }
-  if (d->wellbalanced) {
-    rc = init_wb(e, id, d->Uwb, 0);
-    if (rc) return rc;
-  }
-  rc = upload_state(e, J, d->U0, 0, J.dev.h, J.dev.hu, J.dev.hv);
-  if (rc) return rc;
   if (d->nsteps == 0) J.state = JOB_DONE;  // nothing to run; holds U0 until released
   return SWM_OK;
 }
@@ -1461,6 +1535,12 @@ int swm_init_wb(swm_solver_t* s, const double* Uwb, int ghosts) {
   if (ghosts != 0 && ghosts != 3) FAIL(e, SWM_E_ARG, "ghosts must be 0 or 3 (got %d)", ghosts);
   if (!plain_job(s)->dev.wb) FAIL(e, SWM_E_STATE, "handle was created without wellbalanced");
   return init_wb(e, s->job, Uwb, ghosts);
+}
+
+int swm_fill_initial_data(swm_solver_t* s, int initial_condition, int bathymetry_flag,
+                          const double x0[2], const double x1[2]) {
+  if (!s) return SWM_E_ARG;
+  return fill_initial_data(s->e, s->job, initial_condition, bathymetry_flag, x0, x1);
 }
 
 int swm_upload(swm_solver_t* s, const double* U, int ghosts) {

@@ -183,6 +183,21 @@ inline const SchemeDef* find_scheme(int timescheme) {
   return nullptr;
 }
 
+// The reference's initial data (swm_initial.hip): InitialCondition,
+// BathymetryFlag, MESH_X0, MESH_SX, Mesh_DX, Gravity, PI and the 4-point
+// Gauss rule of BuildMesh (quadNodes1D, quadWeights1D; src/mesh.f90:127-132).
+struct InitialSetup {
+  int ic, bflag;
+  double x0[2], sx[2], dx[2], g, pi;
+  double node[4], w[4];
+};
+
+// h/hu/hv: the state; hwb/huwb/hvwb (may be null): the well-balanced
+// state; bx/by (may be null): the bathymetry tables.
+void launch_initial_data(const InitialSetup& c, int nx, int ny, double* h, double* hu, double* hv,
+                         double* hwb, double* huwb, double* hvwb, double* bx, double* by,
+                         hipStream_t s);
+
 // ---- launches (swm_kernels.hip) -------------------------------------------
 // xt / yt: face tiles of shape kShapes[shape]; ct: k_fvtd_update tiles.
 void launch_fvtd(const EngineArgs& E, int shape, const TileRef* xt, int nxt, const TileRef* yt,

@@ -1,0 +1,181 @@
+// swm_initial.hip -- the reference's initial data on the device: the cell
+// averages of ExactFunction (FillInitialConditions, src/finitevolume.f90:281-333)
+// and of ExactFunctionWB (InitializeWBVariables, :339-396) over the 4x4 Gauss
+// points, and Bathymetry_X/_Y at every Gauss point (src/equation.f90:419-509),
+// written straight into a job's device arrays.
+//
+// Every expression keeps the reference's order, with its compiler's lowering
+// of powers: X**2. and X**2 are X*X, a**3. and a**4 are the left-to-right
+// products a*a*a and a*a*a*a.  sqrt is IEEE; exp, sin, cos and atan are the
+// device library's, which may differ from the host libm in the last bit, so
+// data that go through them agree with the reference's to a few ulps rather
+// than bitwise (the dam breaks 4 and 5 need only sqrt and are bitwise).
+#include <hip/hip_runtime.h>
+
+#include "swm_engine.hpp"
+#include "swm_numerics.hpp"
+
+namespace swm {
+
+namespace {
+
+// Bathymetry, src/equation.f90:419-446
+__device__ double bathymetry(const InitialSetup& c, double x, double y) {
+  if (c.bflag == 1) return 0.1 * sin(2. * c.pi * x) * cos(2. * c.pi * y);
+  if (c.bflag == 2) {
+    double r2 = x * x + y * y;
+    return (r2 < 1) ? exp(1.0 - 1.0 / (1.0 - r2)) : 0.0;
+  }
+  return 0.;
+}
+
+// Bathymetry_X, src/equation.f90:451-478
+__device__ double bathymetry_x(const InitialSetup& c, double x, double y) {
+  if (c.bflag == 1) return 0.1 * 2. * c.pi * cos(2. * c.pi * x) * cos(2. * c.pi * y);
+  if (c.bflag == 2) {
+    double r2 = x * x + y * y;
+    return (r2 < 1) ? -2.0 * x / ((1.0 - r2) * (1.0 - r2)) * exp(1.0 - 1.0 / (1.0 - r2)) : 0.0;
+  }
+  return 0.;
+}
+
+// Bathymetry_Y, src/equation.f90:484-509
+__device__ double bathymetry_y(const InitialSetup& c, double x, double y) {
+  if (c.bflag == 1) return -(0.1 * 2. * c.pi * sin(2. * c.pi * x) * sin(2. * c.pi * y));
+  if (c.bflag == 2) {
+    double r2 = x * x + y * y;
+    return (r2 < 1) ? -2.0 * y / ((1.0 - r2) * (1.0 - r2)) * exp(1.0 - 1.0 / (1.0 - r2)) : 0.0;
+  }
+  return 0.;
+}
+
+// MODULO(a, p) for reals, exact through fmod
+__device__ double modulo_r(double a, double p) {
+  double m = fmod(a, p);
+  if (m != 0. && ((m < 0.) != (p < 0.))) m += p;
+  return m;
+}
+
+// ExactFunction at t = 0, src/equation.f90:70-230: conservative variables.
+__device__ void exact_function(const InitialSetup& c, double x, double y, double cons[3]) {
+  double h = 0., u = 0., v = 0.;
+  switch (c.ic) {
+    case 1: {  // unsteady smooth vortex; hSmoothAuxiliary, hDerivSmoothAuxiliary (:216-228)
+      const double u_inf = 2., v_inf = 3., h_inf = 1.;
+      const double xm = c.x0[0] + 0.5 * c.sx[0], ym = c.x0[1] + 0.5 * c.sx[1];
+      const double xc = modulo_r(x - u_inf * 0.0 - c.x0[0], c.sx[0]) + c.x0[0] - xm;
+      const double yc = modulo_r(y - v_inf * 0.0 - c.x0[1], c.sx[1]) + c.x0[1] - ym;
+      const double r = xc * xc + yc * yc;
+      const double a = atan(r - 1.);
+      const double hd =
+          3. * 0.1 * exp(1. / (a * a * a)) / (a * a * a * a * ((r - 1.) * (r - 1.) + 1.));
+      const double omega = sqrt(2. * c.g * hd);
+      h = h_inf;
+      u = u_inf;
+      v = v_inf;
+      if (r < 1) {
+        const double b = atan(1. - r);
+        h = 1. - 0.1 * exp(-1. / (b * b * b));
+        u = u + omega * (+yc);
+        v = v + omega * (-xc);
+      }
+      break;
+    }
+    case 2:  // lake at rest
+      h = 1. - 0.1 * sin(2. * c.pi * x) * cos(2. * c.pi * y);
+      break;
+    case 3: {  // perturbation of a wet-dry lake at rest
+      h = max_r(0.7 - bathymetry(c, x, y), kMinDepth);
+      const double r2 = ((x + 2) * (x + 2) + (y - 0.5) * (y - 0.5)) * 9.;
+      if (r2 < 1) h = h + 0.05 * exp(1. - 1. / ((1 - r2) * (1 - r2)));
+      break;
+    }
+    case 4:
+    case 5: {  // circular dam breaks 1 and 2
+      const double xc = x - (c.x0[0] + 0.5 * c.sx[0]), yc = y - (c.x0[1] + 0.5 * c.sx[1]);
+      const double r = sqrt(xc * xc + yc * yc);
+      if (c.ic == 4)
+        h = (r <= 7.) ? 2.5 : kMinDepth;
+      else
+        h = (r <= 15.) ? 10. : 0.5;
+      break;
+    }
+    default: {  // 6: wave over a dry island
+      const double r2 = (x + 2) * (x + 2);
+      h = max_r(0.7 - bathymetry(c, x, y), kMinDepth);
+      if (r2 < 1) h = h + 0.5 * exp(1. - 1. / ((1 - r2) * (1 - r2)));
+      u = (h > kMinDepth * 10) ? 1.0 : 0.0;
+      break;
+    }
+  }
+  cons[0] = h;  // PrimToCons
+  cons[1] = h * u;
+  cons[2] = h * v;
+}
+
+// ExactFunctionWB, src/equation.f90:236-278 (initial conditions 2 and 3)
+__device__ void exact_function_wb(const InitialSetup& c, double x, double y, double cons[3]) {
+  const double h = (c.ic == 2) ? 1. - 0.1 * sin(2. * c.pi * x) * cos(2. * c.pi * y)
+                               : max_r(0.7 - bathymetry(c, x, y), kMinDepth);
+  cons[0] = h;
+  cons[1] = h * 0.;
+  cons[2] = h * 0.;
+}
+
+// One thread per cell (i, j): MeshBary = MeshNodes(i-1, j-1) + DX/2 and
+// MeshGP = MeshBary + quadNodes1D*DX (src/mesh.f90:43-54,136-142); averages
+// accumulated over iGP (outer) and jGP (inner) with WeightsGP = w(iGP)*w(jGP).
+__global__ __launch_bounds__(256) void k_initial_data(InitialSetup c, int nx, int ny, double* h,
+                                                      double* hu, double* hv, double* hwb,
+                                                      double* huwb, double* hvwb, double* bx,
+                                                      double* by) {
+  const size_t n = (size_t)nx * ny;
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i = (int)(k % nx) + 1, j = (int)(k / nx) + 1;
+  const double bxc = (c.x0[0] + (double)(i - 1) * c.dx[0]) + 0.5 * c.dx[0];
+  const double byc = (c.x0[1] + (double)(j - 1) * c.dx[1]) + 0.5 * c.dx[1];
+  double acc[3] = {0., 0., 0.}, awb[3] = {0., 0., 0.};
+  for (int ig = 0; ig < 4; ++ig)
+    for (int jg = 0; jg < 4; ++jg) {
+      const double x = bxc + c.node[ig] * c.dx[0], y = byc + c.node[jg] * c.dx[1];
+      const double w = c.w[ig] * c.w[jg];
+      double t[3];
+      exact_function(c, x, y, t);
+#pragma unroll
+      for (int v = 0; v < 3; ++v) acc[v] = acc[v] + w * t[v];
+      if (hwb) {
+        exact_function_wb(c, x, y, t);
+#pragma unroll
+        for (int v = 0; v < 3; ++v) awb[v] = awb[v] + w * t[v];
+      }
+    }
+  h[k] = acc[0];
+  hu[k] = acc[1];
+  hv[k] = acc[2];
+  if (hwb) {
+    hwb[k] = awb[0];
+    huwb[k] = awb[1];
+    hvwb[k] = awb[2];
+  }
+  if (bx)
+    for (int yg = 0; yg < 4; ++yg)
+      for (int xg = 0; xg < 4; ++xg) {
+        const double x = bxc + c.node[xg] * c.dx[0], y = byc + c.node[yg] * c.dx[1];
+        bx[(size_t)(yg * 4 + xg) * n + k] = bathymetry_x(c, x, y);
+        by[(size_t)(yg * 4 + xg) * n + k] = bathymetry_y(c, x, y);
+      }
+}
+
+}  // namespace
+
+void launch_initial_data(const InitialSetup& c, int nx, int ny, double* h, double* hu, double* hv,
+                         double* hwb, double* huwb, double* hvwb, double* bx, double* by,
+                         hipStream_t s) {
+  const size_t n = (size_t)nx * ny;
+  if (!n) return;
+  hipLaunchKernelGGL(k_initial_data, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c, nx,
+                     ny, h, hu, hv, hwb, huwb, hvwb, bx, by);
+}
+
+}  // namespace swm

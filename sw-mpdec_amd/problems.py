@@ -171,7 +171,8 @@ def cons(h, u, v):
 @dataclass
 class Problem:
     """One solve: initial state, optional bathymetry and well-balanced state,
-    on nx = n by ny = rows cells of the periodic square [x0, x1]^2."""
+    on nx = n by ny = rows cells of the periodic domain [x0, x1] x [y0, y1]
+    (y0, y1 default to x0, x1)."""
     n: int
     U0: np.ndarray                  # (rows, n, 3)
     wellbalanced: bool = False
@@ -185,6 +186,8 @@ class Problem:
     x0: float = 0.0
     x1: float = 1.0
     ny: int = 0                     # 0: square (ny = n)
+    y0: float | None = None         # y extent (None: that of x)
+    y1: float | None = None
 
     @property
     def rows(self) -> int:
@@ -197,7 +200,41 @@ class Problem:
 
     @property
     def mesh_y(self) -> Mesh:
-        return Mesh(self.rows, self.x0, self.x1)
+        y0 = self.x0 if self.y0 is None else self.y0
+        y1 = self.x1 if self.y1 is None else self.y1
+        return Mesh(self.rows, y0, y1)
+
+
+# The reference's own setups per InitialCondition (src/parameters.f90:62-125):
+# MESH_X0, MESH_X1, Gravity and BathymetryFlag.
+REFERENCE_SETUPS = {
+    1: ((0.0, 0.0), (3.0, 3.0), 9.81, 0),     # unsteady smooth vortex
+    2: ((0.0, 0.0), (1.0, 1.0), 9.81, 1),     # lake at rest
+    3: ((-5.0, -2.0), (5.0, 2.0), 9.8, 2),    # perturbation of a wet-dry lake at rest
+    4: ((0.0, 0.0), (40.0, 40.0), 9.8, 0),    # circular dam break 1
+    5: ((0.0, 0.0), (50.0, 50.0), 9.8, 0),    # circular dam break 2
+    6: ((-5.0, -2.0), (5.0, 2.0), 9.8, 2),    # wave over a dry island
+}
+
+
+@dataclass
+class ReferenceSetup:
+    """One of the reference's initial conditions on nx x ny cells of its own
+    domain, with the data to be made on the device (swm_fill_initial_data)
+    rather than by the host.  wellbalanced: the WELLBALANCED variant
+    (initial conditions 2 and 3)."""
+    ic: int
+    nx: int
+    ny: int = 0
+    wellbalanced: bool = False
+    cfl: float = 0.5
+    steps: int = 1
+
+    def __post_init__(self):
+        self.ny = self.ny or self.nx
+        self.x0, self.x1, self.gravity, self.bathy_flag = REFERENCE_SETUPS[self.ic]
+        self.mesh = Mesh(self.nx, self.x0[0], self.x1[0])
+        self.mesh_y = Mesh(self.ny, self.x0[1], self.x1[1])
 
 
 def vortex(n: int, seed: int, steps: int = 50, cfl: float = 0.5, ny: int = 0) -> Problem:

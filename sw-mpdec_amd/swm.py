@@ -73,6 +73,7 @@ def lib():
         getattr(L, name).argtypes = [vp, ctypes.c_double, ctypes.c_double]
     L.swm_set_hp_association.argtypes = [vp, ctypes.c_int]
     L.swm_set_reconstruction_fix.argtypes = [vp, ctypes.c_int]
+    L.swm_fill_initial_data.argtypes = [vp, ctypes.c_int, ctypes.c_int, _dp, _dp]
     L.swm_run_steps.argtypes = [vp, ctypes.c_int, _dp, _ip]
     L.swm_fvtd.argtypes = [vp] + [_dp] * 6
     L.swm_set_jacobi_limits.argtypes = [vp, ctypes.c_int, ctypes.c_double]
@@ -139,6 +140,17 @@ class Solver:
         s.upload(p.U0)
         return s
 
+    @classmethod
+    def from_setup(cls, c, recfix=4):
+        """A handle holding the reference's initial condition c (a
+        problems.ReferenceSetup), its data made on the device."""
+        s = cls(c.nx, c.ny, c.mesh.dx, c.mesh_y.dx, c.gravity, c.cfl, c.wellbalanced,
+                c.mesh.wbnd)
+        if recfix != 4:
+            s.set_reconstruction_fix(recfix)
+        s.fill_initial_data(c.ic, c.bathy_flag, c.x0, c.x1)
+        return s
+
     def _check(self, rc, what):
         if rc < 0:
             raise SwmError(f"{what}: {lib().swm_last_error(self._h).decode()} (rc={rc})")
@@ -185,6 +197,15 @@ class Solver:
 
     def set_hp_association(self, mode):
         self._check(lib().swm_set_hp_association(self._h, int(mode)), "swm_set_hp_association")
+
+    def fill_initial_data(self, ic, bathy_flag, x0, x1):
+        """The reference's initial data of InitialCondition ic made on the
+        device (state, bathymetry tables, well-balanced state) on the domain
+        [x0[0], x1[0]] x [x0[1], x1[1]]."""
+        a0 = np.ascontiguousarray(x0, dtype=np.float64)
+        a1 = np.ascontiguousarray(x1, dtype=np.float64)
+        self._check(lib().swm_fill_initial_data(self._h, int(ic), int(bathy_flag), _ptr(a0),
+                                                _ptr(a1)), "swm_fill_initial_data")
 
     def set_reconstruction_fix(self, recfix):
         """ReconstructionFix for the cells the shock indicator flags: 4 (WENO5,
@@ -336,7 +357,9 @@ class JobDesc(ctypes.Structure):
                 ("wellbalanced", ctypes.c_int), ("wgp_bnd", ctypes.c_double * 4),
                 ("U0", _dp), ("Uwb", _dp), ("dbdx", _dp), ("dbdy", _dp),
                 ("nsteps", ctypes.c_int), ("timescheme", ctypes.c_int),
-                ("hp_association", ctypes.c_int), ("reconstruction_fix", ctypes.c_int)]
+                ("hp_association", ctypes.c_int), ("reconstruction_fix", ctypes.c_int),
+                ("initial_condition", ctypes.c_int), ("bathymetry_flag", ctypes.c_int),
+                ("mesh_x0", ctypes.c_double * 2), ("mesh_x1", ctypes.c_double * 2)]
 
 
 class JobInfo(ctypes.Structure):
@@ -445,6 +468,35 @@ class Engine:
             return None
         self._check(rc, "swm_engine_submit")
         self._keep[jid.value] = (p.rows, p.n)
+        return jid.value
+
+    def submit_setup(self, c, nsteps, timescheme=5, hp_association=HP_AS_REFERENCE, recfix=4):
+        """Submits the reference's initial condition c (a
+        problems.ReferenceSetup), its data made on the device; returns the job
+        id, or None when it does not fit the memory budget right now."""
+        d = JobDesc()
+        d.nx, d.ny = c.nx, c.ny
+        d.dx, d.dy = c.mesh.dx, c.mesh_y.dx
+        d.gravity = c.gravity
+        d.cfl = c.cfl
+        d.wellbalanced = int(bool(c.wellbalanced))
+        for i in range(4):
+            d.wgp_bnd[i] = c.mesh.wbnd[i]
+        d.nsteps = int(nsteps)
+        d.timescheme = int(timescheme)
+        d.hp_association = int(hp_association)
+        d.reconstruction_fix = int(recfix)
+        d.initial_condition = int(c.ic)
+        d.bathymetry_flag = int(c.bathy_flag)
+        for i in range(2):
+            d.mesh_x0[i] = c.x0[i]
+            d.mesh_x1[i] = c.x1[i]
+        jid = ctypes.c_int(-1)
+        rc = _engine_lib().swm_engine_submit(self._e, ctypes.byref(d), ctypes.byref(jid))
+        if rc == SWM_E_BUDGET:
+            return None
+        self._check(rc, "swm_engine_submit")
+        self._keep[jid.value] = (c.ny, c.nx)
         return jid.value
 
     def step(self):

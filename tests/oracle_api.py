@@ -64,6 +64,9 @@ def oracle_lib():
         L.orc_set_hp_by_node.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_set_reconstruction_fix.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_set_reconstruction_fix.restype = ctypes.c_int
+        L.orc_initial_data.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       _dp, _dp, ctypes.c_double, _dp, _dp, _dp, _dp]
+        L.orc_initial_data.restype = ctypes.c_int
         L.orc_step_explicit.restype = ctypes.c_int
         L.orc_step_explicit.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
         L.orc_jacobi.restype = ctypes.c_int
@@ -186,20 +189,22 @@ def have_swref_explicit():
 
 
 def run_swref(problem, mode: int, steps: int = 0, jac=None, timeout=600, scheme: int = 5,
-              patankar=None, recfix: int = 4):
+              patankar=None, recfix: int = 4, bathy_flag=None):
     """Run the reference build on problem.  mode 1: one FVTimeDerivative,
     2: `steps` CFL steps through TimeDiscretization, 3: TimeStep only,
     4: jacobi on jac=(Mass, diag, rhs).  scheme: the reference's timescheme
     (5 mPDeC5, 6 mPEuler, 7 mPDeC2 with the PATANKAR build; 1-4 with the
     build without it, unless patankar=True forces the PATANKAR build).
-    recfix: the ReconstructionFix the build runs with (4: its own setting)."""
+    recfix: the ReconstructionFix the build runs with (4: its own setting).
+    bathy_flag: BathymetryFlag (default: 1 when the problem has bathymetry
+    tables, else 0); the reference evaluates its own Bathymetry_X/_Y."""
     if patankar is None:
         patankar = scheme >= 5
     exe = (SWREF if patankar else SWREF_EXPLICIT)[bool(problem.wellbalanced)]
     if not exe.exists():
         raise RuntimeError(f"{exe} not built")
     n, r = problem.n, problem.rows
-    bflag = 1 if problem.dbdx is not None else 0
+    bflag = (1 if problem.dbdx is not None else 0) if bathy_flag is None else bathy_flag
     hdr = np.array([mode, n, steps, bflag, 1 if problem.wellbalanced else 0, r if r != n else 0,
                     scheme, recfix], dtype="<i4")
     fl = np.array([problem.x0, problem.x1, problem.gravity, problem.cfl], dtype="<f8")
@@ -247,3 +252,45 @@ def run_swref(problem, mode: int, steps: int = 0, jac=None, timeout=600, scheme:
         U = raw[off + 3:off + rec].reshape(r, n, 3)
         off += rec
     return hist, U
+
+
+def run_swref_initial_data(ic: int, nx: int, ny: int, x0, x1, gravity: float, bathy_flag: int,
+                           wellbalanced: bool, timeout=600):
+    """The reference's own initial data (driver mode 6): InitialCondition ic
+    on [x0[0], x1[0]] x [x0[1], x1[1]] with nx x ny cells and BathymetryFlag
+    bathy_flag.  Returns (U, Uwb), each (ny, nx, 3); Uwb is None unless
+    wellbalanced (the WELLBALANCED build's InitializeWBVariables averages)."""
+    exe = SWREF[bool(wellbalanced)]
+    if not exe.exists():
+        raise RuntimeError(f"{exe} not built")
+    hdr = np.array([6, nx, ic, bathy_flag, 1 if wellbalanced else 0, ny, 5, 4], dtype="<i4")
+    fl = np.array([x0[0], x1[0], gravity, 0.5, x0[1], x1[1]], dtype="<f8")
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as td:
+        fin, fout = Path(td) / "in.bin", Path(td) / "out.bin"
+        with open(fin, "wb") as f:
+            f.write(hdr.tobytes())
+            f.write(fl.tobytes())
+        subprocess.run([str(exe), str(fin), str(fout)], check=True, cwd=td, timeout=timeout,
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, preexec_fn=big_stack)
+        raw = np.fromfile(fout, dtype="<f8")
+    k = 3 * nx * ny
+    arrs = [raw[i * k:(i + 1) * k].reshape(ny, nx, 3) for i in range(len(raw) // k)]
+    return (arrs[-1], arrs[0] if wellbalanced else None)
+
+
+def oracle_initial_data(ic: int, nx: int, ny: int, x0, x1, gravity: float, bathy_flag: int,
+                        wellbalanced: bool):
+    """The oracle's restatement of the reference's initial data: (U, Uwb,
+    dbdx, dbdy) with U, Uwb (ny, nx, 3) (Uwb None unless wellbalanced) and the
+    bathymetry tables (16, ny, nx) ([yg*4+xg][j][i])."""
+    L = oracle_lib()
+    U = np.zeros((ny, nx, 3))
+    Uwb = np.zeros((ny, nx, 3)) if wellbalanced else None
+    bx, by = np.zeros((16, ny, nx)), np.zeros((16, ny, nx))
+    a0 = np.array(x0, dtype=np.float64)
+    a1 = np.array(x1, dtype=np.float64)
+    rc = L.orc_initial_data(ic, bathy_flag, nx, ny, _ptr(a0), _ptr(a1), gravity, _ptr(U),
+                            _ptr(Uwb) if Uwb is not None else None, _ptr(bx), _ptr(by))
+    if rc != 0:
+        raise ValueError(f"orc_initial_data: initial condition {ic} not available")
+    return U, Uwb, bx, by

@@ -1,0 +1,154 @@
+"""The reference's initial data made on the device (swm_fill_initial_data and
+the engine's initial_condition jobs): ExactFunction cell averages, the
+well-balanced state and the bathymetry tables, against the oracle's
+restatement (itself bitwise equal to the reference's FillInitialConditions,
+InitializeWBVariables and source term, test_oracle_vs_reference.py).
+
+Tolerance: sqrt, + and * are IEEE, so the dam breaks (4, 5) are bitwise; the
+other initial conditions go through the device's exp, sin, cos and atan,
+which may differ from the host libm in the last bit, so their cell averages
+must agree to 1e-14 relative (a few ulps), and the states after three
+mPDeC5 steps from them to 1e-12 in relative L1 norm."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import problems as P
+import swm
+from oracle_api import Oracle, oracle_initial_data
+from parity_util import describe_diff, same_bits
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(16, 16), (23, 11), (64, 40)]
+RTOL = 1e-14
+
+
+def oracle_data(c):
+    return oracle_initial_data(c.ic, c.nx, c.ny, c.x0, c.x1, c.gravity, c.bathy_flag,
+                               c.wellbalanced)
+
+
+def problem_from_oracle(c):
+    U, Uwb, bx, by = oracle_data(c)
+    p = P.Problem(c.nx, U, wellbalanced=c.wellbalanced, Uwb=Uwb,
+                  dbdx=bx if c.bathy_flag else None, dbdy=by if c.bathy_flag else None,
+                  gravity=c.gravity, cfl=c.cfl, ny=c.ny if c.ny != c.nx else 0,
+                  x0=c.x0[0], x1=c.x1[0], y0=c.x0[1], y1=c.x1[1])
+    assert p.mesh.dx == c.mesh.dx and p.mesh_y.dx == c.mesh_y.dx
+    return p
+
+
+def close(a, b, rtol=RTOL):
+    return np.all(np.abs(a - b) <= rtol * np.maximum(1.0, np.abs(b)))
+
+
+@pytest.mark.parametrize("nx,ny", SIZES)
+@pytest.mark.parametrize("ic", [1, 2, 3, 4, 5, 6])
+def test_initial_state_matches_oracle(ic, nx, ny):
+    c = P.ReferenceSetup(ic, nx, ny, wellbalanced=ic in (2, 3))
+    s = swm.Solver.from_setup(c)
+    U = s.download()
+    U_r = oracle_data(c)[0]
+    if ic in (4, 5):
+        assert same_bits(U, U_r), describe_diff(U, U_r)
+    else:
+        assert close(U, U_r), describe_diff(U, U_r)
+        print(f"ic {ic} {nx}x{ny}: {np.mean(U == U_r):.3f} of the values bitwise")
+
+
+@pytest.mark.parametrize("ic", [2, 3, 6])
+def test_bathymetry_tables_give_the_oracle_source(ic):
+    """The device's Bathymetry_X/_Y tables (flag 1 for ic 2, flag 2 for 3 and
+    6) enter the source term S: the device FVTD of the oracle's state with
+    device tables against the oracle's FVTD with its own tables."""
+    c = P.ReferenceSetup(ic, 40, 24)
+    s = swm.Solver.from_setup(c)
+    p = problem_from_oracle(c)
+    s.upload(p.U0)
+    S = s.fvtd()["S"][..., 1:]
+    S_r = Oracle(p).fvtd()["S"][..., 1:]
+    assert np.abs(S_r).max() > 0
+    scale = np.abs(S_r).max()
+    assert np.all(np.abs(S - S_r) <= 1e-13 * scale), describe_diff(S, S_r)
+
+
+def test_well_balanced_state_keeps_the_lake_at_rest():
+    """For the lake at rest (ic 2) ExactFunction and ExactFunctionWB are the
+    same function, so the device's state equals its well-balanced state:
+    Ut(2:3) = 0 exactly, and steps leave the lake bitwise at rest.  (The
+    well-balanced state of ic 3 enters the steps test below.)"""
+    c = P.ReferenceSetup(2, 32, 20, wellbalanced=True)
+    s = swm.Solver.from_setup(c)
+    U0 = s.download()
+    assert (s.fvtd()["Ut"][..., 1:] == 0).all()
+    s.run_steps(3)
+    assert same_bits(s.download(), U0)
+
+
+@pytest.mark.parametrize("ic,wb", [(1, False), (2, True), (3, True), (4, False), (6, False)])
+def test_steps_from_device_data_match_the_reference_data(ic, wb):
+    """Three mPDeC5 steps from the device's initial data against three
+    oracle steps from the reference's (the oracle's, bitwise equal):
+    relative L1 difference at most 1e-12 (bitwise for the dam break)."""
+    c = P.ReferenceSetup(ic, 40, 24, wellbalanced=wb)
+    s = swm.Solver.from_setup(c)
+    o = Oracle(problem_from_oracle(c))
+    for _ in range(3):
+        dt, _, _ = s.timestep()
+        dt_r, _, _ = o.timestep()
+        if ic == 4:
+            assert dt == dt_r
+        s.step(dt)
+        o.step(dt_r)
+    U, U_r = s.download(), o.get_U()
+    if ic == 4:
+        assert same_bits(U, U_r), describe_diff(U, U_r)
+    else:
+        assert np.abs(U - U_r).sum() <= 1e-12 * np.abs(U_r).sum(), describe_diff(U, U_r)
+
+
+def test_engine_jobs_from_initial_conditions():
+    """Engine jobs submitted as initial conditions (no host data) run as the
+    plain solver does from the same device-made data: bitwise."""
+    setups = [P.ReferenceSetup(1, 30), P.ReferenceSetup(2, 24, 16, wellbalanced=True),
+              P.ReferenceSetup(3, 40, 16, wellbalanced=True), P.ReferenceSetup(4, 32),
+              P.ReferenceSetup(5, 20, 28), P.ReferenceSetup(6, 36, 12)]
+    eng = swm.Engine()
+    ids = [eng.submit_setup(c, 2) for c in setups]
+    while eng.step():
+        pass
+    for c, jid in zip(setups, ids):
+        s = swm.Solver.from_setup(c)
+        s.run_steps(2)
+        assert eng.info(jid)["steps_done"] == 2, c.ic
+        assert same_bits(eng.fetch(jid), s.download()), c.ic
+
+
+def test_bad_arguments_are_rejected():
+    c = P.ReferenceSetup(1, 16)
+    s = swm.Solver.from_setup(c)
+    for ic, bf, x0, x1, what in [(0, 0, c.x0, c.x1, "initial condition"),
+                                 (7, 0, c.x0, c.x1, "initial condition"),
+                                 (1, 3, c.x0, c.x1, "bathymetry flag"),
+                                 (1, 0, (0.0, 0.0), (2.0, 3.0), "Mesh_DX")]:
+        with pytest.raises(swm.SwmError, match=what):
+            s.fill_initial_data(ic, bf, x0, x1)
+    w = swm.Solver.from_setup(P.ReferenceSetup(2, 16, wellbalanced=True))
+    with pytest.raises(swm.SwmError, match="well-balanced"):
+        w.fill_initial_data(1, 0, (0.0, 0.0), (1.0, 1.0))
+
+
+def test_large_grids():
+    """2048^2 lake at rest (ic 2, well-balanced): at rest bitwise after two
+    steps; 1024^2 dam break (ic 4) bitwise the oracle's initial data."""
+    c = P.ReferenceSetup(2, 2048, wellbalanced=True)
+    s = swm.Solver.from_setup(c)
+    U0 = s.download()
+    s.run_steps(2)
+    assert same_bits(s.download(), U0)
+    del s
+    c = P.ReferenceSetup(4, 1024)
+    s = swm.Solver.from_setup(c)
+    assert same_bits(s.download(), oracle_data(c)[0])

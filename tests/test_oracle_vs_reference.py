@@ -6,7 +6,8 @@ import numpy as np
 import pytest
 
 import problems as P
-from oracle_api import Oracle, have_swref, have_swref_explicit, run_swref
+from oracle_api import (Oracle, have_swref, have_swref_explicit, oracle_initial_data, run_swref,
+                        run_swref_initial_data)
 from parity_util import FVTD_KEYS, describe_diff, same_bits
 
 pytestmark = pytest.mark.skipif(not have_swref(), reason="oracle/_ref not built")
@@ -291,3 +292,45 @@ def test_reference_fix_does_not_conserve_mass():
     for recfix in (1, 2):
         _, U = run_swref(p, 2, steps=3, recfix=recfix)
         assert abs(U[..., 0].sum() - m0) > 1e-10 * m0
+
+
+# ---- initial data: ExactFunction, ExactFunctionWB, Bathymetry_X/_Y ----------
+# The reference's own setups (src/parameters.f90:62-125): domain, gravity and
+# BathymetryFlag per InitialCondition.
+IC_SETUPS = {1: ((0., 0.), (3., 3.), 9.81, 0), 2: ((0., 0.), (1., 1.), 9.81, 1),
+             3: ((-5., -2.), (5., 2.), 9.8, 2), 4: ((0., 0.), (40., 40.), 9.8, 0),
+             5: ((0., 0.), (50., 50.), 9.8, 0), 6: ((-5., -2.), (5., 2.), 9.8, 2)}
+
+
+@pytest.mark.parametrize("ic", sorted(IC_SETUPS))
+@pytest.mark.parametrize("nx,ny", [(16, 16), (23, 11), (64, 40)])
+def test_initial_data_bitwise(ic, nx, ny):
+    """FillInitialConditions (and InitializeWBVariables' averages for the
+    lakes 2 and 3) of the reference against the oracle's restatement."""
+    x0, x1, g, bf = IC_SETUPS[ic]
+    wb = ic in (2, 3)
+    U, Uwb = run_swref_initial_data(ic, nx, ny, x0, x1, g, bf, wb)
+    Uo, Uwbo, _, _ = oracle_initial_data(ic, nx, ny, x0, x1, g, bf, wb)
+    assert same_bits(Uo, U), describe_diff(Uo, U)
+    if wb:
+        assert same_bits(Uwbo, Uwb), describe_diff(Uwbo, Uwb)
+
+
+@pytest.mark.parametrize("bflag,x0,x1", [(1, 0.0, 1.0), (2, -2.0, 2.0), (2, -1.5, 0.5)])
+def test_bathymetry_tables_give_the_reference_source(bflag, x0, x1):
+    """The oracle's Bathymetry_X/_Y tables at the Gauss points, fed to the
+    FVTD, give the source term S the reference computes with its own
+    Bathymetry_X/_Y (bitwise), on a rough state over the domain."""
+    n = 20
+    _, _, bx, by = oracle_initial_data(2, n, n, (x0, x0), (x1, x1), 9.81, bflag, False)
+    rng = np.random.default_rng(bflag)
+    U = np.empty((n, n, 3))
+    U[..., 0] = 1.0 + 0.3 * rng.uniform(-1, 1, (n, n))
+    U[..., 1] = U[..., 0] * rng.normal(0, 0.3, (n, n))
+    U[..., 2] = U[..., 0] * rng.normal(0, 0.3, (n, n))
+    p = P.Problem(n, U, kind="random", dbdx=bx, dbdy=by, x0=x0, x1=x1)
+    ref = run_swref(p, 1, bathy_flag=bflag)
+    got = Oracle(p).fvtd()
+    assert np.abs(ref["S"][..., 1:]).max() > 0
+    assert same_bits(got["S"][..., 1:], ref["S"][..., 1:]), describe_diff(got["S"], ref["S"])
+    assert same_bits(got["Ut"][..., 1:], ref["Ut"][..., 1:])
