@@ -224,6 +224,59 @@ def recfix_section(n: int, steps: int, cpu: bool) -> dict:
     return res
 
 
+def initial_data_section(n: int, cpu: bool) -> dict:
+    """The reference's initial data made on the device (swm_fill_initial_data)
+    for n x n cells of each initial condition's own domain: the fill of the
+    state, the bathymetry tables and, for the well-balanced lakes 2 and 3,
+    the well-balanced state and its FVTD.  CPU: the reference's own
+    FillInitialConditions + InitializeWBVariables (its WELLBALANCED build,
+    driver mode 6) for the lake at rest at 256 x 256, wall time of the
+    process."""
+    import problems as P
+    import swm
+
+    res = {"cells": n * n}
+    for ic in (1, 2, 3, 4):
+        c = P.ReferenceSetup(ic, n, wellbalanced=ic in (2, 3))
+        s = swm.Solver(c.nx, c.ny, c.mesh.dx, c.mesh_y.dx, c.gravity, c.cfl, c.wellbalanced,
+                       c.mesh.wbnd)
+        s.fill_initial_data(c.ic, c.bathy_flag, c.x0, c.x1)  # first call: allocations
+        s.sync()
+        t0 = time.perf_counter()
+        s.fill_initial_data(c.ic, c.bathy_flag, c.x0, c.x1)
+        s.sync()
+        el = time.perf_counter() - t0
+        res[f"ic{ic}"] = {"ms": 1e3 * el, "cells_per_s": n * n / el,
+                          "wellbalanced": bool(c.wellbalanced)}
+        log(f"bench: initial data ic {ic} {n}x{n}: {1e3 * el:.2f} ms")
+        del s
+    exe = ROOT / "oracle" / "_ref" / "swref_pwb_fast"
+    if cpu and exe.exists():
+        sys.path.insert(0, str(ROOT / "tests"))
+        from oracle_api import big_stack
+
+        m = 256
+        x0, x1, g, bf = P.REFERENCE_SETUPS[2]
+        hdr = np.array([6, m, 2, bf, 1, m, 5, 4], dtype="<i4")
+        fl = np.array([x0[0], x1[0], g, 0.5, x0[1], x1[1]], dtype="<f8")
+        with tempfile.TemporaryDirectory() as td:
+            fin, fout = Path(td) / "in.bin", Path(td) / "out.bin"
+            with open(fin, "wb") as f:
+                f.write(hdr.tobytes())
+                f.write(fl.tobytes())
+            t0 = time.perf_counter()
+            subprocess.run([str(exe), str(fin), str(fout)], check=True, cwd=td,
+                           stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                           preexec_fn=big_stack, timeout=600)
+            el = time.perf_counter() - t0
+        res["cpu_reference_ic2"] = {"cells": m * m, "s": el, "cells_per_s": m * m / el,
+                                    "cores": 1, "kind": "reference",
+                                    "sample": "amdflang -O3 build, WELLBALANCED, "
+                                              "process wall time incl. start-up"}
+        res["ic2_gpu_vs_cpu"] = res["ic2"]["cells_per_s"] / res["cpu_reference_ic2"]["cells_per_s"]
+    return res
+
+
 def batched_section(count: int, seed: int) -> dict:
     """`count` uneven solves (N = round(exp(U[ln 32, ln 512])), 20..200 CFL
     steps, half vortices, half well-balanced lakes over bumps) run together
@@ -322,6 +375,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--scheme-steps", type=int, default=20,
                     help="steps of each other time scheme at the headline size (0: skip)")
+    ap.add_argument("--initial-data", type=int, default=1,
+                    help="1: time the initial data made on the device at the headline size")
     ap.add_argument("--fix-steps", type=int, default=10,
                     help="steps of each ReconstructionFix run on a dam break at the headline "
                          "size (0: skip)")
@@ -436,6 +491,8 @@ def main():
     del s
     if args.scheme_steps > 0:
         out["schemes"] = schemes_section(n, args.seed, args.scheme_steps, not args.no_cpu)
+    if args.initial_data:
+        out["initial_data"] = initial_data_section(n, not args.no_cpu)
     if args.fix_steps > 0:
         out["reconstruction_fix"] = recfix_section(n, args.fix_steps, not args.no_cpu)
     if args.batch_jobs > 0:

@@ -17,7 +17,7 @@ INTEGER(C_INT), PARAMETER, PUBLIC :: SWM_OK = 0
 PUBLIC :: swm_create, swm_destroy, swm_set_bathy_grad, swm_init_wb
 PUBLIC :: swm_upload, swm_download, swm_timestep, swm_step_mpdec5, swm_advance
 PUBLIC :: swm_step_mpeuler, swm_step_mpdec2, swm_set_timescheme, swm_set_hp_association
-PUBLIC :: swm_set_reconstruction_fix
+PUBLIC :: swm_set_reconstruction_fix, swm_fill_initial_data
 PUBLIC :: swm_step_forward_euler, swm_step_ssprk4, swm_step_rk65, swm_step_dec5
 PUBLIC :: swm_error_message, swm_check
 
@@ -132,6 +132,17 @@ INTERFACE
     IMPORT :: C_INT, C_PTR
     TYPE(C_PTR), VALUE :: h
     INTEGER(C_INT), VALUE :: mode
+  END FUNCTION
+
+  ! FillInitialConditions, InitializeWBVariables and the bathymetry tables
+  ! (src/main.f90:22-26) on the device: InitialCondition, BathymetryFlag and
+  ! MESH_X0, MESH_X1; replaces swm_set_bathy_grad + swm_init_wb + swm_upload
+  INTEGER(C_INT) FUNCTION swm_fill_initial_data(h, ic, bflag, x0, x1) &
+      BIND(C, NAME='swm_fill_initial_data')
+    IMPORT :: C_INT, C_DOUBLE, C_PTR
+    TYPE(C_PTR), VALUE :: h
+    INTEGER(C_INT), VALUE :: ic, bflag
+    REAL(C_DOUBLE), INTENT(IN) :: x0(2), x1(2)
   END FUNCTION
 
   ! ReconstructionFix (src/parameters.f90:142): 4 as the reference, 1 first

@@ -21,7 +21,7 @@ sys.path.insert(0, str(ROOT / "sw-mpdec_amd"))
 sys.path.insert(0, str(ROOT / "tests"))
 
 import problems as P  # noqa: E402
-from oracle_api import run_swref  # noqa: E402
+from oracle_api import run_swref, run_swref_initial_data  # noqa: E402
 
 
 def problem_arrays(p):
@@ -65,6 +65,11 @@ CASES = {
 }
 
 
+# The reference's initial data (driver mode 6): InitialCondition, nx, ny on
+# the reference's own domain for that initial condition.
+INITIAL_DATA = [(1, 23, 11), (2, 16, 16), (3, 20, 8), (6, 24, 10)]
+
+
 def main(only=None):
     for name, (mk, steps, *rest) in CASES.items():
         if only and name not in only:
@@ -85,6 +90,19 @@ def main(only=None):
             out["steps_hist"] = np.array(hist)
             out["steps_U"] = U if U is not None else np.zeros(0)
         out["ts"] = np.array(run_swref(p, 3, recfix=recfix))
+        np.savez_compressed(HERE / f"{name}.npz", **out)
+        print(f"wrote {name}.npz")
+    for ic, nx, ny in INITIAL_DATA:
+        name = f"initial_ic{ic}_{nx}x{ny}"
+        if only and name not in only:
+            continue
+        x0, x1, g, bf = P.REFERENCE_SETUPS[ic]
+        wb = ic in (2, 3)
+        U, Uwb = run_swref_initial_data(ic, nx, ny, x0, x1, g, bf, wb)
+        out = dict(ic=ic, nx=nx, ny=ny, x0=np.array(x0), x1=np.array(x1), gravity=g,
+                   bathy_flag=bf, U=U)
+        if wb:
+            out["Uwb"] = Uwb
         np.savez_compressed(HERE / f"{name}.npz", **out)
         print(f"wrote {name}.npz")
     if only:

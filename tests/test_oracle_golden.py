@@ -9,11 +9,13 @@ import numpy as np
 import pytest
 
 import problems as P
-from oracle_api import Oracle
+from oracle_api import Oracle, oracle_initial_data
 from parity_util import FVTD_KEYS, describe_diff, same_bits
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
-FILES = sorted(f for f in GOLDEN.glob("*.npz") if not f.name.startswith("jacobi"))
+FILES = sorted(f for f in GOLDEN.glob("*.npz")
+               if not f.name.startswith(("jacobi", "initial_")))
+INITIAL = sorted(GOLDEN.glob("initial_*.npz"))
 
 
 def recfix(z):
@@ -94,3 +96,16 @@ def test_jacobi_matches_reference():
     sol, k = o.jacobi(z["mass"], z["diag"], z["rhs"])
     assert k > 1
     assert same_bits(sol, z["sol"])
+
+
+@pytest.mark.parametrize("f", INITIAL, ids=lambda f: f.stem)
+def test_initial_data_matches_reference_fixture(f):
+    """The reference's FillInitialConditions (and InitializeWBVariables'
+    averages) as its own build wrote them."""
+    z = np.load(f)
+    wb = "Uwb" in z
+    U, Uwb, _, _ = oracle_initial_data(int(z["ic"]), int(z["nx"]), int(z["ny"]), z["x0"], z["x1"],
+                                       float(z["gravity"]), int(z["bathy_flag"]), wb)
+    assert same_bits(U, z["U"]), describe_diff(U, z["U"])
+    if wb:
+        assert same_bits(Uwb, z["Uwb"]), describe_diff(Uwb, z["Uwb"])
