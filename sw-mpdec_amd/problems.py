@@ -236,6 +236,14 @@ class ReferenceSetup:
         self.mesh = Mesh(self.nx, self.x0[0], self.x1[0])
         self.mesh_y = Mesh(self.ny, self.x0[1], self.x1[1])
 
+    @property
+    def n(self) -> int:
+        return self.nx
+
+    @property
+    def rows(self) -> int:
+        return self.ny
+
 
 def vortex(n: int, seed: int, steps: int = 50, cfl: float = 0.5, ny: int = 0) -> Problem:
     """Smooth vortex on the periodic unit square, flat bottom:

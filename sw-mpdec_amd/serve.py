@@ -31,7 +31,7 @@ import swm
 class Request:
     rid: int
     arrival: float           # seconds after the stream starts
-    problem: P.Problem
+    problem: P.Problem | P.ReferenceSetup  # host data, or an initial condition made on the device
     nsteps: int
     admitted: float = math.nan
     done: float = math.nan
@@ -90,6 +90,36 @@ def run_batch(problems, nsteps, budget_bytes: int = 0, profile: bool = False):
     return res, t1 - t0, steps, prof, states
 
 
+def reference_mix(count: int, seed: int, rate: float, nmin: int = 64, nmax: int = 1024,
+                  smin: int = 50, smax: int = 500) -> list[Request]:
+    """Like request_mix, but each request names one of the reference's own
+    initial conditions (1-6, the lakes 2 and 3 well-balanced) and carries no
+    host data: its initial data are made on the device at admission."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for k in range(count):
+        t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
+        n = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
+        steps = int(rng.integers(smin, smax + 1))
+        ic = int(rng.integers(1, 7))
+        out.append(Request(k, t, P.ReferenceSetup(ic, n, wellbalanced=ic in (2, 3), steps=steps),
+                           steps))
+    return out
+
+
+def _submit(eng, r: Request):
+    if isinstance(r.problem, P.ReferenceSetup):
+        return eng.submit_setup(r.problem, r.nsteps)
+    return eng.submit(r.problem, r.nsteps)
+
+
+def _solver(r: Request):
+    if isinstance(r.problem, P.ReferenceSetup):
+        return swm.Solver.from_setup(r.problem)
+    return swm.Solver.from_problem(r.problem)
+
+
 def work(r: Request) -> int:
     """Cell-steps of a request."""
     return r.problem.n * r.problem.rows * r.nsteps
@@ -130,7 +160,7 @@ def serve(requests: list[Request], budget_bytes: int, keep_results: bool = True,
             c = r.problem.n * r.problem.rows
             if policy == "sjf" and max_running_cells and running and cells + c > max_running_cells:
                 break
-            jid = eng.submit(r.problem, r.nsteps)
+            jid = _submit(eng, r)
             if jid is None:
                 if not running:
                     raise RuntimeError(f"request {r.rid} alone exceeds the budget")
@@ -184,7 +214,7 @@ def sequential(requests: list[Request], keep_results: bool = True) -> dict:
         while now() < r.arrival:
             time.sleep(min(r.arrival - now(), 0.05))
         ts = time.perf_counter()
-        s = swm.Solver.from_problem(r.problem)
+        s = _solver(r)
         state = swm.JOB_DONE
         try:
             s.run_steps(r.nsteps)

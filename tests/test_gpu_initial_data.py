@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 
 import problems as P
+import serve
 import swm
 from oracle_api import Oracle, oracle_initial_data
 from parity_util import describe_diff, same_bits
@@ -174,3 +175,19 @@ def test_large_grids():
     c = P.ReferenceSetup(4, 1024)
     s = swm.Solver.from_setup(c)
     assert same_bits(s.download(), oracle_data(c)[0])
+
+
+def test_served_requests_by_initial_condition():
+    """The continuous-batching loop with requests that carry no host data
+    (the reference's initial conditions, made on the device at admission):
+    every result bitwise the sequential plain solver's."""
+    reqs = serve.reference_mix(10, 5, rate=0.0, nmin=24, nmax=96, smin=2, smax=6)
+    res = serve.serve(reqs, budget_bytes=0)
+    got = [r.result for r in reqs]
+    states = [r.state for r in reqs]
+    seq_reqs = serve.reference_mix(10, 5, rate=0.0, nmin=24, nmax=96, smin=2, smax=6)
+    serve.sequential(seq_reqs)
+    assert res["jobs"] == 10
+    for a, r, st in zip(got, seq_reqs, states):
+        assert st == r.state, r.problem.ic
+        assert same_bits(a, r.result), (r.problem.ic, r.problem.nx)
