@@ -86,6 +86,23 @@ int swm_init_wb(swm_solver_t *h, const double *Uwb, int ghosts);
 int swm_fill_initial_data(swm_solver_t *h, int initial_condition, int bathymetry_flag,
                           const double x0[2], const double x1[2]);
 
+/* PostProcessing (src/postprocessing.f90:24-64) on the device: the L1 error
+ * of the current state against the 4x4-Gauss cell averages of
+ * ExactFunction(initial_condition, t), err[v] = (sum over cells of
+ * |U(v) - Uexact(v)|) * dx * dy, summed in the reference's cell order, on the
+ * domain of swm_fill_initial_data.  Bitwise the reference's for the dam
+ * breaks (4, 5); for the others ExactFunction's exp/sin/cos/atan are the
+ * device library's (a few ulps). */
+int swm_l1_error(swm_solver_t *h, int initial_condition, int bathymetry_flag, double t,
+                 const double x0[2], const double x1[2], double err[3]);
+
+/* The primitive variables (h, u, v) = ConsToPrim(U) (src/equation.f90:
+ * 912-945) of the current state in the (1:3, nx, ny) layout, for output.
+ * The reference's writer (WriteSolutionToDisk, src/output.f90:29-190) prints
+ * the module array V, i.e. the primitives of the last FVTD input (a DeC node
+ * of the step just taken), not of the state: this gives those of the state. */
+int swm_download_primitive(swm_solver_t *h, double *V);
+
 /* Copy the state U into / out of the device (the module array U). */
 int swm_upload(swm_solver_t *h, const double *U, int ghosts);
 int swm_download(swm_solver_t *h, double *U);

@@ -17,6 +17,8 @@
 !   mode 6: the initial data (InitializeWBVariables' well-balanced averages in
 !           WELLBALANCED builds, then FillInitialConditions) for
 !           InitialCondition = nsteps; no state is read
+!   mode 7: PostProcessing of the given state for InitialCondition = nsteps at
+!           tEnd = t (its ErrorL1_nx_ny.dat in the working directory)
 ! With WELLBALANCED builds the well-balanced reference state is installed the
 ! way InitializeWBVariables does it (src/finitevolume.f90:373-392).
 !
@@ -29,7 +31,8 @@
 !         timescheme (0: 5, mPDeC5; src/parameters.f90:144 numbering),
 !         reconstruction_fix (0: 4 = Reconstruction, as src/parameters.f90:142)
 !   real64 x0, x1, gravity, cfl    (the domain is [x0,x1]^2; dy = dx*nx/ny)
-!   real64 y0, y1                  [mode 6 only: the domain is [x0,x1] x [y0,y1]]
+!   real64 y0, y1                  [modes 6, 7: the domain is [x0,x1] x [y0,y1]]
+!   real64 t                       [mode 7: tEnd]
 !   real64 U(3,nx,ny)              [mode 1-3]   or  Mass(5nx*ny), Diag(nx*ny), rhs(nx*ny) [mode 4]
 !   real64 Uwb(3,nx,ny)            [if has_wb]
 !===============================================================================!
@@ -38,6 +41,7 @@ USE MOD_FiniteVolume2D_vars
 USE MOD_FiniteVolume2D,     ONLY: InitializeFiniteVolume
 USE MOD_FiniteVolume2D,     ONLY: FVTimeDerivative
 USE MOD_FiniteVolume2D,     ONLY: FillInitialConditions
+USE MOD_PostProcessing,     ONLY: PostProcessing
 #ifdef WELLBALANCED
 USE MOD_FiniteVolume2D,     ONLY: InitializeWBVariables
 #endif
@@ -50,7 +54,7 @@ USE MOD_JacobiIteration,    ONLY: jacobi
 IMPLICIT NONE
 INTEGER            :: mode, nx, ny, nsteps, bflag, haswb, k, nrow, scheme, recfix
 INTEGER(KIND=8)    :: c0, c1, crate
-REAL               :: x0, x1, y0, y1, grav, cflin, dtmin
+REAL               :: x0, x1, y0, y1, grav, cflin, dtmin, tpost
 REAL, ALLOCATABLE  :: Uic(:,:,:), Uwb(:,:,:), Mass(:), Diag(:), rhs(:), sol(:)
 CHARACTER(LEN=1024):: fin, fout
 
@@ -65,7 +69,9 @@ IF (recfix .LE. 0) recfix = 4
 READ(10) x0, x1, grav, cflin
 y0 = x0
 y1 = x1
-IF (mode .EQ. 6) READ(10) y0, y1
+tpost = 0.
+IF (mode .EQ. 6 .OR. mode .EQ. 7) READ(10) y0, y1
+IF (mode .EQ. 7) READ(10) tpost
 
 #ifdef WELLBALANCED
 IF (haswb .NE. 1) THEN
@@ -177,6 +183,13 @@ SELECT CASE (mode)
   CASE (3)
     dtmin = TimeStep()
     WRITE(11) dtmin, LambdaMaxX, LambdaMaxY
+  CASE (7)
+    ! PostProcessing (src/postprocessing.f90:24-64) of the given state at
+    ! tEnd = tpost for InitialCondition = nsteps: it writes ErrorL1_nx_ny.dat
+    ! into the working directory.
+    InitialCondition = nsteps
+    tEnd = tpost
+    CALL PostProcessing()
   CASE (5)
     CALL SYSTEM_CLOCK(c0, crate)
     DO k = 1, nsteps

@@ -611,7 +611,7 @@ static void fv_time_derivative(orc_t *o) {
  *      src/equation.f90:70-509, src/finitevolume.f90:281-396) ------------------ */
 typedef struct {
   int ic, bflag;
-  double x0[2], sx[2], g, pi;
+  double x0[2], sx[2], g, pi, t;
 } exact_ctx;
 
 /* Bathymetry, src/equation.f90:419-446 */
@@ -662,15 +662,16 @@ static double fmodulo(double a, double p) {
   return m;
 }
 
-/* ExactFunction at t = 0, src/equation.f90:70-211 (conservative variables) */
+/* ExactFunction at time c->t, src/equation.f90:70-211 (conservative
+ * variables); only the vortex moves */
 static int exact_function(const exact_ctx *c, double x, double y, double *cons) {
   double h = 0., u = 0., v = 0.;
   switch (c->ic) {
     case 1: { /* unsteady smooth vortex */
       const double u_inf = 2., v_inf = 3., h_inf = 1.;
       double xm = c->x0[0] + 0.5 * c->sx[0], ym = c->x0[1] + 0.5 * c->sx[1];
-      double xc = fmodulo(x - u_inf * 0.0 - c->x0[0], c->sx[0]) + c->x0[0] - xm;
-      double yc = fmodulo(y - v_inf * 0.0 - c->x0[1], c->sx[1]) + c->x0[1] - ym;
+      double xc = fmodulo(x - u_inf * c->t - c->x0[0], c->sx[0]) + c->x0[0] - xm;
+      double yc = fmodulo(y - v_inf * c->t - c->x0[1], c->sx[1]) + c->x0[1] - ym;
       double r = xc * xc + yc * yc;
       double omega = sqrt(2. * c->g * h_deriv_smooth(r));
       h = h_inf;
@@ -738,7 +739,7 @@ int orc_initial_data(int ic, int bathy_flag, int nx, int ny, const double x0[2],
                      double *dbdy) {
   if (nx < 1 || ny < 1 || ic < 1 || ic > 6) return -1;
   if (Uwb && ic != 2 && ic != 3) return -1;
-  exact_ctx c = {ic, bathy_flag, {x0[0], x0[1]}, {0., 0.}, gravity, acos(-1.0)};
+  exact_ctx c = {ic, bathy_flag, {x0[0], x0[1]}, {0., 0.}, gravity, acos(-1.0), 0.0};
   double dxy[2];
   const int nel[2] = {nx, ny};
   for (int d = 0; d < 2; ++d) { /* src/mesh.f90:39-41 */
@@ -781,6 +782,43 @@ int orc_initial_data(int ic, int bathy_flag, int nx, int ny, const double x0[2],
           if (dbdy) dbdy[(size_t)(yg * 4 + xg) * n + k] = bathymetry_y(&c, x, y);
         }
     }
+  return 0;
+}
+
+/* PostProcessing, src/postprocessing.f90:24-64: the L1 error of U (3, nx, ny)
+ * against the cell averages of ExactFunction(ic, t), summed over jj (outer)
+ * and ii (inner), times MESH_DX(1)*MESH_DX(2). */
+int orc_l1_error(int ic, int bathy_flag, int nx, int ny, const double x0[2], const double x1[2],
+                 double gravity, double t, const double *U, double err[3]) {
+  if (nx < 1 || ny < 1 || ic < 1 || ic > 6) return -1;
+  exact_ctx c = {ic, bathy_flag, {x0[0], x0[1]}, {0., 0.}, gravity, acos(-1.0), t};
+  double dxy[2];
+  const int nel[2] = {nx, ny};
+  for (int d = 0; d < 2; ++d) {
+    c.sx[d] = fabs(x1[d] - x0[d]);
+    dxy[d] = fabs(c.sx[d]) / (double)nel[d];
+  }
+  const double q1 = 0.5 * sqrt(3. / 7. + 2. / 7. * sqrt(6. / 5.));
+  const double q2 = 0.5 * sqrt(3. / 7. - 2. / 7. * sqrt(6. / 5.));
+  const double node[4] = {-q1, -q2, q2, q1};
+  const double wlo = (18. - sqrt(30.)) / 72., whi = (18. + sqrt(30.)) / 72.;
+  const double wq[4] = {wlo, whi, whi, wlo};
+  double l1[3] = {0., 0., 0.};
+  for (int jj = 1; jj <= ny; ++jj)
+    for (int ii = 1; ii <= nx; ++ii) {
+      double bx = (x0[0] + (double)(ii - 1) * dxy[0]) + 0.5 * dxy[0];
+      double by = (x0[1] + (double)(jj - 1) * dxy[1]) + 0.5 * dxy[1];
+      size_t k = (size_t)(jj - 1) * nx + (ii - 1);
+      double ue[3] = {0., 0., 0.};
+      for (int ig = 0; ig < 4; ++ig)
+        for (int jg = 0; jg < 4; ++jg) {
+          double tt[3];
+          exact_function(&c, bx + node[ig] * dxy[0], by + node[jg] * dxy[1], tt);
+          for (int v = 0; v < 3; ++v) ue[v] = ue[v] + wq[ig] * wq[jg] * tt[v];
+        }
+      for (int v = 0; v < 3; ++v) l1[v] = l1[v] + fabs(U[3 * k + v] - ue[v]);
+    }
+  for (int v = 0; v < 3; ++v) err[v] = l1[v] * dxy[0] * dxy[1];
   return 0;
 }
 

@@ -90,6 +90,13 @@ int orc_initial_data(int ic, int bathy_flag, int nx, int ny, const double x0[2],
                      const double x1[2], double gravity, double *U, double *Uwb, double *dbdx,
                      double *dbdy);
 
+/* PostProcessing (src/postprocessing.f90:24-64): the L1 error of U (3, nx, ny)
+ * against the 4x4-Gauss cell averages of ExactFunction(ic, t) on the domain
+ * of orc_initial_data, err[v] = sum |U - Uexact| * dx * dy.  Returns 0, or -1
+ * for an unknown ic. */
+int orc_l1_error(int ic, int bathy_flag, int nx, int ny, const double x0[2], const double x1[2],
+                 double gravity, double t, const double *U, double err[3]);
+
 /* Jacobi on the reference CSR pattern (src/jacobi_iteration.f90:52-87).
  * Mass (5n), diag (n), rhs (n) -> sol (n).  Returns the sweep count. */
 int orc_jacobi(const orc_t *o, const double *Mass, const double *diag,

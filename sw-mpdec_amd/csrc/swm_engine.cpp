@@ -1104,16 +1104,14 @@ int job_set_bathy(swm_engine* e, int id, const double* bx, const double* by) {
 // flat bottom) and, for a well-balanced job, the well-balanced state and its
 // FVTD.  The domain [x0[0], x1[0]] x [x0[1], x1[1]] must give the job's
 // Mesh_DX (src/mesh.f90:39-41) bitwise.
-int fill_initial_data(swm_engine* e, int id, int ic, int bflag, const double* x0,
-                      const double* x1) {
-  Job& J = *e->jobs[id];
+// ExactFunction's setup for a job: checks the arguments and that the domain
+// gives the job's Mesh_DX bitwise.
+int exact_setup(swm_engine* e, const Job& J, int ic, int bflag, const double* x0,
+                const double* x1, double t, InitialSetup& c) {
   if (ic < 1 || ic > 6) FAIL(e, SWM_E_ARG, "initial condition %d is not one of 1..6", ic);
   if (bflag < 0 || bflag > 2) FAIL(e, SWM_E_ARG, "bathymetry flag %d is not one of 0..2", bflag);
   if (!x0 || !x1) FAIL(e, SWM_E_ARG, "domain corners are NULL");
-  if (J.dev.wb && ic != 2 && ic != 3)
-    FAIL(e, SWM_E_ARG,
-         "initial condition %d has no well-balanced state (ExactFunctionWB covers 2 and 3)", ic);
-  InitialSetup c{};
+  c = InitialSetup{};
   c.ic = ic;
   c.bflag = bflag;
   const int nel[2] = {J.dev.nx, J.dev.ny};
@@ -1135,6 +1133,19 @@ int fill_initial_data(swm_engine* e, int id, int ic, int bflag, const double* x0
     c.node[g] = nodes[g];
     c.w[g] = J.dev.wbnd[g];  // WeightsGP(i,j) = quadWeights1D(i)*quadWeights1D(j)
   }
+  c.t = t;
+  return SWM_OK;
+}
+
+int fill_initial_data(swm_engine* e, int id, int ic, int bflag, const double* x0,
+                      const double* x1) {
+  Job& J = *e->jobs[id];
+  InitialSetup c;
+  int rc0 = exact_setup(e, J, ic, bflag, x0, x1, 0.0, c);
+  if (rc0) return rc0;
+  if (J.dev.wb && ic != 2 && ic != 3)
+    FAIL(e, SWM_E_ARG,
+         "initial condition %d has no well-balanced state (ExactFunctionWB covers 2 and 3)", ic);
   if (bflag) {
     int rc = ensure_bathy(e, J);
     if (rc) return rc;
@@ -1541,6 +1552,47 @@ int swm_fill_initial_data(swm_solver_t* s, int initial_condition, int bathymetry
                           const double x0[2], const double x1[2]) {
   if (!s) return SWM_E_ARG;
   return fill_initial_data(s->e, s->job, initial_condition, bathymetry_flag, x0, x1);
+}
+
+int swm_l1_error(swm_solver_t* s, int initial_condition, int bathymetry_flag, double t,
+                 const double x0[2], const double x1[2], double err[3]) {
+  if (!s) return SWM_E_ARG;
+  swm_engine* e = s->e;
+  if (!err) FAIL(e, SWM_E_ARG, "err is NULL");
+  Job& J = *plain_job(s);
+  InitialSetup c;
+  int rc = exact_setup(e, J, initial_condition, bathymetry_flag, x0, x1, t, c);
+  if (rc) return rc;
+  const size_t n = J.lay.n;
+  double* tmp = nullptr;
+  HIPCK(e, hipMalloc((void**)&tmp, (3 * n + 3) * sizeof(double)));
+  launch_l1_error(c, J.dev.nx, J.dev.ny, J.dev.h, J.dev.hu, J.dev.hv, tmp, tmp + 3 * n,
+                  e->stream);
+  hipError_t er = hipGetLastError();
+  if (er == hipSuccess)
+    er = hipMemcpyAsync(err, tmp + 3 * n, 3 * sizeof(double), hipMemcpyDeviceToHost, e->stream);
+  if (er == hipSuccess) er = hipStreamSynchronize(e->stream);
+  (void)hipFree(tmp);
+  if (er != hipSuccess) FAIL(e, SWM_E_HIP, "L1 error failed: %s", hipGetErrorString(er));
+  return SWM_OK;
+}
+
+int swm_download_primitive(swm_solver_t* s, double* V) {
+  if (!s) return SWM_E_ARG;
+  swm_engine* e = s->e;
+  if (!V) FAIL(e, SWM_E_ARG, "V is NULL");
+  Job& J = *plain_job(s);
+  const size_t n = J.lay.n;
+  double* tmp = nullptr;
+  HIPCK(e, hipMalloc((void**)&tmp, 3 * n * sizeof(double)));
+  launch_primitives(n, J.dev.h, J.dev.hu, J.dev.hv, tmp, e->stream);
+  hipError_t er = hipGetLastError();
+  if (er == hipSuccess)
+    er = hipMemcpyAsync(V, tmp, 3 * n * sizeof(double), hipMemcpyDeviceToHost, e->stream);
+  if (er == hipSuccess) er = hipStreamSynchronize(e->stream);
+  (void)hipFree(tmp);
+  if (er != hipSuccess) FAIL(e, SWM_E_HIP, "primitive download failed: %s", hipGetErrorString(er));
+  return SWM_OK;
 }
 
 int swm_upload(swm_solver_t* s, const double* U, int ghosts) {

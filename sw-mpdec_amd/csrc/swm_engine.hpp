@@ -188,7 +188,7 @@ inline const SchemeDef* find_scheme(int timescheme) {
 // Gauss rule of BuildMesh (quadNodes1D, quadWeights1D; src/mesh.f90:127-132).
 struct InitialSetup {
   int ic, bflag;
-  double x0[2], sx[2], dx[2], g, pi;
+  double x0[2], sx[2], dx[2], g, pi, t;  // t: time of ExactFunction
   double node[4], w[4];
 };
 
@@ -197,6 +197,13 @@ struct InitialSetup {
 void launch_initial_data(const InitialSetup& c, int nx, int ny, double* h, double* hu, double* hv,
                          double* hwb, double* huwb, double* hvwb, double* bx, double* by,
                          hipStream_t s);
+// PostProcessing's L1 error of the state against ExactFunction(c.ic, c.t)
+// into err[3] (device); scratch: 3*nx*ny doubles.
+void launch_l1_error(const InitialSetup& c, int nx, int ny, const double* h, const double* hu,
+                     const double* hv, double* scratch, double* err, hipStream_t s);
+// ConsToPrim of n cells into (1:3, n) order.
+void launch_primitives(size_t n, const double* h, const double* hu, const double* hv, double* V,
+                       hipStream_t s);
 
 // ---- launches (swm_kernels.hip) -------------------------------------------
 // xt / yt: face tiles of shape kShapes[shape]; ct: k_fvtd_update tiles.

@@ -1,8 +1,11 @@
-// swm_initial.hip -- the reference's initial data on the device: the cell
-// averages of ExactFunction (FillInitialConditions, src/finitevolume.f90:281-333)
-// and of ExactFunctionWB (InitializeWBVariables, :339-396) over the 4x4 Gauss
-// points, and Bathymetry_X/_Y at every Gauss point (src/equation.f90:419-509),
-// written straight into a job's device arrays.
+// swm_initial.hip -- the steps before and after the time loop on the device:
+// the reference's initial data -- the cell averages of ExactFunction
+// (FillInitialConditions, src/finitevolume.f90:281-333) and of ExactFunctionWB
+// (InitializeWBVariables, :339-396) over the 4x4 Gauss points, and
+// Bathymetry_X/_Y at every Gauss point (src/equation.f90:419-509), written
+// straight into a job's device arrays -- and its post-processing: the L1
+// error against ExactFunction at the final time (PostProcessing,
+// src/postprocessing.f90:24-64) and the primitive variables for output.
 //
 // Every expression keeps the reference's order, with its compiler's lowering
 // of powers: X**2. and X**2 are X*X, a**3. and a**4 are the left-to-right
@@ -56,15 +59,16 @@ __device__ double modulo_r(double a, double p) {
   return m;
 }
 
-// ExactFunction at t = 0, src/equation.f90:70-230: conservative variables.
+// ExactFunction at time c.t, src/equation.f90:70-230: conservative variables
+// (only the vortex moves).
 __device__ void exact_function(const InitialSetup& c, double x, double y, double cons[3]) {
   double h = 0., u = 0., v = 0.;
   switch (c.ic) {
     case 1: {  // unsteady smooth vortex; hSmoothAuxiliary, hDerivSmoothAuxiliary (:216-228)
       const double u_inf = 2., v_inf = 3., h_inf = 1.;
       const double xm = c.x0[0] + 0.5 * c.sx[0], ym = c.x0[1] + 0.5 * c.sx[1];
-      const double xc = modulo_r(x - u_inf * 0.0 - c.x0[0], c.sx[0]) + c.x0[0] - xm;
-      const double yc = modulo_r(y - v_inf * 0.0 - c.x0[1], c.sx[1]) + c.x0[1] - ym;
+      const double xc = modulo_r(x - u_inf * c.t - c.x0[0], c.sx[0]) + c.x0[0] - xm;
+      const double yc = modulo_r(y - v_inf * c.t - c.x0[1], c.sx[1]) + c.x0[1] - ym;
       const double r = xc * xc + yc * yc;
       const double a = atan(r - 1.);
       const double hd =
@@ -167,7 +171,73 @@ __global__ __launch_bounds__(256) void k_initial_data(InitialSetup c, int nx, in
       }
 }
 
+// |U - Uexact| per cell and variable (PostProcessing's summands), with
+// Uexact the Gauss cell average of ExactFunction(ic, t): d[v*n + k].
+__global__ __launch_bounds__(256) void k_exact_abs_diff(InitialSetup c, int nx, int ny,
+                                                        const double* h, const double* hu,
+                                                        const double* hv, double* d) {
+  const size_t n = (size_t)nx * ny;
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i = (int)(k % nx) + 1, j = (int)(k / nx) + 1;
+  const double bxc = (c.x0[0] + (double)(i - 1) * c.dx[0]) + 0.5 * c.dx[0];
+  const double byc = (c.x0[1] + (double)(j - 1) * c.dx[1]) + 0.5 * c.dx[1];
+  double ue[3] = {0., 0., 0.};
+  for (int ig = 0; ig < 4; ++ig)
+    for (int jg = 0; jg < 4; ++jg) {
+      double t[3];
+      exact_function(c, bxc + c.node[ig] * c.dx[0], byc + c.node[jg] * c.dx[1], t);
+#pragma unroll
+      for (int v = 0; v < 3; ++v) ue[v] = ue[v] + c.w[ig] * c.w[jg] * t[v];
+    }
+  d[k] = fabs(h[k] - ue[0]);
+  d[n + k] = fabs(hu[k] - ue[1]);
+  d[2 * n + k] = fabs(hv[k] - ue[2]);
+}
+
+// L1error(v) = (sum over jj, ii of d) * MESH_DX(1) * MESH_DX(2), summed in
+// the reference's loop order (src/postprocessing.f90:47-60): one thread per
+// variable, cell k = (jj-1)*nx + (ii-1) ascending.  Sequential on purpose:
+// it runs once per output and keeps the reference's rounding.
+__global__ void k_sequential_l1(const double* d, size_t n, double dx, double dy, double* err) {
+  const int v = threadIdx.x;
+  if (v >= 3) return;
+  const double* a = d + (size_t)v * n;
+  double s = 0.;
+  for (size_t k = 0; k < n; ++k) s = s + a[k];
+  err[v] = s * dx * dy;
+}
+
+// ConsToPrim of every cell (src/equation.f90:912-945) into the reference's
+// (1:3, nx, ny) layout.
+__global__ __launch_bounds__(256) void k_primitives(size_t n, const double* h, const double* hu,
+                                                    const double* hv, double* V) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double u, v;
+  cons_to_prim(h[k], hu[k], hv[k], u, v);
+  V[3 * k] = h[k];
+  V[3 * k + 1] = u;
+  V[3 * k + 2] = v;
+}
+
 }  // namespace
+
+void launch_l1_error(const InitialSetup& c, int nx, int ny, const double* h, const double* hu,
+                     const double* hv, double* scratch, double* err, hipStream_t s) {
+  const size_t n = (size_t)nx * ny;
+  if (!n) return;
+  hipLaunchKernelGGL(k_exact_abs_diff, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c, nx,
+                     ny, h, hu, hv, scratch);
+  hipLaunchKernelGGL(k_sequential_l1, dim3(1), dim3(64), 0, s, scratch, n, c.dx[0], c.dx[1], err);
+}
+
+void launch_primitives(size_t n, const double* h, const double* hu, const double* hv, double* V,
+                       hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_primitives, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, h, hu,
+                     hv, V);
+}
 
 void launch_initial_data(const InitialSetup& c, int nx, int ny, double* h, double* hu, double* hv,
                          double* hwb, double* huwb, double* hvwb, double* bx, double* by,

@@ -17,7 +17,7 @@ INTEGER(C_INT), PARAMETER, PUBLIC :: SWM_OK = 0
 PUBLIC :: swm_create, swm_destroy, swm_set_bathy_grad, swm_init_wb
 PUBLIC :: swm_upload, swm_download, swm_timestep, swm_step_mpdec5, swm_advance
 PUBLIC :: swm_step_mpeuler, swm_step_mpdec2, swm_set_timescheme, swm_set_hp_association
-PUBLIC :: swm_set_reconstruction_fix, swm_fill_initial_data
+PUBLIC :: swm_set_reconstruction_fix, swm_fill_initial_data, swm_l1_error, swm_download_primitive
 PUBLIC :: swm_step_forward_euler, swm_step_ssprk4, swm_step_rk65, swm_step_dec5
 PUBLIC :: swm_error_message, swm_check
 
@@ -143,6 +143,25 @@ INTERFACE
     TYPE(C_PTR), VALUE :: h
     INTEGER(C_INT), VALUE :: ic, bflag
     REAL(C_DOUBLE), INTENT(IN) :: x0(2), x1(2)
+  END FUNCTION
+
+  ! PostProcessing (src/postprocessing.f90:24-64): L1 error against
+  ! ExactFunction(ic, t) on the device
+  INTEGER(C_INT) FUNCTION swm_l1_error(h, ic, bflag, t, x0, x1, err) &
+      BIND(C, NAME='swm_l1_error')
+    IMPORT :: C_INT, C_DOUBLE, C_PTR
+    TYPE(C_PTR), VALUE :: h
+    INTEGER(C_INT), VALUE :: ic, bflag
+    REAL(C_DOUBLE), VALUE :: t
+    REAL(C_DOUBLE), INTENT(IN) :: x0(2), x1(2)
+    REAL(C_DOUBLE), INTENT(OUT) :: err(3)
+  END FUNCTION
+
+  ! ConsToPrim of the state, V(1:3,nx,ny), for the output writer
+  INTEGER(C_INT) FUNCTION swm_download_primitive(h, V) BIND(C, NAME='swm_download_primitive')
+    IMPORT :: C_INT, C_DOUBLE, C_PTR
+    TYPE(C_PTR), VALUE :: h
+    REAL(C_DOUBLE) :: V(*)
   END FUNCTION
 
   ! ReconstructionFix (src/parameters.f90:142): 4 as the reference, 1 first

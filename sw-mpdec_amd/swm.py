@@ -74,6 +74,8 @@ def lib():
     L.swm_set_hp_association.argtypes = [vp, ctypes.c_int]
     L.swm_set_reconstruction_fix.argtypes = [vp, ctypes.c_int]
     L.swm_fill_initial_data.argtypes = [vp, ctypes.c_int, ctypes.c_int, _dp, _dp]
+    L.swm_l1_error.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
+    L.swm_download_primitive.argtypes = [vp, _dp]
     L.swm_run_steps.argtypes = [vp, ctypes.c_int, _dp, _ip]
     L.swm_fvtd.argtypes = [vp] + [_dp] * 6
     L.swm_set_jacobi_limits.argtypes = [vp, ctypes.c_int, ctypes.c_double]
@@ -206,6 +208,21 @@ class Solver:
         a1 = np.ascontiguousarray(x1, dtype=np.float64)
         self._check(lib().swm_fill_initial_data(self._h, int(ic), int(bathy_flag), _ptr(a0),
                                                 _ptr(a1)), "swm_fill_initial_data")
+
+    def l1_error(self, ic, bathy_flag, x0, x1, t):
+        """PostProcessing's L1 error of the state against ExactFunction(ic, t)."""
+        a0 = np.ascontiguousarray(x0, dtype=np.float64)
+        a1 = np.ascontiguousarray(x1, dtype=np.float64)
+        err = np.zeros(3)
+        self._check(lib().swm_l1_error(self._h, int(ic), int(bathy_flag), float(t), _ptr(a0),
+                                       _ptr(a1), _ptr(err)), "swm_l1_error")
+        return err
+
+    def download_primitive(self):
+        """(ny, nx, 3) primitives (h, u, v) of the state."""
+        V = np.empty((self.ny, self.nx, 3))
+        self._check(lib().swm_download_primitive(self._h, _ptr(V)), "swm_download_primitive")
+        return V
 
     def set_reconstruction_fix(self, recfix):
         """ReconstructionFix for the cells the shock indicator flags: 4 (WENO5,

@@ -67,6 +67,9 @@ def oracle_lib():
         L.orc_initial_data.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        _dp, _dp, ctypes.c_double, _dp, _dp, _dp, _dp]
         L.orc_initial_data.restype = ctypes.c_int
+        L.orc_l1_error.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                   ctypes.c_double, ctypes.c_double, _dp, _dp]
+        L.orc_l1_error.restype = ctypes.c_int
         L.orc_step_explicit.restype = ctypes.c_int
         L.orc_step_explicit.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
         L.orc_jacobi.restype = ctypes.c_int
@@ -294,3 +297,41 @@ def oracle_initial_data(ic: int, nx: int, ny: int, x0, x1, gravity: float, bathy
     if rc != 0:
         raise ValueError(f"orc_initial_data: initial condition {ic} not available")
     return U, Uwb, bx, by
+
+
+def run_swref_l1_error(U, ic: int, x0, x1, gravity: float, bathy_flag: int, t: float,
+                       timeout=600):
+    """The reference's PostProcessing (driver mode 7): the L1 error of the
+    state U (ny, nx, 3) against ExactFunction(ic, t) cell averages, as it
+    writes it to ErrorL1_<nx>_<ny>.dat (list-directed, so to the digits
+    printed)."""
+    exe = SWREF[False]
+    if not exe.exists():
+        raise RuntimeError(f"{exe} not built")
+    ny, nx = U.shape[:2]
+    hdr = np.array([7, nx, ic, bathy_flag, 0, ny, 5, 4], dtype="<i4")
+    fl = np.array([x0[0], x1[0], gravity, 0.5, x0[1], x1[1], t], dtype="<f8")
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as td:
+        fin, fout = Path(td) / "in.bin", Path(td) / "out.bin"
+        with open(fin, "wb") as f:
+            f.write(hdr.tobytes())
+            f.write(fl.tobytes())
+            f.write(np.ascontiguousarray(U, dtype="<f8").tobytes())
+        subprocess.run([str(exe), str(fin), str(fout)], check=True, cwd=td, timeout=timeout,
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, preexec_fn=big_stack)
+        txt = (Path(td) / f"ErrorL1_{nx:04d}_{ny:04d}.dat").read_text()
+    return np.array([float(v) for v in txt.split()])
+
+
+def oracle_l1_error(U, ic: int, x0, x1, gravity: float, bathy_flag: int, t: float):
+    """The oracle's PostProcessing: L1 error of U (ny, nx, 3) against
+    ExactFunction(ic, t)."""
+    ny, nx = U.shape[:2]
+    err = np.zeros(3)
+    a0 = np.array(x0, dtype=np.float64)
+    a1 = np.array(x1, dtype=np.float64)
+    rc = oracle_lib().orc_l1_error(ic, bathy_flag, nx, ny, _ptr(a0), _ptr(a1), gravity, t,
+                                   _ptr(np.ascontiguousarray(U, dtype=np.float64)), _ptr(err))
+    if rc != 0:
+        raise ValueError(f"orc_l1_error: initial condition {ic} not available")
+    return err

@@ -18,7 +18,7 @@ import pytest
 import problems as P
 import serve
 import swm
-from oracle_api import Oracle, oracle_initial_data
+from oracle_api import Oracle, oracle_initial_data, oracle_l1_error
 from parity_util import describe_diff, same_bits
 
 pytestmark = pytest.mark.gpu
@@ -191,3 +191,50 @@ def test_served_requests_by_initial_condition():
     for a, r, st in zip(got, seq_reqs, states):
         assert st == r.state, r.problem.ic
         assert same_bits(a, r.result), (r.problem.ic, r.problem.nx)
+
+
+
+@pytest.mark.parametrize("ic", [1, 2, 3, 4, 5, 6])
+def test_l1_error_matches_oracle(ic):
+    """PostProcessing's L1 error on the device, of a perturbed state at
+    t = 0, 0.05 and 1.7: bitwise the oracle's for the dam breaks, within
+    1e-13 relative for the others (ExactFunction through the device libm)."""
+    c = P.ReferenceSetup(ic, 40, 24)
+    U = oracle_data(c)[0] * (1 + 1e-3 * np.random.default_rng(ic).standard_normal((24, 40, 3)))
+    s = swm.Solver.from_setup(c)
+    s.upload(U)
+    for t in (0.0, 0.05, 1.7):
+        err = s.l1_error(c.ic, c.bathy_flag, c.x0, c.x1, t)
+        ref = oracle_l1_error(U, c.ic, c.x0, c.x1, c.gravity, c.bathy_flag, t)
+        if ic in (4, 5):
+            assert same_bits(err, ref), (t, err, ref)
+        else:
+            assert np.all(np.abs(err - ref) <= 1e-13 * np.abs(ref)), (t, err, ref)
+
+
+def test_vortex_error_after_steps():
+    """The reference's convergence measurement: the vortex stepped to time t,
+    then its L1 error against the exact (translated) vortex, on the device,
+    against the oracle's error of the same state."""
+    c = P.ReferenceSetup(1, 48, 48)
+    s = swm.Solver.from_setup(c)
+    t = 0.0
+    for _ in range(4):
+        dt, _, _ = s.timestep()
+        s.step(dt)
+        t = t + dt
+    U = s.download()
+    err = s.l1_error(1, 0, c.x0, c.x1, t)
+    ref = oracle_l1_error(U, 1, c.x0, c.x1, c.gravity, 0, t)
+    assert np.all(np.abs(err - ref) <= 1e-13 * np.abs(ref)), (err, ref)
+    assert 0 < err[0] < 1e-2
+
+
+def test_primitives_are_cons_to_prim():
+    c = P.ReferenceSetup(3, 40, 24, wellbalanced=True)
+    s = swm.Solver.from_setup(c)
+    s.run_steps(2)
+    U, V = s.download(), s.download_primitive()
+    ht = U[..., 0] + 1e-6 / U[..., 0]
+    want = np.stack([U[..., 0], U[..., 1] / ht, U[..., 2] / ht], -1)
+    assert same_bits(V, want)

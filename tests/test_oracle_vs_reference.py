@@ -6,8 +6,8 @@ import numpy as np
 import pytest
 
 import problems as P
-from oracle_api import (Oracle, have_swref, have_swref_explicit, oracle_initial_data, run_swref,
-                        run_swref_initial_data)
+from oracle_api import (Oracle, have_swref, have_swref_explicit, oracle_initial_data,
+                        oracle_l1_error, run_swref, run_swref_initial_data, run_swref_l1_error)
 from parity_util import FVTD_KEYS, describe_diff, same_bits
 
 pytestmark = pytest.mark.skipif(not have_swref(), reason="oracle/_ref not built")
@@ -334,3 +334,18 @@ def test_bathymetry_tables_give_the_reference_source(bflag, x0, x1):
     assert np.abs(ref["S"][..., 1:]).max() > 0
     assert same_bits(got["S"][..., 1:], ref["S"][..., 1:]), describe_diff(got["S"], ref["S"])
     assert same_bits(got["Ut"][..., 1:], ref["Ut"][..., 1:])
+
+
+
+@pytest.mark.parametrize("ic", sorted(IC_SETUPS))
+def test_l1_error_bitwise(ic):
+    """PostProcessing's L1 error (read back from the ErrorL1 file it writes)
+    against the oracle's, of a perturbed initial state at t = 0, 0.05 and
+    1.7 (the vortex wraps around the periodic domain)."""
+    x0, x1, g, bf = IC_SETUPS[ic]
+    U = oracle_initial_data(ic, 24, 14, x0, x1, g, bf, False)[0]
+    U = U * (1 + 1e-3 * np.random.default_rng(ic).standard_normal(U.shape))
+    for t in (0.0, 0.05, 1.7):
+        ref = run_swref_l1_error(U, ic, x0, x1, g, bf, t)
+        got = oracle_l1_error(U, ic, x0, x1, g, bf, t)
+        assert same_bits(got, ref), (t, got, ref)
