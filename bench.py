@@ -228,7 +228,8 @@ def initial_data_section(n: int, cpu: bool) -> dict:
     """The reference's initial data made on the device (swm_fill_initial_data)
     for n x n cells of each initial condition's own domain: the fill of the
     state, the bathymetry tables and, for the well-balanced lakes 2 and 3,
-    the well-balanced state and its FVTD.  CPU: the reference's own
+    the well-balanced state and its FVTD; and PostProcessing's L1 error
+    (swm_l1_error, the sum in the reference's sequential order included).  CPU: the reference's own
     FillInitialConditions + InitializeWBVariables (its WELLBALANCED build,
     driver mode 6) for the lake at rest at 256 x 256, wall time of the
     process."""
@@ -249,6 +250,12 @@ def initial_data_section(n: int, cpu: bool) -> dict:
         res[f"ic{ic}"] = {"ms": 1e3 * el, "cells_per_s": n * n / el,
                           "wellbalanced": bool(c.wellbalanced)}
         log(f"bench: initial data ic {ic} {n}x{n}: {1e3 * el:.2f} ms")
+        if ic == 1:  # PostProcessing's L1 error of this state at t = 0.1
+            s.l1_error(1, 0, c.x0, c.x1, 0.1)
+            t0 = time.perf_counter()
+            s.l1_error(1, 0, c.x0, c.x1, 0.1)
+            res["l1_error_ms"] = 1e3 * (time.perf_counter() - t0)
+            log(f"bench: L1 error {n}x{n}: {res['l1_error_ms']:.2f} ms")
         del s
     exe = ROOT / "oracle" / "_ref" / "swref_pwb_fast"
     if cpu and exe.exists():
