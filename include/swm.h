@@ -311,6 +311,10 @@ int swm_engine_step(swm_engine_t *e, int *n_running);
 int swm_engine_job_info(swm_engine_t *e, int job_id, swm_job_info *info);
 /* Current state of a job, (1:3, 1:nx, 1:ny). */
 int swm_engine_fetch(swm_engine_t *e, int job_id, double *U);
+/* swm_l1_error of a job's current state: a served request's error against
+ * its initial condition's exact solution without fetching the state. */
+int swm_engine_l1_error(swm_engine_t *e, int job_id, int initial_condition, int bathymetry_flag,
+                        double t, const double x0[2], const double x1[2], double err[3]);
 /* Frees the job's device memory; its id may be reused. */
 int swm_engine_release(swm_engine_t *e, int job_id);
 int swm_engine_set_jacobi_limits(swm_engine_t *e, int max_iter, double tol);

@@ -1163,6 +1163,27 @@ int fill_initial_data(swm_engine* e, int id, int ic, int bflag, const double* x0
   return wb ? build_wb(e, id) : SWM_OK;
 }
 
+// PostProcessing's L1 error of job J's state (swm_l1_error).
+int job_l1_error(swm_engine* e, Job& J, int ic, int bflag, double t, const double* x0,
+                 const double* x1, double* err) {
+  if (!err) FAIL(e, SWM_E_ARG, "err is NULL");
+  InitialSetup c;
+  int rc = exact_setup(e, J, ic, bflag, x0, x1, t, c);
+  if (rc) return rc;
+  const size_t n = J.lay.n;
+  double* tmp = nullptr;
+  HIPCK(e, hipMalloc((void**)&tmp, (3 * n + 3) * sizeof(double)));
+  launch_l1_error(c, J.dev.nx, J.dev.ny, J.dev.h, J.dev.hu, J.dev.hv, tmp, tmp + 3 * n,
+                  e->stream);
+  hipError_t er = hipGetLastError();
+  if (er == hipSuccess)
+    er = hipMemcpyAsync(err, tmp + 3 * n, 3 * sizeof(double), hipMemcpyDeviceToHost, e->stream);
+  if (er == hipSuccess) er = hipStreamSynchronize(e->stream);
+  (void)hipFree(tmp);
+  if (er != hipSuccess) FAIL(e, SWM_E_HIP, "L1 error failed: %s", hipGetErrorString(er));
+  return SWM_OK;
+}
+
 int engine_add(swm_engine* e, const swm_job_desc* d, int* id_out) {
   *id_out = -1;
   if (!d) FAIL(e, SWM_E_ARG, "job description is NULL");
@@ -1557,24 +1578,16 @@ int swm_fill_initial_data(swm_solver_t* s, int initial_condition, int bathymetry
 int swm_l1_error(swm_solver_t* s, int initial_condition, int bathymetry_flag, double t,
                  const double x0[2], const double x1[2], double err[3]) {
   if (!s) return SWM_E_ARG;
-  swm_engine* e = s->e;
-  if (!err) FAIL(e, SWM_E_ARG, "err is NULL");
-  Job& J = *plain_job(s);
-  InitialSetup c;
-  int rc = exact_setup(e, J, initial_condition, bathymetry_flag, x0, x1, t, c);
+  return job_l1_error(s->e, *plain_job(s), initial_condition, bathymetry_flag, t, x0, x1, err);
+}
+
+int swm_engine_l1_error(swm_engine_t* e, int job_id, int initial_condition, int bathymetry_flag,
+                        double t, const double x0[2], const double x1[2], double err[3]) {
+  if (!e) return SWM_E_ARG;
+  Job* J;
+  int rc = job_from_id(e, job_id, &J);
   if (rc) return rc;
-  const size_t n = J.lay.n;
-  double* tmp = nullptr;
-  HIPCK(e, hipMalloc((void**)&tmp, (3 * n + 3) * sizeof(double)));
-  launch_l1_error(c, J.dev.nx, J.dev.ny, J.dev.h, J.dev.hu, J.dev.hv, tmp, tmp + 3 * n,
-                  e->stream);
-  hipError_t er = hipGetLastError();
-  if (er == hipSuccess)
-    er = hipMemcpyAsync(err, tmp + 3 * n, 3 * sizeof(double), hipMemcpyDeviceToHost, e->stream);
-  if (er == hipSuccess) er = hipStreamSynchronize(e->stream);
-  (void)hipFree(tmp);
-  if (er != hipSuccess) FAIL(e, SWM_E_HIP, "L1 error failed: %s", hipGetErrorString(er));
-  return SWM_OK;
+  return job_l1_error(e, *J, initial_condition, bathymetry_flag, t, x0, x1, err);
 }
 
 int swm_download_primitive(swm_solver_t* s, double* V) {

@@ -404,6 +404,8 @@ def _engine_lib():
     L.swm_engine_step.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     L.swm_engine_job_info.argtypes = [vp, ctypes.c_int, ctypes.POINTER(JobInfo)]
     L.swm_engine_fetch.argtypes = [vp, ctypes.c_int, _dp]
+    L.swm_engine_l1_error.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                      _dp, _dp, _dp]
     L.swm_engine_release.argtypes = [vp, ctypes.c_int]
     L.swm_engine_set_jacobi_limits.argtypes = [vp, ctypes.c_int, ctypes.c_double]
     L.swm_engine_profile_enable.argtypes = [vp, ctypes.c_int]
@@ -532,6 +534,17 @@ class Engine:
         U = np.empty((rows, n, 3))
         self._check(_engine_lib().swm_engine_fetch(self._e, jid, _ptr(U)), "swm_engine_fetch")
         return U
+
+    def l1_error(self, jid, ic, bathy_flag, x0, x1, t):
+        """PostProcessing's L1 error of job jid's state against
+        ExactFunction(ic, t)."""
+        a0 = np.ascontiguousarray(x0, dtype=np.float64)
+        a1 = np.ascontiguousarray(x1, dtype=np.float64)
+        err = np.zeros(3)
+        self._check(_engine_lib().swm_engine_l1_error(self._e, int(jid), int(ic), int(bathy_flag),
+                                                      float(t), _ptr(a0), _ptr(a1), _ptr(err)),
+                    "swm_engine_l1_error")
+        return err
 
     def release(self, jid):
         self._check(_engine_lib().swm_engine_release(self._e, jid), "swm_engine_release")

@@ -145,8 +145,12 @@ def test_engine_jobs_from_initial_conditions():
     for c, jid in zip(setups, ids):
         s = swm.Solver.from_setup(c)
         s.run_steps(2)
-        assert eng.info(jid)["steps_done"] == 2, c.ic
+        info = eng.info(jid)
+        assert info["steps_done"] == 2, c.ic
         assert same_bits(eng.fetch(jid), s.download()), c.ic
+        # PostProcessing's error of the job at its own time, without a fetch
+        err = eng.l1_error(jid, c.ic, c.bathy_flag, c.x0, c.x1, info["t"])
+        assert same_bits(err, s.l1_error(c.ic, c.bathy_flag, c.x0, c.x1, info["t"])), c.ic
 
 
 def test_bad_arguments_are_rejected():
