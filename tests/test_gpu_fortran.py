@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import problems as P
+import swm
 from conftest import ROOT
 from oracle_api import Oracle
 from parity_util import describe_diff, same_bits
@@ -29,8 +30,9 @@ def write_input(path, p, loop=0, scheme=5, hp_by_node=False, recfix=0):
     m = p.mesh
     with open(path, "wb") as f:
         np.array([p.n, p.rows, p.steps, int(p.wellbalanced), int(p.dbdx is not None), loop,
-                  scheme, int(hp_by_node), recfix], dtype=np.int32).tofile(f)
-        np.array([m.dx, p.mesh_y.dx, p.gravity, p.cfl, *m.wbnd], dtype=np.float64).tofile(f)
+                  scheme, int(hp_by_node), recfix, 0, 0], dtype=np.int32).tofile(f)
+        np.array([m.dx, p.mesh_y.dx, p.gravity, p.cfl, *m.wbnd, 0., 0., 0., 0.],
+                 dtype=np.float64).tofile(f)
         np.ascontiguousarray(p.U0, dtype=np.float64).tofile(f)
         if p.wellbalanced:
             np.ascontiguousarray(p.Uwb, dtype=np.float64).tofile(f)
@@ -180,3 +182,39 @@ def test_fortran_driver_reconstruction_fix(tmp_path, recfix, scheme, make):
     assert same_bits(dts, np.array(dts_r))
     U_r = o.get_U()
     assert same_bits(U, U_r), describe_diff(U, U_r)
+
+
+
+@pytest.mark.parametrize("ic,wb,scheme", [(1, False, 5), (2, True, 5), (3, True, 2), (4, False, 5)])
+def test_fortran_driver_initial_data_and_l1_error(tmp_path, ic, wb, scheme):
+    """The driver's initial_condition header: swm_fill_initial_data instead of
+    host data, the time loop, then PostProcessing's L1 error by swm_l1_error
+    at the final time -- all through ISO_C_BINDING -- against the same
+    calls from Python (bitwise)."""
+    if not FDRIVER.exists():
+        pytest.skip("Fortran driver not built (no Fortran compiler at build time)")
+    c = P.ReferenceSetup(ic, 32, 20, wellbalanced=wb)
+    steps = 3
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        np.array([c.nx, c.ny, steps, int(wb), int(c.bathy_flag > 0), 0, scheme, 0, 0, ic,
+                  c.bathy_flag], dtype=np.int32).tofile(f)
+        np.array([c.mesh.dx, c.mesh_y.dx, c.gravity, c.cfl, *c.mesh.wbnd, *c.x0, *c.x1],
+                 dtype=np.float64).tofile(f)
+    r = subprocess.run([str(FDRIVER), str(fin), str(fout)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = open(fout, "rb").read()
+    nsteps = int(np.frombuffer(raw[:4], dtype=np.int32)[0])
+    body = np.frombuffer(raw[8:], dtype=np.float64)
+    dts, U, l1 = body[:nsteps], body[nsteps:nsteps + 3 * c.nx * c.ny], body[-3:]
+    s = swm.Solver.from_setup(c)
+    s.set_timescheme(scheme)
+    t = 0.0
+    for k in range(steps):
+        dt, _, _ = s.timestep()
+        assert dt == dts[k]
+        s.step(dt)
+        t = t + dt
+    assert same_bits(U.reshape(c.ny, c.nx, 3), s.download())
+    assert same_bits(l1, s.l1_error(ic, c.bathy_flag, c.x0, c.x1, t))
