@@ -196,16 +196,41 @@ __global__ __launch_bounds__(256) void k_exact_abs_diff(InitialSetup c, int nx, 
 }
 
 // L1error(v) = (sum over jj, ii of d) * MESH_DX(1) * MESH_DX(2), summed in
-// the reference's loop order (src/postprocessing.f90:47-60): one thread per
-// variable, cell k = (jj-1)*nx + (ii-1) ascending.  Sequential on purpose:
-// it runs once per output and keeps the reference's rounding.
-__global__ void k_sequential_l1(const double* d, size_t n, double dx, double dy, double* err) {
-  const int v = threadIdx.x;
-  if (v >= 3) return;
-  const double* a = d + (size_t)v * n;
+// the reference's loop order (src/postprocessing.f90:47-60): lane v of the
+// first wave adds cell k = (jj-1)*nx + (ii-1) ascending, so the rounding is
+// the reference's.  One thread streaming from HBM would wait on every load,
+// so the other waves stage the summands in LDS chunks, double-buffered: while
+// lanes 0..2 add chunk c, waves 1..3 load chunk c+1 with coalesced loads.
+constexpr int kL1Chunk = 1024;
+__global__ __launch_bounds__(256) void k_sequential_l1(const double* d, size_t n, double dx,
+                                                       double dy, double* err) {
+  __shared__ double buf[2][3][kL1Chunk];  // 48 KB
+  const int tid = threadIdx.x;
+  const size_t nchunk = (n + kL1Chunk - 1) / kL1Chunk;
+  auto load = [&](size_t c, int b, int first, int stride) {
+    const size_t base = c * kL1Chunk;
+    const int m = (int)((n - base) < (size_t)kL1Chunk ? (n - base) : (size_t)kL1Chunk);
+    for (int t = first; t < 3 * kL1Chunk; t += stride) {
+      const int v = t / kL1Chunk, i = t % kL1Chunk;
+      if (i < m) buf[b][v][i] = d[(size_t)v * n + base + i];
+    }
+  };
+  load(0, 0, tid, 256);
+  __syncthreads();
   double s = 0.;
-  for (size_t k = 0; k < n; ++k) s = s + a[k];
-  err[v] = s * dx * dy;
+  for (size_t c = 0; c < nchunk; ++c) {
+    const int b = (int)(c & 1);
+    if (tid >= 64) {
+      if (c + 1 < nchunk) load(c + 1, b ^ 1, tid - 64, 192);
+    } else if (tid < 3) {
+      const size_t base = c * kL1Chunk;
+      const int m = (int)((n - base) < (size_t)kL1Chunk ? (n - base) : (size_t)kL1Chunk);
+      const double* a = buf[b][tid];
+      for (int i = 0; i < m; ++i) s = s + a[i];
+    }
+    __syncthreads();
+  }
+  if (tid < 3) err[tid] = s * dx * dy;
 }
 
 // ConsToPrim of every cell (src/equation.f90:912-945) into the reference's
@@ -229,7 +254,8 @@ void launch_l1_error(const InitialSetup& c, int nx, int ny, const double* h, con
   if (!n) return;
   hipLaunchKernelGGL(k_exact_abs_diff, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c, nx,
                      ny, h, hu, hv, scratch);
-  hipLaunchKernelGGL(k_sequential_l1, dim3(1), dim3(64), 0, s, scratch, n, c.dx[0], c.dx[1], err);
+  hipLaunchKernelGGL(k_sequential_l1, dim3(1), dim3(256), 0, s, scratch, n, c.dx[0], c.dx[1],
+                     err);
 }
 
 void launch_primitives(size_t n, const double* h, const double* hu, const double* hv, double* V,
