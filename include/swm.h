@@ -103,6 +103,16 @@ int swm_l1_error(swm_solver_t *h, int initial_condition, int bathymetry_flag, do
  * of the step just taken), not of the state: this gives those of the state. */
 int swm_download_primitive(swm_solver_t *h, double *V);
 
+/* The same, asynchronous: takes a device snapshot of the primitives in
+ * stream order and copies it to V on a second stream, returning at once, so
+ * that the writer's copy overlaps the steps that follow (steps called after
+ * this do not change what is copied).  V must stay valid until
+ * swm_output_wait; pinned memory (hipHostMalloc) makes the copy truly
+ * asynchronous.  A second call first lets the previous copy finish on the
+ * device. */
+int swm_download_primitive_async(swm_solver_t *h, double *V);
+int swm_output_wait(swm_solver_t *h);
+
 /* Copy the state U into / out of the device (the module array U). */
 int swm_upload(swm_solver_t *h, const double *U, int ghosts);
 int swm_download(swm_solver_t *h, double *U);

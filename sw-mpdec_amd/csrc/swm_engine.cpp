@@ -223,6 +223,12 @@ struct swm_engine {
 struct swm_solver {
   swm_engine* e = nullptr;
   int job = -1;
+  // asynchronous output (swm_download_primitive_async): a device snapshot of
+  // the primitives, copied to the host on a stream of its own
+  double* out_stage = nullptr;
+  hipStream_t out_stream = nullptr;
+  hipEvent_t out_ready = nullptr, out_done = nullptr;
+  bool out_pending = false;
 };
 
 #define FAIL(E, code, ...)                                                     \
@@ -1547,6 +1553,13 @@ int swm_create(int nx, int ny, double dx, double dy, double gravity, double cfl,
 
 void swm_destroy(swm_solver_t* s) {
   if (!s) return;
+  if (s->out_stream) {
+    (void)hipStreamSynchronize(s->out_stream);
+    (void)hipEventDestroy(s->out_ready);
+    (void)hipEventDestroy(s->out_done);
+    (void)hipStreamDestroy(s->out_stream);
+  }
+  if (s->out_stage) (void)hipFree(s->out_stage);
   swm_engine_destroy(s->e);
   delete s;
 }
@@ -1605,6 +1618,40 @@ int swm_download_primitive(swm_solver_t* s, double* V) {
   if (er == hipSuccess) er = hipStreamSynchronize(e->stream);
   (void)hipFree(tmp);
   if (er != hipSuccess) FAIL(e, SWM_E_HIP, "primitive download failed: %s", hipGetErrorString(er));
+  return SWM_OK;
+}
+
+int swm_download_primitive_async(swm_solver_t* s, double* V) {
+  if (!s) return SWM_E_ARG;
+  swm_engine* e = s->e;
+  if (!V) FAIL(e, SWM_E_ARG, "V is NULL");
+  Job& J = *plain_job(s);
+  const size_t n = J.lay.n;
+  if (!s->out_stream) {
+    HIPCK(e, hipMalloc((void**)&s->out_stage, 3 * n * sizeof(double)));
+    HIPCK(e, hipStreamCreateWithFlags(&s->out_stream, hipStreamNonBlocking));
+    HIPCK(e, hipEventCreateWithFlags(&s->out_ready, hipEventDisableTiming));
+    HIPCK(e, hipEventCreateWithFlags(&s->out_done, hipEventDisableTiming));
+  }
+  // the previous copy must have left the snapshot before it is overwritten
+  if (s->out_pending) HIPCK(e, hipStreamWaitEvent(e->stream, s->out_done, 0));
+  launch_primitives(n, J.dev.h, J.dev.hu, J.dev.hv, s->out_stage, e->stream);
+  LAUNCHCK(e);
+  HIPCK(e, hipEventRecord(s->out_ready, e->stream));
+  HIPCK(e, hipStreamWaitEvent(s->out_stream, s->out_ready, 0));
+  HIPCK(e, hipMemcpyAsync(V, s->out_stage, 3 * n * sizeof(double), hipMemcpyDeviceToHost,
+                          s->out_stream));
+  HIPCK(e, hipEventRecord(s->out_done, s->out_stream));
+  s->out_pending = true;
+  return SWM_OK;
+}
+
+int swm_output_wait(swm_solver_t* s) {
+  if (!s) return SWM_E_ARG;
+  if (s->out_pending) {
+    HIPCK(s->e, hipEventSynchronize(s->out_done));
+    s->out_pending = false;
+  }
   return SWM_OK;
 }
 

@@ -76,6 +76,8 @@ def lib():
     L.swm_fill_initial_data.argtypes = [vp, ctypes.c_int, ctypes.c_int, _dp, _dp]
     L.swm_l1_error.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
     L.swm_download_primitive.argtypes = [vp, _dp]
+    L.swm_download_primitive_async.argtypes = [vp, _dp]
+    L.swm_output_wait.argtypes = [vp]
     L.swm_run_steps.argtypes = [vp, ctypes.c_int, _dp, _ip]
     L.swm_fvtd.argtypes = [vp] + [_dp] * 6
     L.swm_set_jacobi_limits.argtypes = [vp, ctypes.c_int, ctypes.c_double]
@@ -223,6 +225,19 @@ class Solver:
         V = np.empty((self.ny, self.nx, 3))
         self._check(lib().swm_download_primitive(self._h, _ptr(V)), "swm_download_primitive")
         return V
+
+    def download_primitive_async(self, V):
+        """Starts copying a snapshot of the primitives into V ((ny, nx, 3)
+        float64, C order; kept alive until output_wait)."""
+        if V.shape != (self.ny, self.nx, 3) or V.dtype != np.float64 or not V.flags.c_contiguous:
+            raise ValueError("V must be a C-contiguous (ny, nx, 3) float64 array")
+        self._out = getattr(self, "_out", []) + [V]  # alive until the copies are done
+        self._check(lib().swm_download_primitive_async(self._h, _ptr(V)),
+                    "swm_download_primitive_async")
+
+    def output_wait(self):
+        self._check(lib().swm_output_wait(self._h), "swm_output_wait")
+        self._out = []
 
     def set_reconstruction_fix(self, recfix):
         """ReconstructionFix for the cells the shock indicator flags: 4 (WENO5,

@@ -242,3 +242,30 @@ def test_primitives_are_cons_to_prim():
     ht = U[..., 0] + 1e-6 / U[..., 0]
     want = np.stack([U[..., 0], U[..., 1] / ht, U[..., 2] / ht], -1)
     assert same_bits(V, want)
+
+
+
+def test_async_primitive_download_is_a_snapshot():
+    """swm_download_primitive_async copies the primitives of the state at the
+    call while the following steps run; a second call reuses the snapshot
+    buffer only after the first copy.  Both into pinned host memory."""
+    import torch
+    c = P.ReferenceSetup(1, 64, 40)
+    s = swm.Solver.from_setup(c)
+    s.run_steps(1)
+    want1 = s.download_primitive()
+    V1 = torch.empty((40, 64, 3), dtype=torch.float64, pin_memory=True).numpy()
+    s.download_primitive_async(V1)
+    s.run_steps(2)
+    s.output_wait()
+    assert same_bits(V1, want1)
+    want2 = s.download_primitive()
+    assert not same_bits(want1, want2)
+    V2 = torch.empty((40, 64, 3), dtype=torch.float64, pin_memory=True).numpy()
+    V3 = np.empty((40, 64, 3))
+    s.download_primitive_async(V2)
+    s.run_steps(1)
+    s.download_primitive_async(V3)  # pageable memory works too
+    s.output_wait()
+    assert same_bits(V2, want2)
+    assert same_bits(V3, s.download_primitive())
