@@ -248,20 +248,33 @@ def test_primitives_are_cons_to_prim():
 def test_async_primitive_download_is_a_snapshot():
     """swm_download_primitive_async copies the primitives of the state at the
     call while the following steps run; a second call reuses the snapshot
-    buffer only after the first copy.  Both into pinned host memory."""
-    import torch
+    buffer only after the first copy.  Into pinned (hipHostMalloc) and
+    pageable host memory."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipHostMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_uint]
+    hip.hipHostFree.argtypes = [ctypes.c_void_p]
+    pinned = []
+
+    def pinned_array():
+        ptr = ctypes.c_void_p()
+        assert hip.hipHostMalloc(ctypes.byref(ptr), 40 * 64 * 3 * 8, 0) == 0
+        pinned.append(ptr)
+        buf = (ctypes.c_double * (40 * 64 * 3)).from_address(ptr.value)
+        return np.ctypeslib.as_array(buf).reshape(40, 64, 3)
+
     c = P.ReferenceSetup(1, 64, 40)
     s = swm.Solver.from_setup(c)
     s.run_steps(1)
     want1 = s.download_primitive()
-    V1 = torch.empty((40, 64, 3), dtype=torch.float64, pin_memory=True).numpy()
+    V1 = pinned_array()
     s.download_primitive_async(V1)
     s.run_steps(2)
     s.output_wait()
     assert same_bits(V1, want1)
     want2 = s.download_primitive()
     assert not same_bits(want1, want2)
-    V2 = torch.empty((40, 64, 3), dtype=torch.float64, pin_memory=True).numpy()
+    V2 = pinned_array()
     V3 = np.empty((40, 64, 3))
     s.download_primitive_async(V2)
     s.run_steps(1)
@@ -269,3 +282,6 @@ def test_async_primitive_download_is_a_snapshot():
     s.output_wait()
     assert same_bits(V2, want2)
     assert same_bits(V3, s.download_primitive())
+    del V1, V2
+    for ptr in pinned:
+        assert hip.hipHostFree(ptr) == 0
