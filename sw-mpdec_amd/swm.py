@@ -378,7 +378,7 @@ def debug_shared_div(a, b):
 # ---------------------------------------------------------------------------
 # batched / served solves
 # ---------------------------------------------------------------------------
-SWM_E_BUDGET = -5
+SWM_E_ARG, SWM_E_HIP, SWM_E_STATE, SWM_E_NOMEM, SWM_E_BUDGET = -1, -2, -3, -4, -5  # include/swm.h
 JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2
 
 

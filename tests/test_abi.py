@@ -65,3 +65,48 @@ def test_profile_names():
     L = swm.lib()
     names = [L.swm_profile_name(c).decode() for c in range(swm.PROF_NCAT)]
     assert names[0] == "fvtd_x" and names[5] == "jacobi_sweep"
+
+
+def test_header_declares_the_rows_around_the_path():
+    names = declared_functions()
+    for must in ("swm_set_timescheme", "swm_set_hp_association", "swm_set_reconstruction_fix",
+                 "swm_fill_initial_data", "swm_l1_error", "swm_download_primitive",
+                 "swm_download_primitive_async", "swm_output_wait", "swm_engine_l1_error",
+                 "swm_run_steps", "swm_advance"):
+        assert must in names
+
+
+def test_null_handles_are_rejected_without_a_device():
+    """The new entry points check their handle before any HIP call."""
+    L = swm._engine_lib()
+    x = (ctypes.c_double * 2)(0.0, 0.0)
+    err = (ctypes.c_double * 3)()
+    assert L.swm_set_reconstruction_fix(None, 2) == swm.SWM_E_ARG
+    assert L.swm_fill_initial_data(None, 1, 0, x, x) == swm.SWM_E_ARG
+    assert L.swm_l1_error(None, 1, 0, ctypes.c_double(0.0), x, x, err) == swm.SWM_E_ARG
+    assert L.swm_download_primitive(None, err) == swm.SWM_E_ARG
+    assert L.swm_download_primitive_async(None, err) == swm.SWM_E_ARG
+    assert L.swm_output_wait(None) == swm.SWM_E_ARG
+    assert L.swm_engine_l1_error(None, 0, 1, 0, ctypes.c_double(0.0), x, x, err) == swm.SWM_E_ARG
+
+
+def test_reference_setups_match_the_reference_mesh():
+    """problems.ReferenceSetup carries the reference's domains (src/parameters.f90:62-125)
+    and Mesh_DX = |X1 - X0| / n (src/mesh.f90:39-41)."""
+    import problems as P
+    for ic, (x0, x1, g, bf) in P.REFERENCE_SETUPS.items():
+        c = P.ReferenceSetup(ic, 40, 24)
+        assert c.mesh.dx == abs(abs(x1[0] - x0[0])) / 40.0
+        assert c.mesh_y.dx == abs(abs(x1[1] - x0[1])) / 24.0
+        assert (c.n, c.rows) == (40, 24) and c.gravity == g and c.bathy_flag == bf
+    assert P.REFERENCE_SETUPS[3][0] == (-5.0, -2.0) and P.REFERENCE_SETUPS[5][1] == (50.0, 50.0)
+
+
+def test_reference_request_mix_is_seeded():
+    import serve
+    a = serve.reference_mix(12, 3, rate=5.0)
+    b = serve.reference_mix(12, 3, rate=5.0)
+    assert [(r.problem.ic, r.problem.nx, r.nsteps, r.arrival) for r in a] == \
+           [(r.problem.ic, r.problem.nx, r.nsteps, r.arrival) for r in b]
+    assert {r.problem.ic for r in a} <= set(range(1, 7))
+    assert all(r.problem.wellbalanced == (r.problem.ic in (2, 3)) for r in a)
