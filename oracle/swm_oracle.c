@@ -31,6 +31,13 @@
  *    and the first-step momenta, which do not read the depths and equal the
  *    reference's.
  *
+ * The initial data (orc_initial_data: ExactFunction, ExactFunctionWB,
+ * Bathymetry_X/_Y and BuildMesh's Gauss rule) and PostProcessing's L1 error
+ * (orc_l1_error) are pinned bitwise against the reference's own
+ * FillInitialConditions / InitializeWBVariables and PostProcessing output
+ * (driver modes 6 and 7, fixtures initial_ic*), and the bathymetry tables
+ * through the reference's source term.
+ *
  * The Jameson shock indicator (src/shocksindicator.f90) and
  * ReconstructionFixX/Y (src/reconstruction.f90:190-342) run only when
  * ReconstructionFix differs from Reconstruction = 4 (orc_set_reconstruction_fix;
