@@ -19,6 +19,8 @@
 !           InitialCondition = nsteps; no state is read
 !   mode 7: PostProcessing of the given state for InitialCondition = nsteps at
 !           tEnd = t (its ErrorL1_nx_ny.dat in the working directory)
+!   mode 8: nsteps steps as mode 2, then V(1:3,1:nx,1:ny), the primitives the
+!           output writer prints (those of the last FVTD input)
 ! With WELLBALANCED builds the well-balanced reference state is installed the
 ! way InitializeWBVariables does it (src/finitevolume.f90:373-392).
 !
@@ -183,6 +185,16 @@ SELECT CASE (mode)
   CASE (3)
     dtmin = TimeStep()
     WRITE(11) dtmin, LambdaMaxX, LambdaMaxY
+  CASE (8)
+    ! like mode 2, then the module array V that WriteSolutionToDisk prints
+    ! (src/output.f90:79): the primitives of the last FVTD input
+    DO k = 1, nsteps
+      dtmin = TimeStep()
+      IF (.NOT. (dtmin .LT. HUGE(1.0))) EXIT
+      tEnd = dtmin
+      CALL TimeDiscretization()
+    END DO
+    WRITE(11) V(1:3,1:nx,1:ny)
   CASE (7)
     ! PostProcessing (src/postprocessing.f90:24-64) of the given state at
     ! tEnd = tpost for InitialCondition = nsteps: it writes ErrorL1_nx_ny.dat

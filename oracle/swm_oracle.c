@@ -906,6 +906,12 @@ void orc_set_U(orc_t *o, const double *U) {
       for (int v = 0; v < NVAR; ++v) o->U[iU(o, v, ii, jj)] = U[iC(o, v, ii, jj)];
 }
 
+void orc_get_V(const orc_t *o, double *V) {
+  for (int jj = 1; jj <= o->ny; ++jj)
+    for (int ii = 1; ii <= o->nx; ++ii)
+      for (int v = 0; v < NVAR; ++v) V[iC(o, v, ii, jj)] = o->V[iU(o, v, ii, jj)];
+}
+
 void orc_get_U(const orc_t *o, double *U) {
   for (int jj = 1; jj <= o->ny; ++jj)
     for (int ii = 1; ii <= o->nx; ++ii)

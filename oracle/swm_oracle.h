@@ -39,6 +39,9 @@ void orc_init_wb(orc_t *o, const double *Uwb);
 
 void orc_set_U(orc_t *o, const double *U);
 void orc_get_U(const orc_t *o, double *U);
+/* The module array V (3, nx, ny) as the last FVTD left it: the primitives
+ * that the reference's output writer prints (src/output.f90:79). */
+void orc_get_V(const orc_t *o, double *V);
 
 /* One FVTimeDerivative on the current U (src/finitevolume.f90:402-440).
  * Any output pointer may be NULL. */

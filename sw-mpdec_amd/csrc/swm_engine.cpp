@@ -152,6 +152,8 @@ struct Job {
   const SchemeDef* scheme = &kSchemes[0];  // time scheme of engine / multi-step runs
   int hp_assoc = SWM_HP_AS_REFERENCE;     // depth view of the two-node schemes
   Tables own;          // tables covering this job alone
+  double* vout_buf = nullptr;  // the writer's state (dev.vout), when kept
+  bool vout_valid = false;     // vout holds the state of the last step
 };
 
 // Jobs step together when they share a scheme and, for the two-node schemes,
@@ -204,6 +206,7 @@ struct swm_engine {
   int tune_phase = 0;            // autotuner: candidates timed so far
   double tune_ms[kTuneCands] = {};
   long long graph_epoch = 0;     // bumped whenever captured arguments change
+  int keep_count = 0;            // jobs that keep the writer's state (k_keep_state launched)
   long long spec_steps = 0, redone_steps = 0, graph_launches = 0, graph_captures = 0;
   struct Captured {
     long long epoch = -1;
@@ -567,6 +570,9 @@ int step_body(swm_engine* e, Tables& T, const StepPlan& P) {
   L0.slot[0] = 0;
   int rc = fvtd_launch(e, T, L0);
   if (rc) return rc;
+  // the writer's state: the input of the step's last FVTD (U^n when the
+  // scheme has a single correction)
+  if (e->keep_count && D.kcorr == 1) launch_keep_state(a, T.dcb.p, T.ncb, 0, s);
   SysList S{};
   for (int r = 0; r < 4; ++r) S.src[r] = 0;
   S.hp_is_un = 1;
@@ -633,6 +639,8 @@ int step_body(swm_engine* e, Tables& T, const StepPlan& P) {
       }
       rc = fvtd_launch(e, T, L);
       if (rc) return rc;
+      if (e->keep_count && kk == D.kcorr - 1)
+        launch_keep_state(a, T.dcb.p, T.ncb, D.msteps - 1, s);
       for (int r = 0; r < 4; ++r) S.src[r] = r;
       S.hp_is_un = 0;
     }
@@ -836,6 +844,7 @@ int engine_step(swm_engine* e, Tables& T, const SchemeDef* sc, int hp_assoc,
     }
     J.have_pred = true;
     J.pred_sc = sc;
+    J.vout_valid = J.vout_buf != nullptr && !is_explicit(*sc);  // kept by the Patankar bodies
   }
   if (iters && T.njobs == 1)
     for (int i = 0; i < sc->nsolve; ++i) iters[i] = e->hstatus[T.jl[0]].iters[i];
@@ -1239,6 +1248,12 @@ void engine_free_job(swm_engine* e, int id) {
   Job& J = *e->jobs[id];
   if (J.state == JOB_FREE) return;
   (void)hipStreamSynchronize(e->stream);
+  if (J.vout_buf) {
+    (void)hipFree(J.vout_buf);
+    J.vout_buf = nullptr;
+    J.vout_valid = false;
+    --e->keep_count;
+  }
   if (J.slab) (void)hipFree(J.slab);
   J.slab = nullptr;
   e->in_use -= (long long)J.lay.bytes;
@@ -1618,6 +1633,47 @@ int swm_download_primitive(swm_solver_t* s, double* V) {
   if (er == hipSuccess) er = hipStreamSynchronize(e->stream);
   (void)hipFree(tmp);
   if (er != hipSuccess) FAIL(e, SWM_E_HIP, "primitive download failed: %s", hipGetErrorString(er));
+  return SWM_OK;
+}
+
+int swm_keep_output_state(swm_solver_t* s, int on) {
+  if (!s) return SWM_E_ARG;
+  swm_engine* e = s->e;
+  Job& J = *plain_job(s);
+  if ((on != 0) == (J.vout_buf != nullptr)) return SWM_OK;
+  if (on) {
+    HIPCK(e, hipMalloc((void**)&J.vout_buf, 3 * J.lay.n * sizeof(double)));
+    ++e->keep_count;
+  } else {
+    HIPCK(e, hipStreamSynchronize(e->stream));
+    (void)hipFree(J.vout_buf);
+    J.vout_buf = nullptr;
+    --e->keep_count;
+  }
+  J.dev.vout = J.vout_buf;
+  J.vout_valid = false;
+  e->dirty = true;  // the descriptor changed; captured steps are rebuilt
+  return SWM_OK;
+}
+
+int swm_download_output_primitive(swm_solver_t* s, double* V) {
+  if (!s) return SWM_E_ARG;
+  swm_engine* e = s->e;
+  if (!V) FAIL(e, SWM_E_ARG, "V is NULL");
+  Job& J = *plain_job(s);
+  if (!J.vout_valid)
+    FAIL(e, SWM_E_STATE,
+         "no kept output state: swm_keep_output_state before a step of a Patankar scheme");
+  const size_t n = J.lay.n;
+  double* tmp = nullptr;
+  HIPCK(e, hipMalloc((void**)&tmp, 3 * n * sizeof(double)));
+  launch_primitives(n, J.vout_buf, J.vout_buf + n, J.vout_buf + 2 * n, tmp, e->stream);
+  hipError_t er = hipGetLastError();
+  if (er == hipSuccess)
+    er = hipMemcpyAsync(V, tmp, 3 * n * sizeof(double), hipMemcpyDeviceToHost, e->stream);
+  if (er == hipSuccess) er = hipStreamSynchronize(e->stream);
+  (void)hipFree(tmp);
+  if (er != hipSuccess) FAIL(e, SWM_E_HIP, "output download failed: %s", hipGetErrorString(er));
   return SWM_OK;
 }
 

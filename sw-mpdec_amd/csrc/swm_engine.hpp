@@ -73,6 +73,7 @@ struct JobDev {
   int nx, ny, n;
   int wb, bathy;
   int recfix;     // ReconstructionFix on flagged cells: 0 none (= WENO5), 1 first order, 2 MUSCL
+  double* vout;   // 3n: h, hu, hv of the last FVTD input of a step (the writer's state), or null
   int nblk;       // cell blocks (kCellBlock cells) of this job
   double dx, dy, g, cfl;
   double wbnd[4];
@@ -227,6 +228,8 @@ void launch_combine(const EngineArgs& E, const TileRef* cb, int ncb, const CombO
 void launch_solve_done(const EngineArgs& E, const int* jl, int njobs, const SysList& S,
                        int first_solve, hipStream_t s);
 void launch_copy_wb(const EngineArgs& E, const TileRef* cb, int ncb, int slot, hipStream_t s);
+// node m's state into vout, for the jobs that keep the writer's state
+void launch_keep_state(const EngineArgs& E, const TileRef* cb, int ncb, int m, hipStream_t s);
 void launch_aos_to_soa(int nx, int ny, int ghost, const double* aos, double* h, double* hu,
                        double* hv, hipStream_t s);
 void launch_soa_to_aos(int nx, int ny, const double* h, const double* hu, const double* hv,

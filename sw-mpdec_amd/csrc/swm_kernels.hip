@@ -1108,6 +1108,27 @@ __global__ void k_weno5(int n, const double* q, double* first, double* second) {
 }
 
 // ------------------------------------------------------------------------
+// The state the reference's output writer prints: WriteSolutionToDisk
+// (src/output.f90:79) writes the module array V, which the last FVTD of the
+// step filled from its input -- for mPDeC5 the fourth DeC node after the
+// fourth correction (src/timediscretization.f90:548-556), not U^{n+1}.  A copy
+// of that node's state for the jobs that asked for it (vout != null).
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_keep_state(EngineArgs E, const TileRef* blocks, int m) {
+  const TileRef T = blocks[blockIdx.x];
+  const JobDev& J = E.jobs[T.job];
+  if (!J.vout) return;
+  const int k = T.tile * kCellBlock + threadIdx.x;
+  if (k >= J.n) return;
+  const gdouble *h, *hu, *hv;
+  node_state(J, m, h, hu, hv);
+  gdouble* o = G(J.vout);
+  o[k] = h[k];
+  o[J.n + k] = hu[k];
+  o[2 * (size_t)J.n + k] = hv[k];
+}
+
+// ------------------------------------------------------------------------
 // launch wrappers
 // ------------------------------------------------------------------------
 static inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
@@ -1134,6 +1155,10 @@ void launch_fvtd(const EngineArgs& E, int shape, const TileRef* xt, int nxt, con
   if (nct)
     hipLaunchKernelGGL(k_fvtd_update, dim3(cdiv(nct, 8) * 8 * L.count), dim3(256), 0, s, E, ct,
                        nct, L);
+}
+
+void launch_keep_state(const EngineArgs& E, const TileRef* cb, int ncb, int m, hipStream_t s) {
+  if (ncb) hipLaunchKernelGGL(k_keep_state, dim3(ncb), dim3(256), 0, s, E, cb, m);
 }
 
 void launch_momentum(const EngineArgs& E, const TileRef* cb, int ncb, const SysList& S,

@@ -77,6 +77,8 @@ def lib():
     L.swm_l1_error.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
     L.swm_download_primitive.argtypes = [vp, _dp]
     L.swm_download_primitive_async.argtypes = [vp, _dp]
+    L.swm_keep_output_state.argtypes = [vp, ctypes.c_int]
+    L.swm_download_output_primitive.argtypes = [vp, _dp]
     L.swm_output_wait.argtypes = [vp]
     L.swm_run_steps.argtypes = [vp, ctypes.c_int, _dp, _ip]
     L.swm_fvtd.argtypes = [vp] + [_dp] * 6
@@ -224,6 +226,18 @@ class Solver:
         """(ny, nx, 3) primitives (h, u, v) of the state."""
         V = np.empty((self.ny, self.nx, 3))
         self._check(lib().swm_download_primitive(self._h, _ptr(V)), "swm_download_primitive")
+        return V
+
+    def keep_output_state(self, on=True):
+        """Keep, from now on, the input of each step's last FVTD (the state
+        the reference's writer prints)."""
+        self._check(lib().swm_keep_output_state(self._h, int(bool(on))), "swm_keep_output_state")
+
+    def download_output_primitive(self):
+        """(ny, nx, 3): the reference writer's V after the last step."""
+        V = np.empty((self.ny, self.nx, 3))
+        self._check(lib().swm_download_output_primitive(self._h, _ptr(V)),
+                    "swm_download_output_primitive")
         return V
 
     def download_primitive_async(self, V):

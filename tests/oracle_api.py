@@ -54,6 +54,7 @@ def oracle_lib():
         L.orc_init_wb.argtypes = [ctypes.c_void_p, _dp]
         L.orc_set_U.argtypes = [ctypes.c_void_p, _dp]
         L.orc_get_U.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_get_V.argtypes = [ctypes.c_void_p, _dp]
         L.orc_fvtd.argtypes = [ctypes.c_void_p] + [_dp] * 6
         L.orc_timestep.restype = ctypes.c_double
         L.orc_timestep.argtypes = [ctypes.c_void_p, _dp, _dp]
@@ -132,6 +133,12 @@ class Oracle:
         U = np.empty((self.rows, self.n, 3))
         oracle_lib().orc_get_U(self.h, _ptr(U))
         return U
+
+    def get_V(self):
+        """The primitives of the last FVTD input (the writer's V)."""
+        V = np.empty((self.rows, self.n, 3))
+        oracle_lib().orc_get_V(self.h, _ptr(V))
+        return V
 
     def fvtd(self):
         n, r = self.n, self.rows
@@ -230,6 +237,8 @@ def run_swref(problem, mode: int, steps: int = 0, jac=None, timeout=600, scheme:
         return raw
     if mode == 3:
         return tuple(raw[:3])
+    if mode == 8:
+        return raw.reshape(r, n, 3)
     if mode == 1:
         sizes = [("Ut", (r, n, 3)), ("FX", (r, n + 1, 3)), ("FY", (r + 1, n, 3)), ("S", (r, n, 3))]
         if patankar:

@@ -285,3 +285,35 @@ def test_async_primitive_download_is_a_snapshot():
     del V1, V2
     for ptr in pinned:
         assert hip.hipHostFree(ptr) == 0
+
+
+
+@pytest.mark.parametrize("scheme,by_node", [(5, False), (6, True), (7, True), (7, False)])
+@pytest.mark.parametrize("case", ["vortex", "lake", "rect"])
+def test_output_state_is_the_writers_v(case, scheme, by_node):
+    """swm_keep_output_state / swm_download_output_primitive: the primitives
+    of the last FVTD input of the step, bitwise the oracle's V (itself
+    bitwise the reference's, test_oracle_vs_reference.py), under every
+    execution mode's step bodies (graph replay included), and not the
+    primitives of the new state."""
+    p = {"vortex": lambda: P.vortex(24, 3), "lake": lambda: P.lake_flag1(20, True, pert=0.05),
+         "rect": lambda: P.vortex(17, 5, ny=26)}[case]()
+    for graph, margin in ((0, 1), (1, 1), (0, -2)):
+        s = swm.Solver.from_problem(p)
+        s.set_execution(graph, margin)
+        s.set_timescheme(scheme)
+        s.set_hp_association(swm.HP_BY_NODE if by_node else swm.HP_AS_REFERENCE)
+        with pytest.raises(swm.SwmError, match="kept output state"):
+            s.download_output_primitive()
+        s.keep_output_state()
+        o = Oracle(p, scheme, hp_by_node=by_node)
+        for _ in range(2):
+            dt, _, _ = s.timestep()
+            if not np.isfinite(dt):
+                break  # the reference view of mPDeC2 can turn NaN
+            s.step(dt)
+            o.step(dt)
+        V = s.download_output_primitive()
+        assert same_bits(V, o.get_V()), (graph, margin)
+        if scheme == 5:
+            assert not same_bits(V, s.download_primitive())
