@@ -107,12 +107,12 @@ int swm_download_primitive(swm_solver_t *h, double *V);
  * (src/output.f90:79) writes the module array V, which the last FVTD of the
  * step filled from its input: for mPDeC5 the fourth DeC node after the fourth
  * correction (src/timediscretization.f90:548-556), for mPDeC2 the second node
- * after the first, for mPEuler U^n -- not the new state.  With
- * swm_keep_output_state(h, 1) every following step of a Patankar scheme
- * (timescheme 5-7) keeps a copy of that input (3 n doubles of device memory);
+ * after the first, for mPEuler U^n, for the explicit schemes the last stage
+ * state -- not the new state.  With swm_keep_output_state(h, 1) every
+ * following step keeps a copy of that input (3 n doubles of device memory);
  * swm_download_output_primitive then gives its primitives in the (1:3, nx, ny)
- * layout, bitwise the reference's V.  SWM_E_STATE when no such step has run
- * since the state was kept (or the last step was an explicit scheme). */
+ * layout, bitwise the reference's V.  SWM_E_STATE when no step has run since
+ * the state was kept. */
 int swm_keep_output_state(swm_solver_t *h, int on);
 int swm_download_output_primitive(swm_solver_t *h, double *V);
 

@@ -471,11 +471,17 @@ int explicit_body(swm_engine* e, Tables& T, const SchemeDef& D) {
   auto comb = [&](const CombOp& op) {
     timed(e, 3, [&] { launch_combine(a, T.dcb.p, T.ncb, op, s); });
   };
+  // the writer's state: the input of the step's last FVTD, before a stage
+  // combination overwrites its slot
+  auto keep = [&](int m) {
+    if (e->keep_count) launch_keep_state(a, T.dcb.p, T.ncb, m, s);
+  };
   constexpr int X = 0, F = 1;
   int rc = SWM_OK;
   switch (D.timescheme) {
     case 1:  // ForwardEuler (:198-232): U = K0 + Ut*dt
       if ((rc = fvtd({{0, 0}}))) return rc;
+      keep(0);
       comb(comb_op(0, 1, 0, {{X, 0, 1.0}, {F, 0, 1.0}}));
       break;
     case 2:  // SSPRK4 (:104-192)
@@ -492,6 +498,7 @@ int explicit_body(swm_engine* e, Tables& T, const SchemeDef& D) {
                    {{X, 0, 0.17807995410773}, {X, 3, 0.82192004589227}, {F, 0, 0.54497475021237}}));
       // K5 = Ut of stage 4 stays in derivative slot 0; stage 5's goes to 1
       if ((rc = fvtd({{4, 1}}))) return rc;
+      keep(4);
       comb(comb_op(0, 1, 0,
                    {{X, 0, 0.00683325884039}, {X, 2, 0.51723167208978}, {X, 3, 0.12759831133288},
                     {X, 4, 0.34833675773694}, {F, 0, 0.08460416338212},
@@ -510,6 +517,7 @@ int explicit_body(swm_engine* e, Tables& T, const SchemeDef& D) {
         comb(op);
         if ((rc = fvtd({{1, st}}))) return rc;
       }
+      keep(1);
       CombOp op = comb_op(1, 1, 0, {});
       for (int j = 0; j < 6; ++j) {
         op.deriv[op.nterm] = F;
@@ -540,6 +548,7 @@ int explicit_body(swm_engine* e, Tables& T, const SchemeDef& D) {
         comb(op);
         if (kk != 5) {
           if ((rc = fvtd({{1, 1}, {2, 2}, {3, 3}}))) return rc;
+          if (kk == 4) keep(3);
           for (int r = 0; r < 4; ++r) src[r] = r;
         }
       }
@@ -844,7 +853,7 @@ int engine_step(swm_engine* e, Tables& T, const SchemeDef* sc, int hp_assoc,
     }
     J.have_pred = true;
     J.pred_sc = sc;
-    J.vout_valid = J.vout_buf != nullptr && !is_explicit(*sc);  // kept by the Patankar bodies
+    J.vout_valid = J.vout_buf != nullptr;  // kept by every step body
   }
   if (iters && T.njobs == 1)
     for (int i = 0; i < sc->nsolve; ++i) iters[i] = e->hstatus[T.jl[0]].iters[i];

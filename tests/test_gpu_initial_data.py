@@ -288,7 +288,8 @@ def test_async_primitive_download_is_a_snapshot():
 
 
 
-@pytest.mark.parametrize("scheme,by_node", [(5, False), (6, True), (7, True), (7, False)])
+@pytest.mark.parametrize("scheme,by_node", [(5, False), (6, True), (7, True), (7, False),
+                                            (1, False), (2, False), (3, False), (4, False)])
 @pytest.mark.parametrize("case", ["vortex", "lake", "rect"])
 def test_output_state_is_the_writers_v(case, scheme, by_node):
     """swm_keep_output_state / swm_download_output_primitive: the primitives
@@ -315,5 +316,5 @@ def test_output_state_is_the_writers_v(case, scheme, by_node):
             o.step(dt)
         V = s.download_output_primitive()
         assert same_bits(V, o.get_V()), (graph, margin)
-        if scheme == 5:
+        if scheme in (2, 3, 4, 5):
             assert not same_bits(V, s.download_primitive())

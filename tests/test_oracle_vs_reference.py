@@ -366,3 +366,19 @@ def test_writer_state_bitwise(case, scheme):
             break  # the reference's loop stops there too
         o.step(dt)
     assert same_bits(o.get_V(), V_ref)
+
+
+
+@explicit
+@pytest.mark.parametrize("scheme", [1, 2, 3, 4])
+@pytest.mark.parametrize("case", ["vortex21", "lakeflag1_wb14"])
+def test_writer_state_explicit_bitwise(case, scheme):
+    """The same for the explicit schemes of the build without PATANKAR: V is
+    the last stage state's primitives."""
+    p = CASES[case]()
+    V_ref = run_swref(p, 8, steps=2, scheme=scheme)
+    o = Oracle(p, scheme)
+    for _ in range(2):
+        dt, _, _ = o.timestep()
+        o.step(dt)
+    assert same_bits(o.get_V(), V_ref)
