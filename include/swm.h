@@ -338,6 +338,10 @@ int swm_engine_fetch(swm_engine_t *e, int job_id, double *U);
  * its initial condition's exact solution without fetching the state. */
 int swm_engine_l1_error(swm_engine_t *e, int job_id, int initial_condition, int bathymetry_flag,
                         double t, const double x0[2], const double x1[2], double err[3]);
+/* swm_keep_output_state / swm_download_output_primitive for a job: the state
+ * the reference's writer prints, per served request. */
+int swm_engine_keep_output_state(swm_engine_t *e, int job_id, int on);
+int swm_engine_fetch_output_primitive(swm_engine_t *e, int job_id, double *V);
 /* Frees the job's device memory; its id may be reused. */
 int swm_engine_release(swm_engine_t *e, int job_id);
 int swm_engine_set_jacobi_limits(swm_engine_t *e, int max_iter, double tol);

@@ -1645,10 +1645,8 @@ int swm_download_primitive(swm_solver_t* s, double* V) {
   return SWM_OK;
 }
 
-int swm_keep_output_state(swm_solver_t* s, int on) {
-  if (!s) return SWM_E_ARG;
-  swm_engine* e = s->e;
-  Job& J = *plain_job(s);
+// Keeping (or not) the writer's state of job J (swm_keep_output_state).
+static int job_keep_output(swm_engine* e, Job& J, int on) {
   if ((on != 0) == (J.vout_buf != nullptr)) return SWM_OK;
   if (on) {
     HIPCK(e, hipMalloc((void**)&J.vout_buf, 3 * J.lay.n * sizeof(double)));
@@ -1665,11 +1663,9 @@ int swm_keep_output_state(swm_solver_t* s, int on) {
   return SWM_OK;
 }
 
-int swm_download_output_primitive(swm_solver_t* s, double* V) {
-  if (!s) return SWM_E_ARG;
-  swm_engine* e = s->e;
+// The primitives of job J's kept writer state (swm_download_output_primitive).
+static int job_output_primitive(swm_engine* e, Job& J, double* V) {
   if (!V) FAIL(e, SWM_E_ARG, "V is NULL");
-  Job& J = *plain_job(s);
   if (!J.vout_valid)
     FAIL(e, SWM_E_STATE,
          "no kept output state: swm_keep_output_state before a step of a Patankar scheme");
@@ -1684,6 +1680,32 @@ int swm_download_output_primitive(swm_solver_t* s, double* V) {
   (void)hipFree(tmp);
   if (er != hipSuccess) FAIL(e, SWM_E_HIP, "output download failed: %s", hipGetErrorString(er));
   return SWM_OK;
+}
+
+int swm_keep_output_state(swm_solver_t* s, int on) {
+  if (!s) return SWM_E_ARG;
+  return job_keep_output(s->e, *plain_job(s), on);
+}
+
+int swm_download_output_primitive(swm_solver_t* s, double* V) {
+  if (!s) return SWM_E_ARG;
+  return job_output_primitive(s->e, *plain_job(s), V);
+}
+
+int swm_engine_keep_output_state(swm_engine_t* e, int job_id, int on) {
+  if (!e) return SWM_E_ARG;
+  Job* J;
+  int rc = job_from_id(e, job_id, &J);
+  if (rc) return rc;
+  return job_keep_output(e, *J, on);
+}
+
+int swm_engine_fetch_output_primitive(swm_engine_t* e, int job_id, double* V) {
+  if (!e) return SWM_E_ARG;
+  Job* J;
+  int rc = job_from_id(e, job_id, &J);
+  if (rc) return rc;
+  return job_output_primitive(e, *J, V);
 }
 
 int swm_download_primitive_async(swm_solver_t* s, double* V) {

@@ -435,6 +435,8 @@ def _engine_lib():
     L.swm_engine_fetch.argtypes = [vp, ctypes.c_int, _dp]
     L.swm_engine_l1_error.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                       _dp, _dp, _dp]
+    L.swm_engine_keep_output_state.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.swm_engine_fetch_output_primitive.argtypes = [vp, ctypes.c_int, _dp]
     L.swm_engine_release.argtypes = [vp, ctypes.c_int]
     L.swm_engine_set_jacobi_limits.argtypes = [vp, ctypes.c_int, ctypes.c_double]
     L.swm_engine_profile_enable.argtypes = [vp, ctypes.c_int]
@@ -574,6 +576,17 @@ class Engine:
                                                       float(t), _ptr(a0), _ptr(a1), _ptr(err)),
                     "swm_engine_l1_error")
         return err
+
+    def keep_output_state(self, jid, on=True):
+        self._check(_engine_lib().swm_engine_keep_output_state(self._e, int(jid), int(bool(on))),
+                    "swm_engine_keep_output_state")
+
+    def fetch_output_primitive(self, jid):
+        rows, n = self._keep[jid]
+        V = np.empty((rows, n, 3))
+        self._check(_engine_lib().swm_engine_fetch_output_primitive(self._e, int(jid), _ptr(V)),
+                    "swm_engine_fetch_output_primitive")
+        return V
 
     def release(self, jid):
         self._check(_engine_lib().swm_engine_release(self._e, jid), "swm_engine_release")

@@ -318,3 +318,31 @@ def test_output_state_is_the_writers_v(case, scheme, by_node):
         assert same_bits(V, o.get_V()), (graph, margin)
         if scheme in (2, 3, 4, 5):
             assert not same_bits(V, s.download_primitive())
+
+
+
+def test_engine_jobs_keep_the_writers_v():
+    """Per-job writer state in a mixed engine: jobs that keep it (of several
+    schemes, sharing launches with jobs that do not) give the oracle's V."""
+    jobs = [(P.vortex(20, 1), 5, True), (P.lake_flag1(18, True, pert=0.05), 5, False),
+            (P.vortex(22, 2), 2, True), (P.vortex(16, 3, ny=24), 7, True),
+            (P.lake_bumps(20, 4), 5, True)]
+    eng = swm.Engine()
+    ids = []
+    for p, sc, keep in jobs:
+        jid = eng.submit(p, 2, timescheme=sc, hp_association=swm.HP_BY_NODE)
+        if keep:
+            eng.keep_output_state(jid)
+        ids.append(jid)
+    while eng.step():
+        pass
+    for (p, sc, keep), jid in zip(jobs, ids):
+        if not keep:
+            with pytest.raises(swm.SwmError, match="kept output state"):
+                eng.fetch_output_primitive(jid)
+            continue
+        o = Oracle(p, sc, hp_by_node=True)
+        for _ in range(2):
+            dt, _, _ = o.timestep()
+            o.step(dt)
+        assert same_bits(eng.fetch_output_primitive(jid), o.get_V()), (p.kind, sc)
