@@ -18,6 +18,7 @@ PUBLIC :: swm_create, swm_destroy, swm_set_bathy_grad, swm_init_wb
 PUBLIC :: swm_upload, swm_download, swm_timestep, swm_step_mpdec5, swm_advance
 PUBLIC :: swm_step_mpeuler, swm_step_mpdec2, swm_set_timescheme, swm_set_hp_association
 PUBLIC :: swm_set_reconstruction_fix, swm_fill_initial_data, swm_l1_error, swm_download_primitive
+PUBLIC :: swm_keep_output_state, swm_download_output_primitive
 PUBLIC :: swm_step_forward_euler, swm_step_ssprk4, swm_step_rk65, swm_step_dec5
 PUBLIC :: swm_error_message, swm_check
 
@@ -155,6 +156,20 @@ INTERFACE
     REAL(C_DOUBLE), VALUE :: t
     REAL(C_DOUBLE), INTENT(IN) :: x0(2), x1(2)
     REAL(C_DOUBLE), INTENT(OUT) :: err(3)
+  END FUNCTION
+
+  ! the state WriteSolutionToDisk prints (src/output.f90:79): the input of
+  ! each step's last FVTD, kept from now on; then its primitives V(1:3,nx,ny)
+  INTEGER(C_INT) FUNCTION swm_keep_output_state(h, on) BIND(C, NAME='swm_keep_output_state')
+    IMPORT :: C_INT, C_PTR
+    TYPE(C_PTR), VALUE :: h
+    INTEGER(C_INT), VALUE :: on
+  END FUNCTION
+  INTEGER(C_INT) FUNCTION swm_download_output_primitive(h, V) &
+      BIND(C, NAME='swm_download_output_primitive')
+    IMPORT :: C_INT, C_DOUBLE, C_PTR
+    TYPE(C_PTR), VALUE :: h
+    REAL(C_DOUBLE) :: V(*)
   END FUNCTION
 
   ! ConsToPrim of the state, V(1:3,nx,ny), for the output writer

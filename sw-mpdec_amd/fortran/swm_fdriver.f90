@@ -21,7 +21,8 @@
 !   int32  nsteps_taken, nonconverged_total
 !   real64 dt(nsteps_taken), U(3,nx,ny)
 !   real64 L1error(3) at the final time [initial_condition > 0: PostProcessing,
-!          src/postprocessing.f90:24-64, by swm_l1_error]
+!          src/postprocessing.f90:24-64, by swm_l1_error], then V(3,nx,ny), the
+!          primitives WriteSolutionToDisk prints (swm_download_output_primitive)
 ! Usage: swm_fdriver <input> <output>
 !===============================================================================!
 PROGRAM swm_fdriver
@@ -34,7 +35,7 @@ INTEGER(C_INT) :: iters(13)
 REAL(C_DOUBLE) :: par(12), dx, dy, g, cfl, wgp(4), t, dt, lmx, lmy, x0(2), x1(2), l1(3)
 REAL(C_DOUBLE) :: tend, tanalyze, dtanalyze
 REAL(C_DOUBLE), ALLOCATABLE, TARGET :: chunk(:)
-REAL(C_DOUBLE), ALLOCATABLE :: U(:,:,:), Uwb(:,:,:), bx(:,:,:,:), by(:,:,:,:), dts(:)
+REAL(C_DOUBLE), ALLOCATABLE :: U(:,:,:), Uwb(:,:,:), bx(:,:,:,:), by(:,:,:,:), dts(:), V(:,:,:)
 TYPE(C_PTR) :: h
 CHARACTER(LEN=1024) :: fin, fout
 INTEGER :: uin, uout
@@ -75,6 +76,7 @@ IF (ic .GT. 0) THEN
   ! FillInitialConditions / InitializeWBVariables / the bathymetry of
   ! src/main.f90:22-26, on the device
   CALL swm_check(h, swm_fill_initial_data(h, ic, bflag, x0, x1), "swm_fill_initial_data")
+  CALL swm_check(h, swm_keep_output_state(h, 1_C_INT), "swm_keep_output_state")
 ELSE
   IF (hasb .NE. 0) THEN
     CALL swm_check(h, swm_set_bathy_grad(h, bx, by), "swm_set_bathy_grad")
@@ -149,6 +151,9 @@ CALL swm_check(h, swm_download(h, U), "swm_download")
 IF (ic .GT. 0) THEN
   ! PostProcessing at the final time (src/main.f90:28)
   CALL swm_check(h, swm_l1_error(h, ic, bflag, t, x0, x1, l1), "swm_l1_error")
+  ! the writer's V after the last step (src/output.f90:79)
+  ALLOCATE(V(3,nx,ny))
+  CALL swm_check(h, swm_download_output_primitive(h, V), "swm_download_output_primitive")
 END IF
 CALL swm_destroy(h)
 
@@ -156,7 +161,7 @@ OPEN(NEWUNIT=uout, FILE=TRIM(fout), ACCESS='STREAM', FORM='UNFORMATTED', STATUS=
 WRITE(uout) nrec, nonconv
 WRITE(uout) dts(1:nrec)
 WRITE(uout) U
-IF (ic .GT. 0) WRITE(uout) l1
+IF (ic .GT. 0) WRITE(uout) l1, V
 CLOSE(uout)
 END PROGRAM swm_fdriver
 !===============================================================================!

@@ -189,8 +189,8 @@ def test_fortran_driver_reconstruction_fix(tmp_path, recfix, scheme, make):
 def test_fortran_driver_initial_data_and_l1_error(tmp_path, ic, wb, scheme):
     """The driver's initial_condition header: swm_fill_initial_data instead of
     host data, the time loop, then PostProcessing's L1 error by swm_l1_error
-    at the final time -- all through ISO_C_BINDING -- against the same
-    calls from Python (bitwise).  (The wet-dry lake 3 runs mPDeC5: the
+    at the final time and the writer's V -- all through ISO_C_BINDING --
+    against the same calls from Python (bitwise).  (The wet-dry lake 3 runs mPDeC5: the
     explicit schemes are not positivity preserving, and SSPRK4 turns its dry
     cells negative within three steps, in the reference as here.)"""
     if not FDRIVER.exists():
@@ -209,8 +209,11 @@ def test_fortran_driver_initial_data_and_l1_error(tmp_path, ic, wb, scheme):
     raw = open(fout, "rb").read()
     nsteps = int(np.frombuffer(raw[:4], dtype=np.int32)[0])
     body = np.frombuffer(raw[8:], dtype=np.float64)
-    dts, U, l1 = body[:nsteps], body[nsteps:nsteps + 3 * c.nx * c.ny], body[-3:]
+    n3 = 3 * c.nx * c.ny
+    dts, U = body[:nsteps], body[nsteps:nsteps + n3]
+    l1, V = body[nsteps + n3:nsteps + n3 + 3], body[nsteps + n3 + 3:]
     s = swm.Solver.from_setup(c)
+    s.keep_output_state()
     s.set_timescheme(scheme)
     t = 0.0
     for k in range(steps):
@@ -220,3 +223,4 @@ def test_fortran_driver_initial_data_and_l1_error(tmp_path, ic, wb, scheme):
         t = t + dt
     assert same_bits(U.reshape(c.ny, c.nx, 3), s.download())
     assert same_bits(l1, s.l1_error(ic, c.bathy_flag, c.x0, c.x1, t))
+    assert same_bits(V.reshape(c.ny, c.nx, 3), s.download_output_primitive())
