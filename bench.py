@@ -284,6 +284,34 @@ def initial_data_section(n: int, cpu: bool) -> dict:
     return res
 
 
+def served_ic_section(count: int, seed: int, rate: float) -> dict:
+    """Continuous batching of `count` requests that name one of the
+    reference's initial conditions (1-6) and carry no host data: each job's
+    initial data are made on the device at admission (swm_fill_initial_data),
+    against the sequential plain solver doing the same per request.  N ~
+    logU[64, 512], 20..200 CFL steps, mPDeC5 (FIFO admission)."""
+    import serve
+
+    reqs = serve.reference_mix(count, seed, rate, nmin=64, nmax=512, smin=20, smax=200)
+    work = sum(r.problem.nx * r.problem.ny * r.nsteps for r in reqs)
+    sv = serve.serve(reqs, budget_bytes=0, keep_results=True, log=log)
+    served = [r.result for r in reqs]
+    for r in reqs:
+        r.result, r.done, r.admitted = None, float("nan"), float("nan")
+    sq = serve.sequential(reqs, keep_results=True)
+    same = all(np.array_equal(a.view(np.uint64), r.result.view(np.uint64))
+               for a, r in zip(served, reqs))
+    log(f"bench: served by initial condition: {sv['jobs_per_s']:.2f} jobs/s, "
+        f"sequential {sq['jobs_per_s']:.2f} jobs/s, identical={same}")
+    return {"workload": f"{count} requests by InitialCondition 1-6, Poisson {rate}/s, "
+                        f"N~logU[64,512], steps~U{{20..200}}, seed {seed}",
+            "cell_steps": work, "jobs_per_s": sv["jobs_per_s"],
+            "p50_latency_s": sv["p50_latency_s"], "p99_latency_s": sv["p99_latency_s"],
+            "failed": sv["failed"], "sequential_jobs_per_s": sq["jobs_per_s"],
+            "speedup_vs_sequential": sv["jobs_per_s"] / sq["jobs_per_s"],
+            "bitwise_identical_to_sequential": bool(same)}
+
+
 def batched_section(count: int, seed: int) -> dict:
     """`count` uneven solves (N = round(exp(U[ln 32, ln 512])), 20..200 CFL
     steps, half vortices, half well-balanced lakes over bumps) run together
@@ -391,6 +419,8 @@ def main():
                     help="jobs of the batched run (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=48,
                     help="requests of the served run (0: skip)")
+    ap.add_argument("--serve-ic-requests", type=int, default=24,
+                    help="requests of the served run by initial condition (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0,
                     help="Poisson arrival rate of the served run, per second")
     ap.add_argument("--serve-rates", type=str, default="",
@@ -506,6 +536,9 @@ def main():
         out["batched"] = batched_section(args.batch_jobs, args.seed)
     if args.serve_requests > 0:
         out["served"] = served_section(args.serve_requests, args.seed, args.serve_rate)
+        if args.serve_ic_requests > 0:
+            out["served_by_initial_condition"] = served_ic_section(args.serve_ic_requests,
+                                                                   args.seed, args.serve_rate)
         rates = [float(r) for r in args.serve_rates.split(",") if r.strip()]
         if rates:
             out["served_sweep"] = [served_section(args.serve_requests, args.seed, r)
