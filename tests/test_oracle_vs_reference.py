@@ -382,3 +382,22 @@ def test_writer_state_explicit_bitwise(case, scheme):
         dt, _, _ = o.timestep()
         o.step(dt)
     assert same_bits(o.get_V(), V_ref)
+
+
+def test_reference_dam_break_reaches_a_non_finite_time_step():
+    """The reference's own circular dam break 1 (InitialCondition 4, its
+    domain [0,40]^2, 64^2 cells) under mPDeC5 reaches a non-finite TimeStep
+    at step 40 (index 39); the oracle follows it dt for dt.  Served requests
+    of this initial condition therefore end FAILED, as in the reference."""
+    x0, x1, g, bf = IC_SETUPS[4]
+    U = oracle_initial_data(4, 64, 64, x0, x1, g, bf, False)[0]
+    p = P.Problem(64, U, gravity=g, x0=x0[0], x1=x1[0])
+    hist, _ = run_swref(p, 2, steps=45)
+    o = Oracle(p)
+    for k, row in enumerate(hist):
+        dt, lx, ly = o.timestep()
+        assert same_bits([dt, lx, ly], row), k
+        if not np.isfinite(dt):
+            break
+        o.step(dt)
+    assert k == 39 and not np.isfinite(hist[-1][0])
