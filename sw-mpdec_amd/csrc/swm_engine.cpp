@@ -1261,6 +1261,7 @@ void engine_free_job(swm_engine* e, int id) {
     (void)hipFree(J.vout_buf);
     J.vout_buf = nullptr;
     J.vout_valid = false;
+    e->in_use -= (long long)(3 * J.lay.n * sizeof(double));
     --e->keep_count;
   }
   if (J.slab) (void)hipFree(J.slab);
@@ -1414,7 +1415,8 @@ int swm_engine_job_info(swm_engine_t* e, int job_id, swm_job_info* info) {
   info->t = J->t;
   info->jacobi_sweeps = J->jacobi_sweeps;
   info->nonconverged = J->nonconv;
-  info->bytes = (long long)J->lay.bytes;
+  info->bytes = (long long)J->lay.bytes + (J->bathy_buf ? (long long)bathy_bytes(J->lay.n) : 0) +
+                (J->vout_buf ? (long long)(3 * J->lay.n * sizeof(double)) : 0);
   return SWM_OK;
 }
 
@@ -1648,13 +1650,18 @@ int swm_download_primitive(swm_solver_t* s, double* V) {
 // Keeping (or not) the writer's state of job J (swm_keep_output_state).
 static int job_keep_output(swm_engine* e, Job& J, int on) {
   if ((on != 0) == (J.vout_buf != nullptr)) return SWM_OK;
+  const long long b = (long long)(3 * J.lay.n * sizeof(double));
   if (on) {
-    HIPCK(e, hipMalloc((void**)&J.vout_buf, 3 * J.lay.n * sizeof(double)));
+    if (e->budget > 0 && e->in_use + b > e->budget)
+      FAIL(e, SWM_E_BUDGET, "the kept output state of %lld bytes does not fit", b);
+    HIPCK(e, hipMalloc((void**)&J.vout_buf, (size_t)b));
+    e->in_use += b;
     ++e->keep_count;
   } else {
     HIPCK(e, hipStreamSynchronize(e->stream));
     (void)hipFree(J.vout_buf);
     J.vout_buf = nullptr;
+    e->in_use -= b;
     --e->keep_count;
   }
   J.dev.vout = J.vout_buf;

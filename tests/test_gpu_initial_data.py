@@ -346,3 +346,22 @@ def test_engine_jobs_keep_the_writers_v():
             dt, _, _ = o.timestep()
             o.step(dt)
         assert same_bits(eng.fetch_output_primitive(jid), o.get_V()), (p.kind, sc)
+
+
+def test_job_bytes_count_every_buffer():
+    """A job's reported bytes and the engine's use cover the state slab, the
+    bathymetry tables and a kept output state; the budget covers the last."""
+    p = P.lake_bumps(40, 1)  # well-balanced, with bathymetry
+    fp = swm.job_footprint_bytes(40, 40, True, True)
+    eng = swm.Engine()
+    jid = eng.submit(p, 1)
+    assert eng.info(jid)["bytes"] == fp == eng.bytes_in_use()
+    eng.keep_output_state(jid)
+    extra = 3 * 40 * 40 * 8
+    assert eng.info(jid)["bytes"] == fp + extra == eng.bytes_in_use()
+    eng.release(jid)
+    assert eng.bytes_in_use() == 0
+    tight = swm.Engine(budget_bytes=fp)
+    j2 = tight.submit(p, 1)
+    with pytest.raises(swm.SwmError, match="does not fit"):
+        tight.keep_output_state(j2)
