@@ -1787,7 +1787,8 @@ static int plain_step(swm_solver_t* s, int timescheme, double dt, int* jacobi_it
   swm_engine* e = s->e;
   Job& J = *plain_job(s);
   if (J.dev.wb && !J.wb_ready)
-    FAIL(e, SWM_E_STATE, "well-balanced state not initialised (swm_init_wb)");
+    FAIL(e, SWM_E_STATE,
+         "well-balanced state not initialised (swm_init_wb or swm_fill_initial_data)");
   const SchemeDef* sc = find_scheme(timescheme);
   int rc = refresh(e);
   if (rc) return rc;
@@ -1876,7 +1877,8 @@ int swm_run_steps(swm_solver_t* s, int nsteps, double* dts, int* jacobi_iters) {
   if (nsteps < 0) FAIL(e, SWM_E_ARG, "nsteps < 0");
   Job& J = *plain_job(s);
   if (J.dev.wb && !J.wb_ready)
-    FAIL(e, SWM_E_STATE, "well-balanced state not initialised (swm_init_wb)");
+    FAIL(e, SWM_E_STATE,
+         "well-balanced state not initialised (swm_init_wb or swm_fill_initial_data)");
   int rc = refresh(e);
   if (rc) return rc;
   const int before = J.nonconv;
@@ -1908,7 +1910,8 @@ int swm_advance(swm_solver_t* s, double* t, double t_analyze, double t_end, int 
   if (max_steps < 0) FAIL(e, SWM_E_ARG, "max_steps < 0");
   Job& J = *plain_job(s);
   if (J.dev.wb && !J.wb_ready)
-    FAIL(e, SWM_E_STATE, "well-balanced state not initialised (swm_init_wb)");
+    FAIL(e, SWM_E_STATE,
+         "well-balanced state not initialised (swm_init_wb or swm_fill_initial_data)");
   int rc = refresh(e);
   if (rc) return rc;
   const int before = J.nonconv;
