@@ -3,7 +3,9 @@
  * PATANKAR-build time step: timescheme=5 (mPDeC5) with Reconstruction=4
  * (WENO5 at 4-point Gauss nodes), periodic boundaries, FP64 -- and, on the
  * same kernels, the reference's other time schemes (timescheme 1-4 of the
- * build without PATANKAR, 6 and 7 of the PATANKAR build).
+ * build without PATANKAR, 6 and 7 of the PATANKAR build), its
+ * ReconstructionFix 1 and 2, and the steps before and after the time loop:
+ * initial data, PostProcessing's L1 error and the writer's primitives.
  *
  * The reference (accdavlo/sw-mpdec, Fortran) has no FFI; its hot path sits
  * behind module procedures that share module-global state.  Each entry point
@@ -22,7 +24,9 @@
  * All functions return SWM_OK (0) or a negative SWM_E* code; the message is
  * available from swm_last_error().  Calls on one handle must come from one
  * host thread at a time.  The library owns device memory and one HIP stream
- * per handle; no host pointer is kept after a call returns.
+ * per handle (a second one once swm_download_primitive_async is used); no
+ * host pointer is kept after a call returns, except the destination of
+ * swm_download_primitive_async, until swm_output_wait.
  */
 #ifndef SWM_H
 #define SWM_H
