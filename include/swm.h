@@ -90,6 +90,41 @@ int swm_init_wb(swm_solver_t *h, const double *Uwb, int ghosts);
 int swm_fill_initial_data(swm_solver_t *h, int initial_condition, int bathymetry_flag,
                           const double x0[2], const double x1[2]);
 
+/* Initial data of the benchmark workloads made on the device (no counterpart
+ * in the reference, whose own initial conditions are swm_fill_initial_data's):
+ * on the periodic unit square [0,1]^2 (the handle's dx, dy must be 1/nx, 1/ny
+ * exactly), the 4x4-Gauss cell averages of
+ *   SWM_FAMILY_VORTEX      h = 1 - eps^2/(2g) exp(1 - r^2),
+ *                          (u, v) = eps exp((1 - r^2)/2) (-y', x'),
+ *                          x' = (x - 1/2)/0.1, y' = (y - 1/2)/0.1, r^2 = x'^2 + y'^2;
+ *   SWM_FAMILY_LAKE_BUMPS  lake at rest h + b = 1, u = v = 0, over
+ *                          b = sum_k A_k G(x; cx_k, s_k) G(y; cy_k, s_k), plus
+ *                          amp G(x; px, ps) G(y; py, ps) on h, where
+ *                          G(x; c, s) = sum over a = -1, 0, 1 of exp(-(x-c-a)^2/s^2)
+ *                          (periodic images), with the bathymetry tables of b's
+ *                          exact gradient and, for a well-balanced handle, the
+ *                          well-balanced state h = 1 - b and its FVTD;
+ *   SWM_FAMILY_DAM_BREAK   h = h_in where (x-1/2)^2 + (y-1/2)^2 < radius^2, else
+ *                          h_out; u = v = 0.
+ * exp is a fixed method made of IEEE operations only (error < 0.85 ulp;
+ * swm_debug_exp), so these data are bitwise the same as the host problem
+ * builder's on any host.  One call replaces swm_set_bathy_grad + swm_init_wb +
+ * swm_upload with host-built arrays of the same family. */
+#define SWM_FAMILY_VORTEX 1
+#define SWM_FAMILY_LAKE_BUMPS 2
+#define SWM_FAMILY_DAM_BREAK 3
+#define SWM_FAMILY_MAX_BUMPS 8
+typedef struct {
+  int kind;                  /* SWM_FAMILY_* */
+  int nbumps;                /* lake: bumps used, 0..SWM_FAMILY_MAX_BUMPS */
+  double eps;                /* vortex */
+  double A[SWM_FAMILY_MAX_BUMPS], cx[SWM_FAMILY_MAX_BUMPS], cy[SWM_FAMILY_MAX_BUMPS],
+      s[SWM_FAMILY_MAX_BUMPS];  /* lake: bumps */
+  double amp, px, py, ps;    /* lake: surface perturbation */
+  double radius, h_in, h_out;  /* dam break */
+} swm_family;
+int swm_fill_family(swm_solver_t *h, const swm_family *family);
+
 /* PostProcessing (src/postprocessing.f90:24-64) on the device: the L1 error
  * of the current state against the 4x4-Gauss cell averages of
  * ExactFunction(initial_condition, t), err[v] = (sum over cells of
@@ -309,6 +344,9 @@ typedef struct {
   int initial_condition;     /* 1..6 */
   int bathymetry_flag;       /* 0..2 */
   double mesh_x0[2], mesh_x1[2];
+  /* With U0 == NULL and family != NULL the job's initial data are made on the
+   * device as swm_fill_family does (Uwb, dbdx, dbdy, initial_condition unused). */
+  const swm_family *family;
 } swm_job_desc;
 
 #define SWM_JOB_RUNNING 0
@@ -386,6 +424,10 @@ int swm_debug_divsqrt(int n, const double *a, const double *b, double *q, double
  * [2^-400, 2^400]: q = a/b through one refined reciprocal of b shared by
  * every quotient with that denominator (no range check here). */
 int swm_debug_shared_div(int n, const double *a, const double *b, double *q);
+
+/* Self-check of the exponential of swm_fill_family: y = exp(x) by its fixed
+ * method, for comparison with the host problem builder's. */
+int swm_debug_exp(int n, const double *x, double *y);
 
 /* Self-check of the WENO5 arithmetic as the kernels evaluate it
  * (WENO5_FirstSweep, src/reconstruction.f90:673-755, and WENO5_SecondSweep,

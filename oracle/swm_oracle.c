@@ -978,7 +978,8 @@ void orc_set_jacobi_limits(orc_t *o, int max_iter, double tol) {
  * pattern of InitializeFiniteVolume (src/finitevolume.f90:219-267):
  * row K holds columns S, W, self, E, N at 5(K-1)+0..4. */
 static int jacobi_impl(const orc_t *o, const double *mat, const double *diag,
-                       const double *rhs, double *sol, double *solp, double *prod) {
+                       const double *rhs, double *sol, double *solp, double *prod,
+                       double *final_residual) {
   const int nx = o->nx, ny = o->ny;
   const int n = nx * ny;
   int k = 0;
@@ -1004,6 +1005,7 @@ static int jacobi_impl(const orc_t *o, const double *mat, const double *diag,
     residual = residual / (double)n;
     for (int i = 0; i < n; ++i) solp[i] = sol[i];
   }
+  if (final_residual) *final_residual = residual;
   return k;
 }
 
@@ -1011,7 +1013,7 @@ int orc_jacobi(const orc_t *o, const double *Mass, const double *diag, const dou
                double *sol) {
   size_t n = (size_t)o->nx * o->ny;
   double *solp = malloc(n * sizeof(double)), *prod = malloc(n * sizeof(double));
-  int k = jacobi_impl(o, Mass, diag, rhs, sol, solp, prod);
+  int k = jacobi_impl(o, Mass, diag, rhs, sol, solp, prod, NULL);
   free(solp);
   free(prod);
   return k;
@@ -1070,7 +1072,10 @@ static double hp_at(const orc_t *o, int msteps, int m, int ii, int jj) {
 
 /* MODIFIED_PATANKAR_MATRIX_INVERSION, src/timediscretization.f90:771-908.
  * m is the 0-based node (m_substep - 1); ha -> Ua(m,1). */
-static int patankar_solve(orc_t *o, const dec_scheme *sc, int m, double dt) {
+/* Returns the sweep count; *nonconv = 1 when the loop ended with the
+ * residual above the tolerance ("Jacobi not converged",
+ * src/jacobi_iteration.f90:82-84). */
+static int patankar_solve(orc_t *o, const dec_scheme *sc, int m, double dt, int *nonconv) {
   const int nx = o->nx, ny = o->ny, n = nx * ny, ms = sc->msteps;
   for (int k = 0; k < 5 * n; ++k) o->Mass[k] = 0.;
   for (int k = 0; k < n; ++k) o->Diag[k] = 1.;
@@ -1099,7 +1104,9 @@ static int patankar_solve(orc_t *o, const dec_scheme *sc, int m, double dt) {
       }
       o->rhs[K] = hp_at(o, ms, 1, ii, jj);
     }
-  int k = jacobi_impl(o, o->Mass, o->Diag, o->rhs, o->haVec, o->solp, o->prod);
+  double residual = 0.;
+  int k = jacobi_impl(o, o->Mass, o->Diag, o->rhs, o->haVec, o->solp, o->prod, &residual);
+  *nonconv = (o->tol < residual) ? 1 : 0;
   for (int jj = 1; jj <= ny; ++jj)
     for (int ii = 1; ii <= nx; ++ii) o->Ua[iN(o, m, 0, ii, jj)] = o->haVec[global_elem(o, ii, jj) - 1];
   return k;
@@ -1158,9 +1165,10 @@ int orc_step_patankar(orc_t *o, int timescheme, double dt, int *jacobi_iters) {
     int m0 = (kk == kc) ? ms - 1 : 1;
     for (int m = m0; m < ms; ++m) {
       momentum_update(o, sc, m, dt);
-      int it = patankar_solve(o, sc, m, dt);
+      int nc = 0;
+      int it = patankar_solve(o, sc, m, dt, &nc);
       if (jacobi_iters) jacobi_iters[nsolve] = it;
-      if (it >= o->max_iter) ++nfail;
+      nfail += nc;
       ++nsolve;
     }
     if (kk != kc) {

@@ -53,7 +53,8 @@ double orc_timestep(orc_t *o, double *lmx, double *lmy);
 
 /* One mPDeC5 step with the given dt (src/timediscretization.f90:458-566).
  * jacobi_iters[13] (may be NULL) receives the sweep count of each solve in
- * execution order.  Returns the number of solves that hit max_iter. */
+ * execution order.  Returns the number of solves that ended with the residual
+ * above the tolerance ("Jacobi not converged", src/jacobi_iteration.f90:82-84). */
 int orc_step_mpdec5(orc_t *o, double dt, int *jacobi_iters);
 
 /* One step of a modified-Patankar scheme, by the reference's timescheme

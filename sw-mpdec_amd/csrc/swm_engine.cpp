@@ -204,7 +204,8 @@ struct swm_engine {
   int tuned_graph = 1, tuned_margin = 1, tuned_shape = 0;
   int cur_shape = 0;             // the shape the launches use now
   int tune_phase = 0;            // autotuner: candidates timed so far
-  double tune_ms[kTuneCands] = {};
+  double tune_ms[kTuneCands] = {};  // per candidate: ms per million cell-steps
+  int tune_skips = 0;            // steps of the current candidate not timed (graph capture)
   long long graph_epoch = 0;     // bumped whenever captured arguments change
   int keep_count = 0;            // jobs that keep the writer's state (k_keep_state launched)
   long long spec_steps = 0, redone_steps = 0, graph_launches = 0, graph_captures = 0;
@@ -718,8 +719,11 @@ int step_body_graph(swm_engine* e, Tables& T, const StepPlan& P) {
 
 // Autotuner over (tile shape, graph, margin).  The first speculative steps of
 // an engine time one candidate each: the face-kernel tile shapes first,
-// without a graph (so no capture time enters that comparison), then graph
-// on/off x margin 1/2 with the faster shape; the fastest candidate is kept.
+// without a graph, then graph on/off x margin 1/2 with the faster shape; the
+// fastest candidate is kept.  A step that captured a graph is not timed (the
+// candidate is timed on its next step, a replay, unless it captured twice in
+// a row), and each time is divided by the cell-steps of the job set it ran
+// on, so that candidates timed while jobs join and leave compare fairly.
 // Explicit settings (swm_engine_set_execution, swm_engine_set_tile_shape)
 // override their dimension.
 struct TuneCand {
@@ -814,6 +818,7 @@ int engine_step(swm_engine* e, Tables& T, const SchemeDef* sc, int hp_assoc,
       first += ns;
     }
     const auto t0 = std::chrono::steady_clock::now();
+    const long long captures0 = e->graph_captures;
     int rc = (graph && !e->prof.on) ? step_body_graph(e, T, P) : step_body(e, T, P);
     if (rc) return rc;
     rc = read_status(e);
@@ -823,10 +828,19 @@ int engine_step(swm_engine* e, Tables& T, const SchemeDef* sc, int hp_assoc,
     ++e->spec_steps;
     accepted = true;
     for (int q = 0; q < T.njobs; ++q) accepted = accepted && !e->hstatus[T.jl[q]].overrun;
-    if (accepted)
-      tune_record(e, ms);
-    else
+    if (accepted) {
+      const bool captured = e->graph_captures != captures0;
+      if (captured && e->tune_skips < 2) {
+        ++e->tune_skips;
+      } else {
+        double cells = 0.;
+        for (int q = 0; q < T.njobs; ++q) cells += (double)e->jobs[T.jl[q]]->dev.n;
+        tune_record(e, ms / (cells * 1e-6));
+        e->tune_skips = 0;
+      }
+    } else {
       ++e->redone_steps;
+    }
   }
   if (!accepted) {
     P.speculative = false;
@@ -1187,6 +1201,65 @@ int fill_initial_data(swm_engine* e, int id, int ic, int bflag, const double* x0
   return wb ? build_wb(e, id) : SWM_OK;
 }
 
+// The benchmark families on the device (swm_fill_family): the state, for a
+// lake the bathymetry tables, and for a well-balanced job the well-balanced
+// state (the lake's h = 1 - b) and its FVTD.
+int fill_family(swm_engine* e, int id, const swm_family* f) {
+  Job& J = *e->jobs[id];
+  if (!f) FAIL(e, SWM_E_ARG, "family is NULL");
+  if (f->kind < SWM_FAMILY_VORTEX || f->kind > SWM_FAMILY_DAM_BREAK)
+    FAIL(e, SWM_E_ARG, "family kind %d is not one of SWM_FAMILY_*", f->kind);
+  const bool lake = f->kind == SWM_FAMILY_LAKE_BUMPS;
+  if (lake && (f->nbumps < 0 || f->nbumps > SWM_FAMILY_MAX_BUMPS))
+    FAIL(e, SWM_E_ARG, "nbumps %d is not in 0..%d", f->nbumps, SWM_FAMILY_MAX_BUMPS);
+  if (J.dev.wb && !lake)
+    FAIL(e, SWM_E_ARG, "family %d has no well-balanced state (only the lake has one)", f->kind);
+  if (!(J.dev.dx == 1.0 / (double)J.dev.nx) || !(J.dev.dy == 1.0 / (double)J.dev.ny))
+    FAIL(e, SWM_E_ARG, "the families live on the unit square: dx, dy must be 1/nx, 1/ny");
+  FamilySetup c{};
+  c.kind = f->kind;
+  c.nbumps = lake ? f->nbumps : 0;
+  c.dx = J.dev.dx;
+  c.dy = J.dev.dy;
+  c.g = J.dev.g;
+  const double q1 = 0.5 * std::sqrt(3. / 7. + 2. / 7. * std::sqrt(6. / 5.));
+  const double q2 = 0.5 * std::sqrt(3. / 7. - 2. / 7. * std::sqrt(6. / 5.));
+  const double nodes[4] = {-q1, -q2, q2, q1};
+  for (int g = 0; g < 4; ++g) {
+    c.node[g] = nodes[g];
+    c.w[g] = J.dev.wbnd[g];
+  }
+  c.eps = f->eps;
+  for (int k = 0; k < SWM_FAMILY_MAX_BUMPS; ++k) {
+    c.A[k] = f->A[k];
+    c.cx[k] = f->cx[k];
+    c.cy[k] = f->cy[k];
+    c.s[k] = f->s[k];
+  }
+  c.amp = f->amp;
+  c.px = f->px;
+  c.py = f->py;
+  c.ps = f->ps;
+  c.radius = f->radius;
+  c.h_in = f->h_in;
+  c.h_out = f->h_out;
+  if (lake) {
+    int rc = ensure_bathy(e, J);
+    if (rc) return rc;
+  }
+  const bool wb = J.dev.wb != 0;
+  launch_family_data(c, J.dev.nx, J.dev.ny, J.dev.h, J.dev.hu, J.dev.hv,
+                     wb ? J.dev.hres[1] : nullptr, wb ? J.dev.hun[1] : nullptr,
+                     wb ? J.dev.hvn[1] : nullptr, lake ? J.dev.bx : nullptr,
+                     lake ? J.dev.by : nullptr, e->stream);
+  LAUNCHCK(e);
+  HIPCK(e, hipStreamSynchronize(e->stream));
+  J.dev.bathy = lake ? 1 : 0;
+  J.wb_ready = false;
+  e->dirty = true;
+  return wb ? build_wb(e, id) : SWM_OK;
+}
+
 // PostProcessing's L1 error of job J's state (swm_l1_error).
 int job_l1_error(swm_engine* e, Job& J, int ic, int bflag, double t, const double* x0,
                  const double* x1, double* err) {
@@ -1211,9 +1284,13 @@ int job_l1_error(swm_engine* e, Job& J, int ic, int bflag, double t, const doubl
 int engine_add(swm_engine* e, const swm_job_desc* d, int* id_out) {
   *id_out = -1;
   if (!d) FAIL(e, SWM_E_ARG, "job description is NULL");
-  const bool exact = !d->U0 && d->initial_condition > 0;  // initial data made on the device
-  if (!d->U0 && !exact) FAIL(e, SWM_E_ARG, "U0 is NULL and no initial_condition is set");
-  if (d->wellbalanced && !d->Uwb && !exact) FAIL(e, SWM_E_ARG, "wellbalanced job needs Uwb");
+  // initial data made on the device: a benchmark family, or the reference's own
+  const bool fam = !d->U0 && d->family != nullptr;
+  const bool exact = !d->U0 && !fam && d->initial_condition > 0;
+  if (!d->U0 && !exact && !fam)
+    FAIL(e, SWM_E_ARG, "U0 is NULL and neither family nor initial_condition is set");
+  if (d->wellbalanced && !d->Uwb && !exact && !fam)
+    FAIL(e, SWM_E_ARG, "wellbalanced job needs Uwb");
   if (!d->dbdx != !d->dbdy) FAIL(e, SWM_E_ARG, "dbdx and dbdy must both be set or both NULL");
   const SchemeDef* sc = find_scheme(d->timescheme == 0 ? 5 : d->timescheme);
   if (!sc) FAIL(e, SWM_E_ARG, "timescheme %d is not one of 0..7", d->timescheme);
@@ -1222,7 +1299,9 @@ int engine_add(swm_engine* e, const swm_job_desc* d, int* id_out) {
   const int fix = recfix_code(d->reconstruction_fix == 0 ? 4 : d->reconstruction_fix);
   if (fix < 0)
     FAIL(e, SWM_E_ARG, "reconstruction_fix %d is not 0, 4, 1 or 2", d->reconstruction_fix);
-  const bool tables = exact ? d->bathymetry_flag > 0 : d->dbdx != nullptr;
+  const bool tables = fam     ? d->family->kind == SWM_FAMILY_LAKE_BUMPS
+                      : exact ? d->bathymetry_flag > 0
+                              : d->dbdx != nullptr;
   const long long extra = tables ? (long long)bathy_bytes((size_t)d->nx * d->ny) : 0;
   int id;
   int rc = alloc_job(e, d->nx, d->ny, d->dx, d->dy, d->gravity, d->cfl, d->wellbalanced ? 1 : 0,
@@ -1233,7 +1312,10 @@ int engine_add(swm_engine* e, const swm_job_desc* d, int* id_out) {
   J.scheme = sc;
   J.hp_assoc = d->hp_association;
   J.dev.recfix = fix;
-  if (exact) {
+  if (fam) {
+    rc = fill_family(e, id, d->family);
+    if (rc) return rc;
+  } else if (exact) {
     rc = fill_initial_data(e, id, d->initial_condition, d->bathymetry_flag, d->mesh_x0,
                            d->mesh_x1);
     if (rc) return rc;
@@ -1612,6 +1694,11 @@ int swm_fill_initial_data(swm_solver_t* s, int initial_condition, int bathymetry
                           const double x0[2], const double x1[2]) {
   if (!s) return SWM_E_ARG;
   return fill_initial_data(s->e, s->job, initial_condition, bathymetry_flag, x0, x1);
+}
+
+int swm_fill_family(swm_solver_t* s, const swm_family* family) {
+  if (!s) return SWM_E_ARG;
+  return fill_family(s->e, s->job, family);
 }
 
 int swm_l1_error(swm_solver_t* s, int initial_condition, int bathymetry_flag, double t,
@@ -2095,6 +2182,24 @@ int swm_debug_divsqrt(int n, const double* a, const double* b, double* q, double
       rc = SWM_E_HIP;
   }
   (void)hipFree(da);
+  return rc;
+}
+
+int swm_debug_exp(int n, const double* x, double* y) {
+  if (n <= 0 || !x || !y) return SWM_E_ARG;
+  double* dx = nullptr;
+  size_t bytes = (size_t)n * sizeof(double);
+  if (hipMalloc((void**)&dx, 2 * bytes) != hipSuccess) return SWM_E_NOMEM;
+  double* dy = dx + n;
+  int rc = SWM_OK;
+  if (hipMemcpy(dx, x, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = SWM_E_HIP;
+  if (rc == SWM_OK) {
+    launch_det_exp(n, dx, dy, nullptr);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(y, dy, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = SWM_E_HIP;
+  }
+  (void)hipFree(dx);
   return rc;
 }
 

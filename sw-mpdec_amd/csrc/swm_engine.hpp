@@ -54,8 +54,11 @@ struct JobStatus {
   double dt, lmx, lmy;
   int dt_bad;     // dt not finite
   int hmin_set;   // 1 once hmin holds a value
-  double hmin;    // minimum depth over every DeC stage solution since the last
-                  // reset (NaN if any was NaN); kept only while tracking is on
+  double hmin;    // minimum depth over every DeC stage solution of the accepted
+                  // steps since the last reset (NaN if any was NaN); kept only
+                  // while tracking is on
+  int hmin_step_set;  // the current attempt's minimum, merged into hmin by the
+  double hmin_step;   // commit: a redone attempt's stages never reach hmin
   // the current step's solves, recorded after each correction's Jacobi phase
   int iters[13];  // sweeps of each solve, in execution order
   int nonconv;    // solves stopped at the iteration limit
@@ -192,6 +195,22 @@ struct InitialSetup {
   double x0[2], sx[2], dx[2], g, pi, t;  // t: time of ExactFunction
   double node[4], w[4];
 };
+
+// The benchmark families' parameters (swm_family) with the mesh of the
+// periodic unit square and the 4-point Gauss rule.
+struct FamilySetup {
+  int kind, nbumps;
+  double dx, dy, g;
+  double node[4], w[4];
+  double eps;
+  double A[8], cx[8], cy[8], s[8];
+  double amp, px, py, ps;
+  double radius, h_in, h_out;
+};
+void launch_family_data(const FamilySetup& f, int nx, int ny, double* h, double* hu, double* hv,
+                        double* hwb, double* huwb, double* hvwb, double* bx, double* by,
+                        hipStream_t s);
+void launch_det_exp(int n, const double* x, double* y, hipStream_t s);
 
 // h/hu/hv: the state; hwb/huwb/hvwb (may be null): the well-balanced
 // state; bx/by (may be null): the bathymetry tables.

@@ -15,6 +15,7 @@
 // than bitwise (the dam breaks 4 and 5 need only sqrt and are bitwise).
 #include <hip/hip_runtime.h>
 
+#include "swm.h"
 #include "swm_engine.hpp"
 #include "swm_numerics.hpp"
 
@@ -246,7 +247,199 @@ __global__ __launch_bounds__(256) void k_primitives(size_t n, const double* h, c
   V[3 * k + 2] = v;
 }
 
+// ---- initial data of the benchmark families --------------------------------
+//
+// The families (smooth vortex, lake at rest over Gaussian bumps, circular dam
+// break) have no counterpart in the reference; they are defined by the host's
+// problem builder (problems.py) with the exponential det_exp below rather than
+// a platform exp, so that the device makes the same bits as the host, on any
+// host.  det_exp: Cody-Waite reduction x = k ln2 + r, |r| <= ln2/2, with a
+// 32-bit head of ln2 (k*LN2_HI exact) and the rounding residual of r carried
+// separately; the degree-13 Taylor polynomial of exp(r) - 1 - r in Horner
+// form; p * 2^k formed by exact power-of-two scalings, the last one rounding
+// into the subnormal range or overflowing.  Only IEEE +, -, * in a fixed
+// order (the library is built without contraction): error below 0.85 ulp.
+__device__ __forceinline__ double pow2i(int k) {  // 2^k, k in [-1022, 1023]
+  return __longlong_as_double((long long)(k + 1023) << 52);
+}
+
+__device__ double det_exp(double x) {
+  if (x != x) return x;
+  if (x > 709.8) return __longlong_as_double(0x7ff0000000000000LL);
+  if (x < -746.0) return 0.0;
+  const double kInvLn2 = 0x1.71547652b82fep+0, kLn2Hi = 0x1.62e42fee00000p-1,
+               kLn2Lo = 0x1.a39ef35793c76p-33, kShift = 0x1.8p52;
+  const double t = x * kInvLn2 + kShift;
+  const double k = t - kShift;
+  const double hi = x - k * kLn2Hi;
+  const double lo = k * kLn2Lo;
+  const double r = hi - lo;
+  const double dr = (hi - r) - lo;
+  double q = 0x1.6124613a86d09p-33;  // 1/13!
+  q = 0x1.1eed8eff8d898p-29 + r * q;
+  q = 0x1.ae64567f544e4p-26 + r * q;
+  q = 0x1.27e4fb7789f5cp-22 + r * q;
+  q = 0x1.71de3a556c734p-19 + r * q;
+  q = 0x1.a01a01a01a01ap-16 + r * q;
+  q = 0x1.a01a01a01a01ap-13 + r * q;
+  q = 0x1.6c16c16c16c17p-10 + r * q;
+  q = 0x1.1111111111111p-7 + r * q;
+  q = 0x1.5555555555555p-5 + r * q;
+  q = 0x1.5555555555555p-3 + r * q;
+  q = 0x1.0000000000000p-1 + r * q;  // 1/2!
+  const double p = 1.0 + (r + (dr + (r * r) * q));
+  const int ki = (int)k;
+  if (ki > 1000) return (p * pow2i(ki - 100)) * pow2i(100);
+  if (ki < -1000) return (p * pow2i(ki + 100)) * pow2i(-100);
+  return p * pow2i(ki);
+}
+
+__global__ __launch_bounds__(256) void k_det_exp(int n, const double* x, double* y) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) y[k] = det_exp(x[k]);
+}
+
+// sum over the images a = -1, 0, 1 of exp(-(d*d)/(s*s)), d = (x - c) - a, and
+// of its x-derivative e * ((-2 d)/(s*s)), each sum started from +0.
+__device__ __forceinline__ void periodic_gauss(double x, double c, double s, double& g,
+                                               double& dg) {
+  g = 0.0;
+  dg = 0.0;
+  const double ss = s * s;
+  const double off[3] = {-1.0, 0.0, 1.0};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double d = (x - c) - off[a];
+    const double e = det_exp(-(d * d) / ss);
+    g = g + e;
+    dg = dg + e * ((-2.0 * d) / ss);
+  }
+}
+
+// One thread per cell (i, j) of the periodic unit square: Gauss points
+// MeshGP = ((0 + (i-1) dx) + dx/2) + node*dx as the mesh builder forms them;
+// cell averages started from the first Gauss point's term and accumulated over
+// the x point (outer) and the y point (inner) with weights w(ix)*w(iy), the
+// order of FillInitialConditions (src/finitevolume.f90:314-330).
+__global__ __launch_bounds__(256) void k_family_data(FamilySetup f, int nx, int ny, double* h,
+                                                     double* hu, double* hv, double* hwb,
+                                                     double* huwb, double* hvwb, double* bx,
+                                                     double* by) {
+  const size_t n = (size_t)nx * ny;
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i = (int)(k % nx), j = (int)(k / nx);
+  const double bxc = (0.0 + (double)i * f.dx) + 0.5 * f.dx;
+  const double byc = (0.0 + (double)j * f.dy) + 0.5 * f.dy;
+  double xs[4], ys[4];
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    xs[g] = bxc + f.node[g] * f.dx;
+    ys[g] = byc + f.node[g] * f.dy;
+  }
+  double acc[3] = {0., 0., 0.}, awb[3] = {0., 0., 0.};
+  if (f.kind == SWM_FAMILY_LAKE_BUMPS) {
+    // b and its gradient at the 16 points, bump by bump (the sum's order)
+    double b[4][4], gx_[4][4], gy_[4][4];  // [yg][xg]
+#pragma unroll
+    for (int yg = 0; yg < 4; ++yg)
+#pragma unroll
+      for (int xg = 0; xg < 4; ++xg) b[yg][xg] = gx_[yg][xg] = gy_[yg][xg] = 0.0;
+    for (int q = 0; q < f.nbumps; ++q) {
+      double ex[4], dex[4], ey[4], dey[4];
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        periodic_gauss(xs[g], f.cx[q], f.s[q], ex[g], dex[g]);
+        periodic_gauss(ys[g], f.cy[q], f.s[q], ey[g], dey[g]);
+      }
+      const double A = f.A[q];
+#pragma unroll
+      for (int yg = 0; yg < 4; ++yg)
+#pragma unroll
+        for (int xg = 0; xg < 4; ++xg) {
+          b[yg][xg] = b[yg][xg] + (A * ex[xg]) * ey[yg];
+          gx_[yg][xg] = gx_[yg][xg] + (A * dex[xg]) * ey[yg];
+          gy_[yg][xg] = gy_[yg][xg] + (A * ex[xg]) * dey[yg];
+        }
+    }
+    double px[4], py[4], unused;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      periodic_gauss(xs[g], f.px, f.ps, px[g], unused);
+      periodic_gauss(ys[g], f.py, f.ps, py[g], unused);
+    }
+#pragma unroll
+    for (int ig = 0; ig < 4; ++ig)
+#pragma unroll
+      for (int jg = 0; jg < 4; ++jg) {
+        const double w = f.w[ig] * f.w[jg];
+        const double hw = 1.0 - b[jg][ig];
+        const double h0 = hw + (f.amp * px[ig]) * py[jg];
+        const double t0[3] = {h0, h0 * 0.0, h0 * 0.0}, tw[3] = {hw, hw * 0.0, hw * 0.0};
+#pragma unroll
+        for (int v = 0; v < 3; ++v) {
+          acc[v] = (ig | jg) ? acc[v] + w * t0[v] : w * t0[v];
+          awb[v] = (ig | jg) ? awb[v] + w * tw[v] : w * tw[v];
+        }
+      }
+    if (bx)
+#pragma unroll
+      for (int yg = 0; yg < 4; ++yg)
+#pragma unroll
+        for (int xg = 0; xg < 4; ++xg) {
+          bx[(size_t)(yg * 4 + xg) * n + k] = gx_[yg][xg];
+          by[(size_t)(yg * 4 + xg) * n + k] = gy_[yg][xg];
+        }
+  } else {
+#pragma unroll
+    for (int ig = 0; ig < 4; ++ig)
+#pragma unroll
+      for (int jg = 0; jg < 4; ++jg) {
+        const double x = xs[ig], y = ys[jg];
+        const double w = f.w[ig] * f.w[jg];
+        double hh, u, v;
+        if (f.kind == SWM_FAMILY_VORTEX) {
+          const double xp = (x - 0.5) / 0.1, yp = (y - 0.5) / 0.1;
+          const double r2 = xp * xp + yp * yp;
+          hh = 1.0 - f.eps * f.eps / (2.0 * f.g) * det_exp(1.0 - r2);
+          const double sv = f.eps * det_exp(0.5 * (1.0 - r2));
+          u = (-sv) * yp;
+          v = sv * xp;
+        } else {  // dam break
+          const double r2 = (x - 0.5) * (x - 0.5) + (y - 0.5) * (y - 0.5);
+          hh = (r2 < f.radius * f.radius) ? f.h_in : f.h_out;
+          u = v = 0.0;
+        }
+        const double t0[3] = {hh, hh * u, hh * v};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[c] = (ig | jg) ? acc[c] + w * t0[c] : w * t0[c];
+      }
+  }
+  h[k] = acc[0];
+  hu[k] = acc[1];
+  hv[k] = acc[2];
+  if (hwb) {
+    hwb[k] = awb[0];
+    huwb[k] = awb[1];
+    hvwb[k] = awb[2];
+  }
+}
+
 }  // namespace
+
+void launch_family_data(const FamilySetup& f, int nx, int ny, double* h, double* hu, double* hv,
+                        double* hwb, double* huwb, double* hvwb, double* bx, double* by,
+                        hipStream_t s) {
+  const size_t n = (size_t)nx * ny;
+  if (!n) return;
+  hipLaunchKernelGGL(k_family_data, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, f, nx, ny,
+                     h, hu, hv, hwb, huwb, hvwb, bx, by);
+}
+
+void launch_det_exp(int n, const double* x, double* y, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_det_exp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, x, y);
+}
 
 void launch_l1_error(const InitialSetup& c, int nx, int ny, const double* h, const double* hu,
                      const double* hv, double* scratch, double* err, hipStream_t s) {

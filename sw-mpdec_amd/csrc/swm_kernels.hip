@@ -722,7 +722,9 @@ __global__ __launch_bounds__(256) void k_decide(EngineArgs E, const int* jobs, S
         q.k = 0;
         q.active = 0;
         q.zero = 1;
-        q.nonconv = 0;
+        // no sweep: "not converged" when the initial residual is above the
+        // tolerance (src/jacobi_iteration.f90:82-84)
+        q.nonconv = c0 ? 1 : 0;
         q.residual = tot[kMaxSys] / (double)n;
         continue;
       }
@@ -800,7 +802,9 @@ __global__ __launch_bounds__(256) void k_finalize(EngineArgs E, const TileRef* b
   if (threadIdx.x == 0) J.part[T.tile] = mn;
 }
 
-// One block per job: fold the finalize blocks' minima into JobStatus::hmin.
+// One block per job: fold the finalize blocks' minima into the step's
+// minimum JobStatus::hmin_step (k_commit merges it into hmin once the step is
+// accepted).
 __global__ __launch_bounds__(256) void k_stage_min(EngineArgs E, const int* jobs) {
   __shared__ double red[4];
   const JobDev& J = E.jobs[jobs[blockIdx.x]];
@@ -809,8 +813,8 @@ __global__ __launch_bounds__(256) void k_stage_min(EngineArgs E, const int* jobs
   mn = block_min_all(mn, red);
   if (threadIdx.x == 0) {
     JobStatus* st = J.status;
-    st->hmin = st->hmin_set ? nan_min(st->hmin, mn) : mn;
-    st->hmin_set = 1;
+    st->hmin_step = st->hmin_step_set ? nan_min(st->hmin_step, mn) : mn;
+    st->hmin_step_set = 1;
   }
 }
 
@@ -824,6 +828,7 @@ __global__ void k_solve_done(EngineArgs E, const int* jobs, SysList S, int first
     st->nonconv = 0;
     st->ambiguous = 0;
     st->overrun = 0;
+    st->hmin_step_set = 0;  // this attempt's stages start a new minimum
   }
   for (int s = 0; s < S.count; ++s) {
     const JacState& q = st->st[s];
@@ -840,7 +845,16 @@ __global__ __launch_bounds__(256) void k_commit(EngineArgs E, const TileRef* blo
   const TileRef T = blocks[blockIdx.x];
   const JobDev& J = E.jobs[T.job];
   const int k = T.tile * kCellBlock + threadIdx.x;
-  if (k >= J.n || J.status->dt_bad) return;
+  if (J.status->dt_bad) return;
+  if (k == 0) {  // the accepted step's stage minimum joins the record
+    JobStatus* st = J.status;
+    if (st->hmin_step_set) {
+      st->hmin = st->hmin_set ? nan_min(st->hmin, st->hmin_step) : st->hmin_step;
+      st->hmin_set = 1;
+      st->hmin_step_set = 0;
+    }
+  }
+  if (k >= J.n) return;
   G(J.h)[k] = G(J.hres[mf])[k];
   G(J.hu)[k] = G(J.hun[mf])[k];
   G(J.hv)[k] = G(J.hvn[mf])[k];

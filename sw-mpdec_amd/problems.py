@@ -72,15 +72,70 @@ def cell_average(mesh: Mesh, f, mesh_y: Mesh | None = None) -> np.ndarray:
     return out
 
 
+# ---- the exponential of the benchmark families -----------------------------
+#
+# The families below (vortex, lake over Gaussian bumps) are defined with this
+# exponential rather than the platform's: libm and NumPy's vectorised exp
+# differ between hosts and CPU instruction sets in the last bit, and the
+# WENO5 weights turn last-bit differences of the data into 1e-9 relative
+# differences within a few steps.  det_exp uses only IEEE +, -, *, exact
+# scaling by powers of two and rounding by the 1.5*2^52 shift, in one fixed
+# order, so NumPy here, the device kernel (k_family_data) and any C build
+# without contraction give the same bits.  Method: Cody-Waite reduction
+# x = k ln2 + r, |r| <= ln2/2 (ln2 split into a 32-bit head, so k*LN2_HI is
+# exact), the rounding residual of r carried separately, and the Taylor
+# polynomial of degree 13 of exp(r) - 1 - r in Horner form; measured error
+# below 0.85 ulp against a 60-digit exp.
+EXP_INV_LN2 = float.fromhex("0x1.71547652b82fep+0")
+EXP_LN2_HI = float.fromhex("0x1.62e42fee00000p-1")
+EXP_LN2_LO = float.fromhex("0x1.a39ef35793c76p-33")
+EXP_SHIFT = float.fromhex("0x1.8p52")
+EXP_COEF = tuple(float.fromhex(h) for h in (        # 1/k!, k = 2..13
+    "0x1.0000000000000p-1", "0x1.5555555555555p-3", "0x1.5555555555555p-5",
+    "0x1.1111111111111p-7", "0x1.6c16c16c16c17p-10", "0x1.a01a01a01a01ap-13",
+    "0x1.a01a01a01a01ap-16", "0x1.71de3a556c734p-19", "0x1.27e4fb7789f5cp-22",
+    "0x1.ae64567f544e4p-26", "0x1.1eed8eff8d898p-29", "0x1.6124613a86d09p-33"))
+EXP_MAX = 709.8      # above: +inf
+EXP_MIN = -746.0     # below: +0
+
+
+def det_exp(x):
+    """exp(x) by the fixed method above (float64 scalar or array)."""
+    x = np.asarray(x, dtype=np.float64)
+    xc = np.clip(np.where(np.isnan(x), 0.0, x), EXP_MIN, EXP_MAX)
+    t = xc * EXP_INV_LN2 + EXP_SHIFT
+    k = t - EXP_SHIFT
+    hi = xc - k * EXP_LN2_HI
+    lo = k * EXP_LN2_LO
+    r = hi - lo
+    dr = (hi - r) - lo
+    q = EXP_COEF[-1]
+    for c in EXP_COEF[-2::-1]:
+        q = c + r * q
+    p = 1.0 + (r + (dr + (r * r) * q))
+    ki = k.astype(np.int64)
+    # p * 2^k rounded once: exact scalings by powers of two, the last one
+    # rounding into the subnormal range or overflowing
+    big, small = ki > 1000, ki < -1000
+    k1 = np.where(big, ki - 100, np.where(small, ki + 100, ki))
+    y = p * np.ldexp(1.0, k1)
+    with np.errstate(over="ignore"):
+        y = np.where(big, y * np.ldexp(1.0, 100), np.where(small, y * np.ldexp(1.0, -100), y))
+    y = np.where(x > EXP_MAX, np.inf, np.where(x < EXP_MIN, 0.0, y))
+    y = np.where(np.isnan(x), x, y)
+    return y if y.ndim else float(y)
+
+
 def _periodic_gauss_1d(x, c, sig, L=1.0, images=(-1, 0, 1)):
-    """sum_a exp(-(x - c - a L)^2 / sig^2) and its x-derivative."""
+    """sum_a exp(-(x - c - a L)^2 / sig^2) and its x-derivative, summed over
+    the images a = -1, 0, 1 in that order."""
     g = np.zeros_like(x)
     dg = np.zeros_like(x)
     for a in images:
         d = x - c - a * L
-        e = np.exp(-(d * d) / (sig * sig))
-        g += e
-        dg += e * (-2.0 * d / (sig * sig))
+        e = det_exp(-(d * d) / (sig * sig))
+        g = g + e
+        dg = dg + e * (-2.0 * d / (sig * sig))
     return g, dg
 
 
@@ -94,15 +149,20 @@ class GaussBumps:
     s: np.ndarray
 
     @staticmethod
-    def seeded(rng: np.random.Generator, k: int = 8, max_b: float = 1.0):
+    def seeded(rng: np.random.Generator, k: int = 8, max_b: float = 0.95):
+        """Draws until the bathymetry stays below max_b, below the free
+        surface h + b = 1 with a margin.  The maximum is taken on a 1025^2
+        grid (spacing 1/1024); a bump's curvature is at most 2 A / s^2 <= 320,
+        so the true maximum exceeds the sampled one by less than
+        320 (1/1024)^2 / 8 < 4e-5, far inside the margin: every accepted draw
+        keeps h >= 1 - max_b > 0 at every point."""
+        xs = np.linspace(0.0, 1.0, 1025)
         while True:
             A = rng.uniform(0.1, 0.4, k)
             s = rng.uniform(0.05, 0.2, k)
             cx = rng.uniform(0.0, 1.0, k)
             cy = rng.uniform(0.0, 1.0, k)
             gb = GaussBumps(A, cx, cy, s)
-            # reject bathymetry reaching the free surface (would dry cells)
-            xs = np.linspace(0.0, 1.0, 257)
             if np.max(gb.value(xs[None, :], xs[:, None])) < max_b:
                 return gb
 
@@ -168,6 +228,11 @@ def cons(h, u, v):
 
 # ---- problem families -----------------------------------------------------
 
+# family kinds (include/swm.h SWM_FAMILY_*)
+FAMILY_VORTEX = 1
+FAMILY_LAKE_BUMPS = 2
+FAMILY_DAM_BREAK = 3
+
 @dataclass
 class Problem:
     """One solve: initial state, optional bathymetry and well-balanced state,
@@ -188,6 +253,9 @@ class Problem:
     ny: int = 0                     # 0: square (ny = n)
     y0: float | None = None         # y extent (None: that of x)
     y1: float | None = None
+    # the family's parameters, from which the device makes the same arrays
+    # bitwise (swm_fill_family); None for data defined only by the arrays
+    family: dict | None = None
 
     @property
     def rows(self) -> int:
@@ -245,71 +313,141 @@ class ReferenceSetup:
         return self.ny
 
 
-def vortex(n: int, seed: int, steps: int = 50, cfl: float = 0.5, ny: int = 0) -> Problem:
+@dataclass
+class FamilySpec:
+    """A problem of one of the benchmark families given by its parameters
+    only.  The device makes its arrays from them (swm_fill_family,
+    swm_job_desc.family), so a request carries no host data; build() makes
+    the same arrays, bitwise, on the host.  kind: FAMILY_*; params: the
+    family's parameters (the fields of swm_family); n x ny cells of the
+    periodic unit square."""
+    kind: int
+    n: int
+    params: dict
+    ny: int = 0
+    steps: int = 1
+    cfl: float = 0.5
+    gravity: float = GRAVITY
+
+    @property
+    def rows(self) -> int:
+        return self.ny or self.n
+
+    @property
+    def wellbalanced(self) -> bool:
+        return self.kind == FAMILY_LAKE_BUMPS
+
+    @property
+    def mesh(self) -> Mesh:
+        return Mesh(self.n)
+
+    @property
+    def mesh_y(self) -> Mesh:
+        return Mesh(self.rows)
+
+    @property
+    def name(self) -> str:
+        return {FAMILY_VORTEX: "vortex", FAMILY_LAKE_BUMPS: "lake_bumps",
+                FAMILY_DAM_BREAK: "dam_break"}[self.kind]
+
+    def build(self) -> Problem:
+        """The host arrays of this problem (cell averages at the 4x4 Gauss
+        points; for the lake also the well-balanced state and the bathymetry
+        tables)."""
+        mesh, mesh_y, q, g = self.mesh, self.mesh_y, self.params, self.gravity
+        common = dict(cfl=self.cfl, steps=self.steps, kind=self.name, ny=self.ny,
+                      gravity=g, family={"kind": self.kind, **q})
+        if self.kind == FAMILY_VORTEX:
+            eps = q["eps"]
+
+            def h_u_v(X, Y):
+                xp = (X - 0.5) / 0.1
+                yp = (Y - 0.5) / 0.1
+                r2 = xp * xp + yp * yp
+                h = 1.0 - eps * eps / (2.0 * g) * det_exp(1.0 - r2)
+                s = eps * det_exp(0.5 * (1.0 - r2))
+                return h, -s * yp, s * xp
+
+            U0 = cell_average(mesh, lambda X, Y: cons(*h_u_v(X, Y)), mesh_y)
+            return Problem(self.n, U0, **common)
+        if self.kind == FAMILY_LAKE_BUMPS:
+            gb = GaussBumps(np.array(q["A"]), np.array(q["cx"]), np.array(q["cy"]),
+                            np.array(q["s"]))
+            amp, pcx, pcy, ps = q["amp"], q["px"], q["py"], q["ps"]
+
+            def wb_state(X, Y):
+                b = gb.value(X, Y)
+                z = np.zeros_like(X)
+                return cons(1.0 - b, z, z)
+
+            def ic_state(X, Y):
+                b = gb.value(X, Y)
+                gx, _ = _periodic_gauss_1d(X, pcx, ps)
+                gy, _ = _periodic_gauss_1d(Y, pcy, ps)
+                z = np.zeros_like(X)
+                return cons(1.0 - b + amp * gx * gy, z, z)
+
+            Uwb = cell_average(mesh, wb_state, mesh_y)
+            U0 = cell_average(mesh, ic_state, mesh_y)
+            assert U0[..., 0].min() > 0.0 and Uwb[..., 0].min() > 0.0
+            bx, by = bathy_tables(mesh, gb.grad, mesh_y)
+            return Problem(self.n, U0, wellbalanced=True, Uwb=Uwb, dbdx=bx, dbdy=by, **common)
+        radius, h_in, h_out = q["radius"], q["h_in"], q["h_out"]
+
+        def st(X, Y):
+            r2 = (X - 0.5) ** 2 + (Y - 0.5) ** 2
+            h = np.where(r2 < radius * radius, h_in, h_out)
+            z = np.zeros_like(h)
+            return cons(h, z, z)
+
+        return Problem(self.n, cell_average(mesh, st, mesh_y), **common)
+
+
+def vortex_spec(n: int, seed: int, steps: int = 50, cfl: float = 0.5, ny: int = 0) -> FamilySpec:
     """Smooth vortex on the periodic unit square, flat bottom:
     h = 1 - eps^2/(2g) exp(1 - r^2), (u, v) = eps exp((1 - r^2)/2) (-y', x'),
     x' = (x - 1/2)/0.1, eps ~ U[0.5, 1.5] from the seed; n x ny cells."""
     rng = np.random.default_rng(seed)
-    eps = rng.uniform(0.5, 1.5)
-    mesh = Mesh(n)
-    mesh_y = Mesh(ny or n)
-    g = GRAVITY
-
-    def h_u_v(X, Y):
-        xp = (X - 0.5) / 0.1
-        yp = (Y - 0.5) / 0.1
-        r2 = xp * xp + yp * yp
-        h = 1.0 - eps * eps / (2.0 * g) * np.exp(1.0 - r2)
-        s = eps * np.exp(0.5 * (1.0 - r2))
-        return h, -s * yp, s * xp
-
-    U0 = cell_average(mesh, lambda X, Y: cons(*h_u_v(X, Y)), mesh_y)
-    return Problem(n, U0, cfl=cfl, steps=steps, kind="vortex", ny=ny)
+    eps = float(rng.uniform(0.5, 1.5))
+    return FamilySpec(FAMILY_VORTEX, n, {"eps": eps}, ny=ny, steps=steps, cfl=cfl)
 
 
-def lake_bumps(n: int, seed: int, steps: int = 100, cfl: float = 0.5,
-               amp: float = 1e-3) -> Problem:
+def lake_bumps_spec(n: int, seed: int, steps: int = 100, cfl: float = 0.5,
+                    amp: float = 1e-3) -> FamilySpec:
     """Lake at rest over 8 seeded Gaussian bumps plus a seeded Gaussian
     surface perturbation of amplitude amp, well-balanced variant."""
     rng = np.random.default_rng(seed)
     gb = GaussBumps.seeded(rng)
     pcx, pcy = rng.uniform(0.0, 1.0, 2)
     ps = rng.uniform(0.05, 0.2)
-    mesh = Mesh(n)
+    q = {"A": gb.A.tolist(), "cx": gb.cx.tolist(), "cy": gb.cy.tolist(), "s": gb.s.tolist(),
+         "amp": float(amp), "px": float(pcx), "py": float(pcy), "ps": float(ps)}
+    return FamilySpec(FAMILY_LAKE_BUMPS, n, q, steps=steps, cfl=cfl)
 
-    def wb_state(X, Y):
-        b = gb.value(X, Y)
-        z = np.zeros_like(X)
-        return cons(1.0 - b, z, z)
 
-    def ic_state(X, Y):
-        b = gb.value(X, Y)
-        gx, _ = _periodic_gauss_1d(X, pcx, ps)
-        gy, _ = _periodic_gauss_1d(Y, pcy, ps)
-        z = np.zeros_like(X)
-        return cons(1.0 - b + amp * gx * gy, z, z)
+def dam_break_spec(n: int, steps: int = 200, cfl: float = 0.5, radius: float = 0.25,
+                   dry: float = 0.0) -> FamilySpec:
+    """Circular dam break on the periodic unit square, flat bottom:
+    h = 1 inside radius 0.25 of the centre, h = dry outside, u = v = 0."""
+    return FamilySpec(FAMILY_DAM_BREAK, n, {"radius": float(radius), "h_in": 1.0,
+                                            "h_out": float(dry)}, steps=steps, cfl=cfl)
 
-    Uwb = cell_average(mesh, wb_state)
-    U0 = cell_average(mesh, ic_state)
-    bx, by = bathy_tables(mesh, gb.grad)
-    return Problem(n, U0, wellbalanced=True, Uwb=Uwb, dbdx=bx, dbdy=by, cfl=cfl,
-                   steps=steps, kind="lake_bumps")
+
+def vortex(n: int, seed: int, steps: int = 50, cfl: float = 0.5, ny: int = 0) -> Problem:
+    """vortex_spec(...) with its host arrays."""
+    return vortex_spec(n, seed, steps, cfl, ny).build()
+
+
+def lake_bumps(n: int, seed: int, steps: int = 100, cfl: float = 0.5,
+               amp: float = 1e-3) -> Problem:
+    """lake_bumps_spec(...) with its host arrays."""
+    return lake_bumps_spec(n, seed, steps, cfl, amp).build()
 
 
 def dam_break(n: int, steps: int = 200, cfl: float = 0.5, radius: float = 0.25,
               dry: float = 0.0) -> Problem:
-    """Circular dam break on the periodic unit square, flat bottom:
-    h = 1 inside radius 0.25 of the centre, h = dry outside, u = v = 0."""
-    mesh = Mesh(n)
-
-    def st(X, Y):
-        r2 = (X - 0.5) ** 2 + (Y - 0.5) ** 2
-        h = np.where(r2 < radius * radius, 1.0, dry)
-        z = np.zeros_like(h)
-        return cons(h, z, z)
-
-    U0 = cell_average(mesh, st)
-    return Problem(n, U0, cfl=cfl, steps=steps, kind="dam_break")
+    """dam_break_spec(...) with its host arrays."""
+    return dam_break_spec(n, steps, cfl, radius, dry).build()
 
 
 def lake_flag1(n: int, wellbalanced: bool = True, pert: float = 1e-3, ny: int = 0) -> Problem:

@@ -40,6 +40,37 @@ def _ptr(a):
 
 _lib = None
 
+FAMILY_MAX_BUMPS = 8
+
+
+class Family(ctypes.Structure):
+    """swm_family (include/swm.h)."""
+    _fields_ = [("kind", ctypes.c_int), ("nbumps", ctypes.c_int), ("eps", ctypes.c_double),
+                ("A", ctypes.c_double * FAMILY_MAX_BUMPS),
+                ("cx", ctypes.c_double * FAMILY_MAX_BUMPS),
+                ("cy", ctypes.c_double * FAMILY_MAX_BUMPS),
+                ("s", ctypes.c_double * FAMILY_MAX_BUMPS),
+                ("amp", ctypes.c_double), ("px", ctypes.c_double), ("py", ctypes.c_double),
+                ("ps", ctypes.c_double), ("radius", ctypes.c_double), ("h_in", ctypes.c_double),
+                ("h_out", ctypes.c_double)]
+
+    @classmethod
+    def of(cls, spec) -> "Family":
+        """From a problems.FamilySpec."""
+        f = cls()
+        f.kind = int(spec.kind)
+        q = spec.params
+        for key in ("eps", "amp", "px", "py", "ps", "radius", "h_in", "h_out"):
+            if key in q:
+                setattr(f, key, float(q[key]))
+        if "A" in q:
+            if len(q["A"]) > FAMILY_MAX_BUMPS:
+                raise ValueError(f"at most {FAMILY_MAX_BUMPS} bumps")
+            f.nbumps = len(q["A"])
+            for k in range(f.nbumps):
+                f.A[k], f.cx[k], f.cy[k], f.s[k] = q["A"][k], q["cx"][k], q["cy"][k], q["s"][k]
+        return f
+
 
 def lib():
     """Load libswm.so (raises if it has not been built)."""
@@ -100,6 +131,8 @@ def lib():
     L.swm_profile_read.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
     L.swm_profile_name.restype = ctypes.c_char_p
     L.swm_profile_name.argtypes = [ctypes.c_int]
+    L.swm_fill_family.argtypes = [vp, ctypes.POINTER(Family)]
+    L.swm_debug_exp.argtypes = [ctypes.c_int, _dp, _dp]
     _lib = L
     return L
 
@@ -145,6 +178,22 @@ class Solver:
             s.init_wb(p.Uwb)
         s.upload(p.U0)
         return s
+
+    @classmethod
+    def from_family(cls, spec, recfix=4):
+        """A handle holding the family problem spec (a problems.FamilySpec),
+        its data made on the device (swm_fill_family)."""
+        m = spec.mesh
+        s = cls(spec.n, spec.rows, m.dx, spec.mesh_y.dx, spec.gravity, spec.cfl,
+                spec.wellbalanced, m.wbnd)
+        if recfix != 4:
+            s.set_reconstruction_fix(recfix)
+        s.fill_family(spec)
+        return s
+
+    def fill_family(self, spec):
+        f = Family.of(spec)
+        self._check(lib().swm_fill_family(self._h, ctypes.byref(f)), "swm_fill_family")
 
     @classmethod
     def from_setup(cls, c, recfix=4):
@@ -379,6 +428,16 @@ def debug_weno5(q):
     return first, second
 
 
+def debug_exp(x):
+    """The device's exponential of the benchmark families (swm_debug_exp)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    rc = lib().swm_debug_exp(x.size, _ptr(x), _ptr(y))
+    if rc != SWM_OK:
+        raise SwmError(f"swm_debug_exp rc={rc}")
+    return y
+
+
 def debug_shared_div(a, b):
     a = np.ascontiguousarray(a, dtype=np.float64)
     b = np.ascontiguousarray(b, dtype=np.float64)
@@ -405,7 +464,8 @@ class JobDesc(ctypes.Structure):
                 ("nsteps", ctypes.c_int), ("timescheme", ctypes.c_int),
                 ("hp_association", ctypes.c_int), ("reconstruction_fix", ctypes.c_int),
                 ("initial_condition", ctypes.c_int), ("bathymetry_flag", ctypes.c_int),
-                ("mesh_x0", ctypes.c_double * 2), ("mesh_x1", ctypes.c_double * 2)]
+                ("mesh_x0", ctypes.c_double * 2), ("mesh_x1", ctypes.c_double * 2),
+                ("family", ctypes.POINTER(Family))]
 
 
 class JobInfo(ctypes.Structure):
@@ -518,6 +578,34 @@ class Engine:
             return None
         self._check(rc, "swm_engine_submit")
         self._keep[jid.value] = (p.rows, p.n)
+        return jid.value
+
+    def submit_family(self, spec, nsteps, timescheme=5, hp_association=HP_AS_REFERENCE,
+                      recfix=4):
+        """Submits the family problem spec (a problems.FamilySpec), its data
+        made on the device; returns the job id, or None when it does not fit
+        the memory budget right now."""
+        m = spec.mesh
+        f = Family.of(spec)
+        d = JobDesc()
+        d.nx, d.ny = spec.n, spec.rows
+        d.dx, d.dy = m.dx, spec.mesh_y.dx
+        d.gravity = spec.gravity
+        d.cfl = spec.cfl
+        d.wellbalanced = int(bool(spec.wellbalanced))
+        for i in range(4):
+            d.wgp_bnd[i] = m.wbnd[i]
+        d.nsteps = int(nsteps)
+        d.timescheme = int(timescheme)
+        d.hp_association = int(hp_association)
+        d.reconstruction_fix = int(recfix)
+        d.family = ctypes.pointer(f)
+        jid = ctypes.c_int(-1)
+        rc = _engine_lib().swm_engine_submit(self._e, ctypes.byref(d), ctypes.byref(jid))
+        if rc == SWM_E_BUDGET:
+            return None
+        self._check(rc, "swm_engine_submit")
+        self._keep[jid.value] = (spec.rows, spec.n)
         return jid.value
 
     def submit_setup(self, c, nsteps, timescheme=5, hp_association=HP_AS_REFERENCE, recfix=4):

@@ -3,9 +3,9 @@
 * 64x64 vortex, CFL 0.5, 50 steps (the CPU-runnable case): dt sequence and
   final state bitwise equal to the oracle, hence relative L1 = 0 <= 1e-12.
 * 2048x2048 well-balanced lake over 8 bumps + 1e-3 perturbation (the headline
-  case), a few steps: depth >= 0 at every DeC stage, finite state, total mass
-  conserved to roundoff; without the perturbation the lake at rest stays at
-  rest to 1e-13.
+  case), its 100 steps: depth >= 0 at every DeC stage, finite state, total
+  mass conserved to roundoff; without the perturbation the lake at rest stays
+  at rest to 1e-13.  (Bitwise parity at this size: test_gpu_config_pins.py.)
 * 1024x1024 circular dam break: over a dry bed (h = 0) the first step keeps
   h >= 0 at every stage and TimeStep then turns non-finite, where the
   reference's time loop never ends; over a 1e-3 film, 200 steps with h >= 0
@@ -50,12 +50,12 @@ def test_vortex_64_50_steps_matches_oracle():
 
 
 def test_lake_bumps_2048_stages_positive_and_mass_conserved():
-    p = P.lake_bumps(2048, 2024, steps=4)
-    m0 = mass(p.U0)
-    s = swm.Solver.from_problem(p)
-    del p
+    """The headline configuration over its full 100 steps (data made on the
+    device, bitwise the host builder's)."""
+    s = swm.Solver.from_family(P.lake_bumps_spec(2048, 2024, steps=100))
+    m0 = mass(s.download())
     s.track_stage_min(True)
-    dts, iters, nonconv = s.run_steps(4)
+    dts, iters, nonconv = s.run_steps(100)
     hmin = s.stage_min_h()
     U = s.download()
     assert np.isfinite(U).all()

@@ -69,6 +69,28 @@ def test_underpredicted_step_is_redone():
     assert same_bits(U, U_r), describe_diff(U, U_r)
 
 
+def test_redone_step_keeps_only_the_accepted_stage_minima():
+    """The stage-depth record of a step that is redone after an overrun holds
+    the accepted attempt's minimum only: the same as with polling alone."""
+    p = P.vortex(32, 4, steps=3)
+
+    def run(graph, margin):
+        s = swm.Solver.from_problem(p)
+        s.set_execution(graph, margin)
+        s.set_jacobi_limits(2, 1e-16)
+        s.run_steps(1)
+        s.set_jacobi_limits(1000, 1e-16)
+        s.track_stage_min(True)
+        s.run_steps(2)
+        return s.stage_min_h(), s.execution_stats(), s.download()
+
+    h_spec, st, U_spec = run(1, 0)
+    h_poll, st_poll, U_poll = run(-1, -2)
+    assert st["redone_steps"] >= 1 and st_poll["speculative_steps"] == 0
+    assert same_bits([h_spec], [h_poll]), (h_spec, h_poll)
+    assert same_bits(U_spec, U_poll)
+
+
 def test_engine_graph_mode_matches_oracle():
     jobs = [(P.vortex(20, 1), 4), (P.lake_bumps(33, 2), 3), (P.vortex(47, 3), 5),
             (P.lake_flag1(18, True), 4)]

@@ -197,6 +197,24 @@ def test_jacobi_limit_override_matches_oracle():
     assert same_bits(s.download(), o.get_U())
 
 
+def test_jacobi_limit_zero_reports_every_solve_nonconverged():
+    """maxIter = 0: no sweep runs, the solution is 0, and every solve whose
+    initial residual is above the tolerance is reported as not converged
+    (src/jacobi_iteration.f90:82-84)."""
+    p = P.vortex(16, 4)
+    o = Oracle(p)
+    o.set_jacobi_limits(0, 1e-16)
+    s = swm.Solver.from_problem(p)
+    s.set_jacobi_limits(0, 1e-16)
+    dt_r = o.timestep()[0]
+    it_r, nf_r = o.step(dt_r)
+    dt = s.timestep()[0]
+    it, nf = s.step(dt)
+    assert it == it_r == [0] * 13
+    assert nf == nf_r == 13
+    assert same_bits(s.download(), o.get_U())
+
+
 def test_upload_with_fortran_ghost_layout():
     p = P.vortex(19, 7)
     n = p.n
