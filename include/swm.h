@@ -173,6 +173,22 @@ int swm_download(swm_solver_t *h, double *U);
  * and LambdaMaxX/Y. */
 int swm_timestep(swm_solver_t *h, double *dt_min, double *lambda_max_x, double *lambda_max_y);
 
+/* swm_timestep without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops that forbid hipMalloc).  U is a DEVICE
+ * array U(1:3, 1-g:nx+g, 1-g:ny+g) with g = ghosts (0 or 3, as swm_upload);
+ * out is a DEVICE array that receives {dt_min, LambdaMaxX, LambdaMaxY},
+ * bitwise swm_timestep's values for the same state.  The work is enqueued on
+ * `stream` (a hipStream_t; NULL = the null stream) and the call returns
+ * without waiting.  The per-block partial maxima live in a fixed device array
+ * of the library (1.5 MiB per device, held until the process ends), so grids
+ * above SWM_TIMESTEP_NOALLOC_MAX_CELLS cells are rejected with SWM_E_ARG.
+ * All calls on one device share that array: calls that may overlap in time
+ * on one device (several host threads or unordered streams) race. */
+#define SWM_TIMESTEP_NOALLOC_MAX_CELLS (1 << 24)
+int swm_timestep_noalloc(int nx, int ny, double dx, double dy, double gravity, double cfl,
+                         const double *U, int ghosts, double *out, void *stream);
+
 /* TimeDiscretizationByMPDeC5 (src/timediscretization.f90:458-566) with its
  * 13 FVTimeDerivative evaluations, 13 MODIFIED_PATANKAR_MATRIX_INVERSION
  * assemblies (:771-924) and Jacobi solves (src/jacobi_iteration.f90:52-87).

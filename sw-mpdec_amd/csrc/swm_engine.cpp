@@ -20,8 +20,13 @@
 using namespace swm;
 
 namespace {
-
 thread_local std::string g_error;
+}  // namespace
+
+// the calling thread's message for swm_last_error(NULL)
+void swm::set_thread_error(const std::string& msg) { g_error = msg; }
+
+namespace {
 
 size_t align_up(size_t b) { return (b + 255) & ~size_t(255); }
 int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
@@ -1861,7 +1866,7 @@ int swm_timestep(swm_solver_t* s, double* dt_min, double* lmx, double* lmy) {
   rc = read_status(e);
   if (rc) return rc;
   const JobStatus& js = e->hstatus[s->job];
-  if (dt_min) *dt_min = js.dt;
+  if (dt_min) *dt_min = js.dt_min;  // TimeStep's value: no time-loop clipping
   if (lmx) *lmx = js.lmx;
   if (lmy) *lmy = js.lmy;
   return SWM_OK;

@@ -18,7 +18,12 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 namespace swm {
+
+// Sets the message swm_last_error(NULL) returns on the calling thread.
+void set_thread_error(const std::string& msg);
 
 constexpr int kTile = 16;             // k_fvtd_update tiles: 16x16 cells, 256 threads
 

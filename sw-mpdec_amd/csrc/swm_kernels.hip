@@ -21,6 +21,7 @@
 
 #include "swm_engine.hpp"
 #include "swm_numerics.hpp"
+#include "swm_timestep.hpp"
 
 namespace swm {
 
@@ -960,103 +961,32 @@ __global__ __launch_bounds__(256) void k_copy_wb(EngineArgs E, const TileRef* bl
 }
 
 // ------------------------------------------------------------------------
-// TimeStep (src/equation.f90:791-837).  Without NaN the sequential MAX/MIN
-// fold equals dt = CFL*MIN(MIN(HUGE, dx/LX), dy/LY) with LX, LY the plain
-// maxima of |u + sqrt(g h)|, |v + sqrt(g h)| and 0; with a NaN anywhere the
-// fold is order dependent, so k_timestep_final replays it sequentially.
+// TimeStep (src/equation.f90:791-837): per-block maxima of the wave speeds,
+// then one block per job folds them (swm_timestep.hpp) and applies the time
+// loop's clipping.
 // ------------------------------------------------------------------------
-__device__ __forceinline__ void wave_speeds(const JobDev& J, double h, double hu, double hv,
-                                            double& a, double& b) {
-  double u, v;
-  cons_to_prim(h, hu, hv, u, v);
-  double fx = u + sqrt(J.g * h);
-  double fy = v + sqrt(J.g * h);
-  a = fabs(fx);
-  b = fabs(fy);
-}
-
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = max_r(v, __shfl_xor(v, off, 64));
-  return v;
-}
-
 __global__ __launch_bounds__(256) void k_timestep(EngineArgs E, const TileRef* blocks) {
-  __shared__ double sa[4], sb[4], sn[4];
   const TileRef T = blocks[blockIdx.x];
   const JobDev& J = E.jobs[T.job];
   const int k = T.tile * kCellBlock + threadIdx.x;
-  double a = 0.0, b = 0.0, nan = 0.0;
-  if (k < J.n) {
-    wave_speeds(J, G(J.h)[k], G(J.hu)[k], G(J.hv)[k], a, b);
-    if (isnan(a) || isnan(b)) {
-      nan = 1.0;
-      a = 0.0;
-      b = 0.0;
-    }
-  }
-  a = wave_max(a);
-  b = wave_max(b);
-  nan = wave_max(nan);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) {
-    sa[w] = a;
-    sb[w] = b;
-    sn[w] = nan;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double* p = J.tspart + 3 * (size_t)T.tile;
-    p[0] = max_r(max_r(max_r(sa[0], sa[1]), sa[2]), sa[3]);
-    p[1] = max_r(max_r(max_r(sb[0], sb[1]), sb[2]), sb[3]);
-    p[2] = max_r(max_r(max_r(sn[0], sn[1]), sn[2]), sn[3]);
-  }
+  const bool in = k < J.n;
+  timestep_partial(in, J.g, in ? G(J.h)[k] : 0., in ? G(J.hu)[k] : 0., in ? G(J.hv)[k] : 0.,
+                   J.tspart + 3 * (size_t)T.tile);
 }
 
 __global__ __launch_bounds__(256) void k_timestep_final(EngineArgs E, const int* jobs) {
-  __shared__ double sa[4], sb[4], sn[4];
   const JobDev& J = E.jobs[jobs[blockIdx.x]];
-  double a = 0.0, b = 0.0, nan = 0.0;
-  for (int t = threadIdx.x; t < J.nblk; t += blockDim.x) {
-    a = max_r(a, J.tspart[3 * (size_t)t]);
-    b = max_r(b, J.tspart[3 * (size_t)t + 1]);
-    nan = max_r(nan, J.tspart[3 * (size_t)t + 2]);
-  }
-  a = wave_max(a);
-  b = wave_max(b);
-  nan = wave_max(nan);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) {
-    sa[w] = a;
-    sb[w] = b;
-    sn[w] = nan;
-  }
-  __syncthreads();
+  const gdouble *h = G(J.h), *hu = G(J.hu), *hv = G(J.hv);
+  double ts, LX, LY;
+  timestep_final(
+      J.tspart, J.nblk, J.n,
+      [&](int k, double& a, double& b, double& c) {
+        a = h[k];
+        b = hu[k];
+        c = hv[k];
+      },
+      J.g, J.dx, J.dy, J.cfl, ts, LX, LY);
   if (threadIdx.x != 0) return;
-  double LX = 0.0, LY = 0.0;
-  for (int q = 0; q < 4; ++q) {
-    LX = max_r(LX, sa[q]);
-    LY = max_r(LY, sb[q]);
-  }
-  const bool anynan = (sn[0] + sn[1] + sn[2] + sn[3]) != 0.0;
-  double ts;
-  if (!anynan) {
-    ts = min_r(min_r(kHuge, J.dx / LX), J.dy / LY);
-  } else {
-    // the reference's fold, cell by cell in its loop order
-    LX = 0.0;
-    LY = 0.0;
-    ts = kHuge;
-    for (int k = 0; k < J.n; ++k) {
-      double aa, bb;
-      wave_speeds(J, G(J.h)[k], G(J.hu)[k], G(J.hv)[k], aa, bb);
-      LX = max_r(LX, aa);
-      LY = max_r(LY, bb);
-      ts = min_r(min_r(ts, J.dx / LX), J.dy / LY);
-    }
-  }
-  ts = J.cfl * ts;
-  if (ts < kMinTimestep) ts = kMinTimestep;
   JobStatus* st = J.status;
   st->dt_min = ts;
   // the time loop's dt = MIN(MIN(dt_min, tAnalyze-t), MIN(dt_min, tEnd-t))

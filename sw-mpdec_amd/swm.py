@@ -133,8 +133,14 @@ def lib():
     L.swm_profile_name.argtypes = [ctypes.c_int]
     L.swm_fill_family.argtypes = [vp, ctypes.POINTER(Family)]
     L.swm_debug_exp.argtypes = [ctypes.c_int, _dp, _dp]
+    L.swm_timestep_noalloc.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + \
+        [vp, ctypes.c_int, vp, vp]
     _lib = L
     return L
+
+
+# the largest grid swm_timestep_noalloc accepts (include/swm.h)
+TIMESTEP_NOALLOC_MAX_CELLS = 1 << 24
 
 
 def footprint_bytes(nx, ny, wellbalanced, bathymetry) -> int:

@@ -110,3 +110,20 @@ def test_reference_request_mix_is_seeded():
            [(r.problem.ic, r.problem.nx, r.nsteps, r.arrival) for r in b]
     assert {r.problem.ic for r in a} <= set(range(1, 7))
     assert all(r.problem.wellbalanced == (r.problem.ic in (2, 3)) for r in a)
+
+
+def test_timestep_noalloc_rejects_bad_arguments_without_a_device():
+    """swm_timestep_noalloc checks sizes, ghosts and pointers before any HIP
+    call; a grid above its fixed scratch array is refused with a message."""
+    L = swm.lib()
+    fake = ctypes.c_void_p(0x1000)  # never dereferenced: every case fails its checks first
+    big = swm.TIMESTEP_NOALLOC_MAX_CELLS
+    assert L.swm_timestep_noalloc(4097, 4096, .1, .1, 9.81, .5, fake, 0, fake, None) == swm.SWM_E_ARG
+    msg = L.swm_last_error(None).decode()
+    assert "4097 x 4096" in msg and str(big) in msg and "scratch" in msg
+    assert L.swm_timestep_noalloc(0, 8, .1, .1, 9.81, .5, fake, 0, fake, None) == swm.SWM_E_ARG
+    assert L.swm_timestep_noalloc(8, 8, .1, .1, 9.81, .5, fake, 2, fake, None) == swm.SWM_E_ARG
+    assert b"ghosts" in L.swm_last_error(None)
+    assert L.swm_timestep_noalloc(8, 8, .1, .1, 9.81, .5, None, 0, fake, None) == swm.SWM_E_ARG
+    assert L.swm_timestep_noalloc(8, 8, .1, .1, 9.81, .5, fake, 0, None, None) == swm.SWM_E_ARG
+    assert 4096 * 4096 == big  # the largest grid of the GPU tests fits
