@@ -310,6 +310,7 @@ int swm_stage_min_h(swm_solver_t *h, double *hmin);
  * swm_engine_set_tile_shape / swm_engine_tile_shape for a plain handle. */
 int swm_set_execution(swm_solver_t *h, int use_graph, int margin);
 int swm_execution_stats(const swm_solver_t *h, long long stats[6]);
+int swm_status_reads(const swm_solver_t *h, long long *n);
 int swm_set_tile_shape(swm_solver_t *h, int shape);
 int swm_tile_shape(const swm_solver_t *h, int dims[4]);
 
@@ -389,6 +390,12 @@ int swm_engine_submit(swm_engine_t *e, const swm_job_desc *job, int *job_id);
 /* One step of every running job; *n_running (may be NULL) receives the
  * number still running afterwards. */
 int swm_engine_step(swm_engine_t *e, int *n_running);
+/* Up to max_steps steps of every running job, with one host read of the
+ * device status per batch of up to 16 steps instead of one per step: the
+ * device time loop commits, records and stops each job on its own (its step
+ * count, a non-finite dt).  Results are those of max_steps calls of
+ * swm_engine_step.  *steps_run (may be NULL) receives the steps run. */
+int swm_engine_run(swm_engine_t *e, int max_steps, int *n_running, int *steps_run);
 int swm_engine_job_info(swm_engine_t *e, int job_id, swm_job_info *info);
 /* Current state of a job, (1:3, 1:nx, 1:ny). */
 int swm_engine_fetch(swm_engine_t *e, int job_id, double *U);
@@ -416,6 +423,10 @@ int swm_engine_set_jacobi_limits(swm_engine_t *e, int max_iter, double tol);
  * graph and margin in use. */
 int swm_engine_set_execution(swm_engine_t *e, int use_graph, int margin);
 int swm_engine_execution_stats(const swm_engine_t *e, long long stats[6]);
+/* Times the host has waited for the device status so far (no counterpart in
+ * the reference): one per batch of steps of the device time loop, plus one
+ * per Jacobi check while a step runs in polling mode. */
+int swm_engine_status_reads(const swm_engine_t *e, long long *n);
 /* Tile shape of the face kernels (no counterpart in the reference; results
  * are bitwise the same with every shape): 0 = 16x16 threads for both face
  * directions, 1 = 32x16 for x faces and 16x32 for y faces (fewer redundant

@@ -152,7 +152,7 @@ struct Job {
   int pred[13];
   bool have_pred = false;  // pred holds the sweep counts of an accepted step
   const SchemeDef* pred_sc = nullptr;  // ... of this scheme
-  bool caps_set = false;   // the device status holds time-loop caps
+  std::vector<StepRec>* log = nullptr;  // receives each committed step while set
   bool wb_ready = false;
   const SchemeDef* scheme = &kSchemes[0];  // time scheme of engine / multi-step runs
   int hp_assoc = SWM_HP_AS_REFERENCE;     // depth view of the two-node schemes
@@ -188,12 +188,7 @@ struct swm_engine {
   size_t hstatus_cap = 0;
   double* hdt = nullptr;         // pinned: host-provided dt of a step
   int* hzero = nullptr;          // pinned: 0, for clearing dt_bad
-  struct Caps {
-    int use_cap;
-    int pad;
-    double cap_a, cap_e;
-  };
-  Caps* hcaps = nullptr;         // pinned: the time-loop clipping of one step
+  int gated = 0;                 // 1 while the launches of a batch are issued
   bool dirty = true;             // device descriptors / tables out of date
   Tables groups[kGroups];        // running jobs, by step group (group_of)
   int max_iter = 1000;
@@ -214,6 +209,7 @@ struct swm_engine {
   long long graph_epoch = 0;     // bumped whenever captured arguments change
   int keep_count = 0;            // jobs that keep the writer's state (k_keep_state launched)
   long long spec_steps = 0, redone_steps = 0, graph_launches = 0, graph_captures = 0;
+  long long status_reads = 0;    // host waits for the device status (read_status)
   struct Captured {
     long long epoch = -1;
     int device_dt = 0, track = 0;
@@ -355,6 +351,7 @@ EngineArgs eargs(swm_engine* e) {
   EngineArgs a;
   a.jobs = e->djobs.p;
   a.max_iter = e->max_iter;
+  a.gated = e->gated;
   a.tol = e->tol;
   return a;
 }
@@ -371,6 +368,7 @@ int fvtd_launch(swm_engine* e, Tables& T, const NodeList& L) {
 }
 
 int read_status(swm_engine* e) {
+  ++e->status_reads;
   const size_t nj = e->jobs.size();
   HIPCK(e, hipMemcpyAsync(e->hstatus, e->dstatus.p, nj * sizeof(JobStatus),
                           hipMemcpyDeviceToHost, e->stream));
@@ -628,6 +626,7 @@ int step_body(swm_engine* e, Tables& T, const StepPlan& P) {
         bool any = false;
         for (int q = 0; q < T.njobs; ++q) {
           const JobStatus& js = e->hstatus[T.jl[q]];
+          if (js.halt) continue;  // not in this batch's step
           for (int k = 0; k < S.count; ++k)
             if (js.st[k].active) {
               any = true;
@@ -663,6 +662,18 @@ int step_body(swm_engine* e, Tables& T, const StepPlan& P) {
   return SWM_OK;
 }
 
+// The step's launches, then the commit decision (k_gate) and the commit.
+int step_full(swm_engine* e, Tables& T, const StepPlan& P) {
+  int rc = step_body(e, T, P);
+  if (rc) return rc;
+  EngineArgs a = eargs(e);
+  hipStream_t s = e->stream;
+  timed(e, 7, [&] { launch_gate(a, T.djl.p, T.njobs, P.sc->nsolve, s); });
+  timed(e, 7, [&] { launch_commit(a, T.dcb.p, T.ncb, P.sc->final_state, s); });
+  LAUNCHCK(e);
+  return SWM_OK;
+}
+
 void drop_graph(swm_engine::Captured& c) {
   if (c.exec) (void)hipGraphExecDestroy(c.exec);
   c = swm_engine::Captured{};
@@ -672,7 +683,7 @@ void drop_graphs(swm_engine* e) {
   for (auto& c : e->caps) drop_graph(c);
 }
 
-// Speculative body through a captured graph (capturing it when the job set,
+// Speculative step (body, gate, commit) through a captured graph (capturing it when the job set,
 // the scheme or the sweep counts changed).  Each table set keeps its own
 // graph, so groups of jobs stepped one after another do not recapture.
 int step_body_graph(swm_engine* e, Tables& T, const StepPlan& P) {
@@ -695,7 +706,7 @@ int step_body_graph(swm_engine* e, Tables& T, const StepPlan& P) {
     drop_graph(c);
     c.used = e->cap_clock;
     HIPCK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    int rc = step_body(e, T, P);
+    int rc = step_full(e, T, P);
     hipGraph_t g = nullptr;
     hipError_t er = hipStreamEndCapture(e->stream, &g);
     if (rc) {
@@ -772,33 +783,57 @@ void tune_record(swm_engine* e, double ms) {
   }
 }
 
-// Runs one step of the jobs in T and leaves their status in e->hstatus.
-// host_dt: the step uses this dt for every job (the plain solver's
-// swm_step_mpdec5); otherwise each job's TimeStep is computed on the device.
-int engine_step(swm_engine* e, Tables& T, const SchemeDef* sc, int hp_assoc,
-                const double* host_dt, int iters[13], const double* caps = nullptr) {
+// Consumes the records of a batch: every committed step of every job of T
+// (the job's counters, its sweep-count prediction and, while set, its log);
+// with `only`, just the jobs it marks.
+void consume_batch(swm_engine* e, Tables& T, const SchemeDef* sc,
+                   const std::vector<char>* only = nullptr) {
+  for (int q = 0; q < T.njobs; ++q) {
+    if (only && !(*only)[q]) continue;
+    Job& J = *e->jobs[T.jl[q]];
+    const JobStatus& js = e->hstatus[T.jl[q]];
+    for (int r = 0; r < js.batch_done; ++r) {
+      const StepRec& rec = js.rec[r];
+      J.steps_done += 1;
+      J.t += rec.dt;
+      if (J.steps_left > 0) --J.steps_left;
+      for (int i = 0; i < sc->nsolve; ++i) J.jacobi_sweeps += rec.iters[i];
+      J.nonconv += rec.nonconv;
+      e->ambiguous += rec.ambiguous;
+      if (J.log) J.log->push_back(rec);
+    }
+    if (js.batch_done > 0) {
+      const StepRec& last = js.rec[js.batch_done - 1];
+      for (int i = 0; i < 13; ++i) {
+        J.last_iters[i] = last.iters[i];
+        J.pred[i] = last.iters[i];
+      }
+      J.have_pred = true;
+      J.pred_sc = sc;
+      J.vout_valid = J.vout_buf != nullptr;  // kept by every step body
+    }
+  }
+}
+
+// The device time loop's clipping and stop test (src/timediscretization.f90:
+// 58-59, 83-94) for a plain solve.
+struct LoopArgs {
+  double t, t_an, t_end;
+};
+
+// A batch: up to `steps` steps of every job of T with one read of the status
+// at the end (the device decides, step by step, which jobs go on; see
+// k_gate).  host_dt: one step with this dt for every job (steps = 1).
+// Steps run speculatively when every job has a sweep-count prediction for the
+// scheme, else one step in polling mode (the explicit schemes, which solve
+// nothing, run every step of the batch without host reads).  A job whose
+// speculative step overran its launched sweeps has that step redone in
+// polling mode, alone, before the batch returns.
+int engine_batch(swm_engine* e, Tables& T, const SchemeDef* sc, int hp_assoc,
+                 const double* host_dt, int steps, const LoopArgs* loop = nullptr) {
   if (T.njobs == 0) return SWM_OK;
   hipStream_t s = e->stream;
   e->cur_shape = e->tile_shape >= 0 ? e->tile_shape : e->tuned_shape;  // tune_choice may vary it
-  // the time-loop clipping (caps = {tAnalyze - t, tEnd - t}) or none
-  *e->hcaps = caps ? swm_engine::Caps{1, 0, caps[0], caps[1]} : swm_engine::Caps{0, 0, 0.0, 0.0};
-  for (int q = 0; q < T.njobs; ++q) {
-    Job& J = *e->jobs[T.jl[q]];
-    if (!caps && !J.caps_set) continue;
-    JobStatus* st = J.dev.status;
-    HIPCK(e, hipMemcpyAsync(&st->use_cap, &e->hcaps->use_cap, sizeof(int), hipMemcpyHostToDevice, s));
-    HIPCK(e, hipMemcpyAsync(&st->cap_a, &e->hcaps->cap_a, 2 * sizeof(double),
-                            hipMemcpyHostToDevice, s));
-    J.caps_set = caps != nullptr;
-  }
-  if (host_dt) {
-    *e->hdt = *host_dt;
-    for (int q = 0; q < T.njobs; ++q) {
-      JobStatus* st = e->jobs[T.jl[q]]->dev.status;
-      HIPCK(e, hipMemcpyAsync(&st->dt, e->hdt, sizeof(double), hipMemcpyHostToDevice, s));
-      HIPCK(e, hipMemcpyAsync(&st->dt_bad, e->hzero, sizeof(int), hipMemcpyHostToDevice, s));
-    }
-  }
   StepPlan P{};
   P.sc = sc;
   P.hp_view = (!is_explicit(*sc) && sc->msteps < 4 && hp_assoc == SWM_HP_AS_REFERENCE) ? 1 : 0;
@@ -808,7 +843,31 @@ int engine_step(swm_engine* e, Tables& T, const SchemeDef* sc, int hp_assoc,
     const Job& J = *e->jobs[T.jl[q]];
     spec = spec && J.have_pred && J.pred_sc == sc;
   }
-  bool accepted = false;
+  steps = std::max(1, std::min(steps, kMaxBatch));
+  if (host_dt || (!spec && !is_explicit(*sc)) || (spec && tuning(e))) steps = 1;
+  BatchArgs B{};
+  B.max_steps = steps;
+  if (loop) {
+    B.loop = 1;
+    B.t = loop->t;
+    B.t_an = loop->t_an;
+    B.t_end = loop->t_end;
+  }
+  e->gated = 1;
+  struct Ungate {
+    swm_engine* e;
+    ~Ungate() { e->gated = 0; }
+  } ungate{e};
+  launch_batch_begin(eargs(e), T.djl.p, T.njobs, B, s);
+  LAUNCHCK(e);
+  if (host_dt) {
+    *e->hdt = *host_dt;
+    for (int q = 0; q < T.njobs; ++q) {
+      JobStatus* st = e->jobs[T.jl[q]]->dev.status;
+      HIPCK(e, hipMemcpyAsync(&st->dt, e->hdt, sizeof(double), hipMemcpyHostToDevice, s));
+      HIPCK(e, hipMemcpyAsync(&st->dt_bad, e->hzero, sizeof(int), hipMemcpyHostToDevice, s));
+    }
+  }
   if (spec) {
     int graph, margin;
     tune_choice(e, graph, margin);
@@ -824,58 +883,59 @@ int engine_step(swm_engine* e, Tables& T, const SchemeDef* sc, int hp_assoc,
     }
     const auto t0 = std::chrono::steady_clock::now();
     const long long captures0 = e->graph_captures;
-    int rc = (graph && !e->prof.on) ? step_body_graph(e, T, P) : step_body(e, T, P);
-    if (rc) return rc;
-    rc = read_status(e);
+    for (int b = 0; b < steps; ++b) {
+      int rc = (graph && !e->prof.on) ? step_body_graph(e, T, P) : step_full(e, T, P);
+      if (rc) return rc;
+    }
+    int rc = read_status(e);
     if (rc) return rc;
     const double ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    ++e->spec_steps;
-    accepted = true;
-    for (int q = 0; q < T.njobs; ++q) accepted = accepted && !e->hstatus[T.jl[q]].overrun;
-    if (accepted) {
+    e->spec_steps += steps;
+    bool overrun = false;
+    for (int q = 0; q < T.njobs; ++q) overrun = overrun || e->hstatus[T.jl[q]].halt == kHaltOverrun;
+    if (!overrun) {
       const bool captured = e->graph_captures != captures0;
       if (captured && e->tune_skips < 2) {
         ++e->tune_skips;
       } else {
         double cells = 0.;
         for (int q = 0; q < T.njobs; ++q) cells += (double)e->jobs[T.jl[q]]->dev.n;
-        tune_record(e, ms / (cells * 1e-6));
+        tune_record(e, ms / ((double)steps * cells * 1e-6));
         e->tune_skips = 0;
       }
-    } else {
-      ++e->redone_steps;
     }
-  }
-  if (!accepted) {
+  } else {
     P.speculative = false;
-    int rc = step_body(e, T, P);
+    for (int b = 0; b < steps; ++b) {
+      int rc = step_full(e, T, P);
+      if (rc) return rc;
+    }
+    int rc = read_status(e);
+    if (rc) return rc;
+  }
+  consume_batch(e, T, sc);
+  std::vector<char> redo(T.njobs, 0);
+  bool overrun = false;
+  for (int q = 0; q < T.njobs; ++q) {
+    redo[q] = e->hstatus[T.jl[q]].halt == kHaltOverrun;
+    overrun = overrun || redo[q];
+  }
+  if (overrun) {
+    // U^n of the overrun jobs is intact (their step was not committed): redo
+    // that step in polling mode; every other job keeps its stop and is skipped
+    B.redo = 1;
+    B.max_steps = 1;
+    launch_batch_begin(eargs(e), T.djl.p, T.njobs, B, s);
+    LAUNCHCK(e);
+    for (int q = 0; q < T.njobs; ++q) e->redone_steps += redo[q];
+    P.speculative = false;
+    int rc = step_full(e, T, P);
     if (rc) return rc;
     rc = read_status(e);
     if (rc) return rc;
+    consume_batch(e, T, sc, &redo);
   }
-  EngineArgs a = eargs(e);
-  timed(e, 7, [&] { launch_commit(a, T.dcb.p, T.ncb, sc->final_state, s); });
-  LAUNCHCK(e);
-  for (int q = 0; q < T.njobs; ++q) {
-    Job& J = *e->jobs[T.jl[q]];
-    const JobStatus& js = e->hstatus[T.jl[q]];
-    for (int i = 0; i < 13; ++i) {
-      const int it = i < sc->nsolve ? js.iters[i] : 0;
-      J.last_iters[i] = it;
-      J.pred[i] = it;
-      J.jacobi_sweeps += it;
-    }
-    if (sc->nsolve > 0) {  // the solve counters are this step's only with solves
-      J.nonconv += js.nonconv;
-      e->ambiguous += js.ambiguous;
-    }
-    J.have_pred = true;
-    J.pred_sc = sc;
-    J.vout_valid = J.vout_buf != nullptr;  // kept by every step body
-  }
-  if (iters && T.njobs == 1)
-    for (int i = 0; i < sc->nsolve; ++i) iters[i] = e->hstatus[T.jl[0]].iters[i];
   return SWM_OK;
 }
 
@@ -1085,7 +1145,7 @@ int alloc_job(swm_engine* e, int nx, int ny, double dx, double dy, double g, dou
   for (int i = 0; i < 13; ++i) J.pred[i] = 4;
   J.have_pred = false;
   J.pred_sc = nullptr;
-  J.caps_set = false;
+  J.log = nullptr;
   J.scheme = &kSchemes[0];
   J.hp_assoc = SWM_HP_AS_REFERENCE;
   e->dirty = true;
@@ -1094,6 +1154,9 @@ int alloc_job(swm_engine* e, int nx, int ny, double dx, double dy, double g, dou
   int rc = upload_tables(e, J.own);
   if (rc) return rc;
   HIPCK(e, hipMemsetAsync(e->dstatus.p + id, 0, sizeof(JobStatus), e->stream));
+  // the step-count stop of the device time loop (-1: none)
+  launch_set_steps_left(e->dstatus.p + id, nsteps > 0 ? nsteps : -1, e->stream);
+  LAUNCHCK(e);
   HIPCK(e, hipStreamSynchronize(e->stream));
   *id_out = id;
   return SWM_OK;
@@ -1394,16 +1457,7 @@ int swm_engine_create(long long budget_bytes, swm_engine_t** out) {
     g_error = "hipHostMalloc failed";
     rc = SWM_E_HIP;
   }
-  if (rc == SWM_OK &&
-      hipHostMalloc((void**)&e->hcaps, sizeof(swm_engine::Caps), hipHostMallocDefault) != hipSuccess) {
-    e->hcaps = nullptr;
-    g_error = "hipHostMalloc failed";
-    rc = SWM_E_HIP;
-  }
-  if (rc == SWM_OK) {
-    *e->hzero = 0;
-    *e->hcaps = swm_engine::Caps{0, 0, 0.0, 0.0};
-  }
+  if (rc == SWM_OK) *e->hzero = 0;
   if (rc) {
     swm_engine_destroy(e);
     return rc;
@@ -1421,7 +1475,6 @@ void swm_engine_destroy(swm_engine_t* e) {
   if (e->hstatus) (void)hipHostFree(e->hstatus);
   if (e->hdt) (void)hipHostFree(e->hdt);
   if (e->hzero) (void)hipHostFree(e->hzero);
-  if (e->hcaps) (void)hipHostFree(e->hcaps);
   if (e->dstatus.p) (void)hipFree(e->dstatus.p);
   e->dstatus.p = nullptr;
   if (e->djobs.p) (void)hipFree(e->djobs.p);
@@ -1457,38 +1510,59 @@ int swm_engine_submit(swm_engine_t* e, const swm_job_desc* d, int* job_id) {
   return SWM_OK;
 }
 
-int swm_engine_step(swm_engine_t* e, int* n_running) {
-  if (!e) return SWM_E_ARG;
+// Up to max_steps steps of every running job, each group of jobs in batches
+// of one host read; a job stops on its own at its step count or when its dt
+// is not finite (it then leaves the tables before the next batch).
+static int engine_run(swm_engine* e, int max_steps, int* n_running, int* steps_run) {
   int rc = refresh(e);
   if (rc) return rc;
-  int running = 0;
-  for (Tables& T : e->groups) {
-    if (T.njobs == 0) continue;
-    const Job& J0 = *e->jobs[T.jl[0]];
-    rc = engine_step(e, T, J0.scheme, J0.hp_assoc, nullptr, nullptr);
-    if (rc) return rc;
-    HIPCK(e, hipStreamSynchronize(e->stream));
-    for (int q = 0; q < T.njobs; ++q) {
-      const int id = T.jl[q];
-      Job& J = *e->jobs[id];
-      const JobStatus& js = e->hstatus[id];
-      if (js.dt_bad) {  // the step was not committed: the job keeps its last state
-        J.state = JOB_FAILED;
-        e->dirty = true;
-        continue;
+  int most = 0;
+  for (int g = 0; g < kGroups; ++g) {
+    Tables& T = e->groups[g];
+    std::vector<std::pair<int, int>> start;  // (job, steps_done) of the group
+    for (int q = 0; q < T.njobs; ++q) start.push_back({T.jl[q], e->jobs[T.jl[q]]->steps_done});
+    int done = 0;
+    while (done < max_steps && T.njobs > 0) {
+      const Job& J0 = *e->jobs[T.jl[0]];
+      rc = engine_batch(e, T, J0.scheme, J0.hp_assoc, nullptr, max_steps - done);
+      if (rc) return rc;
+      const bool changed = e->dirty;
+      const int done_before = done;
+      for (int q = 0; q < T.njobs; ++q) {
+        const int id = T.jl[q];
+        Job& J = *e->jobs[id];
+        if (e->hstatus[id].halt == kHaltDtBad) {  // not committed: keeps its last state
+          J.state = JOB_FAILED;
+          e->dirty = true;
+        } else if (J.steps_left == 0) {
+          J.state = JOB_DONE;
+          e->dirty = true;
+        }
       }
-      J.steps_done += 1;
-      J.t += js.dt;
-      if (J.steps_left > 0 && --J.steps_left == 0) {
-        J.state = JOB_DONE;
-        e->dirty = true;
-      } else {
-        ++running;
-      }
+      for (const auto& js : start) done = std::max(done, e->jobs[js.first]->steps_done - js.second);
+      if (done == done_before && !e->dirty && !changed)
+        FAIL(e, SWM_E_STATE, "internal: a batch of %d jobs committed no step", T.njobs);
+      rc = refresh(e);  // finished and failed jobs leave the tables
+      if (rc) return rc;
     }
+    most = std::max(most, done);
   }
+  int running = 0;
+  for (const auto& J : e->jobs) running += J->state == JOB_RUNNING;
   if (n_running) *n_running = running;
+  if (steps_run) *steps_run = most;
   return SWM_OK;
+}
+
+int swm_engine_step(swm_engine_t* e, int* n_running) {
+  if (!e) return SWM_E_ARG;
+  return engine_run(e, 1, n_running, nullptr);
+}
+
+int swm_engine_run(swm_engine_t* e, int max_steps, int* n_running, int* steps_run) {
+  if (!e) return SWM_E_ARG;
+  if (max_steps < 1) FAIL(e, SWM_E_ARG, "max_steps must be >= 1 (got %d)", max_steps);
+  return engine_run(e, max_steps, n_running, steps_run);
 }
 
 int swm_engine_job_info(swm_engine_t* e, int job_id, swm_job_info* info) {
@@ -1603,6 +1677,12 @@ int swm_engine_execution_stats(const swm_engine_t* e, long long out[6]) {
   out[3] = e->graph_captures;
   out[4] = e->use_graph >= 0 ? e->use_graph : e->tuned_graph;
   out[5] = e->margin != -1 ? e->margin : e->tuned_margin;
+  return SWM_OK;
+}
+
+int swm_engine_status_reads(const swm_engine_t* e, long long* n) {
+  if (!e || !n) return SWM_E_ARG;
+  *n = e->status_reads;
   return SWM_OK;
 }
 
@@ -1885,12 +1965,10 @@ static int plain_step(swm_solver_t* s, int timescheme, double dt, int* jacobi_it
   int rc = refresh(e);
   if (rc) return rc;
   const int before = J.nonconv;
-  int iters[13] = {0};
-  rc = engine_step(e, J.own, sc, J.hp_assoc, &dt, iters);
+  rc = engine_batch(e, J.own, sc, J.hp_assoc, &dt, 1);
   if (rc) return rc;
   if (jacobi_iters)
-    for (int i = 0; i < sc->nsolve; ++i) jacobi_iters[i] = iters[i];
-  J.steps_done += 1;
+    for (int i = 0; i < sc->nsolve; ++i) jacobi_iters[i] = J.last_iters[i];
   return J.nonconv - before;
 }
 
@@ -1963,6 +2041,14 @@ int swm_set_hp_association(swm_solver_t* s, int mode) {
   return SWM_OK;
 }
 
+// Sets a job's step log for the duration of one API call.
+struct LogScope {
+  Job& J;
+  std::vector<StepRec> log;
+  explicit LogScope(Job& j) : J(j) { J.log = &log; }
+  ~LogScope() { J.log = nullptr; }
+};
+
 int swm_run_steps(swm_solver_t* s, int nsteps, double* dts, int* jacobi_iters) {
   if (!s) return SWM_E_ARG;
   swm_engine* e = s->e;
@@ -1974,22 +2060,27 @@ int swm_run_steps(swm_solver_t* s, int nsteps, double* dts, int* jacobi_iters) {
   int rc = refresh(e);
   if (rc) return rc;
   const int before = J.nonconv;
-  for (int k = 0; k < nsteps; ++k) {
+  const int ns = J.scheme->nsolve;
+  LogScope L(J);
+  int k = 0;
+  while (k < nsteps) {
     // dt from TimeStep on the device, within the step; a step whose dt is
     // not finite is not committed, so the state stays that of step k
-    int iters[13] = {0};
-    rc = engine_step(e, J.own, J.scheme, J.hp_assoc, nullptr, iters);
+    L.log.clear();
+    rc = engine_batch(e, J.own, J.scheme, J.hp_assoc, nullptr, nsteps - k);
     if (rc) return rc;
+    for (const StepRec& r : L.log) {
+      if (dts) dts[k] = r.dt;
+      if (jacobi_iters)
+        for (int i = 0; i < ns; ++i) jacobi_iters[ns * k + i] = r.iters[i];
+      ++k;
+    }
     const JobStatus& js = e->hstatus[s->job];
-    if (dts) dts[k] = js.dt;
-    if (js.dt_bad) {
-      HIPCK(e, hipStreamSynchronize(e->stream));
+    if (js.halt == kHaltDtBad) {
+      if (dts) dts[k] = js.dt;
       FAIL(e, SWM_E_STATE, "time step %d: dt is not finite (%g)", k, js.dt);
     }
-    const int ns = J.scheme->nsolve;
-    if (jacobi_iters)
-      for (int i = 0; i < ns; ++i) jacobi_iters[ns * k + i] = iters[i];
-    J.steps_done += 1;
+    if (L.log.empty()) FAIL(e, SWM_E_STATE, "internal: a batch committed no step (halt %d)", js.halt);
   }
   HIPCK(e, hipStreamSynchronize(e->stream));
   return J.nonconv - before;
@@ -2007,24 +2098,35 @@ int swm_advance(swm_solver_t* s, double* t, double t_analyze, double t_end, int 
   int rc = refresh(e);
   if (rc) return rc;
   const int before = J.nonconv;
+  LogScope L(J);
   int k = 0;
   if (steps_done) *steps_done = 0;
+  double dt_prev = 0.0;
   while (k < max_steps) {
-    // src/timediscretization.f90:58-59 and :83-94
-    const double caps[2] = {t_analyze - *t, t_end - *t};
-    rc = engine_step(e, J.own, J.scheme, J.hp_assoc, nullptr, nullptr, caps);
-    if (rc) return rc;
-    const JobStatus& js = e->hstatus[s->job];
-    if (js.dt_bad) {
-      HIPCK(e, hipStreamSynchronize(e->stream));
-      FAIL(e, SWM_E_STATE, "dt is not finite (%g) at t = %.17g", js.dt, *t);
+    // src/timediscretization.f90:58-59 and :83-94 on the device; a batch
+    // covers about the steps left to the next stop (k_gate ends it there)
+    int batch = 1;
+    if (dt_prev > 0.0) {
+      const double left = std::min(t_analyze, t_end) - *t;
+      batch = (int)std::min(16.0, std::max(1.0, std::floor(left / dt_prev) + 1.0));
     }
-    if (dts) dts[k] = js.dt;
-    *t = *t + js.dt;
-    ++k;
-    J.steps_done += 1;
+    batch = std::min(batch, max_steps - k);
+    const LoopArgs la{*t, t_analyze, t_end};
+    L.log.clear();
+    rc = engine_batch(e, J.own, J.scheme, J.hp_assoc, nullptr, batch, &la);
+    if (rc) return rc;
+    for (const StepRec& r : L.log) {
+      if (dts) dts[k] = r.dt;
+      *t = *t + r.dt;  // as k_gate: the loop's t = t + dt
+      dt_prev = r.dt;
+      ++k;
+    }
     if (steps_done) *steps_done = k;
-    if (std::fabs(t_analyze - *t) < 1.0E-10 || std::fabs(*t - t_end) < 1.0E-10) break;
+    const JobStatus& js = e->hstatus[s->job];
+    if (js.halt == kHaltDtBad)
+      FAIL(e, SWM_E_STATE, "dt is not finite (%g) at t = %.17g", js.dt, *t);
+    if (js.halt == kHaltReached) break;
+    if (L.log.empty()) FAIL(e, SWM_E_STATE, "internal: a batch committed no step (halt %d)", js.halt);
   }
   HIPCK(e, hipStreamSynchronize(e->stream));
   return J.nonconv - before;
@@ -2144,6 +2246,11 @@ int swm_tile_shape(const swm_solver_t* s, int dims[4]) {
 int swm_execution_stats(const swm_solver_t* s, long long stats[6]) {
   if (!s) return SWM_E_ARG;
   return swm_engine_execution_stats(s->e, stats);
+}
+
+int swm_status_reads(const swm_solver_t* s, long long* n) {
+  if (!s) return SWM_E_ARG;
+  return swm_engine_status_reads(s->e, n);
 }
 
 int swm_set_jacobi_limits(swm_solver_t* s, int max_iter, double tol) {

@@ -53,6 +53,32 @@ struct JacState {
   double residual;
 };
 
+// Steps of one job that one host synchronisation can cover: the device time
+// loop (k_batch_begin, k_gate) runs up to kMaxBatch steps between reads of
+// the status, recording each committed step.
+constexpr int kMaxBatch = 16;
+
+// Why a job's batch stopped (JobStatus::halt); while halt != 0 every gated
+// kernel of the job returns at once and k_commit commits nothing.
+enum HaltCode {
+  kHaltNone = 0,
+  kHaltBatch = 1,    // the batch's step count was reached
+  kHaltCount = 2,    // the job's step count (steps_left) was reached
+  kHaltReached = 3,  // t reached t_analyze or t_end (src/timediscretization.f90:85-94)
+  kHaltOverrun = 4,  // a speculative step needed more Jacobi sweeps than launched
+  kHaltDtBad = 5,    // TimeStep gave a non-finite dt (the step is not committed)
+  kHaltMasked = 6,   // not part of the redo of another job's step
+};
+
+// One committed step of a batch.
+struct StepRec {
+  double dt;
+  int iters[13];  // sweeps of each solve, in execution order
+  int nonconv;    // solves stopped at the iteration limit
+  int ambiguous;  // stop decisions settled by the exact sequential sum
+  int pad;
+};
+
 // Per-job status the host reads back after every Jacobi batch.
 struct JobStatus {
   JacState st[kMaxSys];
@@ -63,17 +89,25 @@ struct JobStatus {
                   // steps since the last reset (NaN if any was NaN); kept only
                   // while tracking is on
   int hmin_step_set;  // the current attempt's minimum, merged into hmin by the
-  double hmin_step;   // commit: a redone attempt's stages never reach hmin
+  double hmin_step;   // gate: a redone attempt's stages never reach hmin
   // the current step's solves, recorded after each correction's Jacobi phase
   int iters[13];  // sweeps of each solve, in execution order
   int nonconv;    // solves stopped at the iteration limit
   int ambiguous;  // stop decisions settled by the exact sequential sum
   int overrun;    // a solve was still running when its phase ended (the
                   // step ran with too few sweeps launched and must be redone)
-  int use_cap;    // 1: dt = MIN(MIN(dt_min, cap_a), MIN(dt_min, cap_e)), the
-                  // clipping of the reference's time loop
-  double cap_a, cap_e;  // tAnalyze - t, tEnd - t (set by the host per step)
-  double dt_min;  // TimeStep's value before the clipping
+  double dt_min;  // TimeStep's value before the time loop's clipping
+  // the device time loop (k_batch_begin sets, k_gate advances)
+  int halt;        // HaltCode
+  int commit;      // 1: k_commit commits the step just run
+  int batch_done;  // steps committed in this batch (records rec[0..batch_done))
+  int batch_max;   // steps this batch may commit
+  int loop;        // 1: dt = MIN(MIN(dt_min, t_an - t), MIN(dt_min, t_end - t))
+                   // (src/timediscretization.f90:58-59) and t advances here
+  int steps_left;  // step-count stop, counted down by k_gate: > 0 steps to go,
+                   // 0 all run (k_batch_begin keeps the job halted), < 0 none
+  double t, t_an, t_end;
+  StepRec rec[kMaxBatch];
 };
 
 struct JobDev {
@@ -123,11 +157,22 @@ struct TileRef {
   int tile;
 };
 
-// Arguments shared by every launch of one engine step.
+// Arguments shared by every launch of one engine step.  gated = 1 in the
+// steps of a batch: a job whose JobStatus::halt is set is skipped.
 struct EngineArgs {
   const JobDev* jobs;
   int max_iter;
+  int gated;
   double tol;
+};
+
+// k_batch_begin's settings for every job of a batch (redo = 1: only jobs
+// halted by an overrun run, the others are masked).
+struct BatchArgs {
+  int max_steps;
+  int redo;
+  int loop;
+  double t, t_an, t_end;
 };
 
 // Which DeC nodes an FVTD launch evaluates (blockIdx.y picks one).
@@ -246,6 +291,12 @@ void launch_timestep(const EngineArgs& E, const TileRef* cb, int ncb, const int*
                      hipStream_t s);
 void launch_commit(const EngineArgs& E, const TileRef* cb, int ncb, int final_node,
                    hipStream_t s);
+// k_gate: the commit decision and record of each job's step
+void launch_gate(const EngineArgs& E, const int* jl, int njobs, int nsolve, hipStream_t s);
+void launch_batch_begin(const EngineArgs& E, const int* jl, int njobs, const BatchArgs& B,
+                        hipStream_t s);
+// sets JobStatus::steps_left of one job (its step-count stop)
+void launch_set_steps_left(JobStatus* st, int steps_left, hipStream_t s);
 void launch_hp_view(const EngineArgs& E, const TileRef* cb, int ncb, int from_un, hipStream_t s);
 void launch_combine(const EngineArgs& E, const TileRef* cb, int ncb, const CombOp& op,
                     hipStream_t s);

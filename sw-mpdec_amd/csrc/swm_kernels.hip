@@ -74,6 +74,12 @@ __device__ __forceinline__ void cell_ij(int i0, int j0, int t, int nx, int& i, i
 
 __device__ __forceinline__ int cdiv_d(int a, int b) { return (a + b - 1) / b; }
 
+// In the steps of a batch (E.gated) a job whose batch has stopped
+// (JobStatus::halt) is skipped by every kernel; the test is uniform per block.
+__device__ __forceinline__ bool skipped(const EngineArgs& E, const JobDev& J) {
+  return E.gated && J.status->halt != 0;
+}
+
 // State slot m of a job: m = 0 is U^n, 1..3 the DeC nodes.
 __device__ __forceinline__ void node_state(const JobDev& J, int m, const gdouble*& h,
                                            const gdouble*& hu, const gdouble*& hv) {
@@ -118,6 +124,7 @@ __global__ __launch_bounds__(TW * TH) void k_fvtd_x(EngineArgs E, const TileRef*
   double* sQP = sQM + 3 * NQ;
   const TileRef T = tiles[blockIdx.x];
   const JobDev& J = E.jobs[T.job];
+  if (skipped(E, J)) return;
   const int m = L.m[blockIdx.y], slot = L.slot[blockIdx.y];
   const int nx = J.nx, ny = J.ny;
   const int ntx = cdiv_d(nx + 1, TW - 1);
@@ -214,6 +221,7 @@ __global__ __launch_bounds__(TW * TH) void k_fvtd_y(EngineArgs E, const TileRef*
   double* sQP = sQM + 3 * NQ;
   const TileRef T = tiles[blockIdx.x];
   const JobDev& J = E.jobs[T.job];
+  if (skipped(E, J)) return;
   const int m = L.m[blockIdx.y], slot = L.slot[blockIdx.y];
   const int nx = J.nx, ny = J.ny;
   const int ntx = cdiv_d(nx, TW);
@@ -312,6 +320,7 @@ __global__ __launch_bounds__(256) void k_fvtd_update(EngineArgs E, const TileRef
   if (tile >= ntiles) return;
   const TileRef T = tiles[tile];
   const JobDev& J = E.jobs[T.job];
+  if (skipped(E, J)) return;
   const int nx = J.nx, ny = J.ny;
   const int ntx = cdiv_d(nx, kTile);
   const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
@@ -425,6 +434,7 @@ __global__ __launch_bounds__(256) void k_momentum(EngineArgs E, const TileRef* b
                                                   SysList S) {
   const TileRef T = blocks[blockIdx.x];
   const JobDev& J = E.jobs[T.job];
+  if (skipped(E, J)) return;
   const int k = T.tile * kCellBlock + threadIdx.x;
   if (k >= J.n) return;
   const double dt = J.status->dt;
@@ -569,6 +579,7 @@ __global__ __launch_bounds__(256) void k_assemble(EngineArgs E, const TileRef* b
   __shared__ double sq[4 * R * 256];  // [slot*R + node][thread]: face mass fluxes
   const TileRef T = blocks[blockIdx.x];
   const JobDev& J = E.jobs[T.job];
+  if (skipped(E, J)) return;
   const int nx = J.nx, ny = J.ny, n = J.n;
   const int tid = threadIdx.x;
   const int k = T.tile * kCellBlock + tid;
@@ -645,6 +656,7 @@ __global__ __launch_bounds__(256) void k_sweep(EngineArgs E, const TileRef* bloc
   __shared__ double red[4];
   const TileRef T = blocks[blockIdx.x];
   const JobDev& J = E.jobs[T.job];
+  if (skipped(E, J)) return;
   int act[kMaxSys] = {0, 0, 0}, cur[kMaxSys] = {0, 0, 0};
   bool any = false;
   for (int s = 0; s < S.count; ++s) {
@@ -694,6 +706,7 @@ __global__ __launch_bounds__(256) void k_decide(EngineArgs E, const int* jobs, S
   __shared__ double red[4];
   const JobDev& J = E.jobs[jobs[blockIdx.x]];
   JobStatus* st = J.status;
+  if (skipped(E, J)) return;
   const int n = J.n, nb = J.nblk;
   double tot[kMaxSys + 1] = {0., 0., 0., 0.};
   int act[kMaxSys] = {0, 0, 0}, cur[kMaxSys] = {0, 0, 0};
@@ -787,6 +800,7 @@ __global__ __launch_bounds__(256) void k_finalize(EngineArgs E, const TileRef* b
   __shared__ double red[4];
   const TileRef T = blocks[blockIdx.x];
   const JobDev& J = E.jobs[T.job];
+  if (skipped(E, J)) return;
   const int k = T.tile * kCellBlock + threadIdx.x;
   double mn = __builtin_inf();
   if (k < J.n) {
@@ -809,6 +823,7 @@ __global__ __launch_bounds__(256) void k_finalize(EngineArgs E, const TileRef* b
 __global__ __launch_bounds__(256) void k_stage_min(EngineArgs E, const int* jobs) {
   __shared__ double red[4];
   const JobDev& J = E.jobs[jobs[blockIdx.x]];
+  if (skipped(E, J)) return;
   double mn = __builtin_inf();
   for (int t = threadIdx.x; t < J.nblk; t += blockDim.x) mn = nan_min(mn, J.part[t]);
   mn = block_min_all(mn, red);
@@ -825,6 +840,7 @@ __global__ __launch_bounds__(256) void k_stage_min(EngineArgs E, const int* jobs
 __global__ void k_solve_done(EngineArgs E, const int* jobs, SysList S, int first_solve) {
   if (threadIdx.x != 0) return;
   JobStatus* st = E.jobs[jobs[blockIdx.x]].status;
+  if (E.gated && st->halt) return;
   if (first_solve == 0) {
     st->nonconv = 0;
     st->ambiguous = 0;
@@ -841,24 +857,92 @@ __global__ void k_solve_done(EngineArgs E, const int* jobs, SysList S, int first
 }
 
 // U^{n+1} = Ua(MSteps) (src/timediscretization.f90:564,662,762), node
-// mf = MSteps - 1 here; a job whose dt was not finite keeps U^n.
+// mf = MSteps - 1 here, for the jobs whose step k_gate accepted.
 __global__ __launch_bounds__(256) void k_commit(EngineArgs E, const TileRef* blocks, int mf) {
   const TileRef T = blocks[blockIdx.x];
   const JobDev& J = E.jobs[T.job];
   const int k = T.tile * kCellBlock + threadIdx.x;
-  if (J.status->dt_bad) return;
-  if (k == 0) {  // the accepted step's stage minimum joins the record
-    JobStatus* st = J.status;
-    if (st->hmin_step_set) {
-      st->hmin = st->hmin_set ? nan_min(st->hmin, st->hmin_step) : st->hmin_step;
-      st->hmin_set = 1;
-      st->hmin_step_set = 0;
-    }
-  }
-  if (k >= J.n) return;
+  if (!J.status->commit || k >= J.n) return;
   G(J.h)[k] = G(J.hres[mf])[k];
   G(J.hu)[k] = G(J.hun[mf])[k];
   G(J.hv)[k] = G(J.hvn[mf])[k];
+}
+
+// ------------------------------------------------------------------------
+// The device time loop: up to kMaxBatch steps between host reads.
+//
+// k_batch_begin starts a batch (one thread per job).  After each step k_gate
+// (one thread per job) decides whether the step is committed: not when the
+// job's batch has already stopped, when a speculative step overran its
+// launched Jacobi sweeps (U^n is intact and the host redoes the step), or when
+// dt is not finite.  A committed step is recorded and advances the loop: t =
+// t + dt and the output/end test of src/timediscretization.f90:83-94, the
+// job's step count and the batch's.  Whatever stops the batch sets halt, and
+// every later kernel of the batch skips the job.
+// ------------------------------------------------------------------------
+__global__ void k_batch_begin(EngineArgs E, const int* jobs, BatchArgs B) {
+  if (threadIdx.x != 0) return;
+  JobStatus* st = E.jobs[jobs[blockIdx.x]].status;
+  if (B.redo && st->halt != kHaltOverrun) {  // keeps its own stop
+    if (!st->halt) st->halt = kHaltMasked;
+    st->commit = 0;
+    return;
+  }
+  if (st->steps_left == 0) {  // the job has run all its steps
+    st->halt = kHaltCount;
+    st->commit = 0;
+    return;
+  }
+  st->halt = kHaltNone;
+  st->commit = 0;
+  st->overrun = 0;
+  st->batch_done = 0;
+  st->batch_max = B.max_steps;
+  if (B.redo) return;  // the redone step keeps the loop's t
+  st->loop = B.loop;
+  if (B.loop) {
+    st->t = B.t;
+    st->t_an = B.t_an;
+    st->t_end = B.t_end;
+  }
+}
+
+__global__ void k_gate(EngineArgs E, const int* jobs, int nsolve) {
+  if (threadIdx.x != 0) return;
+  JobStatus* st = E.jobs[jobs[blockIdx.x]].status;
+  st->commit = 0;
+  if (st->halt) return;
+  if (st->overrun) {
+    st->halt = kHaltOverrun;
+    return;
+  }
+  if (st->dt_bad) {
+    st->halt = kHaltDtBad;
+    return;
+  }
+  st->commit = 1;
+  StepRec& r = st->rec[st->batch_done];
+  r.dt = st->dt;
+  for (int i = 0; i < 13; ++i) r.iters[i] = i < nsolve ? st->iters[i] : 0;
+  r.nonconv = nsolve > 0 ? st->nonconv : 0;
+  r.ambiguous = nsolve > 0 ? st->ambiguous : 0;
+  st->batch_done += 1;
+  if (st->hmin_step_set) {  // the accepted step's stage minimum joins the record
+    st->hmin = st->hmin_set ? nan_min(st->hmin, st->hmin_step) : st->hmin_step;
+    st->hmin_set = 1;
+    st->hmin_step_set = 0;
+  }
+  if (st->loop) {
+    st->t = st->t + st->dt;
+    if (fabs(st->t_an - st->t) < 1.0E-10 || fabs(st->t - st->t_end) < 1.0E-10)
+      st->halt = kHaltReached;
+  }
+  if (st->steps_left > 0 && --st->steps_left == 0 && !st->halt) st->halt = kHaltCount;  // < 0: none
+  if (!st->halt && st->batch_done >= st->batch_max) st->halt = kHaltBatch;
+}
+
+__global__ void k_set_steps_left(JobStatus* st, int steps_left) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) st->steps_left = steps_left;
 }
 
 // The reference's view of the depths in the two-node schemes.
@@ -877,6 +961,7 @@ __global__ __launch_bounds__(256) void k_hp_view(EngineArgs E, const TileRef* bl
                                                  int from_un) {
   const TileRef T = blocks[blockIdx.x];
   const JobDev& J = E.jobs[T.job];
+  if (skipped(E, J)) return;
   const int k = T.tile * kCellBlock + threadIdx.x;
   if (k >= J.n) return;
   const gdouble* up2 = G(from_un ? J.h : J.hres[1]);
@@ -901,6 +986,7 @@ __device__ __forceinline__ const gdouble* deriv_var(const JobDev& J, int v, int 
 __global__ __launch_bounds__(256) void k_combine(EngineArgs E, const TileRef* blocks, CombOp op) {
   const TileRef T = blocks[blockIdx.x];
   const JobDev& J = E.jobs[T.job];
+  if (skipped(E, J)) return;
   const int k = T.tile * kCellBlock + threadIdx.x;
   if (k >= J.n) return;
   const double dt = J.status->dt;
@@ -968,6 +1054,7 @@ __global__ __launch_bounds__(256) void k_copy_wb(EngineArgs E, const TileRef* bl
 __global__ __launch_bounds__(256) void k_timestep(EngineArgs E, const TileRef* blocks) {
   const TileRef T = blocks[blockIdx.x];
   const JobDev& J = E.jobs[T.job];
+  if (skipped(E, J)) return;
   const int k = T.tile * kCellBlock + threadIdx.x;
   const bool in = k < J.n;
   timestep_partial(in, J.g, in ? G(J.h)[k] : 0., in ? G(J.hu)[k] : 0., in ? G(J.hv)[k] : 0.,
@@ -976,6 +1063,7 @@ __global__ __launch_bounds__(256) void k_timestep(EngineArgs E, const TileRef* b
 
 __global__ __launch_bounds__(256) void k_timestep_final(EngineArgs E, const int* jobs) {
   const JobDev& J = E.jobs[jobs[blockIdx.x]];
+  if (skipped(E, J)) return;
   const gdouble *h = G(J.h), *hu = G(J.hu), *hv = G(J.hv);
   double ts, LX, LY;
   timestep_final(
@@ -990,8 +1078,9 @@ __global__ __launch_bounds__(256) void k_timestep_final(EngineArgs E, const int*
   JobStatus* st = J.status;
   st->dt_min = ts;
   // the time loop's dt = MIN(MIN(dt_min, tAnalyze-t), MIN(dt_min, tEnd-t))
-  // (src/timediscretization.f90:58-59), when the host set the caps
-  const double dt = st->use_cap ? min_r(min_r(ts, st->cap_a), min_r(ts, st->cap_e)) : ts;
+  // (src/timediscretization.f90:58-59) when the device runs the loop
+  const double dt =
+      st->loop ? min_r(min_r(ts, st->t_an - st->t), min_r(ts, st->t_end - st->t)) : ts;
   st->dt = dt;
   st->lmx = LX;
   st->lmy = LY;
@@ -1061,6 +1150,7 @@ __global__ void k_weno5(int n, const double* q, double* first, double* second) {
 __global__ __launch_bounds__(256) void k_keep_state(EngineArgs E, const TileRef* blocks, int m) {
   const TileRef T = blocks[blockIdx.x];
   const JobDev& J = E.jobs[T.job];
+  if (skipped(E, J)) return;
   if (!J.vout) return;
   const int k = T.tile * kCellBlock + threadIdx.x;
   if (k >= J.n) return;
@@ -1157,6 +1247,19 @@ void launch_timestep(const EngineArgs& E, const TileRef* cb, int ncb, const int*
 void launch_commit(const EngineArgs& E, const TileRef* cb, int ncb, int final_node,
                    hipStream_t s) {
   if (ncb) hipLaunchKernelGGL(k_commit, dim3(ncb), dim3(256), 0, s, E, cb, final_node);
+}
+
+void launch_gate(const EngineArgs& E, const int* jl, int njobs, int nsolve, hipStream_t s) {
+  if (njobs) hipLaunchKernelGGL(k_gate, dim3(njobs), dim3(64), 0, s, E, jl, nsolve);
+}
+
+void launch_batch_begin(const EngineArgs& E, const int* jl, int njobs, const BatchArgs& B,
+                        hipStream_t s) {
+  if (njobs) hipLaunchKernelGGL(k_batch_begin, dim3(njobs), dim3(64), 0, s, E, jl, B);
+}
+
+void launch_set_steps_left(JobStatus* st, int steps_left, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_steps_left, dim3(1), dim3(64), 0, s, st, steps_left);
 }
 
 void launch_solve_done(const EngineArgs& E, const int* jl, int njobs, const SysList& S,

@@ -125,6 +125,7 @@ def lib():
     L.swm_set_tile_shape.argtypes = [vp, ctypes.c_int]
     L.swm_tile_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     L.swm_execution_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
+    L.swm_status_reads.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
     L.swm_stage_min_h.argtypes = [vp, _dp]
     L.swm_debug_weno5.argtypes = [ctypes.c_int, _dp, _dp, _dp]
     L.swm_profile_enable.argtypes = [vp, ctypes.c_int]
@@ -365,6 +366,12 @@ class Solver:
         self._check(lib().swm_execution_stats(self._h, out), "swm_execution_stats")
         return dict(zip(EXEC_STATS, list(out)))
 
+    def status_reads(self):
+        """Host waits for the device status so far."""
+        n = ctypes.c_longlong(0)
+        self._check(lib().swm_status_reads(self._h, ctypes.byref(n)), "swm_status_reads")
+        return n.value
+
     def set_tile_shape(self, shape=-1):
         """Face-kernel tile shape: 0 (16x16), 1 (32x16 / 16x32), -1 autotuned."""
         self._check(lib().swm_set_tile_shape(self._h, int(shape)), "swm_set_tile_shape")
@@ -497,6 +504,8 @@ def _engine_lib():
     L.swm_engine_bytes_in_use.argtypes = [vp]
     L.swm_engine_submit.argtypes = [vp, ctypes.POINTER(JobDesc), ctypes.POINTER(ctypes.c_int)]
     L.swm_engine_step.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    L.swm_engine_run.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                 ctypes.POINTER(ctypes.c_int)]
     L.swm_engine_job_info.argtypes = [vp, ctypes.c_int, ctypes.POINTER(JobInfo)]
     L.swm_engine_fetch.argtypes = [vp, ctypes.c_int, _dp]
     L.swm_engine_l1_error.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -512,6 +521,7 @@ def _engine_lib():
     L.swm_engine_set_tile_shape.argtypes = [vp, ctypes.c_int]
     L.swm_engine_tile_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     L.swm_engine_execution_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
+    L.swm_engine_status_reads.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
     L.swm_engine_stage_min_h.argtypes = [vp, ctypes.c_int, _dp]
     L._engine_bound = True
     return L
@@ -648,6 +658,14 @@ class Engine:
         self._check(_engine_lib().swm_engine_step(self._e, ctypes.byref(n)), "swm_engine_step")
         return n.value
 
+    def run(self, max_steps):
+        """Up to max_steps engine steps with one host read per batch; returns
+        (jobs still running, steps run)."""
+        n, k = ctypes.c_int(0), ctypes.c_int(0)
+        self._check(_engine_lib().swm_engine_run(self._e, int(max_steps), ctypes.byref(n),
+                                                 ctypes.byref(k)), "swm_engine_run")
+        return n.value, k.value
+
     def info(self, jid):
         i = JobInfo()
         self._check(_engine_lib().swm_engine_job_info(self._e, jid, ctypes.byref(i)),
@@ -714,6 +732,12 @@ class Engine:
         self._check(_engine_lib().swm_engine_execution_stats(self._e, out),
                     "swm_engine_execution_stats")
         return dict(zip(EXEC_STATS, list(out)))
+
+    def status_reads(self):
+        n = ctypes.c_longlong(0)
+        self._check(_engine_lib().swm_engine_status_reads(self._e, ctypes.byref(n)),
+                    "swm_engine_status_reads")
+        return n.value
 
     def set_tile_shape(self, shape=-1):
         self._check(_engine_lib().swm_engine_set_tile_shape(self._e, int(shape)),
