@@ -29,6 +29,10 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "sw-mpdec_amd"))
 
+# untimed steps in which the engine's autotuner times its candidates (tile
+# shape, graph replay, speculative margin), before the warmup steps
+TUNING_STEPS = 10
+
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
@@ -133,11 +137,35 @@ def cpu_baseline(budget_s: float = 20.0, scheme: int = 5, recfix: int = 0):
     steps = max(1, min(50, int(budget_s / max(t1, 1e-3))))
     t, k = run(steps)
     rate = n * n * k / t
-    return {"value": rate, "unit": "cell-steps/s", "cores": 1, "kind": "reference",
-            "sample": (f"reference Fortran ({kind_note}), timescheme {scheme}, "
-                       f"{f'ReconstructionFix {recfix}, ' if recfix else ''}WELLBALANCED, "
-                       f"BathymetryFlag=1 lake at rest + 1e-3 bump, {n}x{n}, {k} CFL steps, "
-                       f"{t:.1f} s, time loop only")}
+    out = {"value": rate, "unit": "cell-steps/s", "cores": 1, "kind": "reference",
+           "cpu_model": cpu_model(),
+           "sample": (f"reference Fortran ({kind_note}), timescheme {scheme}, "
+                      f"{f'ReconstructionFix {recfix}, ' if recfix else ''}WELLBALANCED, "
+                      f"BathymetryFlag=1 lake at rest + 1e-3 bump, {n}x{n}, {k} CFL steps, "
+                      f"{t:.1f} s, time loop only")}
+    if scheme >= 5:
+        # the run's Jacobi sweeps: the same steps on the GPU, whose sweep
+        # counts are the reference's solve by solve
+        import swm
+        g = swm.Solver.from_problem(p)
+        g.set_timescheme(scheme)
+        _, its, _ = g.run_steps(k)
+        g.close()
+        out["jacobi_sweeps_per_step"] = float(np.sum(its)) / k
+    return out
+
+
+def cpu_model() -> str:
+    """The host CPU's model name and the cores this process may use."""
+    name = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                name = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return f"{name}; {len(os.sched_getaffinity(0))} cores available, 1 used (taskset -c 1)"
 
 
 def schemes_section(n: int, seed: int, steps: int, cpu: bool) -> dict:
@@ -149,11 +177,11 @@ def schemes_section(n: int, seed: int, steps: int, cpu: bool) -> dict:
     import problems as P
     import swm
 
-    prob = P.lake_bumps(n, seed)
+    spec = P.lake_bumps_spec(n, seed)
     res = {}
     for ts, name in ((1, "forward_euler"), (2, "ssprk4"), (3, "rk65"), (4, "dec5"),
                      (6, "mpeuler"), (7, "mpdec2")):
-        s = swm.Solver.from_problem(prob)
+        s = swm.Solver.from_family(spec)
         s.set_timescheme(ts)
         s.set_hp_association(swm.HP_BY_NODE)
         s.run_steps(1)
@@ -314,10 +342,11 @@ def served_ic_section(count: int, seed: int, rate: float) -> dict:
 
 def batched_section(count: int, seed: int) -> dict:
     """`count` uneven solves (N = round(exp(U[ln 32, ln 512])), 20..200 CFL
-    steps, half vortices, half well-balanced lakes over bumps) run together
-    in one engine, against the same solves run one after another with the
-    plain solver.  Both timings exclude host-side setup and upload.  Every
-    job's final state must be bitwise the same both ways."""
+    steps, half vortices, half well-balanced lakes over bumps, their data made
+    on the device) run together in one engine, against the same solves run
+    one after another with the plain solver.  Both timings exclude the
+    initial data.  Every job's final state must be bitwise the same both
+    ways."""
     import problems as P
     import serve
     import swm
@@ -332,7 +361,7 @@ def batched_section(count: int, seed: int) -> dict:
     seq = 0.0
     same = True
     for p, r in zip(probs, res):
-        sv = swm.Solver.from_problem(p)
+        sv = serve.solver_for(p)
         sv.sync()
         ts = time.perf_counter()
         sv.run_steps(p.steps)
@@ -445,32 +474,41 @@ def main():
 
     n = args.size
     t0 = time.time()
-    prob = P.lake_bumps(n, args.seed, steps=args.steps)
-    log(f"bench: problem {n}x{n} lake_bumps seed {args.seed} built in {time.time() - t0:.1f} s")
-    s = swm.Solver.from_problem(prob)
+    # the state, its well-balanced reference and the bathymetry tables made on
+    # the device, bitwise the host builder's problems.lake_bumps(n, seed)
+    s = swm.Solver.from_family(P.lake_bumps_spec(n, args.seed, steps=args.steps))
     s.set_execution(args.graph, args.margin)
     s.set_tile_shape(args.tile_shape)
-    del prob
-    for _ in range(args.warmup):
-        dt, _, _ = s.timestep()
-        s.step(dt)
     s.sync()
-    log(f"bench: {args.warmup} warmup steps done")
+    log(f"bench: problem {n}x{n} lake_bumps seed {args.seed} made in {time.time() - t0:.1f} s")
+    # the autotuner's candidate steps (execution mode and tile shape), then
+    # the warmup steps; TimeStep and the step run on the device
+    # (swm_run_steps: the device time loop, one host read per batch)
+    s.run_steps(TUNING_STEPS)
+    if args.warmup:
+        s.run_steps(args.warmup)
+    s.sync()
+    log(f"bench: {TUNING_STEPS} tuning + {args.warmup} warmup steps done")
 
+    # the timed region, with a HIP event pair around every kernel launch
     s.profile(True)
     s.sync()
-    iters = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        dt, _, _ = s.timestep()
-        it, _ = s.step(dt)
-        iters.append(it)
+    _, iters, _ = s.run_steps(args.steps)
     s.sync()
     t1 = time.perf_counter()
     prof = s.profile_read()
     s.profile(False)
     elapsed = t1 - t0
     log(f"bench: {args.steps} timed steps in {elapsed:.3f} s")
+    # the same number of steps without events (graph replay allowed): the
+    # production execution mode, reported beside the timed figure
+    s.sync()
+    t2 = time.perf_counter()
+    s.run_steps(args.steps)
+    s.sync()
+    el_off = time.perf_counter() - t2
+    log(f"bench: {args.steps} steps without events in {el_off:.3f} s")
 
     value = n * n * args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
@@ -521,7 +559,13 @@ def main():
                                f"8 Gaussian bumps, CFL 0.5",
                    "grid": [n, n], "cfl": 0.5, "wellbalanced": True, "seed": args.seed},
         "jacobi_sweeps_per_step": jsum / args.steps,
-        "execution": dict(s.execution_stats(), tile_shape=s.tile_shape()[1]),
+        "tuning_steps": TUNING_STEPS,
+        "events_off": {"cell_steps_per_s": n * n * args.steps / el_off,
+                       "ms_per_step": 1e3 * el_off / args.steps,
+                       "note": "the same run_steps call with no per-launch HIP events "
+                               "(HIP-graph replay allowed), right after the timed region"},
+        "execution": dict(s.execution_stats(), tile_shape=s.tile_shape()[1],
+                          status_reads=s.status_reads()),
         "kernels": kern,
         "roofline": roof,
     }

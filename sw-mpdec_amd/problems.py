@@ -473,9 +473,11 @@ def lake_flag1(n: int, wellbalanced: bool = True, pert: float = 1e-3, ny: int = 
 
 
 def mixed_batch(count: int, seed: int, nmin: int = 32, nmax: int = 512,
-                smin: int = 20, smax: int = 200) -> list[Problem]:
+                smin: int = 20, smax: int = 200) -> list[FamilySpec]:
     """Seeded mix of uneven solves: N = round(exp(U[ln nmin, ln nmax])),
-    steps ~ U{smin..smax}, 50/50 vortex and lake-at-rest-over-bumps."""
+    steps ~ U{smin..smax}, 50/50 vortex and lake-at-rest-over-bumps, as
+    family parameters (the device makes the arrays; .build() makes them on
+    the host)."""
     rng = np.random.default_rng(seed)
     out = []
     for k in range(count):
@@ -483,7 +485,7 @@ def mixed_batch(count: int, seed: int, nmin: int = 32, nmax: int = 512,
         steps = int(rng.integers(smin, smax + 1))
         s = int(rng.integers(0, 2**31 - 1))
         if rng.uniform() < 0.5:
-            out.append(vortex(n, s, steps=steps))
+            out.append(vortex_spec(n, s, steps=steps))
         else:
-            out.append(lake_bumps(n, s, steps=steps))
+            out.append(lake_bumps_spec(n, s, steps=steps))
     return out

@@ -48,7 +48,10 @@ def request_mix(count: int, seed: int, rate: float, nmin: int = 64, nmax: int = 
                 smin: int = 50, smax: int = 500) -> list[Request]:
     """Seeded Poisson arrivals at `rate` per second; N log-uniform on
     [nmin, nmax], steps uniform on {smin..smax}; initial states drawn from the
-    vortex, lake-at-rest-over-bumps and shallow dam-break families."""
+    vortex, lake-at-rest-over-bumps and shallow dam-break families.  A request
+    carries only its family's parameters (problems.FamilySpec): its arrays are
+    made on the device at admission (spec.build() makes the same ones on the
+    host)."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -59,29 +62,33 @@ def request_mix(count: int, seed: int, rate: float, nmin: int = 64, nmax: int = 
         fam = int(rng.integers(0, 3))
         s = int(rng.integers(0, 2**31 - 1))
         if fam == 0:
-            p = P.vortex(n, s, steps=steps)
+            p = P.vortex_spec(n, s, steps=steps)
         elif fam == 1:
-            p = P.lake_bumps(n, s, steps=steps)
+            p = P.lake_bumps_spec(n, s, steps=steps)
         else:
-            p = P.dam_break(n, steps=steps, dry=0.1)
+            p = P.dam_break_spec(n, steps=steps, dry=0.1)
         out.append(Request(k, t, p, steps))
     return out
 
 
-def run_batch(problems, nsteps, budget_bytes: int = 0, profile: bool = False):
-    """All problems in one engine from the start; returns (results, seconds,
-    engine steps, profile)."""
+def run_batch(problems, nsteps, budget_bytes: int = 0, profile: bool = False,
+              steps_per_call: int = 16):
+    """All problems (Problem or FamilySpec) in one engine from the start;
+    returns (results, seconds, engine steps, profile).  Each engine call runs
+    up to steps_per_call steps with one host read per batch."""
     eng = swm.Engine(budget_bytes)
-    ids = [eng.submit(p, k) for p, k in zip(problems, nsteps)]
+    ids = [_submit_problem(eng, p, k) for p, k in zip(problems, nsteps)]
     if any(i is None for i in ids):
         raise RuntimeError("batch does not fit the memory budget")
     if profile:
         eng.profile(True)
     t0 = time.perf_counter()
     steps = 0
-    while eng.step():
-        steps += 1
-    steps += 1
+    while True:
+        running, k = eng.run(steps_per_call)
+        steps += k
+        if not running:
+            break
     t1 = time.perf_counter()
     prof = eng.profile_read() if profile else None
     res = [eng.fetch(i) for i in ids]
@@ -108,16 +115,30 @@ def reference_mix(count: int, seed: int, rate: float, nmin: int = 64, nmax: int 
     return out
 
 
+def _submit_problem(eng, p, nsteps):
+    if isinstance(p, P.ReferenceSetup):
+        return eng.submit_setup(p, nsteps)
+    if isinstance(p, P.FamilySpec):
+        return eng.submit_family(p, nsteps)
+    return eng.submit(p, nsteps)
+
+
 def _submit(eng, r: Request):
-    if isinstance(r.problem, P.ReferenceSetup):
-        return eng.submit_setup(r.problem, r.nsteps)
-    return eng.submit(r.problem, r.nsteps)
+    return _submit_problem(eng, r.problem, r.nsteps)
+
+
+def solver_for(p):
+    """The plain solver holding problem p (Problem, FamilySpec or
+    ReferenceSetup), its data made where p says."""
+    if isinstance(p, P.ReferenceSetup):
+        return swm.Solver.from_setup(p)
+    if isinstance(p, P.FamilySpec):
+        return swm.Solver.from_family(p)
+    return swm.Solver.from_problem(p)
 
 
 def _solver(r: Request):
-    if isinstance(r.problem, P.ReferenceSetup):
-        return swm.Solver.from_setup(r.problem)
-    return swm.Solver.from_problem(r.problem)
+    return solver_for(r.problem)
 
 
 def work(r: Request) -> int:
@@ -126,7 +147,8 @@ def work(r: Request) -> int:
 
 
 def serve(requests: list[Request], budget_bytes: int, keep_results: bool = True,
-          log=None, policy: str = "fifo", max_running_cells: int = 0) -> dict:
+          log=None, policy: str = "fifo", max_running_cells: int = 0,
+          steps_per_call: int = 4) -> dict:
     """Continuous batching.  Arrived requests are admitted while their device
     footprint fits the budget, stepped together with all running solves, and
     retired as soon as they finish.
@@ -136,7 +158,12 @@ def serve(requests: list[Request], budget_bytes: int, keep_results: bool = True,
     one with the least work (cells x steps) first, and no admission beyond
     max_running_cells running cells (0: no cap) -- the GPU is saturated well
     below the memory limit, and short requests then finish without waiting
-    behind long ones."""
+    behind long ones.
+
+    Between admissions the engine runs up to steps_per_call steps of the
+    running solves (swm_engine_run: one host read per batch of steps); a
+    solve that finishes inside them stops on the device at its own step
+    count."""
     if policy not in ("fifo", "sjf"):
         raise ValueError(policy)
     eng = swm.Engine(budget_bytes)
@@ -176,8 +203,7 @@ def serve(requests: list[Request], budget_bytes: int, keep_results: bool = True,
             continue
         peak_bytes = max(peak_bytes, eng.bytes_in_use())
         peak_running = max(peak_running, len(running))
-        eng.step()
-        steps += 1
+        steps += eng.run(steps_per_call)[1]
         t = now()
         for jid in list(running):
             info = eng.info(jid)
@@ -189,7 +215,7 @@ def serve(requests: list[Request], budget_bytes: int, keep_results: bool = True,
                 if keep_results:
                     r.result = eng.fetch(jid)
                 eng.release(jid)
-        if log and steps % 50 == 0:
+        if log and steps // 50 != (steps - 1) // 50:
             log(f"serve: t={t:.2f}s step {steps} running {len(running)} "
                 f"waiting {len(pending) + len(arrived)}")
     makespan = now()
