@@ -2152,7 +2152,7 @@ int swm_fvtd(swm_solver_t* s, double* Ut, double* FX, double* FY, double* S, dou
   const size_t n = J.lay.n, nfx = J.lay.nfx, nfy = J.lay.nfy;
   const JobDev& D = J.dev;
   std::vector<double> ut2(n), ut3(n), s2(n), s3(n), qx(nfx), qy(nfy);
-  std::vector<double> fx[3], fy[3], fxwb[3], fywb[3];
+  std::vector<double> fx[3], fy[3];
   auto get = [&](std::vector<double>& v, const double* p, size_t cnt) {
     v.resize(cnt);
     return hipMemcpyAsync(v.data(), p, cnt * sizeof(double), hipMemcpyDeviceToHost, e->stream);
@@ -2160,10 +2160,6 @@ int swm_fvtd(swm_solver_t* s, double* Ut, double* FX, double* FY, double* S, dou
   for (int v = 0; v < 3; ++v) {
     HIPCK(e, get(fx[v], D.FX[0][v], nfx));
     HIPCK(e, get(fy[v], D.FY[0][v], nfy));
-    if (D.wb) {
-      HIPCK(e, get(fxwb[v], D.FXWB[v], nfx));
-      HIPCK(e, get(fywb[v], D.FYWB[v], nfy));
-    }
   }
   HIPCK(e, get(ut2, D.FU2[0], n));
   HIPCK(e, get(ut3, D.FU3[0], n));
@@ -2186,13 +2182,14 @@ int swm_fvtd(swm_solver_t* s, double* Ut, double* FX, double* FY, double* S, dou
         S[3 * k + 2] = s3[k];
       }
     }
-  // the reference's module FX/FY after UpdateTimeDerivative (WB subtracted)
+  // the reference's module FX/FY after UpdateTimeDerivative: the face
+  // kernels store them with FXWB/FYWB already subtracted
   if (FX)
     for (size_t k = 0; k < nfx; ++k)
-      for (int v = 0; v < 3; ++v) FX[3 * k + v] = D.wb ? fx[v][k] - fxwb[v][k] : fx[v][k];
+      for (int v = 0; v < 3; ++v) FX[3 * k + v] = fx[v][k];
   if (FY)
     for (size_t k = 0; k < nfy; ++k)
-      for (int v = 0; v < 3; ++v) FY[3 * k + v] = D.wb ? fy[v][k] - fywb[v][k] : fy[v][k];
+      for (int v = 0; v < 3; ++v) FY[3 * k + v] = fy[v][k];
   if (Prod || Dest)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i) {

@@ -199,6 +199,11 @@ __global__ __launch_bounds__(TW * TH) void k_fvtd_x(EngineArgs E, const TileRef*
       acc3 = acc3 + J.wbnd[g] * F3;
     }
     size_t o = (size_t)(j - 1) * (nx + 1) + c;
+    if (J.wb && !L.wb_init) {  // FX = FX - FXWB (src/finitevolume.f90:485-486)
+      acc1 = acc1 - G(J.FXWB[0])[o];
+      acc2 = acc2 - G(J.FXWB[1])[o];
+      acc3 = acc3 - G(J.FXWB[2])[o];
+    }
     G(J.FX[slot][0])[o] = acc1;
     G(J.FX[slot][1])[o] = acc2;
     G(J.FX[slot][2])[o] = acc3;
@@ -293,6 +298,11 @@ __global__ __launch_bounds__(TW * TH) void k_fvtd_y(EngineArgs E, const TileRef*
       acc3 = acc3 + J.wbnd[g] * F3;
     }
     size_t o = (size_t)r * nx + (i - 1);
+    if (J.wb && !L.wb_init) {  // FY = FY - FYWB (src/finitevolume.f90:487-488)
+      acc1 = acc1 - G(J.FYWB[0])[o];
+      acc2 = acc2 - G(J.FYWB[1])[o];
+      acc3 = acc3 - G(J.FYWB[2])[o];
+    }
     G(J.FY[slot][0])[o] = acc1;
     G(J.FY[slot][1])[o] = acc2;
     G(J.FY[slot][2])[o] = acc3;
@@ -378,8 +388,7 @@ __global__ __launch_bounds__(256) void k_fvtd_update(EngineArgs E, const TileRef
     }
     const size_t fe = (size_t)(j - 1) * (nx + 1) + i, fw = fe - 1;  // x faces i, i-1
     const size_t gn = (size_t)j * nx + (i - 1), gs = gn - nx;        // y faces j, j-1
-    const bool use_wb = J.wb != 0;
-    if (use_wb) {
+    if (J.wb) {
       S2 = S2 - G(J.SWB2)[k];
       S3 = S3 - G(J.SWB3)[k];
     }
@@ -387,6 +396,7 @@ __global__ __launch_bounds__(256) void k_fvtd_update(EngineArgs E, const TileRef
       G(J.M[0][0])[k] = S2;
       G(J.M[0][1])[k] = S3;
     }
+    // the face fluxes, with FXWB / FYWB already subtracted by the face kernels
     double fxe[3], fxw[3], fyn[3], fys[3];
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
@@ -394,21 +404,21 @@ __global__ __launch_bounds__(256) void k_fvtd_update(EngineArgs E, const TileRef
       fxw[v] = G(J.FX[slot][v])[fw];
       fyn[v] = G(J.FY[slot][v])[gn];
       fys[v] = G(J.FY[slot][v])[gs];
-      if (use_wb) {  // FX = FX - FXWB; FY = FY - FYWB (src/finitevolume.f90:485-488)
-        fxe[v] = fxe[v] - G(J.FXWB[v])[fe];
-        fxw[v] = fxw[v] - G(J.FXWB[v])[fw];
-        fyn[v] = fyn[v] - G(J.FYWB[v])[gn];
-        fys[v] = fys[v] - G(J.FYWB[v])[gs];
-      }
     }
+    // the divisions by dx and dy share one reciprocal each (uniform per job)
+    const double dx = J.dx, dy = J.dy;
+    const bool shared = recip_range(dx) & recip_range(dy);
+    const double rdx = recip_refined(dx), rdy = recip_refined(dy);
+    auto over_dx = [&](double a) { return shared ? div_pos(a, dx, rdx) : a / dx; };
+    auto over_dy = [&](double a) { return shared ? div_pos(a, dy, rdy) : a / dy; };
     // Ut(2:3) = S - (FX(i)-FX(i-1))/dx - (FY(j)-FY(j-1))/dy  (:525-527)
     const int o = L.out[node];
-    const double ut2 = S2 - (fxe[1] - fxw[1]) / J.dx - (fyn[1] - fys[1]) / J.dy;
-    const double ut3 = S3 - (fxe[2] - fxw[2]) / J.dx - (fyn[2] - fys[2]) / J.dy;
+    const double ut2 = S2 - over_dx(fxe[1] - fxw[1]) - over_dy(fyn[1] - fys[1]);
+    const double ut3 = S3 - over_dx(fxe[2] - fxw[2]) - over_dy(fyn[2] - fys[2]);
     if (L.ut1) {
       // the build without PATANKAR: Ut(1:3) (:529-533), with S(1) = 0 exactly
       // (SourceFunc's first component is 0., src/equation.f90:408)
-      G(J.d1[o])[k] = 0.0 - (fxe[0] - fxw[0]) / J.dx - (fyn[0] - fys[0]) / J.dy;
+      G(J.d1[o])[k] = 0.0 - over_dx(fxe[0] - fxw[0]) - over_dy(fyn[0] - fys[0]);
       G(J.d2[o])[k] = ut2;
       G(J.d3[o])[k] = ut3;
       return;
@@ -417,10 +427,10 @@ __global__ __launch_bounds__(256) void k_fvtd_update(EngineArgs E, const TileRef
     G(J.FU3[o])[k] = ut3;
     // mass flux per face / dx: the production/destruction entries of
     // src/finitevolume.f90:496-523 are max(+-q, 0) and -min(+-q, 0) of these.
-    G(J.qx[o])[fe] = fxe[0] / J.dx;
-    G(J.qy[o])[gn] = fyn[0] / J.dy;
-    if (i == 1) G(J.qx[o])[fw] = fxw[0] / J.dx;
-    if (j == 1) G(J.qy[o])[gs] = fys[0] / J.dy;
+    G(J.qx[o])[fe] = over_dx(fxe[0]);
+    G(J.qy[o])[gn] = over_dy(fyn[0]);
+    if (i == 1) G(J.qx[o])[fw] = over_dx(fxw[0]);
+    if (j == 1) G(J.qy[o])[gs] = over_dy(fys[0]);
   }
 }
 

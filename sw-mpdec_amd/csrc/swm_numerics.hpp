@@ -65,6 +65,19 @@ __device__ __forceinline__ double div_by(double a, double b, double rb) {
   return fma(e, rb, q);
 }
 
+// a/b for a positive b in [2^-400, 2^400] with rb = recip_refined(b): the
+// same bits as '/'.  A zero numerator needs no division (the quotient is a
+// itself); numerators beyond the range of div_by (tiny, huge, NaN, Inf) take
+// the IEEE quotient.
+__device__ __forceinline__ double div_pos(double a, double b, double rb) {
+#if SWM_SHARED_RECIP
+  const double m = fabs(a);
+  if ((int)(m >= kRecipLo) & (int)(m <= kRecipHi)) return div_by(a, b, rb);
+  if (a == 0.) return a;
+#endif
+  return a / b;
+}
+
 // ConsToPrim, src/equation.f90:912-945:  ht = h + MIN_DEPTH/h; u = hu/ht.
 __device__ __forceinline__ void cons_to_prim(double h, double hu, double hv, double& u,
                                              double& v) {
@@ -256,6 +269,10 @@ __device__ __forceinline__ void weno5_second(double qm2, double qm1, double q0, 
   w[3] = weno5_second_gp<3, false>(qm2, qm1, q0, qp1, qp2, d1, d2, d3, 0., 0., 0.);
 }
 
+#ifndef SWM_LIMITER_SKIP
+#define SWM_LIMITER_SKIP 1
+#endif
+
 // PositivityLimiterX/Y body, src/finitevolume.f90:671-691: theta-scaling of
 // the h values at the Gauss points of one cell towards the cell mean vh.
 __device__ __forceinline__ void positivity_limit(double vh, const double wbnd[4], double wm[4],
@@ -270,11 +287,19 @@ __device__ __forceinline__ void positivity_limit(double vh, const double wbnd[4]
 #pragma unroll
   for (int g = 0; g < 4; ++g) {
     double mmin = min_r(min_r(csi, wm[g]), wp[g]);
+    const double num = vh - kMinDepth, den = vh - mmin;
     double theta;
     if (vh == mmin)
       theta = 1.;
+#if SWM_LIMITER_SKIP
+    // |num| >= |den| with num finite makes the rounded |num/den| >= 1, so
+    // MIN(1, |num/den|) is 1 without the division (the usual case: the
+    // Gauss values lie close to the mean)
+    else if ((fabs(num) >= fabs(den)) & (fabs(num) <= kHuge))
+      theta = 1.;
+#endif
     else
-      theta = min_r(1., fabs((vh - kMinDepth) / (vh - mmin)));
+      theta = min_r(1., fabs(num / den));
     wp[g] = vh + theta * (wp[g] - vh);
     wm[g] = vh + theta * (wm[g] - vh);
   }
