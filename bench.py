@@ -217,6 +217,54 @@ def flagged_fraction(h: np.ndarray) -> float:
     return float(np.mean(ind > 0.5e-3))
 
 
+def time_loop_section(steps: int, sizes=(64, 128, 256, 512)) -> dict:
+    """The device time loop on small grids, where host round trips weigh:
+    ms per mPDeC5 step of swm_run_steps called once per step (one host read
+    of the device status per step) against one call for all `steps` steps
+    (one host read per batch of up to 16), both after the autotuner's steps,
+    on a seeded vortex (data made on the device)."""
+    import problems as P
+    import swm
+
+    res = {"workload": f"vortex N x N, mPDeC5, {steps} steps each way"}
+    for n in sizes:
+        s = swm.Solver.from_family(P.vortex_spec(n, 7))
+        s.run_steps(TUNING_STEPS)
+        s.sync()
+        r0 = s.status_reads()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.run_steps(1)
+        s.sync()
+        one = (time.perf_counter() - t0) / steps
+        r1 = s.status_reads()
+        t0 = time.perf_counter()
+        s.run_steps(steps)
+        s.sync()
+        batched = (time.perf_counter() - t0) / steps
+        r2 = s.status_reads()
+        res[str(n)] = {"ms_per_step_one_read_per_step": 1e3 * one,
+                       "ms_per_step_device_loop": 1e3 * batched,
+                       "status_reads": [r1 - r0, r2 - r1],
+                       "speedup": one / batched}
+        s.close()
+    return res
+
+
+def sync_cost_section() -> dict:
+    """What a step boundary costs (swm_debug_sync_cost): microseconds per
+    round of a chain through every block, with one kernel launch per round,
+    the launches replayed from a HIP graph, or a grid barrier inside one
+    cooperative launch -- the alternative a persistent kernel would use."""
+    import swm
+
+    res = {}
+    for nblk in (256, 1024, 2048):
+        launch, graph, barrier = swm.sync_cost(nblk, 2000)
+        res[str(nblk)] = {"launch_us": launch, "graph_us": graph, "grid_barrier_us": barrier}
+    return res
+
+
 def recfix_section(n: int, steps: int, cpu: bool) -> dict:
     """ReconstructionFix 1 (first order) and 2 (MUSCL) on the shock indicator's
     flagged cells, against 4 (WENO5 everywhere, the reference's setting), on a
@@ -441,6 +489,10 @@ def main():
                     help="steps of each other time scheme at the headline size (0: skip)")
     ap.add_argument("--initial-data", type=int, default=1,
                     help="1: time the initial data made on the device at the headline size")
+    ap.add_argument("--sync-cost", type=int, default=1,
+                    help="1: measure a step boundary's cost (launch, graph, grid barrier)")
+    ap.add_argument("--time-loop-steps", type=int, default=48,
+                    help="steps per grid of the device time loop section (0: skip)")
     ap.add_argument("--fix-steps", type=int, default=10,
                     help="steps of each ReconstructionFix run on a dam break at the headline "
                          "size (0: skip)")
@@ -574,6 +626,10 @@ def main():
         out["schemes"] = schemes_section(n, args.seed, args.scheme_steps, not args.no_cpu)
     if args.initial_data:
         out["initial_data"] = initial_data_section(n, not args.no_cpu)
+    if args.time_loop_steps > 0:
+        out["time_loop"] = time_loop_section(args.time_loop_steps)
+    if args.sync_cost:
+        out["sync_cost"] = sync_cost_section()
     if args.fix_steps > 0:
         out["reconstruction_fix"] = recfix_section(n, args.fix_steps, not args.no_cpu)
     if args.batch_jobs > 0:

@@ -456,6 +456,15 @@ int swm_debug_shared_div(int n, const double *a, const double *b, double *q);
  * method, for comparison with the host problem builder's. */
 int swm_debug_exp(int n, const double *x, double *y);
 
+/* Measurement (no counterpart in the reference): what a step boundary costs,
+ * in microseconds per round of a chain in which every one of nblk blocks reads
+ * its neighbour's value of the previous round -- [0] one kernel launch per
+ * round, [1] the same launches replayed from a HIP graph, [2] one cooperative
+ * launch with a grid-wide barrier per round.  SWM_E_STATE if a mode does not
+ * carry the chain to every block; SWM_E_ARG if nblk blocks cannot all be
+ * resident at once or rounds is outside 1..100000. */
+int swm_debug_sync_cost(int nblk, int rounds, double us_per_round[3]);
+
 /* Self-check of the WENO5 arithmetic as the kernels evaluate it
  * (WENO5_FirstSweep, src/reconstruction.f90:673-755, and WENO5_SecondSweep,
  * :999-1136) on n five-point stencils q[5i..5i+4]: first[2i..2i+1] = (wm, wp),

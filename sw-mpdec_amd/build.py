@@ -21,7 +21,8 @@ CSRC = PKG / "csrc"
 LIBDIR = PKG / "lib"
 LIB = LIBDIR / "libswm.so"
 
-SOURCES = ["swm_kernels.hip", "swm_initial.hip", "swm_noalloc.hip", "swm_engine.cpp"]
+SOURCES = ["swm_kernels.hip", "swm_initial.hip", "swm_noalloc.hip", "swm_probe.hip",
+           "swm_engine.cpp"]
 
 HIPFLAGS = [
     "--offload-arch=gfx950",

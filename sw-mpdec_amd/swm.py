@@ -136,12 +136,24 @@ def lib():
     L.swm_debug_exp.argtypes = [ctypes.c_int, _dp, _dp]
     L.swm_timestep_noalloc.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + \
         [vp, ctypes.c_int, vp, vp]
+    L.swm_debug_sync_cost.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
     _lib = L
     return L
 
 
 # the largest grid swm_timestep_noalloc accepts (include/swm.h)
 TIMESTEP_NOALLOC_MAX_CELLS = 1 << 24
+
+
+def sync_cost(nblk: int, rounds: int = 1000):
+    """Microseconds per round of a step boundary with nblk blocks: (one launch
+    per round, launches replayed from a HIP graph, a grid barrier inside one
+    cooperative launch)."""
+    out = np.zeros(3)
+    rc = lib().swm_debug_sync_cost(int(nblk), int(rounds), _ptr(out))
+    if rc != SWM_OK:
+        raise SwmError(f"swm_debug_sync_cost: {lib().swm_last_error(None).decode()} (rc={rc})")
+    return tuple(float(v) for v in out)
 
 
 def footprint_bytes(nx, ny, wellbalanced, bathymetry) -> int:

@@ -127,3 +127,12 @@ def test_timestep_noalloc_rejects_bad_arguments_without_a_device():
     assert L.swm_timestep_noalloc(8, 8, .1, .1, 9.81, .5, None, 0, fake, None) == swm.SWM_E_ARG
     assert L.swm_timestep_noalloc(8, 8, .1, .1, 9.81, .5, fake, 0, None, None) == swm.SWM_E_ARG
     assert 4096 * 4096 == big  # the largest grid of the GPU tests fits
+
+
+def test_sync_cost_checks_arguments_before_the_device():
+    L = swm.lib()
+    out = (ctypes.c_double * 3)()
+    assert L.swm_debug_sync_cost(0, 10, out) == swm.SWM_E_ARG
+    assert L.swm_debug_sync_cost(8, 0, out) == swm.SWM_E_ARG
+    assert L.swm_debug_sync_cost(8, 100001, out) == swm.SWM_E_ARG
+    assert b"rounds" in L.swm_last_error(None)
