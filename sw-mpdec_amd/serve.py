@@ -11,13 +11,25 @@
 * ``sequential``: the comparison baseline, one plain solver per request, run
   one after another in arrival order.
 
+Run as a program it serves (or solves sequentially) a prefix of the seeded
+2000-request stream and writes one JSON record per request (arrival,
+admission, completion, service time, state, SHA-256 of the final state), so
+that runs on different boxes can be compared result by result:
+
+    python sw-mpdec_amd/serve.py --mode fifo|sjf|sequential --out FILE [--first K --count M]
+    python sw-mpdec_amd/serve.py --compare SERVED.json SEQUENTIAL.json [...]
+
 The solver itself is the C ABI of include/swm.h (swm.Engine / swm.Solver);
 this module only schedules.
 """
 from __future__ import annotations
 
+import argparse
 import collections
+import hashlib
+import json
 import math
+import sys
 import time
 from dataclasses import dataclass, field
 
@@ -31,13 +43,15 @@ import swm
 class Request:
     rid: int
     arrival: float           # seconds after the stream starts
-    problem: P.Problem | P.ReferenceSetup  # host data, or an initial condition made on the device
+    problem: P.Problem | P.FamilySpec | P.ReferenceSetup  # host data, or data made on the device
     nsteps: int
     admitted: float = math.nan
     done: float = math.nan
     result: np.ndarray | None = None
     state: int = -1
     bytes: int = 0
+    digest: str = ""         # SHA-256 of the final state's bytes (keep="hash")
+    service: float = math.nan  # sequential: seconds the request's own solve took
 
     @property
     def latency(self):
@@ -141,6 +155,11 @@ def _solver(r: Request):
     return solver_for(r.problem)
 
 
+def digest(U: np.ndarray) -> str:
+    """SHA-256 of a state's bytes (a result compared across runs)."""
+    return hashlib.sha256(np.ascontiguousarray(U).tobytes()).hexdigest()
+
+
 def work(r: Request) -> int:
     """Cell-steps of a request."""
     return r.problem.n * r.problem.rows * r.nsteps
@@ -176,6 +195,7 @@ def serve(requests: list[Request], budget_bytes: int, keep_results: bool = True,
     peak_bytes = 0
     peak_running = 0
     cells = 0
+    last_log = 0.0
     while pending or arrived or running:
         t = now()
         while pending and pending[0].arrival <= t:
@@ -212,10 +232,13 @@ def serve(requests: list[Request], budget_bytes: int, keep_results: bool = True,
                 cells -= r.problem.n * r.problem.rows
                 r.done = t
                 r.state = info["state"]
-                if keep_results:
+                if keep_results == "hash":
+                    r.digest = digest(eng.fetch(jid))
+                elif keep_results:
                     r.result = eng.fetch(jid)
                 eng.release(jid)
-        if log and steps // 50 != (steps - 1) // 50:
+        if log and t - last_log > 20.0:
+            last_log = t
             log(f"serve: t={t:.2f}s step {steps} running {len(running)} "
                 f"waiting {len(pending) + len(arrived)}")
     makespan = now()
@@ -228,7 +251,7 @@ def serve(requests: list[Request], budget_bytes: int, keep_results: bool = True,
             "failed": int(sum(r.state == swm.JOB_FAILED for r in requests))}
 
 
-def sequential(requests: list[Request], keep_results: bool = True) -> dict:
+def sequential(requests: list[Request], keep_results: bool = True, log=None) -> dict:
     """The baseline: one plain solver per request, in arrival order, each
     started when both it has arrived and the previous one has finished."""
     reqs = sorted(requests, key=lambda r: r.arrival)
@@ -236,7 +259,11 @@ def sequential(requests: list[Request], keep_results: bool = True) -> dict:
     now = lambda: time.perf_counter() - t0  # noqa: E731
     lat = []
     busy = 0.0
-    for r in reqs:
+    last_log = 0.0
+    for k, r in enumerate(reqs):
+        if log and now() - last_log > 20.0:
+            last_log = now()
+            log(f"sequential: t={last_log:.1f}s request {k} of {len(reqs)}")
         while now() < r.arrival:
             time.sleep(min(r.arrival - now(), 0.05))
         ts = time.perf_counter()
@@ -246,10 +273,13 @@ def sequential(requests: list[Request], keep_results: bool = True) -> dict:
             s.run_steps(r.nsteps)
         except swm.SwmError:
             state = swm.JOB_FAILED
-        if keep_results:
+        if keep_results == "hash":
+            r.digest = digest(s.download())
+        elif keep_results:
             r.result = s.download()
         s.close()
-        busy += time.perf_counter() - ts
+        r.service = time.perf_counter() - ts
+        busy += r.service
         r.done = now()
         r.state = state
         lat.append(r.latency)
@@ -258,3 +288,104 @@ def sequential(requests: list[Request], keep_results: bool = True) -> dict:
     return {"jobs": len(reqs), "makespan_s": makespan, "jobs_per_s": len(reqs) / makespan,
             "busy_s": busy, "p50_latency_s": float(np.percentile(lat, 50)),
             "p99_latency_s": float(np.percentile(lat, 99))}
+
+
+# ---------------------------------------------------------------------------
+# The full served stream, run by parts and compared result by result
+# ---------------------------------------------------------------------------
+STREAM = dict(count=2000, seed=2024, rate=20.0, nmin=64, nmax=1024, smin=50, smax=500)
+BUDGET = 240 * 10**9  # bytes admitted at once: 288 GB of HBM less headroom
+
+
+def stream(first: int = 0, count: int = 0) -> list[Request]:
+    reqs = request_mix(STREAM["count"], STREAM["seed"], STREAM["rate"], nmin=STREAM["nmin"],
+                       nmax=STREAM["nmax"], smin=STREAM["smin"], smax=STREAM["smax"])
+    return reqs[first:first + count] if count else reqs[first:]
+
+
+def record(r: Request) -> dict:
+    p = r.problem
+    return {"rid": r.rid, "n": p.n, "rows": p.rows, "kind": getattr(p, "kind", 0),
+            "steps": r.nsteps, "arrival": r.arrival, "admitted": r.admitted, "done": r.done,
+            "service": r.service, "state": r.state, "sha256": r.digest}
+
+
+def queue_latencies(recs: list[dict]) -> list[float]:
+    """Latencies of a sequential solver serving the requests in arrival order,
+    from their measured service times: start = max(arrival, previous finish)."""
+    t, out = 0.0, []
+    for r in sorted(recs, key=lambda r: r["arrival"]):
+        t = max(t, r["arrival"]) + r["service"]
+        out.append(t - r["arrival"])
+    return out
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--mode", choices=["fifo", "sjf", "sequential"])
+    ap.add_argument("--out")
+    ap.add_argument("--first", type=int, default=0)
+    ap.add_argument("--count", type=int, default=0, help="requests from --first (0: all)")
+    ap.add_argument("--sjf-cells", type=int, default=1 << 22)
+    ap.add_argument("--compare", nargs="+", metavar="JSON")
+    a = ap.parse_args(argv)
+    if a.compare:
+        return compare(a.compare)
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    reqs = stream(a.first, a.count)
+    if a.first:  # a later part of the stream: arrival times from its own start
+        t0 = reqs[0].arrival
+        for r in reqs:
+            r.arrival -= t0
+    log(f"serve: {len(reqs)} requests from {a.first}, "
+        f"{sum(work(r) for r in reqs) / 1e9:.1f} G cell-steps, mode {a.mode}")
+    if a.mode == "sequential":
+        out = sequential(reqs, keep_results="hash", log=log)
+    else:
+        out = serve(reqs, BUDGET, keep_results="hash", log=log, policy=a.mode,
+                    max_running_cells=a.sjf_cells if a.mode == "sjf" else 0)
+    out.update(mode=a.mode, first=a.first, stream=STREAM, budget_bytes=BUDGET,
+               requests=[record(r) for r in reqs])
+    with open(a.out, "w") as f:
+        json.dump(out, f)
+    log(f"serve: {a.mode} done, {out['jobs_per_s']:.3f} jobs/s, makespan {out['makespan_s']:.1f} s")
+    return 0
+
+
+def compare(files: list[str]) -> int:
+    """Prints a summary of served runs against sequential parts of the same
+    stream: jobs/s, p50/p99 latency, and whether every result's hash agrees."""
+    runs = [json.load(open(f)) for f in files]
+    seq = {r["rid"]: r for run in runs if run["mode"] == "sequential" for r in run["requests"]}
+    summary = {}
+    if seq:
+        recs = sorted(seq.values(), key=lambda r: r["rid"])
+        lat = queue_latencies(recs)
+        busy = sum(r["service"] for r in recs)
+        last = max(r["arrival"] for r in recs)
+        summary["sequential"] = {
+            "requests": len(recs), "busy_s": busy,
+            "jobs_per_s": len(recs) / busy,
+            "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)),
+            "note": "service times measured per request; latencies by the FIFO queue "
+                    "recursion over the stream's arrivals (last arrival "
+                    f"{last:.1f} s)"}
+    for run in runs:
+        if run["mode"] == "sequential":
+            continue
+        same = [seq[r["rid"]]["sha256"] == r["sha256"] for r in run["requests"] if r["rid"] in seq]
+        summary[run["mode"]] = {
+            "requests": run["jobs"], "jobs_per_s": run["jobs_per_s"],
+            "makespan_s": run["makespan_s"], "p50_latency_s": run["p50_latency_s"],
+            "p99_latency_s": run["p99_latency_s"], "peak_running": run["peak_running"],
+            "peak_bytes": run["peak_bytes"], "failed": run["failed"],
+            "results_compared": len(same), "bitwise_identical_to_sequential": all(same),
+            "speedup_vs_sequential": (run["jobs_per_s"] / summary["sequential"]["jobs_per_s"]
+                                      if seq else None)}
+    print(json.dumps(summary, indent=1))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
