@@ -298,9 +298,13 @@ BUDGET = 240 * 10**9  # bytes admitted at once: 288 GB of HBM less headroom
 
 
 def stream(first: int = 0, count: int = 0) -> list[Request]:
-    reqs = request_mix(STREAM["count"], STREAM["seed"], STREAM["rate"], nmin=STREAM["nmin"],
+    """Requests first .. first+count-1 (count 0: to the end) of the stream.
+    Each request's draws follow the previous ones', so a prefix of the
+    stream is the stream of that length."""
+    total = min(first + count, STREAM["count"]) if count else STREAM["count"]
+    reqs = request_mix(total, STREAM["seed"], STREAM["rate"], nmin=STREAM["nmin"],
                        nmax=STREAM["nmax"], smin=STREAM["smin"], smax=STREAM["smax"])
-    return reqs[first:first + count] if count else reqs[first:]
+    return reqs[first:]
 
 
 def record(r: Request) -> dict:
@@ -333,10 +337,12 @@ def main(argv=None) -> int:
         return compare(a.compare)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     reqs = stream(a.first, a.count)
-    if a.first:  # a later part of the stream: arrival times from its own start
-        t0 = reqs[0].arrival
+    arrival = {r.rid: r.arrival for r in reqs}
+    if a.mode == "sequential":
+        # back to back: each request's service time is what is measured; the
+        # queue over the stream's arrivals is computed from them (compare)
         for r in reqs:
-            r.arrival -= t0
+            r.arrival = 0.0
     log(f"serve: {len(reqs)} requests from {a.first}, "
         f"{sum(work(r) for r in reqs) / 1e9:.1f} G cell-steps, mode {a.mode}")
     if a.mode == "sequential":
@@ -344,6 +350,8 @@ def main(argv=None) -> int:
     else:
         out = serve(reqs, BUDGET, keep_results="hash", log=log, policy=a.mode,
                     max_running_cells=a.sjf_cells if a.mode == "sjf" else 0)
+    for r in reqs:
+        r.arrival = arrival[r.rid]
     out.update(mode=a.mode, first=a.first, stream=STREAM, budget_bytes=BUDGET,
                requests=[record(r) for r in reqs])
     with open(a.out, "w") as f:
