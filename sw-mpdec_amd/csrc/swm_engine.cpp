@@ -166,8 +166,12 @@ struct Job {
 // by-node view), 5-8 the explicit schemes 1-4.
 constexpr int kGroups = 9;
 // autotuner candidates: each tile shape with (graph 0, margin 1), then
-// (1,1), (1,2), (0,2) with the faster shape
-constexpr int kTuneCands = kNumShapes + 3;
+// (graph, margin) = (1,1), (0,1), (1,2), (0,2) with the faster shape; each
+// set is timed in kTuneRounds interleaved rounds
+constexpr int kTuneExec = 4;
+constexpr int kTuneCands = kNumShapes + kTuneExec;
+constexpr int kTuneRounds = 2;
+constexpr int kTunePhases = kTuneRounds * kTuneCands;
 int group_of(const SchemeDef* d, int hp_assoc) {
   const int si = (int)(d - kSchemes);
   if (si == 0) return 0;
@@ -203,8 +207,8 @@ struct swm_engine {
   int tile_shape = -1;           // face-kernel tile shape (kShapes), -1 = autotuned
   int tuned_graph = 1, tuned_margin = 1, tuned_shape = 0;
   int cur_shape = 0;             // the shape the launches use now
-  int tune_phase = 0;            // autotuner: candidates timed so far
-  double tune_ms[kTuneCands] = {};  // per candidate: ms per million cell-steps
+  int tune_phase = 0;            // autotuner: timed steps so far
+  double tune_ms[kTuneCands];    // per candidate: ms per million cell-steps, the fastest round
   int tune_skips = 0;            // steps of the current candidate not timed (graph capture)
   long long graph_epoch = 0;     // bumped whenever captured arguments change
   int keep_count = 0;            // jobs that keep the writer's state (k_keep_state launched)
@@ -734,35 +738,42 @@ int step_body_graph(swm_engine* e, Tables& T, const StepPlan& P) {
 }
 
 // Autotuner over (tile shape, graph, margin).  The first speculative steps of
-// an engine time one candidate each: the face-kernel tile shapes first,
-// without a graph, then graph on/off x margin 1/2 with the faster shape; the
-// fastest candidate is kept.  A step that captured a graph is not timed (the
-// candidate is timed on its next step, a replay, unless it captured twice in
-// a row), and each time is divided by the cell-steps of the job set it ran
-// on, so that candidates timed while jobs join and leave compare fairly.
+// an engine time the candidates one step each, in interleaved rounds (so that
+// a drift of the clocks during tuning favours none of them; a candidate's
+// fastest round counts): the face-kernel tile shapes first, without a graph,
+// then graph on/off x margin 1/2 with the faster shape; the fastest of the
+// latter is kept.  A step that captured a graph is not timed (the candidate
+// is timed on its next step, a replay, unless it captured twice in a row),
+// and each time is divided by the cell-steps of the job set it ran on, so
+// that candidates timed while jobs join and leave compare fairly.
 // Explicit settings (swm_engine_set_execution, swm_engine_set_tile_shape)
 // override their dimension.
 struct TuneCand {
   int graph, margin, shape;
 };
 
-TuneCand tune_cand(const swm_engine* e, int phase) {
-  if (phase < kNumShapes) return {0, 1, phase};
-  int best = 0;  // the faster shape of the first phases
+// Candidate id of tuning step `phase`: kTuneRounds rounds over the shapes,
+// then kTuneRounds rounds over the execution settings.
+int tune_id(int phase) {
+  if (phase < kTuneRounds * kNumShapes) return phase % kNumShapes;
+  return kNumShapes + (phase - kTuneRounds * kNumShapes) % kTuneExec;
+}
+
+TuneCand tune_cand(const swm_engine* e, int id) {
+  if (id < kNumShapes) return {0, 1, id};
+  int best = 0;  // the faster shape
   for (int i = 1; i < kNumShapes; ++i)
     if (e->tune_ms[i] < e->tune_ms[best]) best = i;
-  static const int cg[4] = {1, 0, 1, 0}, cm[4] = {1, 1, 2, 2};
-  const int q = phase - kNumShapes;  // (0,1) was timed with each shape already
-  const int idx = q == 0 ? 0 : q + 1;
-  return {cg[idx], cm[idx], best};
+  static const int cg[kTuneExec] = {1, 0, 1, 0}, cm[kTuneExec] = {1, 1, 2, 2};
+  return {cg[id - kNumShapes], cm[id - kNumShapes], best};
 }
 
 bool tuning(const swm_engine* e) {
-  return e->tune_phase < kTuneCands && (e->use_graph < 0 || e->margin < 0 || e->tile_shape < 0);
+  return e->tune_phase < kTunePhases && (e->use_graph < 0 || e->margin < 0 || e->tile_shape < 0);
 }
 
 void tune_choice(swm_engine* e, int& graph, int& margin) {
-  TuneCand c = tuning(e) ? tune_cand(e, e->tune_phase)
+  TuneCand c = tuning(e) ? tune_cand(e, tune_id(e->tune_phase))
                          : TuneCand{e->tuned_graph, e->tuned_margin, e->tuned_shape};
   graph = e->use_graph >= 0 ? e->use_graph : c.graph;
   margin = e->margin >= 0 ? e->margin : c.margin;
@@ -771,10 +782,11 @@ void tune_choice(swm_engine* e, int& graph, int& margin) {
 
 void tune_record(swm_engine* e, double ms) {
   if (!tuning(e)) return;
-  e->tune_ms[e->tune_phase++] = ms;
-  if (e->tune_phase == kTuneCands) {
-    int best = 0;
-    for (int i = 1; i < kTuneCands; ++i)
+  double& t = e->tune_ms[tune_id(e->tune_phase)];
+  t = std::min(t, ms);
+  if (++e->tune_phase == kTunePhases) {
+    int best = kNumShapes;
+    for (int i = kNumShapes + 1; i < kTuneCands; ++i)
       if (e->tune_ms[i] < e->tune_ms[best]) best = i;
     const TuneCand c = tune_cand(e, best);
     e->tuned_graph = c.graph;
@@ -1440,6 +1452,7 @@ int swm_engine_create(long long budget_bytes, swm_engine_t** out) {
   *out = nullptr;
   swm_engine* e = new swm_engine();
   e->budget = budget_bytes;
+  std::fill(std::begin(e->tune_ms), std::end(e->tune_ms), HUGE_VAL);
   hipError_t er = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
   if (er != hipSuccess) {
     g_error = std::string("hipStreamCreate failed: ") + hipGetErrorString(er);

@@ -29,12 +29,28 @@ namespace swm {
 //   b = 0  mPDeC5  (src/timediscretization.f90:491-506)
 //   b = 1  mPEuler and mPDeC2, reshape((/0, 1/2, 0, 1/2/), (/2,2/))
 //          (src/timediscretization.f90:604,704)
-__constant__ double THETA[2][4][4] = {
-    {{0.0000000000000000000, 0.0000000000000000000, 0.0000000000000000000, 0.0000000000000000000},
-     {0.1103005664791649049, 0.1896994335208351257, -0.0339073642291439076, 0.0103005664791649201},
-     {0.0730327668541684294, 0.4505740308958107176, 0.2269672331458314485, -0.0269672331458315692},
-     {0.0833333333333333287, 0.4166666666666666852, 0.4166666666666666852, 0.0833333333333333287}},
-    {{0.0, 0.0, 0.0, 0.0}, {0.5, 0.5, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}}};
+#define SWM_THETA_TABLE                                                                         \
+  {{{0.0000000000000000000, 0.0000000000000000000, 0.0000000000000000000, 0.0000000000000000000},  \
+    {0.1103005664791649049, 0.1896994335208351257, -0.0339073642291439076, 0.0103005664791649201}, \
+    {0.0730327668541684294, 0.4505740308958107176, 0.2269672331458314485, -0.0269672331458315692}, \
+    {0.0833333333333333287, 0.4166666666666666852, 0.4166666666666666852, 0.0833333333333333287}}, \
+   {{0.0, 0.0, 0.0, 0.0}, {0.5, 0.5, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}}}
+__constant__ double THETA[2][4][4] = SWM_THETA_TABLE;
+
+// Sign pattern of THETA[b][m][.]: bit r set when thetaDeC(m+1, r+1) > 0.  The
+// assembly branches on it, uniformly for a whole launch.
+constexpr double kThetaHost[2][4][4] = SWM_THETA_TABLE;
+constexpr int theta_pos(int b, int m) {
+  int p = 0;
+  for (int r = 0; r < 4; ++r)
+    if (kThetaHost[b][m][r] > 0.) p |= 1 << r;
+  return p;
+}
+__constant__ int THETA_POS[2][4] = {{theta_pos(0, 0), theta_pos(0, 1), theta_pos(0, 2), theta_pos(0, 3)},
+                                    {theta_pos(1, 0), theta_pos(1, 1), theta_pos(1, 2), theta_pos(1, 3)}};
+static_assert(theta_pos(0, 1) == 0xB && theta_pos(0, 2) == 0x7 && theta_pos(0, 3) == 0xF &&
+                  theta_pos(1, 1) == 0x3,
+              "thetaDeC sign pattern");
 
 // Global-address-space view of a device array.  The arrays' pointers are read
 // from the job table in memory, so the compiler cannot infer their address
@@ -536,40 +552,91 @@ __device__ __forceinline__ double patankar_div_zero(double z, double h) {
   return (h < kMinDepth) ? 0. : z * (h * 0. + 1.);
 }
 
-template <int R>
-__device__ __forceinline__ void patankar_slot(const double (&th_m)[4], double dt, const double q[R],
-                                              bool east, double hK, double hL, double& M,
+// The parts of PATANKAR_DIV (src/timediscretization.f90:913-924) that depend
+// on the depth h alone: 2*h*prod/(h*h + MAX(h*h, MIN_DEPTH)) is (2h)*prod / den
+// evaluated left to right, 0 when h < MIN_DEPTH; and its value for a +0 and a
+// -0 numerator.
+struct PdDepth {
+  double twoh, den, zp, zm;
+  bool dry;
+};
+__device__ __forceinline__ PdDepth pd_depth(double h) {
+  PdDepth d;
+  d.dry = h < kMinDepth;
+  d.twoh = 2. * h;
+  d.den = h * h + max_r(h * h, kMinDepth);
+  d.zp = patankar_div_zero(0., h);
+  d.zm = patankar_div_zero(-0., h);
+  return d;
+}
+
+// One stencil slot of a row at the R DeC nodes (src/timediscretization.f90:
+// 800-884): with P and De the slot's production and destruction,
+//   theta > 0: M = M - c*PD(P, hL), D = D + c*PD(De, hK);
+//   otherwise: D = D - c*PD(P, hK), M = M + c*PD(De, hL).
+// s is the slot's oriented face mass flux: q = F/dx for the west and south
+// slots, -q for the east and north ones, so that P = s > 0 ? s : +0 and
+// De = s < 0 ? -s : -0 (src/finitevolume.f90:496-523).  At most one of P and
+// De is nonzero, and the nonzero one is |s|, so each node needs one division;
+// the zero one takes PdDepth's zp or zm, the value PATANKAR_DIV gives it.
+// The sign of theta (bit r of pos) is the same for the whole launch.  WET:
+// no depth of the row is below MIN_DEPTH, so no quotient is replaced by 0.
+template <int R, bool WET>
+__device__ __forceinline__ double pd_quot(bool useL, double a, const PdDepth& L,
+                                          const PdDepth& K) {
+  const double q = ((useL ? L.twoh : K.twoh) * a) / (useL ? L.den : K.den);
+  return (!WET && (useL ? L.dry : K.dry)) ? 0. : q;
+}
+template <int R, bool WET>
+__device__ __forceinline__ void patankar_slot(int pos, const double c[R], const double s[R],
+                                              const PdDepth& L, const PdDepth& K, double& M,
                                               double& D) {
-  // production / destruction of the slot at the R DeC nodes, from the face
-  // mass flux q = F/dx: west/south slots take max(F,0)/dx, -min(F,0)/dx; the
-  // east/north slots the same of -F (src/finitevolume.f90:496-523).
-  // theta > 0: M = M - c*PD(P, hL), D = D + c*PD(De, hK);
-  // otherwise: D = D - c*PD(P, hK), M = M + c*PD(De, hL)  (:800-884).
-  // At most one of P and De is nonzero, so each node needs one division; the
-  // zero one goes through patankar_div_zero, which gives the same value.
   M = 0.;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const double qr = q[r];
-    double P, De;
-    if (!east) {
-      P = (qr > 0.) ? qr : 0.;
-      De = (qr < 0.) ? -qr : -0.;
-    } else {
-      P = (qr < 0.) ? -qr : 0.;
-      De = (qr > 0.) ? qr : -0.;
+    const double sr = s[r];
+    const bool gt = sr > 0., lt = sr < 0.;
+    if ((pos >> r) & 1) {  // L takes P, K takes De
+      const double dv = pd_quot<R, WET>(gt, fabs(sr), L, K);
+      const double tL = c[r] * (gt ? dv : L.zp);
+      const double tK = c[r] * (lt ? dv : K.zm);
+      M = M - tL;
+      D = D + tK;
+    } else {  // L takes De, K takes P
+      const double dv = pd_quot<R, WET>(lt, fabs(sr), L, K);
+      const double tL = c[r] * (lt ? dv : L.zm);
+      const double tK = c[r] * (gt ? dv : K.zp);
+      M = M + tL;
+      D = D - tK;
     }
-    const double th = th_m[r];
-    const double c = dt * th;
-    const bool pos = th > 0.;
-    const double numL = pos ? P : De, numK = pos ? De : P;
-    const bool l_nz = numL != 0.;
-    const double dv = patankar_div(l_nz ? numL : numK, l_nz ? hL : hK);
-    const double tL = c * (l_nz ? dv : patankar_div_zero(numL, hL));
-    const double tK = c * (l_nz ? patankar_div_zero(numK, hK) : dv);
-    M = pos ? (M - tL) : (M + tL);
-    D = pos ? (D + tK) : (D - tK);
   }
+}
+
+// The four stencil slots of one row-system (sparse_matmul in CSR order S, W,
+// self (value 0), E, N): writes the slots' matrix entries, accumulates the
+// diagonal into D and returns the product with the neighbours' rhs values.
+template <int R, bool WET>
+__device__ __forceinline__ double patankar_row(const JobDev& J, int s, size_t k, const double* sq,
+                                               int pos, const double c[R], const PdDepth& pK,
+                                               double hS, double hW, double hE, double hN,
+                                               double bS, double bW, double bE, double bN,
+                                               double b, double& D) {
+  double p = 0.;
+#pragma unroll 1
+  for (int sl = 0; sl < 4; ++sl) {
+    // the slot's neighbour values, selected from the ones loaded by the caller
+    const double hL = sl == 0 ? hS : sl == 1 ? hW : sl == 2 ? hE : hN;
+    const double bL = sl == 0 ? bS : sl == 1 ? bW : sl == 2 ? bE : bN;
+    double q[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) q[r] = sq[(sl * R + r) * 256];
+    double M;
+    patankar_slot<R, WET>(pos, c, q, pd_depth(hL), pK, M, D);
+    G(J.M[s][sl])[k] = M;
+    p = p + M * bL;
+    if (sl == 1) p = p + 0. * b;
+  }
+  return p;
 }
 
 // The denominators and right-hand side of the systems: hp(m,.) is h^n in the
@@ -611,8 +678,8 @@ __global__ __launch_bounds__(256) void k_assemble(EngineArgs E, const TileRef* b
     for (int r = 0; r < R; ++r) {
       sq[(0 * R + r) * 256 + tid] = G(J.qy[S.src[r]])[gS];
       sq[(1 * R + r) * 256 + tid] = G(J.qx[S.src[r]])[fW];
-      sq[(2 * R + r) * 256 + tid] = G(J.qx[S.src[r]])[fE];
-      sq[(3 * R + r) * 256 + tid] = G(J.qy[S.src[r]])[gN];
+      sq[(2 * R + r) * 256 + tid] = -G(J.qx[S.src[r]])[fE];  // east and north slots:
+      sq[(3 * R + r) * 256 + tid] = -G(J.qy[S.src[r]])[gN];  // the flux oriented into the cell
     }
     const gdouble* rhs = system_rhs(J, S);
     b = rhs[k];
@@ -631,24 +698,21 @@ __global__ __launch_bounds__(256) void k_assemble(EngineArgs E, const TileRef* b
       const int m = S.m[s];
       const gdouble* hp = system_hp(J, S, m);
       const double hK = hp[k], hS = hp[kS], hW = hp[kW], hE = hp[kE], hN = hp[kN];
-      double D = 1.;
-      // sparse_matmul in CSR order S, W, self (value 0), E, N; then
-      // sol = (rhs - prod)/diag
-      double p = 0.;
-#pragma unroll 1
-      for (int sl = 0; sl < 4; ++sl) {
-        // the slot's neighbour values, selected from the ones loaded above
-        const double hL = sl == 0 ? hS : sl == 1 ? hW : sl == 2 ? hE : hN;
-        const double bL = sl == 0 ? bS : sl == 1 ? bW : sl == 2 ? bE : bN;
-        double q[R];
+      const PdDepth pK = pd_depth(hK);
+      const int pos = THETA_POS[S.theta][m];
+      double c[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) q[r] = sq[(sl * R + r) * 256 + tid];
-        double M;
-        patankar_slot<R>(THETA[S.theta][m], dt, q, sl >= 2, hK, hL, M, D);
-        G(J.M[s][sl])[k] = M;
-        p = p + M * bL;
-        if (sl == 1) p = p + 0. * b;
-      }
+      for (int r = 0; r < R; ++r) c[r] = dt * THETA[S.theta][m][r];
+      // sol = (rhs - prod)/diag; rows of a wave with no depth below
+      // MIN_DEPTH take the branch without the dry-depth selects
+      double D = 1.;
+      const bool dry = pK.dry | (hS < kMinDepth) | (hW < kMinDepth) | (hE < kMinDepth) |
+                       (hN < kMinDepth);
+      const double p =
+          __any(dry) ? patankar_row<R, false>(J, s, k, sq + tid, pos, c, pK, hS, hW, hE, hN, bS,
+                                              bW, bE, bN, b, D)
+                     : patankar_row<R, true>(J, s, k, sq + tid, pos, c, pK, hS, hW, hE, hN, bS,
+                                             bW, bE, bN, b, D);
       G(J.M[s][4])[k] = D;
       double x1 = (b - p) / D;
       G(J.xj[m][1])[k] = x1;
@@ -1188,9 +1252,9 @@ static void launch_faces(const EngineArgs& E, const TileRef* xt, int nxt, const 
 
 void launch_fvtd(const EngineArgs& E, int shape, const TileRef* xt, int nxt, const TileRef* yt,
                  int nyt, const TileRef* ct, int nct, const NodeList& L, hipStream_t s) {
-  static_assert(kShapes[0].xw == 16 && kShapes[0].xh == 16 && kShapes[0].yw == 16 &&
-                    kShapes[0].yh == 16 && kShapes[1].xw == 32 && kShapes[1].xh == 16 &&
-                    kShapes[1].yw == 16 && kShapes[1].yh == 32,
+  static_assert(kNumShapes == 2 && kShapes[0].xw == 16 && kShapes[0].xh == 16 &&
+                    kShapes[0].yw == 16 && kShapes[0].yh == 16 && kShapes[1].xw == 32 &&
+                    kShapes[1].xh == 16 && kShapes[1].yw == 16 && kShapes[1].yh == 32,
                 "the instantiations below follow kShapes");
   if (shape == 1)
     launch_faces<32, 16, 16, 32>(E, xt, nxt, yt, nyt, L, s);

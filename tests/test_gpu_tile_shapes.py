@@ -68,12 +68,16 @@ def test_fvtd_outputs_are_shape_independent():
 def test_autotuner_settles_on_a_shape():
     p = P.vortex(96, 7)
     s = swm.Solver.from_problem(p)
-    s.run_steps(8)  # one polling step, then the tuning candidates
+    # one polling step, then two rounds over the 2 shapes and over the 4
+    # (graph, margin) settings, plus the steps that capture a graph
+    s.run_steps(20)
     shape, dims = s.tile_shape()
     assert shape in (0, 1) and len(dims) == 4
+    st = s.execution_stats()
+    assert st["graph"] in (0, 1) and st["margin"] in (1, 2)
     U = s.download()
     o = Oracle(p)
-    o.run(8)
+    o.run(20)
     assert same_bits(U, o.get_U())
     eng = swm.Engine()
     with pytest.raises(swm.SwmError):
