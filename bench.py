@@ -41,15 +41,16 @@ FP64_PEAK_TFLOPS = 78.6
 #   fvtd_x / fvtd_y   per face of one node: read h, hu, hv (24), write F1..F3 (24)
 #   fvtd_update       per cell of one node: read one x and one y face flux (48),
 #                     write Ut(2:3) (16) and the east/north mass flux (16)
-#   momentum          per cell-node: read Ua(1,2:3) and FUp(1:4,2:3) (80), write 16
 #   patankar_assemble per row-system: face mass fluxes of 4 nodes (64), h (8),
-#                     write 4 off-diagonals, the diagonal and x1 (48)
+#                     write 4 off-diagonals, the diagonal and x1 (48); and the
+#                     DeC momentum update of the same node, which runs in the
+#                     same kernel: read Ua(1,2:3) and FUp(1:4,2:3) (80), write 16
 #   jacobi_sweep      per row-sweep: 5 matrix planes, x_old, b (56), write x_new (8)
 #   timestep          per cell: read h, hu, hv (24)
 # The 13 FVTDs / momentum updates / Patankar systems of a step cover 13
 # node-evaluations of every cell; Jacobi covers J_step sweeps of every row.
-ALGO_BYTES = {"fvtd_x": 48.0, "fvtd_y": 48.0, "fvtd_update": 80.0, "momentum": 96.0,
-              "patankar_assemble": 120.0, "jacobi_sweep": 64.0, "timestep": 24.0}
+ALGO_BYTES = {"fvtd_x": 48.0, "fvtd_y": 48.0, "fvtd_update": 80.0,
+              "patankar_assemble": 120.0 + 96.0, "jacobi_sweep": 64.0, "timestep": 24.0}
 # FP64 operations per face of one node in fvtd_x / fvtd_y, counted from the
 # source (each +, -, *, /, sqrt = 1): ConsToPrim 4, three first sweeps 348,
 # six second sweeps 1320, limiter 70, Rusanov at 4 Gauss points 264.

@@ -320,9 +320,11 @@ long long swm_ambiguous_decisions(const swm_solver_t *h);
 
 /* Measurement (no counterpart in the reference, which has no timers): HIP
  * events around every kernel launch of the handle, summed per category --
- * 0 fvtd_x, 1 fvtd_y, 2 fvtd_update, 3 momentum (and the explicit schemes'
- * stage combinations), 4 patankar_assemble (and the two-node schemes' depth
- * view), 5 jacobi_sweep, 6 timestep, 7 other.  swm_profile_read waits for the
+ * 0 fvtd_x, 1 fvtd_y, 2 fvtd_update, 3 momentum (the explicit schemes'
+ * stage combinations; the modified-Patankar schemes' momentum update runs
+ * inside the assembly kernel), 4 patankar_assemble (with that momentum
+ * update, and the two-node schemes' depth view), 5 jacobi_sweep, 6 timestep,
+ * 7 other.  swm_profile_read waits for the
  * stream, returns milliseconds and launch counts, and clears the record. */
 #define SWM_PROF_NCAT 8
 int swm_profile_enable(swm_solver_t *h, int on);

@@ -605,7 +605,7 @@ int step_body(swm_engine* e, Tables& T, const StepPlan& P) {
     else
       for (int m = 1; m < D.msteps; ++m) S.m[S.count++] = m;
     if (S.hp_view) timed(e, 4, [&] { launch_hp_view(a, T.dcb.p, T.ncb, kk == 1, s); });
-    timed(e, 3, [&] { launch_momentum(a, T.dcb.p, T.ncb, S, s); });
+    // the DeC momentum update runs inside the assembly kernel
     timed(e, 4, [&] { launch_assemble(a, T.dcb.p, T.ncb, T.djl.p, T.njobs, S, s); });
     LAUNCHCK(e);
     if (P.speculative) {

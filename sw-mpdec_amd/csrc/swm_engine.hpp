@@ -279,8 +279,6 @@ void launch_primitives(size_t n, const double* h, const double* hu, const double
 // xt / yt: face tiles of shape kShapes[shape]; ct: k_fvtd_update tiles.
 void launch_fvtd(const EngineArgs& E, int shape, const TileRef* xt, int nxt, const TileRef* yt,
                  int nyt, const TileRef* ct, int nct, const NodeList& L, hipStream_t s);
-void launch_momentum(const EngineArgs& E, const TileRef* cb, int ncb, const SysList& S,
-                     hipStream_t s);
 void launch_assemble(const EngineArgs& E, const TileRef* cb, int ncb, const int* jl, int njobs,
                      const SysList& S, hipStream_t s);
 void launch_sweep(const EngineArgs& E, const TileRef* cb, int ncb, const int* jl, int njobs,

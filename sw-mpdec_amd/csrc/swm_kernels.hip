@@ -12,7 +12,8 @@
 //   k_fvtd_y       the same for y faces
 //   k_fvtd_update  source term (h-only WENO5 point values times -g db) and
 //                  UpdateTimeDerivative: Ut(2:3) and face mass fluxes / dx
-// The Patankar assembly fuses the first Jacobi sweep.  Every Jacobi sweep
+// The Patankar assembly fuses the first Jacobi sweep and carries the DeC
+// momentum update of the same nodes (disjoint arrays).  Every Jacobi sweep
 // writes per-block sums of |x_new - x_old|; a one-block-per-job kernel takes
 // the stop decision of src/jacobi_iteration.f90:71 on the device.  No grid-wide
 // fences are used: on a multi-XCD device an agent-scope fence writes back the
@@ -455,23 +456,27 @@ __global__ __launch_bounds__(256) void k_fvtd_update(EngineArgs E, const TileRef
 // :717-721): Ua(m,2:3) = Ua(1,2:3); DO r=1,R: Ua(m) = Ua(m) + (dt*theta(m,r))*FUp(r)
 // with R = MSteps.
 // ------------------------------------------------------------------------
+// One cell, every system's node m.  It runs inside the assembly kernel (the
+// two read and write disjoint arrays), so that its loads overlap the
+// assembly's latency-bound work.
 template <int R>
-__global__ __launch_bounds__(256) void k_momentum(EngineArgs E, const TileRef* blocks,
-                                                  SysList S) {
-  const TileRef T = blocks[blockIdx.x];
-  const JobDev& J = E.jobs[T.job];
-  if (skipped(E, J)) return;
-  const int k = T.tile * kCellBlock + threadIdx.x;
-  if (k >= J.n) return;
-  const double dt = J.status->dt;
+__device__ __forceinline__ void momentum_cell(const JobDev& J, const SysList& S, int k,
+                                              double dt) {
+  double fu2[R], fu3[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    fu2[r] = G(J.FU2[S.src[r]])[k];
+    fu3[r] = G(J.FU3[S.src[r]])[k];
+  }
+  const double u0 = G(J.hu)[k], v0 = G(J.hv)[k];
   for (int s = 0; s < S.count; ++s) {
     const int m = S.m[s];
-    double u = G(J.hu)[k], v = G(J.hv)[k];
+    double u = u0, v = v0;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const double c = dt * THETA[S.theta][m][r];
-      u = u + c * G(J.FU2[S.src[r]])[k];
-      v = v + c * G(J.FU3[S.src[r]])[k];
+      u = u + c * fu2[r];
+      v = v + c * fu3[r];
     }
     G(J.hun[m])[k] = u;
     G(J.hvn[m])[k] = v;
@@ -687,6 +692,7 @@ __global__ __launch_bounds__(256) void k_assemble(EngineArgs E, const TileRef* b
     bW = rhs[kW];
     bE = rhs[kE];
     bN = rhs[kN];
+    momentum_cell<R>(J, S, k, dt);
   }
   // One system at a time, and within it one stencil slot at a time (S, W, E,
   // N: the CSR order), so that only one slot's divisions are live.  Each
@@ -1267,15 +1273,6 @@ void launch_fvtd(const EngineArgs& E, int shape, const TileRef* xt, int nxt, con
 
 void launch_keep_state(const EngineArgs& E, const TileRef* cb, int ncb, int m, hipStream_t s) {
   if (ncb) hipLaunchKernelGGL(k_keep_state, dim3(ncb), dim3(256), 0, s, E, cb, m);
-}
-
-void launch_momentum(const EngineArgs& E, const TileRef* cb, int ncb, const SysList& S,
-                     hipStream_t s) {
-  if (!ncb) return;
-  if (S.nr == 2)
-    hipLaunchKernelGGL(k_momentum<2>, dim3(ncb), dim3(256), 0, s, E, cb, S);
-  else
-    hipLaunchKernelGGL(k_momentum<4>, dim3(ncb), dim3(256), 0, s, E, cb, S);
 }
 
 void launch_assemble(const EngineArgs& E, const TileRef* cb, int ncb, const int* jl, int njobs,
