@@ -740,8 +740,9 @@ int step_body_graph(swm_engine* e, Tables& T, const StepPlan& P) {
 // Autotuner over (tile shape, graph, margin).  The first speculative steps of
 // an engine time the candidates one step each, in interleaved rounds (so that
 // a drift of the clocks during tuning favours none of them; a candidate's
-// fastest round counts): the face-kernel tile shapes first, without a graph,
-// then graph on/off x margin 1/2 with the faster shape; the fastest of the
+// fastest round counts): the face-kernel tile shapes first, without a graph
+// and by the face kernels' own time (HIP events), then graph on/off x margin
+// 1/2 with the faster shape, by the step's wall time; the fastest of the
 // latter is kept.  A step that captured a graph is not timed (the candidate
 // is timed on its next step, a replay, unless it captured twice in a row),
 // and each time is divided by the cell-steps of the job set it ran on, so
@@ -893,16 +894,31 @@ int engine_batch(swm_engine* e, Tables& T, const SchemeDef* sc, int hp_assoc,
       P.sweeps[kk] = std::min(mx + margin, e->max_iter);
       first += ns;
     }
+    // a tile-shape candidate is timed by HIP events around the face kernels
+    // (the only launches the shape changes), unless the caller is recording
+    // events itself or has set graph replay on (events keep it off); the
+    // other candidates by the step's wall time
+    const bool face_events =
+        tuning(e) && tune_id(e->tune_phase) < kNumShapes && !e->prof.on && !graph;
+    if (face_events) e->prof.on = true;
     const auto t0 = std::chrono::steady_clock::now();
     const long long captures0 = e->graph_captures;
-    for (int b = 0; b < steps; ++b) {
-      int rc = (graph && !e->prof.on) ? step_body_graph(e, T, P) : step_full(e, T, P);
-      if (rc) return rc;
-    }
-    int rc = read_status(e);
-    if (rc) return rc;
-    const double ms =
+    int rc = SWM_OK;
+    for (int b = 0; b < steps && !rc; ++b)
+      rc = (graph && !e->prof.on) ? step_body_graph(e, T, P) : step_full(e, T, P);
+    if (!rc) rc = read_status(e);
+    double ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (face_events) {
+      e->prof.on = false;
+      double cat_ms[SWM_PROF_NCAT] = {};
+      long long cat_n[SWM_PROF_NCAT] = {};
+      if (!rc) rc = swm_engine_profile_read(e, cat_ms, cat_n);  // also clears the record
+      e->prof.cats.clear();
+      e->prof.used = 0;
+      ms = cat_ms[0] + cat_ms[1];  // fvtd_x + fvtd_y
+    }
+    if (rc) return rc;
     e->spec_steps += steps;
     bool overrun = false;
     for (int q = 0; q < T.njobs; ++q) overrun = overrun || e->hstatus[T.jl[q]].halt == kHaltOverrun;
