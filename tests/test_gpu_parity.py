@@ -176,6 +176,44 @@ def test_dam_break_dry_bed_nan_semantics():
         s.run_steps(1)
 
 
+def threshold_state(n, ny, seed, levels):
+    """Depths at and around MIN_DEPTH (1e-6) in the lower half of the grid,
+    wet ones in the upper half: the Patankar assembly sees 64-cell waves with
+    no depth below MIN_DEPTH, and waves that mix both."""
+    rng = np.random.default_rng(seed)
+    U = np.empty((ny, n, 3))
+    h = 0.5 + 0.5 * rng.random((ny, n))
+    h[ny // 2:] = rng.choice(np.asarray(levels), size=(ny - ny // 2, n))
+    U[..., 0] = h
+    U[..., 1] = h * rng.normal(0, 0.2, (ny, n))
+    U[..., 2] = h * rng.normal(0, 0.2, (ny, n))
+    return P.Problem(n, U, kind="random", ny=ny)
+
+
+@pytest.mark.parametrize("levels", [
+    (1e-9, 9.9e-7, 1e-6, 1.0000001e-6, 3e-6, 0.3, 1.0),
+    (0.0, 1e-9, 9.9e-7, 1e-6, 1.0000001e-6, 3e-6, 0.3, 1.0)], ids=["tiny", "with_zero"])
+@pytest.mark.parametrize("scheme", [5, 7])
+def test_depths_around_min_depth_match_oracle(levels, scheme):
+    """PATANKAR_DIV's h < MIN_DEPTH branch (src/timediscretization.f90:
+    913-924) in the assembly, beside rows with no such depth.  The momentum
+    fluxes turn NaN near the tiny depths (as in the reference); the depths
+    stay finite and are compared bit for bit, with every solve's sweeps."""
+    p = threshold_state(40, 24, 3, levels)
+    o = Oracle(p, scheme)
+    s = swm.Solver.from_problem(p)
+    s.set_timescheme(scheme)
+    dt_r = o.timestep()
+    dt = s.timestep()
+    assert same_bits(dt, dt_r)
+    it_r, nf_r = o.step(dt_r[0])
+    it, nf = s.step(dt[0])
+    assert list(it) == list(it_r) and nf == nf_r
+    U_r, U = o.get_U(), s.download()
+    assert np.isfinite(U_r[..., 0]).all() and (U_r[..., 0] < 1e-6).any()
+    assert same_bits(U, U_r), describe_diff(U, U_r)
+
+
 def test_lake_at_rest_is_preserved_exactly():
     p = P.lake_bumps(48, 11, amp=0.0)
     s = swm.Solver.from_problem(p)
