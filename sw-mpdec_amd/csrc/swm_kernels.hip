@@ -575,6 +575,15 @@ __device__ __forceinline__ PdDepth pd_depth(double h) {
   return d;
 }
 
+// PATANKAR_DIV(a, h) with h the neighbour's depth (useL) or the row's own.
+// WET: no depth of the row is below MIN_DEPTH, so no quotient becomes 0.
+template <bool WET>
+__device__ __forceinline__ double pd_quot(bool useL, double a, const PdDepth& L,
+                                          const PdDepth& K) {
+  const double q = ((useL ? L.twoh : K.twoh) * a) / (useL ? L.den : K.den);
+  return (!WET && (useL ? L.dry : K.dry)) ? 0. : q;
+}
+
 // One stencil slot of a row at the R DeC nodes (src/timediscretization.f90:
 // 800-884): with P and De the slot's production and destruction,
 //   theta > 0: M = M - c*PD(P, hL), D = D + c*PD(De, hK);
@@ -584,14 +593,8 @@ __device__ __forceinline__ PdDepth pd_depth(double h) {
 // De = s < 0 ? -s : -0 (src/finitevolume.f90:496-523).  At most one of P and
 // De is nonzero, and the nonzero one is |s|, so each node needs one division;
 // the zero one takes PdDepth's zp or zm, the value PATANKAR_DIV gives it.
-// The sign of theta (bit r of pos) is the same for the whole launch.  WET:
-// no depth of the row is below MIN_DEPTH, so no quotient is replaced by 0.
-template <int R, bool WET>
-__device__ __forceinline__ double pd_quot(bool useL, double a, const PdDepth& L,
-                                          const PdDepth& K) {
-  const double q = ((useL ? L.twoh : K.twoh) * a) / (useL ? L.den : K.den);
-  return (!WET && (useL ? L.dry : K.dry)) ? 0. : q;
-}
+// The sign of theta (bit r of pos) is the same for the whole launch, so the
+// branch on it is uniform.
 template <int R, bool WET>
 __device__ __forceinline__ void patankar_slot(int pos, const double c[R], const double s[R],
                                               const PdDepth& L, const PdDepth& K, double& M,
@@ -602,13 +605,13 @@ __device__ __forceinline__ void patankar_slot(int pos, const double c[R], const 
     const double sr = s[r];
     const bool gt = sr > 0., lt = sr < 0.;
     if ((pos >> r) & 1) {  // L takes P, K takes De
-      const double dv = pd_quot<R, WET>(gt, fabs(sr), L, K);
+      const double dv = pd_quot<WET>(gt, fabs(sr), L, K);
       const double tL = c[r] * (gt ? dv : L.zp);
       const double tK = c[r] * (lt ? dv : K.zm);
       M = M - tL;
       D = D + tK;
     } else {  // L takes De, K takes P
-      const double dv = pd_quot<R, WET>(lt, fabs(sr), L, K);
+      const double dv = pd_quot<WET>(lt, fabs(sr), L, K);
       const double tL = c[r] * (lt ? dv : L.zm);
       const double tK = c[r] * (gt ? dv : K.zp);
       M = M + tL;
