@@ -114,9 +114,15 @@ extern "C" int swm_timestep_noalloc(int nx, int ny, double dx, double dy, double
   }
   int dev = -1;
   hipError_t e = hipGetDevice(&dev);
-  if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+  if (e != hipSuccess) {
     set_thread_error(std::string("swm_timestep_noalloc: hipGetDevice failed: ") +
                      hipGetErrorString(e));
+    return SWM_E_HIP;
+  }
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_thread_error("swm_timestep_noalloc: device " + std::to_string(dev) +
+                     " is beyond the " + std::to_string(kMaxDevices) +
+                     " devices the library keeps scratch addresses for");
     return SWM_E_HIP;
   }
   double* part = nullptr;
