@@ -900,6 +900,7 @@ int engine_batch(swm_engine* e, Tables& T, const SchemeDef* sc, int hp_assoc,
     // other candidates by the step's wall time
     const bool face_events =
         tuning(e) && tune_id(e->tune_phase) < kNumShapes && !e->prof.on && !graph;
+    const size_t rec0 = e->prof.cats.size();  // records a caller has not read yet stay
     if (face_events) e->prof.on = true;
     const auto t0 = std::chrono::steady_clock::now();
     const long long captures0 = e->graph_captures;
@@ -911,12 +912,22 @@ int engine_batch(swm_engine* e, Tables& T, const SchemeDef* sc, int hp_assoc,
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (face_events) {
       e->prof.on = false;
-      double cat_ms[SWM_PROF_NCAT] = {};
-      long long cat_n[SWM_PROF_NCAT] = {};
-      if (!rc) rc = swm_engine_profile_read(e, cat_ms, cat_n);  // also clears the record
-      e->prof.cats.clear();
-      e->prof.used = 0;
-      ms = cat_ms[0] + cat_ms[1];  // fvtd_x + fvtd_y
+      double face_ms = 0.;  // fvtd_x + fvtd_y of this step's records, which are then dropped
+      for (size_t q = rec0; !rc && q < e->prof.cats.size(); ++q) {
+        if (e->prof.cats[q] > 1) continue;
+        float t = 0.f;
+        const hipError_t er =
+            hipEventElapsedTime(&t, e->prof.pool[2 * q], e->prof.pool[2 * q + 1]);
+        if (er != hipSuccess) {
+          e->err = std::string("hipEventElapsedTime failed: ") + hipGetErrorString(er);
+          g_error = e->err;
+          rc = SWM_E_HIP;
+        }
+        face_ms += t;
+      }
+      e->prof.cats.resize(rec0);
+      e->prof.used = 2 * rec0;
+      ms = face_ms;
     }
     if (rc) return rc;
     e->spec_steps += steps;

@@ -295,7 +295,7 @@ __device__ __forceinline__ void positivity_limit(double vh, const double wbnd[4]
     // |num| >= |den| with num finite makes the rounded |num/den| >= 1, so
     // MIN(1, |num/den|) is 1 without the division (the usual case: the
     // Gauss values lie close to the mean)
-    else if ((fabs(num) >= fabs(den)) & (fabs(num) <= kHuge))
+    else if ((int)(fabs(num) >= fabs(den)) & (int)(fabs(num) <= kHuge))
       theta = 1.;
 #endif
     else
