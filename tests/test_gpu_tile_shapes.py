@@ -84,6 +84,24 @@ def test_autotuner_settles_on_a_shape():
         eng.set_tile_shape(2)
 
 
+def test_tuner_leaves_unread_profile_records():
+    """The tuner times the tile shapes by HIP events of its own; records a
+    caller took with profiling on and has not read yet stay as they were."""
+    p = P.vortex(48, 5)
+    s = swm.Solver.from_problem(p)
+    s.profile(True)
+    s.run_steps(1)  # the polling step: 5 launches of each face kernel
+    s.profile(False)
+    s.run_steps(20)  # the tuning steps
+    rec = s.profile_read()
+    assert rec["fvtd_x"][1] == 5 and rec["fvtd_y"][1] == 5
+    assert rec["fvtd_x"][0] > 0.
+    U = s.download()
+    o = Oracle(p)
+    o.run(21)
+    assert same_bits(U, o.get_U())
+
+
 def test_engine_with_wide_tiles_matches_oracle():
     """An engine forced to the wide shape, graph replay on, with tiny,
     rectangular and larger grids under several schemes."""
