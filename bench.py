@@ -157,15 +157,26 @@ def cpu_baseline(budget_s: float = 20.0, scheme: int = 5, recfix: int = 0):
 
 
 def cpu_model() -> str:
-    """The host CPU's model name and the cores this process may use."""
-    name = "unknown"
+    """The host CPU's model name (with its family, model and stepping numbers,
+    which identify the core generation where a virtual machine shortens the
+    name) and the cores this process may use."""
+    info = {}
     try:
         for line in Path("/proc/cpuinfo").read_text().splitlines():
-            if line.startswith("model name"):
-                name = line.split(":", 1)[1].strip()
-                break
+            key, _, val = line.partition(":")
+            key = key.strip()
+            if key in ("model name", "cpu family", "model", "stepping", "cpu MHz") \
+                    and key not in info:
+                info[key] = val.strip()
+            if not line.strip() and info:
+                break  # the first processor's block
     except OSError:
         pass
+    name = info.get("model name", "unknown")
+    ids = ", ".join(f"{k} {info[k]}" for k in ("cpu family", "model", "stepping", "cpu MHz")
+                    if k in info)
+    if ids:
+        name = f"{name} ({ids})"
     return f"{name}; {len(os.sched_getaffinity(0))} cores available, 1 used (taskset -c 1)"
 
 
