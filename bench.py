@@ -636,6 +636,26 @@ def main():
         "kernels": kern,
         "roofline": roof,
     }
+    # the boundary's host transfers: U in the reference's own layout through
+    # swm_upload / swm_download (pageable host memory, as a Fortran caller
+    # hands it over); not part of `value`, whose state stays on the device
+    U = s.download()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        U = s.download()
+    el_d = (time.perf_counter() - t0) / 3
+    t0 = time.perf_counter()
+    for _ in range(3):
+        s.upload(U)
+    s.sync()
+    el_u = (time.perf_counter() - t0) / 3
+    host_ms = 1e3 * (el_d + el_u)
+    out["boundary_transfer"] = {
+        "bytes_each_way": int(U.nbytes), "download_ms": 1e3 * el_d, "upload_ms": 1e3 * el_u,
+        "cell_steps_per_s_with_upload_and_download_every_step":
+            n * n / (1e-3 * (ms_per_step + host_ms)),
+        "note": "U (ny, nx, 3) float64 through swm_download / swm_upload from pageable host "
+                "memory; the drop-in moves U only at output times"}
     del s
     if args.scheme_steps > 0:
         out["schemes"] = schemes_section(n, args.seed, args.scheme_steps, not args.no_cpu)
