@@ -319,7 +319,11 @@ int swm_tile_shape(const swm_solver_t *h, int dims[4]);
 long long swm_ambiguous_decisions(const swm_solver_t *h);
 
 /* Measurement (no counterpart in the reference, which has no timers): HIP
- * events around every kernel launch of the handle, summed per category --
+ * events around every kernel launch of the handle (swm_profile_enable), or
+ * around the launches of the categories whose bit is set in mask
+ * (swm_profile_categories; bit c for category c, 0 turns recording off),
+ * summed per category.  While any category is recorded, steps run their
+ * launches one by one (no HIP-graph replay).  Categories --
  * 0 fvtd_x, 1 fvtd_y, 2 fvtd_update, 3 momentum (the explicit schemes'
  * stage combinations; the modified-Patankar schemes' momentum update runs
  * inside the assembly kernel), 4 patankar_assemble (with that momentum
@@ -328,6 +332,7 @@ long long swm_ambiguous_decisions(const swm_solver_t *h);
  * stream, returns milliseconds and launch counts, and clears the record. */
 #define SWM_PROF_NCAT 8
 int swm_profile_enable(swm_solver_t *h, int on);
+int swm_profile_categories(swm_solver_t *h, unsigned mask);
 int swm_profile_read(swm_solver_t *h, double ms[SWM_PROF_NCAT], long long launches[SWM_PROF_NCAT]);
 const char *swm_profile_name(int category);
 
@@ -443,6 +448,7 @@ int swm_engine_tile_shape(const swm_engine_t *e, int dims[4]);
 int swm_engine_track_stage_min(swm_engine_t *e, int on);
 int swm_engine_stage_min_h(swm_engine_t *e, int job_id, double *hmin);
 int swm_engine_profile_enable(swm_engine_t *e, int on);
+int swm_engine_profile_categories(swm_engine_t *e, unsigned mask);
 int swm_engine_profile_read(swm_engine_t *e, double ms[SWM_PROF_NCAT],
                             long long launches[SWM_PROF_NCAT]);
 

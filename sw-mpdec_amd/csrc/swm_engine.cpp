@@ -37,7 +37,7 @@ const char* const PROF_NAMES[SWM_PROF_NCAT] = {
 
 // HIP-event timing of kernel launches by category.
 struct Profiler {
-  bool on = false;
+  unsigned mask = 0;  // bit c: record the launches of category c; 0: off
   std::vector<hipEvent_t> pool;
   size_t used = 0;
   std::vector<int> cats;
@@ -267,7 +267,7 @@ namespace {
 
 template <class F>
 void timed(swm_engine* e, int cat, F&& launch) {
-  if (!e->prof.on) {
+  if (!((e->prof.mask >> cat) & 1u)) {
     launch();
     return;
   }
@@ -899,19 +899,19 @@ int engine_batch(swm_engine* e, Tables& T, const SchemeDef* sc, int hp_assoc,
     // events itself or has set graph replay on (events keep it off); the
     // other candidates by the step's wall time
     const bool face_events =
-        tuning(e) && tune_id(e->tune_phase) < kNumShapes && !e->prof.on && !graph;
+        tuning(e) && tune_id(e->tune_phase) < kNumShapes && !e->prof.mask && !graph;
     const size_t rec0 = e->prof.cats.size();  // records a caller has not read yet stay
-    if (face_events) e->prof.on = true;
+    if (face_events) e->prof.mask = 0x3u;     // fvtd_x, fvtd_y
     const auto t0 = std::chrono::steady_clock::now();
     const long long captures0 = e->graph_captures;
     int rc = SWM_OK;
     for (int b = 0; b < steps && !rc; ++b)
-      rc = (graph && !e->prof.on) ? step_body_graph(e, T, P) : step_full(e, T, P);
+      rc = (graph && !e->prof.mask) ? step_body_graph(e, T, P) : step_full(e, T, P);
     if (!rc) rc = read_status(e);
     double ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (face_events) {
-      e->prof.on = false;
+      e->prof.mask = 0;
       double face_ms = 0.;  // fvtd_x + fvtd_y of this step's records, which are then dropped
       for (size_t q = rec0; !rc && q < e->prof.cats.size(); ++q) {
         if (e->prof.cats[q] > 1) continue;
@@ -1727,8 +1727,14 @@ int swm_engine_status_reads(const swm_engine_t* e, long long* n) {
 }
 
 int swm_engine_profile_enable(swm_engine_t* e, int on) {
+  return swm_engine_profile_categories(e, on ? (1u << SWM_PROF_NCAT) - 1u : 0u);
+}
+
+int swm_engine_profile_categories(swm_engine_t* e, unsigned mask) {
   if (!e) return SWM_E_ARG;
-  e->prof.on = on != 0;
+  if (mask >> SWM_PROF_NCAT)
+    FAIL(e, SWM_E_ARG, "profile mask 0x%x has bits beyond the %d categories", mask, SWM_PROF_NCAT);
+  e->prof.mask = mask;
   return SWM_OK;
 }
 
@@ -2300,6 +2306,11 @@ long long swm_ambiguous_decisions(const swm_solver_t* s) { return s ? s->e->ambi
 int swm_profile_enable(swm_solver_t* s, int on) {
   if (!s) return SWM_E_ARG;
   return swm_engine_profile_enable(s->e, on);
+}
+
+int swm_profile_categories(swm_solver_t* s, unsigned mask) {
+  if (!s) return SWM_E_ARG;
+  return swm_engine_profile_categories(s->e, mask);
 }
 
 int swm_profile_read(swm_solver_t* s, double ms[SWM_PROF_NCAT], long long launches[SWM_PROF_NCAT]) {

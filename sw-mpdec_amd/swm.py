@@ -24,6 +24,18 @@ _ip = ctypes.POINTER(ctypes.c_int)
 
 SWM_OK = 0
 PROF_NCAT = 8
+# the event categories, by index (swm_profile_name)
+PROF_NAMES = ("fvtd_x", "fvtd_y", "fvtd_update", "momentum", "patankar_assemble",
+              "jacobi_sweep", "timestep", "other")
+
+
+def _prof_mask(categories):
+    mask = 0
+    for c in categories:
+        if c not in PROF_NAMES:
+            raise ValueError(f"unknown profile category {c!r}; one of {PROF_NAMES}")
+        mask |= 1 << PROF_NAMES.index(c)
+    return mask
 
 
 class SwmError(RuntimeError):
@@ -129,6 +141,7 @@ def lib():
     L.swm_stage_min_h.argtypes = [vp, _dp]
     L.swm_debug_weno5.argtypes = [ctypes.c_int, _dp, _dp, _dp]
     L.swm_profile_enable.argtypes = [vp, ctypes.c_int]
+    L.swm_profile_categories.argtypes = [vp, ctypes.c_uint]
     L.swm_profile_read.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
     L.swm_profile_name.restype = ctypes.c_char_p
     L.swm_profile_name.argtypes = [ctypes.c_int]
@@ -420,8 +433,14 @@ class Solver:
     def sync(self):
         self._check(lib().swm_sync(self._h), "swm_sync")
 
-    def profile(self, on=True):
-        self._check(lib().swm_profile_enable(self._h, int(bool(on))), "swm_profile_enable")
+    def profile(self, on=True, categories=None):
+        """HIP events around every launch (on), or only around the launches
+        of the named categories (PROF_NAMES entries)."""
+        if categories is None:
+            self._check(lib().swm_profile_enable(self._h, int(bool(on))), "swm_profile_enable")
+        else:
+            self._check(lib().swm_profile_categories(self._h, _prof_mask(categories) if on else 0),
+                        "swm_profile_categories")
 
     def profile_read(self):
         """{kernel category: (milliseconds, launches)} since the last read."""
@@ -527,6 +546,7 @@ def _engine_lib():
     L.swm_engine_release.argtypes = [vp, ctypes.c_int]
     L.swm_engine_set_jacobi_limits.argtypes = [vp, ctypes.c_int, ctypes.c_double]
     L.swm_engine_profile_enable.argtypes = [vp, ctypes.c_int]
+    L.swm_engine_profile_categories.argtypes = [vp, ctypes.c_uint]
     L.swm_engine_profile_read.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
     L.swm_engine_track_stage_min.argtypes = [vp, ctypes.c_int]
     L.swm_engine_set_execution.argtypes = [vp, ctypes.c_int, ctypes.c_int]
@@ -770,9 +790,13 @@ class Engine:
                     "swm_engine_stage_min_h")
         return v.value
 
-    def profile(self, on=True):
-        self._check(_engine_lib().swm_engine_profile_enable(self._e, int(bool(on))),
-                    "swm_engine_profile_enable")
+    def profile(self, on=True, categories=None):
+        if categories is None:
+            self._check(_engine_lib().swm_engine_profile_enable(self._e, int(bool(on))),
+                        "swm_engine_profile_enable")
+        else:
+            self._check(_engine_lib().swm_engine_profile_categories(
+                self._e, _prof_mask(categories) if on else 0), "swm_engine_profile_categories")
 
     def profile_read(self):
         ms = (ctypes.c_double * PROF_NCAT)()

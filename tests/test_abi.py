@@ -65,6 +65,19 @@ def test_profile_names():
     L = swm.lib()
     names = [L.swm_profile_name(c).decode() for c in range(swm.PROF_NCAT)]
     assert names[0] == "fvtd_x" and names[5] == "jacobi_sweep"
+    assert tuple(names) == swm.PROF_NAMES
+
+
+def test_profile_categories_arguments():
+    """The category mask names categories by bit; a null handle is refused
+    before anything touches the GPU."""
+    L = swm.lib()
+    assert L.swm_profile_categories(None, 3) == -1
+    assert L.swm_engine_profile_categories(None, 3) == -1
+    assert swm._prof_mask(["fvtd_x", "fvtd_y"]) == 3
+    assert swm._prof_mask(["other"]) == 1 << 7
+    with pytest.raises(ValueError):
+        swm._prof_mask(["fvtd_z"])
 
 
 def test_header_declares_the_rows_around_the_path():
