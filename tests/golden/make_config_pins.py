@@ -4,8 +4,8 @@ full sizes, where a fixture of the arrays themselves would be too large.
 * 2048 x 2048 lake at rest over 8 seeded Gaussian bumps (seed 2024) with a
   1e-3 surface perturbation, well-balanced, CFL 0.5 (the headline workload):
   SHA-256 of the inputs U0, Uwb, dbdx, dbdy as the host problem builder makes
-  them, then two mPDeC5 steps of the C oracle: each step's dt, its 13 Jacobi
-  sweep counts and the SHA-256 of the state after it.
+  them, then LAKE_STEPS mPDeC5 steps of the C oracle: each step's dt, its 13
+  Jacobi sweep counts and the SHA-256 of the state after it.
 * 1024 x 1024 circular dam break over a dry bed (h = 1 inside radius 0.25,
   h = 0 outside): SHA-256 of U0, then one mPDeC5 step of the C oracle AND of
   the reference Fortran itself (oracle/_ref/swref_p, built unchanged from its
@@ -16,7 +16,7 @@ full sizes, where a fixture of the arrays themselves would be too large.
 The inputs depend only on IEEE arithmetic (the families' exponential is the
 fixed method of problems.det_exp), so any host rebuilds them bitwise; the GPU
 test checks the input hashes first.  Run from the repository root after
-__graft_entry__.build() (about 8 minutes on one core):
+__graft_entry__.build() (about 25 minutes on one core):
 
     python tests/golden/make_config_pins.py
 """
@@ -39,6 +39,7 @@ import problems as P  # noqa: E402
 from oracle_api import Oracle, run_swref  # noqa: E402
 
 OUT = HERE / "config_pins.json"
+LAKE_STEPS = 8
 NAN = np.frombuffer(np.uint64(0x7FF8000000000000).tobytes(), dtype=np.float64)[0]
 
 
@@ -60,7 +61,7 @@ def lake_2048() -> dict:
     print(f"lake 2048: inputs built in {time.time() - t0:.1f} s", flush=True)
     o = Oracle(p)
     steps = []
-    for k in range(2):
+    for k in range(LAKE_STEPS):
         t0 = time.time()
         dt, lmx, lmy = o.timestep()
         iters, nonconv = o.step(dt)

@@ -8,10 +8,11 @@ and the benchmark families made on the device against the host builder.
   that swm_fill_family makes are bitwise the host builder's (directly for the
   state; through the FVTD outputs and a step for the others).
 * 2048^2 lake over bumps (the headline workload): the host-built inputs hash
-  to the pinned values (so this host rebuilt them bitwise); two mPDeC5 steps
-  from them, and from the same data made on the device, give the oracle's dt,
-  its 26 Jacobi sweep counts and its state hashes.  This covers the certified
-  Jacobi stop at n = 4.19M cells and the 64-bit indexing at that size.
+  to the pinned values (so this host rebuilt them bitwise); eight mPDeC5
+  steps from them, and from the same data made on the device, give the
+  oracle's dt, its 104 Jacobi sweep counts and its state hashes after every
+  step.  This covers the certified Jacobi stop at n = 4.19M cells, the
+  speculative steps after the first, and the 64-bit indexing at that size.
 * 1024^2 dam break over a dry bed: the oracle's (and the reference
   Fortran's, equal to it) first step, NaN velocities included.
 """
