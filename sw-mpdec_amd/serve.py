@@ -332,9 +332,13 @@ def main(argv=None) -> int:
     ap.add_argument("--count", type=int, default=0, help="requests from --first (0: all)")
     ap.add_argument("--sjf-cells", type=int, default=1 << 22)
     ap.add_argument("--compare", nargs="+", metavar="JSON")
+    ap.add_argument("--profile-batch", type=int, metavar="JOBS",
+                    help="kernel times of the seeded batched mix of JOBS jobs")
     a = ap.parse_args(argv)
     if a.compare:
         return compare(a.compare)
+    if a.profile_batch:
+        return profile_batch(a.profile_batch)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     reqs = stream(a.first, a.count)
     arrival = {r.rid: r.arrival for r in reqs}
@@ -357,6 +361,23 @@ def main(argv=None) -> int:
     with open(a.out, "w") as f:
         json.dump(out, f)
     log(f"serve: {a.mode} done, {out['jobs_per_s']:.3f} jobs/s, makespan {out['makespan_s']:.1f} s")
+    return 0
+
+
+def profile_batch(jobs: int, seed: int = 2024) -> int:
+    """Prints the batched mix's kernel time per category (HIP events around
+    every launch), per 10^9 cell-steps, and the mix's rate with and without
+    the events."""
+    specs = P.mixed_batch(jobs, seed)
+    W = sum(p.n * p.rows * p.steps for p in specs)
+    _, secs, esteps, prof, _ = run_batch(specs, [p.steps for p in specs], profile=True)
+    _, secs_plain, _, _, _ = run_batch(specs, [p.steps for p in specs])
+    print(json.dumps({
+        "workload": f"{jobs} jobs, seed {seed} (the batched mix)", "cell_steps": W,
+        "engine_steps": esteps,
+        "rate_with_events": W / secs, "rate_without_events": W / secs_plain,
+        "ms_per_1e9_cell_steps": {k: v[0] * 1e9 / W for k, v in prof.items() if v[1]},
+        "launches": {k: v[1] for k, v in prof.items() if v[1]}}, indent=1), flush=True)
     return 0
 
 
