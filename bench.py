@@ -155,7 +155,45 @@ def cpu_baseline(budget_s: float = 20.0, scheme: int = 5, recfix: int = 0):
         _, its, _ = g.run_steps(k)
         g.close()
         out["jacobi_sweeps_per_step"] = float(np.sum(its)) / k
+    if scheme == 5 and not recfix:
+        out["port_on_headline_family"] = port_baseline(budget_s=min(budget_s, 15.0))
     return out
+
+
+PORT_RUN = """
+import json, sys, time
+sys.path[:0] = [sys.argv[1] + '/sw-mpdec_amd', sys.argv[1] + '/tests']
+import problems as P
+from oracle_api import Oracle
+n, budget = int(sys.argv[2]), float(sys.argv[3])
+o = Oracle(P.lake_bumps_spec(n, 2024).build())
+t0, k, sweeps = time.perf_counter(), 0, 0
+while k == 0 or time.perf_counter() - t0 < budget:
+    dt, _, _ = o.timestep()
+    its, _ = o.step(dt)
+    k, sweeps = k + 1, sweeps + sum(its)
+el = time.perf_counter() - t0
+print(json.dumps({"steps": k, "seconds": el, "sweeps": sweeps}))
+"""
+
+
+def port_baseline(budget_s: float = 15.0, n: int = 256):
+    """The C restatement of the reference (oracle/, FP contraction off, one
+    core) on the headline's own family -- the lake over 8 Gaussian bumps,
+    seed 2024 -- at n x n, which the reference's Fortran cannot take (its
+    bathymetry is built in): cell-steps/s over as many steps as fit the
+    budget."""
+    cmd = [sys.executable, "-c", PORT_RUN, str(ROOT), str(n), str(budget_s)]
+    if os.path.exists("/usr/bin/taskset"):
+        cmd = ["taskset", "-c", "1"] + cmd
+    r = subprocess.run(cmd, check=True, capture_output=True, text=True, timeout=600,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    return {"value": n * n * res["steps"] / res["seconds"], "unit": "cell-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"the C oracle (gcc, -ffp-contract=off) on lake_bumps {n}x{n} seed 2024, "
+                      f"{res['steps']} CFL steps, {res['seconds']:.1f} s",
+            "jacobi_sweeps_per_step": res["sweeps"] / res["steps"]}
 
 
 def cpu_model() -> str:
@@ -711,6 +749,9 @@ def main():
         if cb:
             out["cpu_baseline"] = cb
             out["gpu_vs_cpu"] = value / cb["value"]
+            port = cb.get("port_on_headline_family")
+            if port:
+                out["gpu_vs_port_on_headline_family"] = value / port["value"]
     print(json.dumps(out), flush=True)
 
 
